@@ -1,0 +1,46 @@
+"""In-tree builds: the HIP library (gfx950) and, for tests, the CPU oracle."""
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+SOURCES = ["idw_api.hip", "idw_kernels.hip", "idw_tree.cpp"]
+LIB = os.path.join(HERE, "libidwarp_hip.so")
+
+HIPCC_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-munsafe-fp-atomics",
+    # keep packed-FP32 instructions out of the generated code
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+
+def _run(cmd, cwd):
+    r = subprocess.run(cmd, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l)
+    if r.returncode != 0:
+        raise RuntimeError("command failed (%d): %s\n%s" % (r.returncode, " ".join(cmd), out))
+    return out
+
+
+def build_library():
+    """Compile libidwarp_hip.so with hipcc for gfx950 (always rebuilds)."""
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    tmp = LIB + ".tmp"
+    _run([hipcc] + HIPCC_FLAGS + ["-o", tmp] + SOURCES, CSRC)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle(reference="/root/reference/src"):
+    """Compile oracle/liboracle.so and, where the reference sources exist,
+    oracle/_ref/libidwref.so (test-only checkers)."""
+    targets = ["all"]
+    if os.path.isdir(reference):
+        targets.append("ref")
+    _run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "REF=" + reference] + targets, ROOT)
+
+
+if __name__ == "__main__":
+    print(build_library())
+    build_oracle()

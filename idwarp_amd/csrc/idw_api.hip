@@ -1,0 +1,629 @@
+// Context management and C ABI of libidwarp_hip.so (declared in
+// include/idwarp_hip.h).  One context holds one mesh instance: the volume
+// nodes in traversal order, the surface tree and all per-call buffers, on one
+// device and one stream.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "idw_internal.h"
+#include "idw_kernels.h"
+
+namespace {
+
+thread_local std::string t_err;
+
+int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    t_err = buf;
+    return code;
+}
+
+#define HIPCHK(x)                                                                          \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess) return fail(IDW_EHIP, "%s: %s", #x, hipGetErrorString(e_));  \
+    } while (0)
+
+// getSpherePts.F90:20-40 nominal vertices (phi = 2.1180339887498949,
+// constants.F90:32, the reference's own constant).
+void sphere_nominal(double *v) {
+    const double o = 1.0, z = 0.0, p = 2.1180339887498949, ip = 1.0 / p;
+    const double t[20][3] = {{-o, -o, -o}, {o, -o, -o}, {-o, o, -o}, {o, o, -o}, {-o, -o, o}, {o, -o, o}, {-o, o, o},
+                             {o, o, o},    {z, -ip, -p}, {z, ip, -p}, {z, -ip, p}, {z, ip, p}, {-ip, -p, z}, {ip, -p, z},
+                             {-ip, p, z},  {ip, p, z},  {-p, z, -ip}, {-p, z, ip}, {p, z, -ip}, {p, z, ip}};
+    std::memcpy(v, t, sizeof(t));
+}
+
+bool same_numeric_options(const idw_options &a, const idw_options &b) {
+    return a.LdefFact == b.LdefFact && a.alpha == b.alpha && a.bExp == b.bExp && a.errTol == b.errTol &&
+           a.evalMode == b.evalMode && a.precision == b.precision && a.nearWallFactor == b.nearWallFactor;
+}
+
+}  // namespace
+
+struct idw_ctx {
+    int device = 0;
+    hipStream_t st = nullptr;
+    idw_options opt{};
+    idw_options initOpt{};
+    idw_options denOpt{};
+    bool volSet = false, surfInit = false, denValid = false, haveDxs = false;
+    idw::HostSurface hs;
+    idw::DevSurface S;
+    idw::DevVolume V;
+    int64_t npts = 0;
+    double *xv0w = nullptr, *xv = nullptr, *xs = nullptr, *dxs = nullptr, *dxv = nullptr;
+    double *partial = nullptr;
+    int nchunk = 0;
+    std::vector<void *> allocs;
+    std::vector<int32_t> solverInd;
+    hipEvent_t ev[6] = {};
+    double ms[4] = {0, 0, 0, 0};
+};
+
+namespace {
+
+template <class T>
+int dalloc(idw_ctx *c, T **p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, count * sizeof(T));
+    if (e != hipSuccess) return fail(IDW_ENOMEM, "hipMalloc(%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
+    c->allocs.push_back(q);
+    *p = static_cast<T *>(q);
+    return IDW_OK;
+}
+
+template <class T>
+void dfree(idw_ctx *c, T *&p) {
+    if (!p) return;
+    auto it = std::find(c->allocs.begin(), c->allocs.end(), (void *)p);
+    if (it != c->allocs.end()) c->allocs.erase(it);
+    (void)hipFree(p);
+    p = nullptr;
+}
+
+#define TRY(x)                  \
+    do {                        \
+        int r_ = (x);           \
+        if (r_ != IDW_OK) return r_; \
+    } while (0)
+
+idw::WParams make_params(const idw_ctx *c, bool init) {
+    const idw_options &o = init ? c->initOpt : c->opt;
+    idw::WParams P{};
+    P.Ldef = c->hs.ldef0 * o.LdefFact;
+    P.a5 = std::pow(o.alpha, o.bExp);
+    P.aExp = c->initOpt.aExp;
+    P.bExp = c->initOpt.bExp;
+    P.errTol = o.errTol;
+    const double nw = o.nearWallFactor * P.Ldef;
+    P.nearT2 = nw * nw;
+    const double a = P.aExp, b = P.bExp;
+    if (a == 3.0 && b == 5.0) {
+        P.expMode = idw::EXP_35;
+    } else if (a == std::floor(a) && b == std::floor(b) && a >= 0 && b >= 0 && a <= 64 && b <= 64) {
+        P.expMode = idw::EXP_INT;
+        P.ia = (int)a;
+        P.ib = (int)b;
+    } else {
+        P.expMode = idw::EXP_REAL;
+    }
+    return P;
+}
+
+int ensure_device(idw_ctx *c) {
+    HIPCHK(hipSetDevice(c->device));
+    return IDW_OK;
+}
+
+int compute_den0(idw_ctx *c) {
+    if (!c->volSet || !c->surfInit || c->V.n == 0) return IDW_OK;
+    idw::launch_den0(c->st, c->S, c->V, make_params(c, true));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->st));
+    c->denValid = false;
+    return IDW_OK;
+}
+
+// Surface nodal properties and, in fast mode, the cluster data.
+int surface_pass(idw_ctx *c, const double *xs, bool clusters) {
+    idw::launch_face_props(c->st, c->S, xs);
+    idw::launch_node_props(c->st, c->S, xs, 0, c->opt.useRotations);
+    if (clusters) idw::launch_cluster_data(c->st, c->S);
+    HIPCHK(hipGetLastError());
+    return IDW_OK;
+}
+
+int run_warp(idw_ctx *c, const double *xs, double *xv) {
+    if (!c->surfInit) return fail(IDW_ESTATE, "warp before initialize_warping");
+    if (!c->volSet) return fail(IDW_ESTATE, "no volume nodes set");
+    const int fast = c->opt.evalMode == IDW_EVAL_FAST;
+    HIPCHK(hipEventRecord(c->ev[0], c->st));
+    TRY(surface_pass(c, xs, fast));
+    HIPCHK(hipEventRecord(c->ev[1], c->st));
+    if (c->V.n) idw::launch_forward(c->st, c->S, c->V, make_params(c, false), fast, c->opt.precision, xv);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c->ev[2], c->st));
+    c->denOpt = c->opt;
+    c->denValid = true;
+    return IDW_OK;
+}
+
+int ensure_partial(idw_ctx *c) {
+    const int64_t nbs = (c->S.n + 255) / 256;
+    int nchunk = (int)std::min<int64_t>(128, std::max<int64_t>(1, (4096 + nbs - 1) / nbs));
+    while (nchunk > 1 && c->V.n / nchunk < 64) nchunk /= 2;
+    if (c->partial && nchunk == c->nchunk) return IDW_OK;
+    dfree(c, c->partial);
+    TRY(dalloc(c, &c->partial, (size_t)nchunk * c->S.n * 12));
+    c->nchunk = nchunk;
+    return IDW_OK;
+}
+
+int run_deriv(idw_ctx *c, const double *xs, const double *dxv, double *dxs) {
+    if (!c->surfInit) return fail(IDW_ESTATE, "warp_deriv before initialize_warping");
+    if (!c->volSet) return fail(IDW_ESTATE, "no volume nodes set");
+    const int fast = c->opt.evalMode == IDW_EVAL_FAST;
+    // The denominator depends on the options, not on the surface coordinates.
+    if (!c->denValid || !same_numeric_options(c->denOpt, c->opt)) TRY(run_warp(c, xs, c->xv));
+    if (!fast) TRY(ensure_partial(c));
+    HIPCHK(hipEventRecord(c->ev[3], c->st));
+    TRY(surface_pass(c, xs, false));
+    if (c->V.n) {
+        idw::launch_prep_numb(c->st, c->V, dxv);
+        idw::launch_reverse(c->st, c->S, c->V, make_params(c, false), fast, c->opt.precision, c->partial, c->nchunk);
+    } else {
+        idw::launch_fill(c->st, c->S.padj, (int64_t)c->S.n * idw::ADJ_STRIDE, 0.0);
+        idw::launch_fill(c->st, c->S.nadj, (int64_t)c->S.nnodes * idw::ADJ_STRIDE, 0.0);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c->ev[4], c->st));
+    idw::launch_nodal_reverse(c->st, c->S, xs, fast, c->opt.useRotations, dxs, c->npts);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c->ev[5], c->st));
+    return IDW_OK;
+}
+
+void collect_timings(idw_ctx *c, bool fwd, bool rev) {
+    float t;
+    if (fwd) {
+        if (hipEventElapsedTime(&t, c->ev[0], c->ev[1]) == hipSuccess) c->ms[0] = t;
+        if (hipEventElapsedTime(&t, c->ev[1], c->ev[2]) == hipSuccess) c->ms[1] = t;
+    }
+    if (rev) {
+        if (hipEventElapsedTime(&t, c->ev[3], c->ev[4]) == hipSuccess) c->ms[2] = t;
+        if (hipEventElapsedTime(&t, c->ev[4], c->ev[5]) == hipSuccess) c->ms[3] = t;
+    }
+}
+
+bool valid_opts(const idw_options *o) {
+    return o && o->bucketSize >= 1 && (o->evalMode == 0 || o->evalMode == 1) && (o->precision == 0 || o->precision == 1) &&
+           std::isfinite(o->aExp) && std::isfinite(o->bExp) && o->LdefFact > 0 && o->errTol >= 0 &&
+           o->nearWallFactor >= 0;
+}
+
+}  // namespace
+
+namespace idw {
+hipError_t sort_pairs(hipStream_t st, int64_t n, uint64_t *keys, int32_t *ids, uint64_t *keys_out, int32_t *ids_out) {
+    size_t bytes = 0;
+    hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, keys, keys_out, ids, ids_out, (int)n, 0, 63, st);
+    if (e != hipSuccess) return e;
+    void *tmp = nullptr;
+    e = hipMalloc(&tmp, bytes);
+    if (e != hipSuccess) return e;
+    e = hipcub::DeviceRadixSort::SortPairs(tmp, bytes, keys, keys_out, ids, ids_out, (int)n, 0, 63, st);
+    hipError_t e2 = hipStreamSynchronize(st);
+    (void)hipFree(tmp);
+    return e != hipSuccess ? e : e2;
+}
+}  // namespace idw
+
+extern "C" {
+
+void idw_default_options(idw_options *o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->aExp = 3.0;
+    o->bExp = 5.0;
+    o->LdefFact = 1.0;
+    o->alpha = 0.25;
+    o->errTol = 5e-4;
+    o->cornerAngle = 30.0;
+    o->evalMode = IDW_EVAL_FAST;
+    o->useRotations = 1;
+    o->zeroCornerRotations = 1;
+    o->bucketSize = 8;
+    o->precision = IDW_PREC_FP64;
+    o->nearWallFactor = 1e-3;
+}
+
+const char *idw_last_error(void) { return t_err.c_str(); }
+
+int idw_create(idw_ctx **out, int device, const idw_options *opt) {
+    if (!out) return fail(IDW_EINVAL, "null output pointer");
+    *out = nullptr;
+    if (!valid_opts(opt)) return fail(IDW_EINVAL, "invalid options");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(IDW_ENODEV, "no HIP device available");
+    if (device < 0 || device >= ndev) return fail(IDW_ENODEV, "device %d out of range (%d devices)", device, ndev);
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(IDW_ENODEV, "device %d is %s; this library is built for gfx950", device, prop.gcnArchName);
+    idw_ctx *c = new (std::nothrow) idw_ctx();
+    if (!c) return fail(IDW_ENOMEM, "out of host memory");
+    c->device = device;
+    c->opt = *opt;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
+    for (int k = 0; k < 6 && e == hipSuccess; ++k) e = hipEventCreate(&c->ev[k]);
+    if (e != hipSuccess) {
+        idw_destroy(c);
+        return fail(IDW_EHIP, "stream/event creation: %s", hipGetErrorString(e));
+    }
+    *out = c;
+    return IDW_OK;
+}
+
+int idw_destroy(idw_ctx *c) {
+    if (!c) return IDW_OK;
+    (void)hipSetDevice(c->device);
+    if (c->st) (void)hipStreamSynchronize(c->st);
+    for (void *p : c->allocs) (void)hipFree(p);
+    for (auto &e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->st) (void)hipStreamDestroy(c->st);
+    delete c;
+    return IDW_OK;
+}
+
+int idw_set_options(idw_ctx *c, const idw_options *opt) {
+    if (!c) return fail(IDW_EINVAL, "null context");
+    if (!valid_opts(opt)) return fail(IDW_EINVAL, "invalid options");
+    c->opt = *opt;
+    return IDW_OK;
+}
+
+int idw_set_volume_nodes(idw_ctx *c, const double *xv0, int64_t ndof) {
+    if (!c || (!xv0 && ndof)) return fail(IDW_EINVAL, "null argument");
+    if (ndof < 0 || ndof % 3) return fail(IDW_EINVAL, "volume DOF count %lld is not a multiple of 3", (long long)ndof);
+    const int64_t n = ndof / 3;
+    if (n >= INT32_MAX / 2) return fail(IDW_EINVAL, "too many volume nodes for 32-bit indexing");
+    TRY(ensure_device(c));
+    dfree(c, c->xv0w); dfree(c, c->xv); dfree(c, c->dxv);
+    dfree(c, c->V.vol); dfree(c, c->V.perm); dfree(c, c->V.den); dfree(c, c->V.numb);
+    dfree(c, c->partial);
+    c->volSet = false;
+    c->denValid = false;
+    c->V.n = n;
+    TRY(dalloc(c, &c->xv0w, 3 * n));
+    TRY(dalloc(c, &c->xv, 3 * n));
+    TRY(dalloc(c, &c->dxv, 3 * n));
+    TRY(dalloc(c, &c->V.vol, n));
+    TRY(dalloc(c, &c->V.perm, n));
+    TRY(dalloc(c, &c->V.den, n));
+    TRY(dalloc(c, &c->V.numb, n));
+    if (n) {
+        HIPCHK(hipMemcpyAsync(c->xv0w, xv0, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->st));
+        HIPCHK(hipMemcpyAsync(c->xv, c->xv0w, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, c->st));
+        // Morton order of the volume nodes over their bounding box
+        double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int64_t j = 0; j < n; ++j)
+            for (int k = 0; k < 3; ++k) {
+                lo[k] = std::min(lo[k], xv0[3 * j + k]);
+                hi[k] = std::max(hi[k], xv0[3 * j + k]);
+            }
+        double span = 0.0;
+        for (int k = 0; k < 3; ++k) span = std::max(span, hi[k] - lo[k]);
+        double scale[3];
+        for (int k = 0; k < 3; ++k) scale[k] = span > 0 ? 2097151.0 / span : 0.0;
+        uint64_t *keys = nullptr, *keys2 = nullptr;
+        int32_t *ids = nullptr;
+        TRY(dalloc(c, &keys, n));
+        TRY(dalloc(c, &keys2, n));
+        TRY(dalloc(c, &ids, n));
+        idw::launch_morton(c->st, n, c->xv0w, lo, scale, keys, ids);
+        HIPCHK(hipGetLastError());
+        HIPCHK(idw::sort_pairs(c->st, n, keys, ids, keys2, c->V.perm));
+        dfree(c, keys);
+        dfree(c, keys2);
+        dfree(c, ids);
+        idw::launch_build_vol(c->st, n, c->xv0w, c->V.perm, c->V.vol);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->st));
+    }
+    c->volSet = true;
+    return compute_den0(c);
+}
+
+int idw_point_reduce(const double *pts, int64_t n, double tol, double *unique_pts, int32_t *link, int64_t *n_unique) {
+    if (n < 0 || (n && (!pts || !unique_pts || !link)) || !n_unique) return fail(IDW_EINVAL, "bad argument");
+    if (n >= INT32_MAX) return fail(IDW_EINVAL, "too many points");
+    *n_unique = idw::point_reduce(pts, n, tol, unique_pts, link);
+    return IDW_OK;
+}
+
+int idw_initialize_warping(idw_ctx *c, const double *pts, int64_t n_pts, const double *unique_pts, int64_t n_unique,
+                           const int32_t *link, const int32_t *face_sizes, int64_t n_faces, const int32_t *face_conn,
+                           int64_t n_conn) {
+    if (!c || !pts || !unique_pts || !link || (n_faces && (!face_sizes || !face_conn)))
+        return fail(IDW_EINVAL, "null argument");
+    if (n_pts <= 0 || n_unique <= 0 || n_unique > n_pts) return fail(IDW_EINVAL, "bad point counts");
+    TRY(ensure_device(c));
+    std::string msg;
+    idw::HostSurface hs;
+    if (!idw::build_surface(unique_pts, n_unique, link, n_pts, face_sizes, n_faces, face_conn, n_conn,
+                            c->opt.bucketSize, hs, msg))
+        return fail(IDW_EINVAL, "%s", msg.c_str());
+    // free a previous surface
+    idw::DevSurface &S = c->S;
+    void *olds[] = {S.nodes, S.nd, S.nd32, S.internal, S.ps, S.pd, S.pd32, S.normals0, S.nprops, S.corner, S.xuInd,
+                    S.facePtr, S.faceConn, S.n2eStart, S.n2e, S.slotStart, S.slots, S.fprops, S.padj, S.nadj,
+                    S.ntmp, S.pb, c->xs, c->dxs, c->partial};
+    for (void *p : olds) {
+        void *q = p;
+        dfree(c, q);
+    }
+    S = idw::DevSurface();
+    c->xs = c->dxs = c->partial = nullptr;
+    c->surfInit = false;
+    c->denValid = false;
+    c->hs = std::move(hs);
+    c->initOpt = c->opt;
+    c->npts = n_pts;
+    const idw::HostSurface &H = c->hs;
+    S.n = (int)H.n;
+    S.nnodes = (int)H.nodes.size();
+    S.nInternal = H.nInternal;
+    S.nface = (int)H.nface;
+    TRY(dalloc(c, &S.nodes, S.nnodes));
+    TRY(dalloc(c, &S.nd, S.nnodes));
+    TRY(dalloc(c, &S.nd32, S.nnodes));
+    TRY(dalloc(c, &S.internal, S.nInternal));
+    TRY(dalloc(c, &S.ps, S.n));
+    TRY(dalloc(c, &S.pd, S.n));
+    TRY(dalloc(c, &S.pd32, S.n));
+    TRY(dalloc(c, &S.normals0, 3 * (size_t)S.n));
+    TRY(dalloc(c, &S.nprops, S.n));
+    TRY(dalloc(c, &S.corner, S.n));
+    TRY(dalloc(c, &S.xuInd, S.n));
+    TRY(dalloc(c, &S.facePtr, H.nface + 1));
+    TRY(dalloc(c, &S.faceConn, H.nconn));
+    TRY(dalloc(c, &S.n2eStart, S.n + 1));
+    TRY(dalloc(c, &S.n2e, H.nconn));
+    TRY(dalloc(c, &S.slotStart, S.n + 1));
+    TRY(dalloc(c, &S.slots, H.nconn));
+    TRY(dalloc(c, &S.fprops, H.nface));
+    TRY(dalloc(c, &S.padj, (size_t)S.n * idw::ADJ_STRIDE));
+    TRY(dalloc(c, &S.nadj, (size_t)S.nnodes * idw::ADJ_STRIDE));
+    TRY(dalloc(c, &S.ntmp, (size_t)S.n * 8));
+    TRY(dalloc(c, &S.pb, 3 * (size_t)H.nconn));
+    TRY(dalloc(c, &c->xs, 3 * (size_t)n_pts));
+    TRY(dalloc(c, &c->dxs, 3 * (size_t)n_pts));
+    std::vector<int32_t> internal;
+    for (int k = 0; k < S.nnodes; ++k)
+        if (!H.nodes[k].leaf) internal.push_back(k);
+    std::vector<double4> ps(S.n);
+    for (int i = 0; i < S.n; ++i) ps[i] = make_double4(H.xu0[3 * i], H.xu0[3 * i + 1], H.xu0[3 * i + 2], 0.0);
+    hipStream_t st = c->st;
+    HIPCHK(hipMemcpyAsync(S.internal, internal.data(), sizeof(int32_t) * internal.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(S.ps, ps.data(), sizeof(double4) * S.n, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(S.xuInd, H.xuInd.data(), sizeof(int32_t) * S.n, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(S.facePtr, H.facePtr.data(), sizeof(int32_t) * (H.nface + 1), hipMemcpyHostToDevice, st));
+    if (H.nconn) {
+        HIPCHK(hipMemcpyAsync(S.faceConn, H.faceConn.data(), sizeof(int32_t) * H.nconn, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(S.n2e, H.n2e.data(), sizeof(int32_t) * H.nconn, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(S.slots, H.slots.data(), sizeof(int32_t) * H.nconn, hipMemcpyHostToDevice, st));
+    }
+    HIPCHK(hipMemcpyAsync(S.n2eStart, H.n2eStart.data(), sizeof(int32_t) * (S.n + 1), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(S.slotStart, H.slotStart.data(), sizeof(int32_t) * (S.n + 1), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(c->xs, pts, sizeof(double) * 3 * n_pts, hipMemcpyHostToDevice, st));
+    // initial nodal properties: normals0, A_i (initNodalProperties, kd_tree.F90:1547-1578)
+    idw::launch_face_props(st, S, c->xs);
+    idw::launch_node_props(st, S, c->xs, 1, 0);
+    const double PI = 3.1415926535897931;
+    const double thr = c->opt.zeroCornerRotations ? std::cos(c->opt.cornerAngle * PI / 180.0) : -1.0;
+    idw::launch_corners(st, S, thr);
+    HIPCHK(hipGetLastError());
+    std::vector<double4> ps2(S.n);
+    HIPCHK(hipMemcpyAsync(ps2.data(), S.ps, sizeof(double4) * S.n, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    std::vector<double> Ai(S.n);
+    for (int i = 0; i < S.n; ++i) Ai[i] = ps2[i].w;
+    idw::set_node_areas(c->hs, Ai.data());
+    HIPCHK(hipMemcpyAsync(S.nodes, c->hs.nodes.data(), sizeof(idw::NodeRec) * S.nnodes, hipMemcpyHostToDevice, st));
+    static double sphere[60];
+    sphere_nominal(sphere);
+    idw::launch_errors(st, S, make_params(c, true), sphere);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    c->surfInit = true;
+    return compute_den0(c);
+}
+
+int idw_set_surface_coordinates(idw_ctx *c, const double *xs, int64_t ndof) {
+    if (!c || !xs) return fail(IDW_EINVAL, "null argument");
+    if (!c->surfInit) return fail(IDW_ESTATE, "surface not initialized");
+    if (ndof != 3 * c->npts)
+        return fail(IDW_EINVAL, "Incorrect length of coordinates: expected %lld, received %lld", (long long)(3 * c->npts),
+                    (long long)ndof);
+    TRY(ensure_device(c));
+    HIPCHK(hipMemcpyAsync(c->xs, xs, sizeof(double) * ndof, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return IDW_OK;
+}
+
+int idw_get_surface_coordinates(idw_ctx *c, double *xs, int64_t ndof) {
+    if (!c || !xs) return fail(IDW_EINVAL, "null argument");
+    if (!c->surfInit) return fail(IDW_ESTATE, "surface not initialized");
+    if (ndof != 3 * c->npts) return fail(IDW_EINVAL, "expected %lld values", (long long)(3 * c->npts));
+    TRY(ensure_device(c));
+    HIPCHK(hipMemcpyAsync(xs, c->xs, sizeof(double) * ndof, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return IDW_OK;
+}
+
+int idw_warp_mesh(idw_ctx *c) {
+    if (!c) return fail(IDW_EINVAL, "null context");
+    TRY(ensure_device(c));
+    TRY(run_warp(c, c->xs, c->xv));
+    HIPCHK(hipStreamSynchronize(c->st));
+    collect_timings(c, true, false);
+    return IDW_OK;
+}
+
+int idw_get_volume_coordinates(idw_ctx *c, double *xv, int64_t ndof) {
+    if (!c || (!xv && ndof)) return fail(IDW_EINVAL, "null argument");
+    if (!c->volSet) return fail(IDW_ESTATE, "no volume nodes set");
+    if (ndof != 3 * c->V.n) return fail(IDW_EINVAL, "expected %lld values", (long long)(3 * c->V.n));
+    TRY(ensure_device(c));
+    if (ndof) HIPCHK(hipMemcpyAsync(xv, c->xv, sizeof(double) * ndof, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return IDW_OK;
+}
+
+int idw_set_external_mesh_indices(idw_ctx *c, const int32_t *ind, int64_t n) {
+    if (!c || (n && !ind) || n < 0) return fail(IDW_EINVAL, "bad argument");
+    if (!c->volSet) return fail(IDW_ESTATE, "no volume nodes set");
+    for (int64_t k = 0; k < n; ++k)
+        if (ind[k] < 0 || ind[k] >= 3 * c->V.n) return fail(IDW_EINVAL, "index %lld out of range", (long long)k);
+    c->solverInd.assign(ind, ind + n);
+    return IDW_OK;
+}
+
+int idw_get_solver_dof(idw_ctx *c, int64_t *n) {
+    if (!c || !n) return fail(IDW_EINVAL, "null argument");
+    *n = (int64_t)c->solverInd.size();
+    return IDW_OK;
+}
+
+// gridTransform.F90:4-51 (one process: common == warp ordering)
+int idw_warp_to_solver_grid(idw_ctx *c, const double *warp, int64_t wdof, double *solver, int64_t sdof) {
+    if (!c || (wdof && !warp) || (sdof && !solver)) return fail(IDW_EINVAL, "null argument");
+    if (wdof != 3 * c->V.n || sdof != (int64_t)c->solverInd.size()) return fail(IDW_EINVAL, "size mismatch");
+    for (int64_t k = 0; k < sdof; ++k) solver[k] = warp[c->solverInd[k]];
+    return IDW_OK;
+}
+
+// gridTransform.F90:53-99: scatter-add of the solver vector (ADD_VALUES).
+int idw_solver_to_warp_grid(idw_ctx *c, const double *solver, int64_t sdof, double *warp, int64_t wdof) {
+    if (!c || (wdof && !warp) || (sdof && !solver)) return fail(IDW_EINVAL, "null argument");
+    if (wdof != 3 * c->V.n || sdof != (int64_t)c->solverInd.size()) return fail(IDW_EINVAL, "size mismatch");
+    std::fill(warp, warp + wdof, 0.0);
+    for (int64_t k = 0; k < sdof; ++k) warp[c->solverInd[k]] += solver[k];
+    return IDW_OK;
+}
+
+int idw_warp_deriv(idw_ctx *c, const double *dxv, int64_t ndof) {
+    if (!c || (!dxv && ndof)) return fail(IDW_EINVAL, "null argument");
+    if (!c->volSet) return fail(IDW_ESTATE, "no volume nodes set");
+    if (ndof != 3 * c->V.n) return fail(IDW_EINVAL, "expected %lld values", (long long)(3 * c->V.n));
+    TRY(ensure_device(c));
+    if (ndof) HIPCHK(hipMemcpyAsync(c->dxv, dxv, sizeof(double) * ndof, hipMemcpyHostToDevice, c->st));
+    TRY(run_deriv(c, c->xs, c->dxv, c->dxs));
+    HIPCHK(hipStreamSynchronize(c->st));
+    collect_timings(c, false, true);
+    c->haveDxs = true;
+    return IDW_OK;
+}
+
+int idw_get_dxs(idw_ctx *c, double *dxs, int64_t ndof) {
+    if (!c || !dxs) return fail(IDW_EINVAL, "null argument");
+    if (!c->haveDxs) return fail(IDW_ESTATE, "no warp_deriv result");
+    if (ndof != 3 * c->npts) return fail(IDW_EINVAL, "expected %lld values", (long long)(3 * c->npts));
+    TRY(ensure_device(c));
+    HIPCHK(hipMemcpyAsync(dxs, c->dxs, sizeof(double) * ndof, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return IDW_OK;
+}
+
+int idw_warp_mesh_dev(idw_ctx *c, const double *d_xs, double *d_xv, void *stream) {
+    if (!c || !d_xs || !d_xv) return fail(IDW_EINVAL, "null argument");
+    TRY(ensure_device(c));
+    hipStream_t user = static_cast<hipStream_t>(stream);
+    // order the context stream after the caller's stream, and back
+    hipEvent_t e;
+    HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    HIPCHK(hipEventRecord(e, user));
+    HIPCHK(hipStreamWaitEvent(c->st, e, 0));
+    int r = run_warp(c, d_xs, d_xv);
+    if (r == IDW_OK) {
+        HIPCHK(hipEventRecord(e, c->st));
+        HIPCHK(hipStreamWaitEvent(user, e, 0));
+    }
+    (void)hipEventDestroy(e);
+    return r;
+}
+
+int idw_warp_deriv_dev(idw_ctx *c, const double *d_xs, const double *d_dxv, double *d_dxs, void *stream) {
+    if (!c || !d_xs || !d_dxv || !d_dxs) return fail(IDW_EINVAL, "null argument");
+    TRY(ensure_device(c));
+    hipStream_t user = static_cast<hipStream_t>(stream);
+    hipEvent_t e;
+    HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    HIPCHK(hipEventRecord(e, user));
+    HIPCHK(hipStreamWaitEvent(c->st, e, 0));
+    int r = run_deriv(c, d_xs, d_dxv, d_dxs);
+    if (r == IDW_OK) {
+        HIPCHK(hipEventRecord(e, c->st));
+        HIPCHK(hipStreamWaitEvent(user, e, 0));
+    }
+    (void)hipEventDestroy(e);
+    return r;
+}
+
+int idw_get_sizes(idw_ctx *c, int64_t *n_vol, int64_t *n_pts, int64_t *n_unique, int64_t *n_tree_nodes) {
+    if (!c) return fail(IDW_EINVAL, "null context");
+    if (n_vol) *n_vol = c->V.n;
+    if (n_pts) *n_pts = c->npts;
+    if (n_unique) *n_unique = c->S.n;
+    if (n_tree_nodes) *n_tree_nodes = c->S.nnodes;
+    return IDW_OK;
+}
+
+int idw_get_ldef0(idw_ctx *c, double *ldef0) {
+    if (!c || !ldef0) return fail(IDW_EINVAL, "null argument");
+    if (!c->surfInit) return fail(IDW_ESTATE, "surface not initialized");
+    *ldef0 = c->hs.ldef0;
+    return IDW_OK;
+}
+
+int idw_get_denominator0(idw_ctx *c, double *den0, int64_t n) {
+    if (!c || (!den0 && n)) return fail(IDW_EINVAL, "null argument");
+    if (!c->volSet || !c->surfInit) return fail(IDW_ESTATE, "not initialized");
+    if (n != c->V.n) return fail(IDW_EINVAL, "expected %lld values", (long long)c->V.n);
+    TRY(ensure_device(c));
+    std::vector<double4> vol(n);
+    std::vector<int32_t> perm(n);
+    if (n) {
+        HIPCHK(hipMemcpyAsync(vol.data(), c->V.vol, sizeof(double4) * n, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipMemcpyAsync(perm.data(), c->V.perm, sizeof(int32_t) * n, hipMemcpyDeviceToHost, c->st));
+    }
+    HIPCHK(hipStreamSynchronize(c->st));
+    for (int64_t k = 0; k < n; ++k) den0[perm[k]] = vol[k].w;
+    return IDW_OK;
+}
+
+int idw_get_timings(idw_ctx *c, double *ms4) {
+    if (!c || !ms4) return fail(IDW_EINVAL, "null argument");
+    for (int k = 0; k < 4; ++k) ms4[k] = c->ms[k];
+    return IDW_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,82 @@
+// Internal layout of the volume-warp library: host-built surface/tree data
+// and the device records the kernels read.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace idw {
+
+constexpr int NERR = 12;
+
+// kd-tree node in depth-first pre-order (parent, left subtree, right subtree)
+// as created by build_tree (kd_tree.F90:140-182).  One record per node; the
+// traversal kernels read it with wave-uniform (scalar) loads.
+struct alignas(16) NodeRec {
+    double X[3];      // centroid of the node's points (setCenters, kd_tree.F90:558-593)
+    double orad;      // 1 / max distance from X to a point of the node
+    double Ai;        // summed area weights (setAi, kd_tree.F90:536-556)
+    double err[NERR]; // RMS denominator error at rstar radii (computeErrors, :832-880)
+    int32_t l, u;     // points [l, u] (0-based, inclusive) in tree order
+    int32_t skip;     // index of the next node after this subtree (== node count at the end)
+    int32_t depth;    // root = 0
+    int32_t leaf;     // 1 for a bucket
+    int32_t n;        // u - l + 1
+};
+static_assert(sizeof(NodeRec) == 160, "NodeRec layout");
+
+// Per-call cluster data of an internal node (setData, kd_tree.F90:595-617),
+// stored as Kbar = mean(M_i) - I and e = Kbar X + mean(b_i), so that the
+// cluster displacement mean(M) r + mean(b) - r equals Kbar (r - X) + e.
+struct alignas(16) NodeDyn {
+    double K[9];  // row-major
+    double e[3];
+    double pad[4];
+};
+struct alignas(16) NodeDyn32 {
+    float K[9];
+    float e[3];
+};
+
+// Per-call surface point data in tree order: K = M_i - I and d = Xu_i - Xu0_i,
+// so that M_i r + b_i - r == K (r - Xu0_i) + d (b_i = Xu_i - M_i Xu0_i,
+// kd_tree.F90:1648).
+struct alignas(16) PointDyn {
+    double K[9];
+    double d[3];
+    double pad[4];
+};
+struct alignas(16) PointDyn32 {
+    float K[9];
+    float d[3];
+};
+
+// Host-side result of the surface definition (myCreateTree, kd_tree.F90:1365-1545).
+struct HostSurface {
+    int64_t npts = 0;       // surface points incl. duplicates (Xs size / 3)
+    int64_t n = 0;          // unique nodes
+    int64_t nface = 0, nconn = 0;
+    std::vector<int32_t> facePtr;    // nface + 1
+    std::vector<int32_t> faceConn;   // 0-based tree index
+    std::vector<int32_t> n2eStart, n2e;      // node -> faces (0-based), duplicates kept, face order
+    std::vector<int32_t> slotStart, slots;   // node -> positions in faceConn (for the reverse gather)
+    std::vector<int32_t> xuInd;      // tree node -> first point index (0-based)
+    std::vector<double> xu0;         // 3*n tree order
+    std::vector<NodeRec> nodes;      // DFS pre-order
+    int32_t maxDepth = 0, nInternal = 0;
+    double ldef0 = 0.0;
+};
+
+// Build the tree, permutation, connectivity maps and Ldef0 from the reduced
+// surface.  uniquePts: 3*nu (already lexsorted), link 1-based (npts),
+// faceConn 0-based into the npts points.  Returns false with msg on error.
+bool build_surface(const double *uniquePts, int64_t nu, const int32_t *link, int64_t npts, const int32_t *faceSizes,
+                   int64_t nface, const int32_t *faceConn, int64_t nconn, int bucket, HostSurface &out, std::string &msg);
+
+// setAi (kd_tree.F90:536-556) on the host tree once nodal areas are known.
+void set_node_areas(HostSurface &s, const double *Ai);
+
+// pointReduce (src/utils/pointReduce.F90): returns number of unique points.
+int64_t point_reduce(const double *pts, int64_t n, double tol, double *unique, int32_t *link);
+
+}  // namespace idw

@@ -1,0 +1,70 @@
+// Device-side structures and kernel launchers of the volume warp.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/idwarp_hip.h"
+#include "idw_internal.h"
+
+namespace idw {
+
+constexpr int ADJ_STRIDE = 16;  // doubles per adjoint record (12 used, 128 B aligned)
+
+enum { EXP_35 = 0, EXP_INT = 1, EXP_REAL = 2 };
+
+// Scalars of one warp (warpMesh.F90:39-41): Ldef = Ldef0 * LdefFact and
+// alpha^bExp with the current options; the exponents are those frozen into
+// the tree at initialisation.
+struct WParams {
+    double Ldef, a5, aExp, bExp, errTol, nearT2;
+    int32_t expMode, ia, ib, pad;
+};
+
+struct DevSurface {
+    int n = 0, nnodes = 0, nInternal = 0, nface = 0;
+    NodeRec *nodes = nullptr;
+    NodeDyn *nd = nullptr;
+    NodeDyn32 *nd32 = nullptr;
+    int32_t *internal = nullptr;
+    double4 *ps = nullptr;  // Xu0, A (tree order)
+    PointDyn *pd = nullptr;
+    PointDyn32 *pd32 = nullptr;
+    double *normals0 = nullptr;
+    double4 *nprops = nullptr;  // sumArea, sumNormal of the last evaluation
+    int32_t *corner = nullptr, *xuInd = nullptr;
+    int32_t *facePtr = nullptr, *faceConn = nullptr, *n2eStart = nullptr, *n2e = nullptr;
+    int32_t *slotStart = nullptr, *slots = nullptr;
+    double4 *fprops = nullptr;  // dA = area/nPts, unit normal
+    double *padj = nullptr, *nadj = nullptr, *ntmp = nullptr, *pb = nullptr;
+};
+
+struct DevVolume {
+    int64_t n = 0;
+    double4 *vol = nullptr;   // x, y, z, denominator0 in traversal (Morton) order
+    int32_t *perm = nullptr;  // traversal order -> warp order
+    double *den = nullptr;    // denominator of the last warp, traversal order
+    double4 *numb = nullptr;  // reverse seed dXv/den, traversal order
+};
+
+void launch_face_props(hipStream_t st, const DevSurface &S, const double *xs);
+void launch_node_props(hipStream_t st, const DevSurface &S, const double *xs, int initial, int useRot);
+void launch_corners(hipStream_t st, const DevSurface &S, double threshold);
+void launch_cluster_data(hipStream_t st, const DevSurface &S);
+void launch_errors(hipStream_t st, const DevSurface &S, const WParams &P, const double *sphere60);
+void launch_den0(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P);
+void launch_morton(hipStream_t st, int64_t n, const double *xv0, double lo[3], double scale[3], uint64_t *keys,
+                   int32_t *ids);
+void launch_build_vol(hipStream_t st, int64_t n, const double *xv0, const int32_t *perm, double4 *vol);
+void launch_fill(hipStream_t st, double *a, int64_t n, double v);
+void launch_forward(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, int fast, int prec,
+                    double *xv);
+void launch_prep_numb(hipStream_t st, const DevVolume &V, const double *dxv);
+void launch_reverse(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, int fast, int prec,
+                    double *partial, int nchunk);
+void launch_nodal_reverse(hipStream_t st, const DevSurface &S, const double *xs, int fast, int useRot, double *dxs,
+                          int64_t npts);
+// Sort keys/ids pairs (Morton order); temp storage allocated inside.
+hipError_t sort_pairs(hipStream_t st, int64_t n, uint64_t *keys, int32_t *ids, uint64_t *keys_out, int32_t *ids_out);
+
+}  // namespace idw

@@ -1,0 +1,1101 @@
+// HIP kernels of the MI355X volume warp (gfx950, wave64).
+//
+// Work split
+//  * surface kernels: one thread per face or per unique surface node;
+//  * exact mode: one lane per volume node streaming every surface point
+//    (forward), one lane per surface node streaming a chunk of volume nodes
+//    (reverse, deterministic chunk partials);
+//  * fast mode: one lane per volume node, 64 spatially sorted volume nodes per
+//    wave walking the kd-tree together.  The walk is wave-uniform: every wave
+//    visits the union of the nodes its lanes need, in depth-first order, and a
+//    lane only acts on a node if it descended through all of the node's
+//    ancestors.  That per-lane condition is kept in one integer (the depth at
+//    which the lane stopped), so no stack is needed and node and point records
+//    are fetched with scalar loads.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "idw_internal.h"
+#include "idw_kernels.h"
+
+namespace idw {
+
+__constant__ double c_rstar[NERR] = {2.0, 3.0, 4.0, 5.0, 7.5, 10.0, 20.0, 40.0, 80.0, 160.0, 320.0, 640.0};
+// {rstar(i), orstar(i)} pairs; orstar(i) = 1/(rstar(i+1) - rstar(i)) as
+// myCreateTree computes it (kd_tree.F90:1539-1541).
+__constant__ double2 c_rseg[NERR - 1] = {
+    {2.0, 1.0 / (3.0 - 2.0)},     {3.0, 1.0 / (4.0 - 3.0)},       {4.0, 1.0 / (5.0 - 4.0)},
+    {5.0, 1.0 / (7.5 - 5.0)},     {7.5, 1.0 / (10.0 - 7.5)},      {10.0, 1.0 / (20.0 - 10.0)},
+    {20.0, 1.0 / (40.0 - 20.0)},  {40.0, 1.0 / (80.0 - 40.0)},    {80.0, 1.0 / (160.0 - 80.0)},
+    {160.0, 1.0 / (320.0 - 160.0)}, {320.0, 1.0 / (640.0 - 320.0)}};
+
+constexpr double EPS_W = 1e-15;    // constants.F90:33
+constexpr double EPS_DEN0 = 1e-16; // getWiEstimate, kd_tree.F90:770,774
+constexpr double ROT_TOL = 1.4901161193847656e-08;  // vectorUtils.f90:44
+
+// ---------------------------------------------------------------------------
+// small helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int xcd_block(int nblocks) {
+    // Workgroups are dealt round-robin over the 8 XCDs; give each XCD a
+    // contiguous range of work so its L2 sees spatially close volume nodes.
+    const int b = blockIdx.x;
+    const int per = (nblocks + 7) >> 3;
+    return (b & 7) * per + (b >> 3);
+}
+
+template <typename T>
+__device__ __forceinline__ T ipow_u(T q, int e) {
+    T r = T(1);
+    while (e > 0) {
+        if (e & 1) r *= q;
+        q *= q;
+        e >>= 1;
+    }
+    return r;
+}
+
+// A_i * (q^aExp + alpha^bExp q^bExp) without the A factor.
+template <typename T>
+__device__ __forceinline__ T wfun(T q, const WParams &P) {
+    if (P.expMode == EXP_35) {
+        const T q2 = q * q, q3 = q2 * q, q5 = q3 * q2;
+        return q3 + T(P.a5) * q5;
+    } else if (P.expMode == EXP_INT) {
+        return ipow_u(q, P.ia) + T(P.a5) * ipow_u(q, P.ib);
+    } else {
+        return pow(q, T(P.aExp)) + T(P.a5) * pow(q, T(P.bExp));
+    }
+}
+
+__device__ __forceinline__ void cross3(const double *a, const double *b, double *c) {
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// ---------------------------------------------------------------------------
+// Surface: face props, nodal props (computeNodalProperties, kd_tree.F90:1580)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void load_xu(const double *xs, const int32_t *xuInd, int32_t node, double *p) {
+    const int64_t j = xuInd[node];
+    p[0] = xs[3 * j];
+    p[1] = xs[3 * j + 1];
+    p[2] = xs[3 * j + 2];
+}
+
+// getElementProps.F90: fan of cross products about the face centroid.
+__global__ void k_face_props(int nface, const int32_t *__restrict__ facePtr, const int32_t *__restrict__ faceConn,
+                             const int32_t *__restrict__ xuInd, const double *__restrict__ xs,
+                             double4 *__restrict__ fprops) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nface) return;
+    const int p0 = facePtr[f], np = facePtr[f + 1] - p0;
+    double c[3] = {0.0, 0.0, 0.0}, p[3];
+    for (int i = 0; i < np; ++i) {
+        load_xu(xs, xuInd, faceConn[p0 + i], p);
+        c[0] += p[0]; c[1] += p[1]; c[2] += p[2];
+    }
+    c[0] /= np; c[1] /= np; c[2] /= np;
+    double first[3], prev[3], nrm[3] = {0.0, 0.0, 0.0}, area = 0.0, cr[3];
+    load_xu(xs, xuInd, faceConn[p0], first);
+    for (int k = 0; k < 3; ++k) first[k] -= c[k];
+    prev[0] = first[0]; prev[1] = first[1]; prev[2] = first[2];
+    for (int i = 0; i < np; ++i) {
+        double nx[3];
+        if (i + 1 < np) {
+            load_xu(xs, xuInd, faceConn[p0 + i + 1], nx);
+            for (int k = 0; k < 3; ++k) nx[k] -= c[k];
+        } else {
+            nx[0] = first[0]; nx[1] = first[1]; nx[2] = first[2];
+        }
+        cross3(prev, nx, cr);
+        const double cn = sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2] + 1e-15);
+        area += 0.5 * cn;
+        nrm[0] += cr[0] / cn; nrm[1] += cr[1] / cn; nrm[2] += cr[2] / cn;
+        prev[0] = nx[0]; prev[1] = nx[1]; prev[2] = nx[2];
+    }
+    const double nn = sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+    fprops[f] = make_double4(area / np, nrm[0] / nn, nrm[1] / nn, nrm[2] / nn);
+}
+
+// getRotationMatrix3d (vectorUtils.f90:31-103), row-major M.
+__device__ void rodrigues(const double *v1, const double *v2, double *M) {
+    double axis[3];
+    cross3(v1, v2, axis);
+    const double axisMag = sqrt(axis[0] * axis[0] + axis[1] * axis[1] + axis[2] * axis[2] + 1e-30);
+    double s = 0.0, oc = 0.0;
+    if (axisMag < ROT_TOL) {
+        axis[0] = 1.0; axis[1] = 0.0; axis[2] = 0.0;
+    } else {
+        const double m1 = sqrt(v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2] + 1e-30);
+        const double m2 = sqrt(v2[0] * v2[0] + v2[1] * v2[1] + v2[2] * v2[2] + 1e-30);
+        for (int k = 0; k < 3; ++k) axis[k] /= axisMag;
+        const double dot = (v1[0] / m1) * (v2[0] / m2) + (v1[1] / m1) * (v2[1] / m2) + (v1[2] / m1) * (v2[2] / m2);
+        const double ang = acos(fmin(1.0, dot));
+        s = sin(ang);
+        oc = 1.0 - cos(ang);
+    }
+    // M = I + sin(a) A + (1 - cos(a)) A^2 with A the skew matrix of the axis
+    const double A[9] = {0.0, -axis[2], axis[1], axis[2], 0.0, -axis[0], -axis[1], axis[0], 0.0};
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+            const double C = A[3 * r] * A[c] + A[3 * r + 1] * A[3 + c] + A[3 * r + 2] * A[6 + c];
+            M[3 * r + c] = (r == c ? 1.0 : 0.0) + s * A[3 * r + c] + oc * C;
+        }
+}
+
+// Per node: area-weighted normal over incident faces; at init it becomes
+// normals0 and A_i, later the rotation M_i (or I), stored as K = M - I and
+// d = Xu - Xu0.  Zero-area nodes get M = I, b = 0 (kd_tree.F90:1651-1663).
+__global__ void k_node_props(int n, int initial, int useRot, const int32_t *__restrict__ n2eStart,
+                             const int32_t *__restrict__ n2e, const double4 *__restrict__ fprops,
+                             const int32_t *__restrict__ xuInd, const int32_t *__restrict__ corner,
+                             const double *__restrict__ xs, double4 *__restrict__ ps, double *__restrict__ normals0,
+                             double4 *__restrict__ nprops, PointDyn *__restrict__ pd, PointDyn32 *__restrict__ pd32) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double sa = 0.0, sn[3] = {0.0, 0.0, 0.0};
+    for (int e = n2eStart[i]; e < n2eStart[i + 1]; ++e) {
+        const double4 fp = fprops[n2e[e]];
+        sa += fp.x;
+        sn[0] += fp.x * fp.y; sn[1] += fp.x * fp.z; sn[2] += fp.x * fp.w;
+    }
+    const double nrm[3] = {sn[0] / sa, sn[1] / sa, sn[2] / sa};
+    nprops[i] = make_double4(sa, sn[0], sn[1], sn[2]);
+    if (initial) {
+        normals0[3 * i] = nrm[0]; normals0[3 * i + 1] = nrm[1]; normals0[3 * i + 2] = nrm[2];
+        double4 p = ps[i];
+        p.w = sa;
+        ps[i] = p;
+        return;
+    }
+    const double4 p = ps[i];
+    PointDyn out;
+    double M[9] = {1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0};
+    double xu[3];
+    load_xu(xs, xuInd, i, xu);
+    if (p.w == 0.0) {
+        for (int k = 0; k < 9; ++k) out.K[k] = 0.0;
+        out.d[0] = out.d[1] = out.d[2] = 0.0;
+    } else {
+        if (useRot && !corner[i]) {
+            const double n0[3] = {normals0[3 * i], normals0[3 * i + 1], normals0[3 * i + 2]};
+            rodrigues(n0, nrm, M);
+        }
+        for (int k = 0; k < 9; ++k) out.K[k] = M[k] - ((k == 0 || k == 4 || k == 8) ? 1.0 : 0.0);
+        out.d[0] = xu[0] - p.x; out.d[1] = xu[1] - p.y; out.d[2] = xu[2] - p.z;
+    }
+    for (int k = 0; k < 4; ++k) out.pad[k] = 0.0;
+    pd[i] = out;
+    PointDyn32 o32;
+    for (int k = 0; k < 9; ++k) o32.K[k] = (float)out.K[k];
+    for (int k = 0; k < 3; ++k) o32.d[k] = (float)out.d[k];
+    pd32[i] = o32;
+}
+
+// determineCorners (kd_tree.F90:1666-1715) against the initial surface.
+__global__ void k_corners(int n, double threshold, const int32_t *__restrict__ n2eStart, const int32_t *__restrict__ n2e,
+                          const double4 *__restrict__ fprops, const double *__restrict__ normals0,
+                          int32_t *__restrict__ corner) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int c = 0;
+    for (int e = n2eStart[i]; e < n2eStart[i + 1]; ++e) {
+        const double4 fp = fprops[n2e[e]];
+        const double dp = fp.y * normals0[3 * i] + fp.z * normals0[3 * i + 1] + fp.w * normals0[3 * i + 2];
+        if (fabs(dp) < threshold) c = 1;
+    }
+    corner[i] = c;
+}
+
+// setData (kd_tree.F90:595-617): mean K and mean b = d - K Xu0 over an
+// internal node's points, stored as Kbar and e = Kbar X + mean(b)
+// = mean(d) + mean(K (X - Xu0)).  One block per internal node.
+__global__ void __launch_bounds__(256) k_cluster_data(const int32_t *__restrict__ internal,
+                                                      const NodeRec *__restrict__ nodes,
+                                                      const double4 *__restrict__ ps,
+                                                      const PointDyn *__restrict__ pd, NodeDyn *__restrict__ nd,
+                                                      NodeDyn32 *__restrict__ nd32) {
+    __shared__ double red[15][256 / 64];
+    const int node = internal[blockIdx.x];
+    const NodeRec &r = nodes[node];
+    const double X0 = r.X[0], X1 = r.X[1], X2 = r.X[2];
+    double acc[15];
+    for (int k = 0; k < 15; ++k) acc[k] = 0.0;
+    for (int i = r.l + threadIdx.x; i <= r.u; i += blockDim.x) {
+        const PointDyn &q = pd[i];
+        const double4 p = ps[i];
+        const double dx = X0 - p.x, dy = X1 - p.y, dz = X2 - p.z;
+        for (int k = 0; k < 9; ++k) acc[k] += q.K[k];
+        for (int k = 0; k < 3; ++k) {
+            acc[9 + k] += q.d[k];
+            acc[12 + k] += q.K[3 * k] * dx + q.K[3 * k + 1] * dy + q.K[3 * k + 2] * dz;
+        }
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int k = 0; k < 15; ++k) {
+        double v = acc[k];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+        if (lane == 0) red[k][w] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 15) {
+        double v = 0.0;
+        for (int q = 0; q < (int)(blockDim.x >> 6); ++q) v += red[threadIdx.x][q];
+        red[threadIdx.x][0] = v / r.n;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        NodeDyn o;
+        NodeDyn32 o32;
+        for (int k = 0; k < 9; ++k) { o.K[k] = red[k][0]; o32.K[k] = (float)o.K[k]; }
+        for (int k = 0; k < 3; ++k) { o.e[k] = red[9 + k][0] + red[12 + k][0]; o32.e[k] = (float)o.e[k]; }
+        for (int k = 0; k < 4; ++k) o.pad[k] = 0.0;
+        nd[node] = o;
+        nd32[node] = o32;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Exact mode (evalNodeExact on the root, kd_tree.F90:664-690)
+// ---------------------------------------------------------------------------
+template <int PREC>
+__global__ void __launch_bounds__(256) k_exact_fwd(int nvol, const double4 *__restrict__ vol,
+                                                   const int32_t *__restrict__ perm, int n,
+                                                   const double4 *__restrict__ ps, const PointDyn *__restrict__ pd,
+                                                   const PointDyn32 *__restrict__ pd32, WParams P,
+                                                   double *__restrict__ xv, double *__restrict__ den) {
+    const int nb = (nvol + 255) / 256;
+    const int blk = xcd_block(nb);
+    if (blk >= nb) return;
+    const int k = blk * 256 + threadIdx.x;
+    const bool valid = k < nvol;
+    const double4 v = valid ? vol[k] : make_double4(0.0, 0.0, 0.0, 0.0);
+    double n0 = 0.0, n1 = 0.0, n2 = 0.0, dd = 0.0;
+    if (PREC == IDW_PREC_FP64) {
+        for (int i = 0; i < n; ++i) {
+            const double4 s = ps[i];
+            const PointDyn &q = pd[i];
+            const double rx = v.x - s.x, ry = v.y - s.y, rz = v.z - s.z;
+            const double lo = P.Ldef / sqrt(rx * rx + ry * ry + rz * rz + EPS_W);
+            const double w = s.w * wfun(lo, P);
+            n0 += w * (q.K[0] * rx + q.K[1] * ry + q.K[2] * rz + q.d[0]);
+            n1 += w * (q.K[3] * rx + q.K[4] * ry + q.K[5] * rz + q.d[1]);
+            n2 += w * (q.K[6] * rx + q.K[7] * ry + q.K[8] * rz + q.d[2]);
+            dd += w;
+        }
+    } else {
+        // FP32 weight and displacement on the FP64 difference r - Xu0; pairs
+        // nearer than nearWallFactor*Ldef for any lane of the wave stay FP64.
+        const float Lf = (float)P.Ldef;
+        const double T2 = P.nearT2;
+        constexpr int G = 8;
+        for (int i0 = 0; i0 < n; i0 += G) {
+            float a0 = 0.f, a1 = 0.f, a2 = 0.f, ad = 0.f;
+            const int ie = min(i0 + G, n);
+            for (int i = i0; i < ie; ++i) {
+                const double4 s = ps[i];
+                const double rx = v.x - s.x, ry = v.y - s.y, rz = v.z - s.z;
+                const double d2 = rx * rx + ry * ry + rz * rz;
+                if (__any(valid && d2 < T2)) {
+                    const PointDyn &q = pd[i];
+                    const double lo = P.Ldef / sqrt(d2 + EPS_W);
+                    const double w = s.w * wfun(lo, P);
+                    n0 += w * (q.K[0] * rx + q.K[1] * ry + q.K[2] * rz + q.d[0]);
+                    n1 += w * (q.K[3] * rx + q.K[4] * ry + q.K[5] * rz + q.d[1]);
+                    n2 += w * (q.K[6] * rx + q.K[7] * ry + q.K[8] * rz + q.d[2]);
+                    dd += w;
+                } else {
+                    const PointDyn32 &q = pd32[i];
+                    const float fx = (float)rx, fy = (float)ry, fz = (float)rz;
+                    const float lo = Lf * __frsqrt_rn(fx * fx + fy * fy + fz * fz);
+                    const float w = (float)s.w * wfun(lo, P);
+                    a0 += w * (q.K[0] * fx + q.K[1] * fy + q.K[2] * fz + q.d[0]);
+                    a1 += w * (q.K[3] * fx + q.K[4] * fy + q.K[5] * fz + q.d[1]);
+                    a2 += w * (q.K[6] * fx + q.K[7] * fy + q.K[8] * fz + q.d[2]);
+                    ad += w;
+                }
+            }
+            n0 += a0; n1 += a1; n2 += a2; dd += ad;
+        }
+    }
+    if (valid) {
+        const int64_t j = perm[k];
+        const double oden = 1.0 / dd;
+        xv[3 * j] = v.x + n0 * oden;
+        xv[3 * j + 1] = v.y + n1 * oden;
+        xv[3 * j + 2] = v.z + n2 * oden;
+        den[k] = dd;
+    }
+}
+
+// Reverse seed numb = dXv / den (warpDeriv.F90:76-78), sorted order.
+__global__ void k_prep_numb(int nvol, const int32_t *__restrict__ perm, const double *__restrict__ dxv,
+                            const double *__restrict__ den, double4 *__restrict__ numb) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nvol) return;
+    const int64_t j = perm[k];
+    const double oden = 1.0 / den[k];
+    numb[k] = make_double4(dxv[3 * j] * oden, dxv[3 * j + 1] * oden, dxv[3 * j + 2] * oden, 0.0);
+}
+
+// evalNodeExact_b (kd_tree.F90:984-1013) transposed: one lane per surface
+// node, a chunk of volume nodes per block row.  Accumulates
+// Mb'(r,c) = sum s_r (x_c - Xu0_c) (the reverse's Mib - Xu0 (x) Bib of
+// COMPUTENODALPROPERTIES_B, kd_tree.F90:1115-1118) and Bb = sum s.
+template <int PREC>
+__global__ void __launch_bounds__(256) k_exact_rev(int nvol, int chunk, const double4 *__restrict__ vol,
+                                                   const double4 *__restrict__ numb, int n,
+                                                   const double4 *__restrict__ ps, WParams P,
+                                                   double *__restrict__ partial) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int c = blockIdx.y;
+    const int j0 = c * chunk, j1 = min(nvol, j0 + chunk);
+    const bool valid = i < n;
+    const double4 s = valid ? ps[i] : make_double4(0.0, 0.0, 0.0, 0.0);
+    double acc[12];
+    for (int k = 0; k < 12; ++k) acc[k] = 0.0;
+    for (int j = j0; j < j1; ++j) {
+        const double4 v = vol[j];
+        const double4 nb = numb[j];
+        const double rx = v.x - s.x, ry = v.y - s.y, rz = v.z - s.z;
+        double w;
+        if (PREC == IDW_PREC_FP64) {
+            const double lo = P.Ldef / sqrt(rx * rx + ry * ry + rz * rz + EPS_W);
+            w = s.w * wfun(lo, P);
+        } else {
+            const double d2 = rx * rx + ry * ry + rz * rz;
+            if (__any(valid && d2 < P.nearT2)) {
+                const double lo = P.Ldef / sqrt(d2 + EPS_W);
+                w = s.w * wfun(lo, P);
+            } else {
+                const float fx = (float)rx, fy = (float)ry, fz = (float)rz;
+                const float lo = (float)P.Ldef * __frsqrt_rn(fx * fx + fy * fy + fz * fz);
+                w = (double)((float)s.w * wfun(lo, P));
+            }
+        }
+        const double s0 = w * nb.x, s1 = w * nb.y, s2 = w * nb.z;
+        acc[0] += s0 * rx; acc[1] += s0 * ry; acc[2] += s0 * rz;
+        acc[3] += s1 * rx; acc[4] += s1 * ry; acc[5] += s1 * rz;
+        acc[6] += s2 * rx; acc[7] += s2 * ry; acc[8] += s2 * rz;
+        acc[9] += s0; acc[10] += s1; acc[11] += s2;
+    }
+    if (valid) {
+        double *o = partial + ((int64_t)c * n + i) * 12;
+        for (int k = 0; k < 12; ++k) o[k] = acc[k];
+    }
+}
+
+__global__ void k_exact_rev_reduce(int n, int nchunk, const double *__restrict__ partial, double *__restrict__ padj) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * 12) return;
+    const int i = t / 12, k = t % 12;
+    double s = 0.0;
+    for (int c = 0; c < nchunk; ++c) s += partial[((int64_t)c * n + i) * 12 + k];
+    padj[(int64_t)i * ADJ_STRIDE + k] = s;
+}
+
+// ---------------------------------------------------------------------------
+// Fast mode: wave-uniform kd-tree walk
+// ---------------------------------------------------------------------------
+// getError (kd_tree.F90:882-906): linear interpolation of the node's error
+// table at dist*oradius.
+__device__ __forceinline__ double get_error(const NodeRec *nd, double dOvrR) {
+    int i = 0;
+#pragma unroll
+    for (int k = 1; k < NERR; ++k) i += (dOvrR >= c_rstar[k]) ? 1 : 0;
+    if (i == NERR - 1) return nd->err[NERR - 1];
+    const double2 seg = c_rseg[i];
+    const double fact = (dOvrR - seg.x) * seg.y;
+    return (1.0 - fact) * nd->err[i] + fact * nd->err[i + 1];
+}
+
+__device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
+// Forward walk (evalNode, kd_tree.F90:619-662; leaves evalNodeExact,
+// accepted clusters evalNodeApprox, :692-713).
+template <int PREC>
+__global__ void __launch_bounds__(256) k_fast_fwd(int nvol, const double4 *__restrict__ vol,
+                                                  const int32_t *__restrict__ perm, int nnodes,
+                                                  const NodeRec *__restrict__ nodes, const NodeDyn *__restrict__ nd,
+                                                  const NodeDyn32 *__restrict__ nd32,
+                                                  const double4 *__restrict__ ps, const PointDyn *__restrict__ pd,
+                                                  const PointDyn32 *__restrict__ pd32, WParams P,
+                                                  double *__restrict__ xv, double *__restrict__ den) {
+    const int nb = (nvol + 255) / 256;
+    const int blk = xcd_block(nb);
+    if (blk >= nb) return;
+    const int k = blk * 256 + threadIdx.x;
+    const bool valid = k < nvol;
+    const double4 v = valid ? vol[k] : make_double4(0.0, 0.0, 0.0, 1.0);
+    const double tol = P.errTol * v.w;  // errTol * denominator0
+    int stop = valid ? INT32_MAX : -1;  // depth at which this lane stopped descending
+    double n0 = 0.0, n1 = 0.0, n2 = 0.0, dd = 0.0;
+    int node = 0;
+    while (node < nnodes) {
+        const NodeRec *r = nodes + node;
+        const int depth = uniform(r->depth);
+        const bool active = depth <= stop;
+        if (uniform(r->leaf)) {
+            const int l = uniform(r->l), u = uniform(r->u);
+            if (PREC == IDW_PREC_FP64) {
+                for (int i = l; i <= u; ++i) {
+                    const double4 s = ps[i];
+                    const PointDyn &q = pd[i];
+                    const double rx = v.x - s.x, ry = v.y - s.y, rz = v.z - s.z;
+                    const double lo = P.Ldef / sqrt(rx * rx + ry * ry + rz * rz + EPS_W);
+                    const double w = active ? s.w * wfun(lo, P) : 0.0;
+                    n0 += w * (q.K[0] * rx + q.K[1] * ry + q.K[2] * rz + q.d[0]);
+                    n1 += w * (q.K[3] * rx + q.K[4] * ry + q.K[5] * rz + q.d[1]);
+                    n2 += w * (q.K[6] * rx + q.K[7] * ry + q.K[8] * rz + q.d[2]);
+                    dd += w;
+                }
+            } else {
+                // FP32 bucket unless an active lane is within nearWallFactor*Ldef
+                // of the bucket's bounding sphere.
+                const double cx = v.x - r->X[0], cy = v.y - r->X[1], cz = v.z - r->X[2];
+                const double dc = sqrt(cx * cx + cy * cy + cz * cz) - 1.0 / r->orad;
+                if (__any(active && dc * fabs(dc) < P.nearT2)) {
+                    for (int i = l; i <= u; ++i) {
+                        const double4 s = ps[i];
+                        const PointDyn &q = pd[i];
+                        const double rx = v.x - s.x, ry = v.y - s.y, rz = v.z - s.z;
+                        const double lo = P.Ldef / sqrt(rx * rx + ry * ry + rz * rz + EPS_W);
+                        const double w = active ? s.w * wfun(lo, P) : 0.0;
+                        n0 += w * (q.K[0] * rx + q.K[1] * ry + q.K[2] * rz + q.d[0]);
+                        n1 += w * (q.K[3] * rx + q.K[4] * ry + q.K[5] * rz + q.d[1]);
+                        n2 += w * (q.K[6] * rx + q.K[7] * ry + q.K[8] * rz + q.d[2]);
+                        dd += w;
+                    }
+                } else {
+                    float a0 = 0.f, a1 = 0.f, a2 = 0.f, ad = 0.f;
+                    const float Lf = (float)P.Ldef;
+                    for (int i = l; i <= u; ++i) {
+                        const double4 s = ps[i];
+                        const PointDyn32 &q = pd32[i];
+                        const float fx = (float)(v.x - s.x), fy = (float)(v.y - s.y), fz = (float)(v.z - s.z);
+                        const float lo = Lf * __frsqrt_rn(fx * fx + fy * fy + fz * fz);
+                        const float w = active ? (float)s.w * wfun(lo, P) : 0.f;
+                        a0 += w * (q.K[0] * fx + q.K[1] * fy + q.K[2] * fz + q.d[0]);
+                        a1 += w * (q.K[3] * fx + q.K[4] * fy + q.K[5] * fz + q.d[1]);
+                        a2 += w * (q.K[6] * fx + q.K[7] * fy + q.K[8] * fz + q.d[2]);
+                        ad += w;
+                    }
+                    n0 += a0; n1 += a1; n2 += a2; dd += ad;
+                }
+            }
+            node = uniform(r->skip);
+            continue;
+        }
+        const double cx = v.x - r->X[0], cy = v.y - r->X[1], cz = v.z - r->X[2];
+        const double dist = sqrt(cx * cx + cy * cy + cz * cz);
+        const double orad = r->orad;
+        bool descend = false;
+        if (active) {
+            const double dOvrR = dist * orad;
+            if (dOvrR < 2.0) {
+                descend = true;
+            } else if (get_error(r, dOvrR) < tol) {
+                const double Ai = r->Ai;
+                if (PREC == IDW_PREC_FP64) {
+                    const NodeDyn &q = nd[node];
+                    const double w = Ai * wfun(P.Ldef / dist, P);
+                    n0 += w * (q.K[0] * cx + q.K[1] * cy + q.K[2] * cz + q.e[0]);
+                    n1 += w * (q.K[3] * cx + q.K[4] * cy + q.K[5] * cz + q.e[1]);
+                    n2 += w * (q.K[6] * cx + q.K[7] * cy + q.K[8] * cz + q.e[2]);
+                    dd += w;
+                } else {
+                    const NodeDyn32 &q = nd32[node];
+                    const float fx = (float)cx, fy = (float)cy, fz = (float)cz;
+                    const float w = (float)Ai * wfun((float)P.Ldef * __frcp_rn((float)dist), P);
+                    n0 += (double)(w * (q.K[0] * fx + q.K[1] * fy + q.K[2] * fz + q.e[0]));
+                    n1 += (double)(w * (q.K[3] * fx + q.K[4] * fy + q.K[5] * fz + q.e[1]));
+                    n2 += (double)(w * (q.K[6] * fx + q.K[7] * fy + q.K[8] * fz + q.e[2]));
+                    dd += (double)w;
+                }
+            } else {
+                descend = true;
+            }
+            stop = descend ? INT32_MAX : depth;
+        }
+        node = __any(descend) ? node + 1 : uniform(r->skip);
+    }
+    if (valid) {
+        const int64_t j = perm[k];
+        const double oden = 1.0 / dd;
+        xv[3 * j] = v.x + n0 * oden;
+        xv[3 * j + 1] = v.y + n1 * oden;
+        xv[3 * j + 2] = v.z + n2 * oden;
+        den[k] = dd;
+    }
+}
+
+// Sum of 16 per-lane values over the wave; afterwards lane 4*q (q = 0..15)
+// holds the total of value q.  Transpose-reduce: at each of the first four
+// steps a lane keeps half of its values and receives the partner's copy of
+// them, so the exchange volume halves every step.
+__device__ __forceinline__ double xchg_dpp(double x, int ctrl) {
+    const int lo = __double2loint(x), hi = __double2hiint(x);
+    int rl, rh;
+    switch (ctrl) {
+        case 0x140:
+            rl = __builtin_amdgcn_update_dpp(0, lo, 0x140, 0xf, 0xf, false);
+            rh = __builtin_amdgcn_update_dpp(0, hi, 0x140, 0xf, 0xf, false);
+            break;
+        case 0x141:
+            rl = __builtin_amdgcn_update_dpp(0, lo, 0x141, 0xf, 0xf, false);
+            rh = __builtin_amdgcn_update_dpp(0, hi, 0x141, 0xf, 0xf, false);
+            break;
+        case 0x4e:  // quad_perm [2,3,0,1]
+            rl = __builtin_amdgcn_update_dpp(0, lo, 0x4e, 0xf, 0xf, false);
+            rh = __builtin_amdgcn_update_dpp(0, hi, 0x4e, 0xf, 0xf, false);
+            break;
+        default:  // 0xb1 quad_perm [1,0,3,2]
+            rl = __builtin_amdgcn_update_dpp(0, lo, 0xb1, 0xf, 0xf, false);
+            rh = __builtin_amdgcn_update_dpp(0, hi, 0xb1, 0xf, 0xf, false);
+            break;
+    }
+    return __hiloint2double(rh, rl);
+}
+
+// a + partner's b across the 32-lane (sw32) or 16-lane (sw16) halves, where
+// lanes in the lower half keep a and lanes in the upper half keep b.
+__device__ __forceinline__ double swap_add32(double a, double b) {
+    const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)__double2loint(a), (unsigned)__double2loint(b), false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)__double2hiint(a), (unsigned)__double2hiint(b), false, false);
+    return __hiloint2double((int)hi[0], (int)lo[0]) + __hiloint2double((int)hi[1], (int)lo[1]);
+}
+__device__ __forceinline__ double swap_add16(double a, double b) {
+    const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)__double2loint(a), (unsigned)__double2loint(b), false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)__double2hiint(a), (unsigned)__double2hiint(b), false, false);
+    return __hiloint2double((int)hi[0], (int)lo[0]) + __hiloint2double((int)hi[1], (int)lo[1]);
+}
+
+__device__ __forceinline__ double wave_reduce16(double *v) {
+    const int lane = threadIdx.x & 63;
+    // bit 5: keep v[k] (lower 32 lanes) or v[k+8] (upper)
+    double a[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) a[k] = swap_add32(v[k], v[k + 8]);
+    // bit 4
+    double b[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) b[k] = swap_add16(a[k], a[k + 4]);
+    // bit 3: partner lane ^ 15 (row mirror)
+    const bool hi3 = (lane >> 3) & 1;
+    double c[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const double keep = hi3 ? b[k + 2] : b[k], send = hi3 ? b[k] : b[k + 2];
+        c[k] = keep + xchg_dpp(send, 0x140);
+    }
+    // bit 2: partner lane ^ 7 (half-row mirror)
+    const bool hi2 = (lane >> 2) & 1;
+    const double keep = hi2 ? c[1] : c[0], send = hi2 ? c[0] : c[1];
+    double d = keep + xchg_dpp(send, 0x141);
+    d += xchg_dpp(d, 0x4e);
+    d += xchg_dpp(d, 0xb1);
+    return d;
+}
+
+// index of the value that lane 4q holds after wave_reduce16
+__device__ __forceinline__ int reduced_index(int lane) {
+    return ((lane >> 5) & 1) * 8 + ((lane >> 4) & 1) * 4 + ((lane >> 3) & 1) * 2 + ((lane >> 2) & 1);
+}
+
+__device__ __forceinline__ void reduce_add12(double *c, double *dst) {
+    double v[16];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) v[k] = c[k];
+    v[12] = v[13] = v[14] = v[15] = 0.0;
+    const double s = wave_reduce16(v);
+    const int lane = threadIdx.x & 63;
+    const int q = reduced_index(lane);
+    if ((lane & 3) == 0 && q < 12 && s != 0.0) atomicAdd(dst + q, s);
+}
+
+// Reverse walk (evalNode_b, kd_tree.F90:942-1031): same decisions as the
+// forward; accepted clusters accumulate C(r,c) = sum s_r (x_c - X_c) and
+// B = sum s per node, leaf points Mb'(r,c) = sum s_r (x_c - Xu0_c) and
+// Bb = sum s, with s = w * numb.
+template <int PREC>
+__global__ void __launch_bounds__(256) k_fast_rev(int nvol, const double4 *__restrict__ vol,
+                                                  const double4 *__restrict__ numb, int nnodes,
+                                                  const NodeRec *__restrict__ nodes, const double4 *__restrict__ ps,
+                                                  WParams P, double *__restrict__ nadj, double *__restrict__ padj) {
+    const int nb = (nvol + 255) / 256;
+    const int blk = xcd_block(nb);
+    if (blk >= nb) return;
+    const int k = blk * 256 + threadIdx.x;
+    const bool valid = k < nvol;
+    const double4 v = valid ? vol[k] : make_double4(0.0, 0.0, 0.0, 1.0);
+    const double4 s = valid ? numb[k] : make_double4(0.0, 0.0, 0.0, 0.0);
+    const double tol = P.errTol * v.w;
+    int stop = valid ? INT32_MAX : -1;
+    int node = 0;
+    double c[12];
+    while (node < nnodes) {
+        const NodeRec *r = nodes + node;
+        const int depth = uniform(r->depth);
+        const bool active = depth <= stop;
+        if (uniform(r->leaf)) {
+            const int l = uniform(r->l), u = uniform(r->u);
+            for (int i = l; i <= u; ++i) {
+                const double4 p = ps[i];
+                const double rx = v.x - p.x, ry = v.y - p.y, rz = v.z - p.z;
+                double w;
+                if (PREC == IDW_PREC_FP64) {
+                    w = active ? p.w * wfun(P.Ldef / sqrt(rx * rx + ry * ry + rz * rz + EPS_W), P) : 0.0;
+                } else {
+                    const double d2 = rx * rx + ry * ry + rz * rz;
+                    if (__any(active && d2 < P.nearT2)) {
+                        w = active ? p.w * wfun(P.Ldef / sqrt(d2 + EPS_W), P) : 0.0;
+                    } else {
+                        const float fx = (float)rx, fy = (float)ry, fz = (float)rz;
+                        const float lo = (float)P.Ldef * __frsqrt_rn(fx * fx + fy * fy + fz * fz);
+                        w = active ? (double)((float)p.w * wfun(lo, P)) : 0.0;
+                    }
+                }
+                const double s0 = w * s.x, s1 = w * s.y, s2 = w * s.z;
+                c[0] = s0 * rx; c[1] = s0 * ry; c[2] = s0 * rz;
+                c[3] = s1 * rx; c[4] = s1 * ry; c[5] = s1 * rz;
+                c[6] = s2 * rx; c[7] = s2 * ry; c[8] = s2 * rz;
+                c[9] = s0; c[10] = s1; c[11] = s2;
+                reduce_add12(c, padj + (int64_t)i * ADJ_STRIDE);
+            }
+            node = uniform(r->skip);
+            continue;
+        }
+        const double cx = v.x - r->X[0], cy = v.y - r->X[1], cz = v.z - r->X[2];
+        const double dist = sqrt(cx * cx + cy * cy + cz * cz);
+        bool descend = false, accept = false;
+        if (active) {
+            const double dOvrR = dist * r->orad;
+            if (dOvrR < 2.0) descend = true;
+            else if (get_error(r, dOvrR) < tol) accept = true;
+            else descend = true;
+            stop = descend ? INT32_MAX : depth;
+        }
+        if (__any(accept)) {
+            double w = 0.0;
+            if (accept) {
+                if (PREC == IDW_PREC_FP64) w = r->Ai * wfun(P.Ldef / dist, P);
+                else w = (double)((float)r->Ai * wfun((float)P.Ldef * __frcp_rn((float)dist), P));
+            }
+            const double s0 = w * s.x, s1 = w * s.y, s2 = w * s.z;
+            c[0] = s0 * cx; c[1] = s0 * cy; c[2] = s0 * cz;
+            c[3] = s1 * cx; c[4] = s1 * cy; c[5] = s1 * cz;
+            c[6] = s2 * cx; c[7] = s2 * cy; c[8] = s2 * cz;
+            c[9] = s0; c[10] = s1; c[11] = s2;
+            reduce_add12(c, nadj + (int64_t)node * ADJ_STRIDE);
+        }
+        node = __any(descend) ? node + 1 : uniform(r->skip);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Nodal reverse (setData_b + COMPUTENODALPROPERTIES_B, kd_tree.F90:1033-1162)
+// ---------------------------------------------------------------------------
+// GETROTATIONMATRIX3D_B (vectorUtils_b.f90:7-154): d<Mb, M>/dv2, row-major Mb.
+__device__ void rodrigues_b(const double *v1, const double *v2, const double *Mb, double *v2b) {
+    double ax0[3];
+    cross3(v1, v2, ax0);
+    const double axmag = sqrt(ax0[0] * ax0[0] + ax0[1] * ax0[1] + ax0[2] * ax0[2] + 1e-30);
+    v2b[0] = v2b[1] = v2b[2] = 0.0;
+    if (axmag < ROT_TOL) return;  // M = I there: no dependence on v2
+    const double m1 = sqrt(v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2] + 1e-30);
+    const double m2 = sqrt(v2[0] * v2[0] + v2[1] * v2[1] + v2[2] * v2[2] + 1e-30);
+    const double ax[3] = {ax0[0] / axmag, ax0[1] / axmag, ax0[2] / axmag};
+    const double u1[3] = {v1[0] / m1, v1[1] / m1, v1[2] / m1};
+    const double u2[3] = {v2[0] / m2, v2[1] / m2, v2[2] / m2};
+    const double dot = u1[0] * u2[0] + u1[1] * u2[1] + u1[2] * u2[2];
+    const bool clipped = !(1.0 > dot);
+    const double arg = clipped ? 1.0 : dot;
+    const double ang = acos(arg), sn = sin(ang), cs = cos(ang);
+    const double A[9] = {0.0, -ax[2], ax[1], ax[2], 0.0, -ax[0], -ax[1], ax[0], 0.0};
+    // angle adjoint: d/da of sin(a) A + (1-cos a) A^2
+    double sA = 0.0, sC = 0.0, Ab[9];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+            const double C = A[3 * r] * A[c] + A[3 * r + 1] * A[3 + c] + A[3 * r + 2] * A[6 + c];
+            sA += A[3 * r + c] * Mb[3 * r + c];
+            sC += C * Mb[3 * r + c];
+        }
+    const double angb = cs * sA + sn * sC;
+    // skew adjoint: sin(a) Mb + (1-cos a) (Mb A^T + A^T Mb)
+    const double oc = 1.0 - cs;
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+            double t = 0.0;
+            for (int m = 0; m < 3; ++m) t += Mb[3 * r + m] * A[3 * c + m] + A[3 * m + r] * Mb[3 * m + c];
+            Ab[3 * r + c] = sn * Mb[3 * r + c] + oc * t;
+        }
+    double axb[3] = {Ab[7] - Ab[5], Ab[2] - Ab[6], Ab[3] - Ab[1]};
+    // angle = acos(u1 . u2)
+    const double argb = (arg == 1.0 || arg == -1.0) ? 0.0 : -(angb / sqrt(1.0 - arg * arg));
+    double u2b[3] = {0.0, 0.0, 0.0};
+    if (!clipped) { u2b[0] = u1[0] * argb; u2b[1] = u1[1] * argb; u2b[2] = u1[2] * argb; }
+    // u2 = v2 / |v2|
+    const double m2b = -((v2[0] * u2b[0] + v2[1] * u2b[1] + v2[2] * u2b[2]) / (m2 * m2));
+    for (int k = 0; k < 3; ++k) v2b[k] += u2b[k] / m2 + v2[k] * (m2b / m2);
+    // axis = (v1 x v2) / |v1 x v2|
+    const double axmagb = -((ax0[0] * axb[0] + ax0[1] * axb[1] + ax0[2] * axb[2]) / (axmag * axmag));
+    double ax0b[3];
+    for (int k = 0; k < 3; ++k) ax0b[k] = axb[k] / axmag + ax0[k] * (axmagb / axmag);
+    // d(v1 x v2)/dv2: v2b += ax0b x v1
+    v2b[0] += ax0b[1] * v1[2] - ax0b[2] * v1[1];
+    v2b[1] += ax0b[2] * v1[0] - ax0b[0] * v1[2];
+    v2b[2] += ax0b[0] * v1[1] - ax0b[1] * v1[0];
+}
+
+// Per node: complete Mb', Bb (leaf/exact part in padj plus the shares of the
+// accepted clusters containing the point, setData_b), then the rotation
+// backward and the backward of normal = sumNormal / sumArea.
+__global__ void k_nodal_rev_a(int n, int fast, int useRot, const NodeRec *__restrict__ nodes,
+                              const double *__restrict__ nadj, const double *__restrict__ padj,
+                              const double4 *__restrict__ ps, const double4 *__restrict__ nprops,
+                              const double *__restrict__ normals0, const int32_t *__restrict__ corner,
+                              double *__restrict__ ntmp) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double Mb[9], Bb[3];
+    const double *pa = padj + (int64_t)i * ADJ_STRIDE;
+    for (int k = 0; k < 9; ++k) Mb[k] = pa[k];
+    for (int k = 0; k < 3; ++k) Bb[k] = pa[9 + k];
+    const double4 p = ps[i];
+    if (fast) {
+        int node = 0;
+        while (!nodes[node].leaf) {
+            const NodeRec &r = nodes[node];
+            const double *na = nadj + (int64_t)node * ADJ_STRIDE;
+            const double ovrN = 1.0 / r.n;
+            const double dx = r.X[0] - p.x, dy = r.X[1] - p.y, dz = r.X[2] - p.z;
+            for (int q = 0; q < 3; ++q) {
+                const double b = na[9 + q];
+                Mb[3 * q] += (na[3 * q] + b * dx) * ovrN;
+                Mb[3 * q + 1] += (na[3 * q + 1] + b * dy) * ovrN;
+                Mb[3 * q + 2] += (na[3 * q + 2] + b * dz) * ovrN;
+                Bb[q] += b * ovrN;
+            }
+            const int left = node + 1;
+            node = (i <= nodes[left].u) ? left : nodes[left].skip;
+        }
+    }
+    double nb[3] = {0.0, 0.0, 0.0};
+    const double4 np4 = nprops[i];
+    const double sa = np4.x, sn[3] = {np4.y, np4.z, np4.w};
+    if (useRot && !corner[i]) {
+        const double n0[3] = {normals0[3 * i], normals0[3 * i + 1], normals0[3 * i + 2]};
+        const double nrm[3] = {sn[0] / sa, sn[1] / sa, sn[2] / sa};
+        rodrigues_b(n0, nrm, Mb, nb);
+    }
+    double *o = ntmp + (int64_t)i * 8;
+    o[0] = Bb[0]; o[1] = Bb[1]; o[2] = Bb[2];
+    o[3] = nb[0] / sa; o[4] = nb[1] / sa; o[5] = nb[2] / sa;
+    o[6] = (-(sn[0] * nb[0] / sa) + -(sn[1] * nb[1] / sa) + -(sn[2] * nb[2] / sa)) / sa;
+    o[7] = 0.0;
+}
+
+// GETELEMENTPROPS_B per face with the seeds of all its slots summed:
+// area seed sum (sab + n_f . snb)/nPts, unit-normal seed dA_f * sum snb.
+__global__ void k_nodal_rev_face(int nface, const int32_t *__restrict__ facePtr, const int32_t *__restrict__ faceConn,
+                                 const int32_t *__restrict__ xuInd, const double *__restrict__ xs,
+                                 const double4 *__restrict__ fprops, const double *__restrict__ ntmp,
+                                 double *__restrict__ pb) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nface) return;
+    const int p0 = facePtr[f], np = facePtr[f + 1] - p0;
+    const double4 fp = fprops[f];
+    double areab = 0.0, nrb[3] = {0.0, 0.0, 0.0};
+    for (int j = 0; j < np; ++j) {
+        const double *t = ntmp + (int64_t)faceConn[p0 + j] * 8;
+        areab += (t[6] + (fp.y * t[3] + fp.z * t[4] + fp.w * t[5])) / np;
+        nrb[0] += t[3]; nrb[1] += t[4]; nrb[2] += t[5];
+    }
+    nrb[0] *= fp.x; nrb[1] *= fp.x; nrb[2] *= fp.x;
+    // recompute the fan (radial vectors, cross products) of getElementProps
+    double P[20][3], c[3] = {0.0, 0.0, 0.0};
+    for (int j = 0; j < np; ++j) {
+        load_xu(xs, xuInd, faceConn[p0 + j], P[j]);
+        c[0] += P[j][0]; c[1] += P[j][1]; c[2] += P[j][2];
+    }
+    c[0] /= np; c[1] /= np; c[2] /= np;
+    for (int j = 0; j < np; ++j) { P[j][0] -= c[0]; P[j][1] -= c[1]; P[j][2] -= c[2]; }
+    double nsum[3] = {0.0, 0.0, 0.0};
+    for (int j = 0; j < np; ++j) {
+        double cr[3];
+        cross3(P[j], P[j + 1 < np ? j + 1 : 0], cr);
+        const double cn = sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2] + 1e-15);
+        nsum[0] += cr[0] / cn; nsum[1] += cr[1] / cn; nsum[2] += cr[2] / cn;
+    }
+    // normal = nsum / |nsum|
+    const double t2 = nsum[0] * nsum[0] + nsum[1] * nsum[1] + nsum[2] * nsum[2], t = sqrt(t2);
+    const double g = -((nsum[0] * nrb[0] + nsum[1] * nrb[1] + nsum[2] * nrb[2]) / (t2 * t));
+    const double nsb[3] = {nrb[0] / t + nsum[0] * g, nrb[1] / t + nsum[1] * g, nrb[2] / t + nsum[2] * g};
+    double rvb[20][3];
+    for (int j = 0; j < np; ++j) rvb[j][0] = rvb[j][1] = rvb[j][2] = 0.0;
+    for (int j = 0; j < np; ++j) {
+        const int j2 = j + 1 < np ? j + 1 : 0;
+        double cr[3];
+        cross3(P[j], P[j2], cr);
+        const double cn = sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2] + 1e-15);
+        // area += cn/2, nsum += cr/cn
+        const double cnb = 0.5 * areab - (cr[0] * nsb[0] + cr[1] * nsb[1] + cr[2] * nsb[2]) / (cn * cn);
+        double crb[3];
+        for (int k = 0; k < 3; ++k) crb[k] = nsb[k] / cn + cr[k] * (cnb / cn);
+        // cr = a x b: ab += b x crb, bb += crb x a
+        const double *a = P[j], *b = P[j2];
+        rvb[j][0] += b[1] * crb[2] - b[2] * crb[1];
+        rvb[j][1] += b[2] * crb[0] - b[0] * crb[2];
+        rvb[j][2] += b[0] * crb[1] - b[1] * crb[0];
+        rvb[j2][0] += crb[1] * a[2] - crb[2] * a[1];
+        rvb[j2][1] += crb[2] * a[0] - crb[0] * a[2];
+        rvb[j2][2] += crb[0] * a[1] - crb[1] * a[0];
+    }
+    double cb[3] = {0.0, 0.0, 0.0};
+    for (int j = 0; j < np; ++j) { cb[0] -= rvb[j][0]; cb[1] -= rvb[j][1]; cb[2] -= rvb[j][2]; }
+    for (int j = 0; j < np; ++j)
+        for (int k = 0; k < 3; ++k) pb[3 * (int64_t)(p0 + j) + k] = rvb[j][k] + cb[k] / np;
+}
+
+// dXs(XuInd(i)) = Bb_i + sum of the face-slot adjoints of node i
+// (kd_tree.F90:1146-1159: only the first duplicate receives sensitivity).
+__global__ void k_nodal_rev_gather(int n, const int32_t *__restrict__ slotStart, const int32_t *__restrict__ slots,
+                                   const double *__restrict__ pb, const double *__restrict__ ntmp,
+                                   const int32_t *__restrict__ xuInd, double *__restrict__ dxs) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double *t = ntmp + (int64_t)i * 8;
+    double x[3] = {t[0], t[1], t[2]};
+    for (int e = slotStart[i]; e < slotStart[i + 1]; ++e) {
+        const double *q = pb + 3 * (int64_t)slots[e];
+        x[0] += q[0]; x[1] += q[1]; x[2] += q[2];
+    }
+    const int64_t j = xuInd[i];
+    dxs[3 * j] = x[0]; dxs[3 * j + 1] = x[1]; dxs[3 * j + 2] = x[2];
+}
+
+__global__ void k_fill(int64_t n, double *__restrict__ a, double v) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) a[t] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Init-time kernels
+// ---------------------------------------------------------------------------
+__constant__ double c_sphere[20][3];
+
+// computeErrors (kd_tree.F90:832-880): for internal node q and radius j,
+// the RMS over the 20 sphere points of (exact - cluster) denominator.  One
+// block per internal node, one thread per (j, sphere point).
+__global__ void __launch_bounds__(256) k_errors(const int32_t *__restrict__ internal, NodeRec *__restrict__ nodes,
+                                                const double4 *__restrict__ ps, WParams P) {
+    __shared__ double e2[NERR][20];
+    const int node = internal[blockIdx.x];
+    const NodeRec &r = nodes[node];
+    const int t = threadIdx.x;
+    if (t < NERR * 20) {
+        const int j = t / 20, ii = t % 20;
+        const double radius = 1.0 / r.orad * c_rstar[j];
+        const double s3 = sqrt(3.0);
+        double vp[3];
+        for (int k = 0; k < 3; ++k) vp[k] = c_sphere[ii][k] * radius / s3 + r.X[k];
+        double ex = 0.0;
+        for (int i = r.l; i <= r.u; ++i) {
+            const double4 p = ps[i];
+            const double rx = vp[0] - p.x, ry = vp[1] - p.y, rz = vp[2] - p.z;
+            ex += p.w * wfun(P.Ldef / sqrt(rx * rx + ry * ry + rz * rz), P);
+        }
+        const double rx = vp[0] - r.X[0], ry = vp[1] - r.X[1], rz = vp[2] - r.X[2];
+        const double ap = r.Ai * wfun(P.Ldef / sqrt(rx * rx + ry * ry + rz * rz), P);
+        const double d = ex - ap;
+        e2[j][ii] = d * d;
+    }
+    __syncthreads();
+    if (t < NERR) {
+        double s = 0.0;
+        for (int ii = 0; ii < 20; ++ii) s += e2[t][ii];
+        nodes[node].err[t] = sqrt(s / 20);
+    }
+}
+
+// getWiEstimate (kd_tree.F90:751-787) for every volume node: far when
+// dist*oradius > 5 with dist including 1e-16.  Writes vol[k].w.
+__global__ void __launch_bounds__(256) k_den0(int nvol, double4 *__restrict__ vol, int nnodes,
+                                              const NodeRec *__restrict__ nodes, const double4 *__restrict__ ps,
+                                              WParams P) {
+    const int nb = (nvol + 255) / 256;
+    const int blk = xcd_block(nb);
+    if (blk >= nb) return;
+    const int k = blk * 256 + threadIdx.x;
+    const bool valid = k < nvol;
+    const double4 v = valid ? vol[k] : make_double4(0.0, 0.0, 0.0, 0.0);
+    int stop = valid ? INT32_MAX : -1;
+    double dd = 0.0;
+    int node = 0;
+    while (node < nnodes) {
+        const NodeRec *r = nodes + node;
+        const int depth = uniform(r->depth);
+        const bool active = depth <= stop;
+        if (uniform(r->leaf)) {
+            const int l = uniform(r->l), u = uniform(r->u);
+            for (int i = l; i <= u; ++i) {
+                const double4 p = ps[i];
+                const double rx = v.x - p.x, ry = v.y - p.y, rz = v.z - p.z;
+                const double w = p.w * wfun(P.Ldef / sqrt(rx * rx + ry * ry + rz * rz + EPS_DEN0), P);
+                if (active) dd += w;
+            }
+            node = uniform(r->skip);
+            continue;
+        }
+        const double cx = v.x - r->X[0], cy = v.y - r->X[1], cz = v.z - r->X[2];
+        const double dist = sqrt(cx * cx + cy * cy + cz * cz + EPS_DEN0);
+        bool descend = false;
+        if (active) {
+            if (dist * r->orad > 5.0) dd += r->Ai * wfun(P.Ldef / dist, P);
+            else descend = true;
+            stop = descend ? INT32_MAX : depth;
+        }
+        node = __any(descend) ? node + 1 : uniform(r->skip);
+    }
+    if (valid) vol[k].w = dd;
+}
+
+__global__ void k_morton(int64_t nvol, const double *__restrict__ xv0, double3 lo, double3 scale,
+                         uint64_t *__restrict__ keys, int32_t *__restrict__ ids) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nvol) return;
+    auto spread = [](uint64_t x) {
+        x &= 0x1fffff;
+        x = (x | x << 32) & 0x1f00000000ffffull;
+        x = (x | x << 16) & 0x1f0000ff0000ffull;
+        x = (x | x << 8) & 0x100f00f00f00f00full;
+        x = (x | x << 4) & 0x10c30c30c30c30c3ull;
+        x = (x | x << 2) & 0x1249249249249249ull;
+        return x;
+    };
+    auto q = [](double t) {
+        const double c = fmin(fmax(t, 0.0), 2097151.0);
+        return (uint64_t)c;
+    };
+    const uint64_t x = q((xv0[3 * k] - lo.x) * scale.x), y = q((xv0[3 * k + 1] - lo.y) * scale.y),
+                   z = q((xv0[3 * k + 2] - lo.z) * scale.z);
+    keys[k] = spread(x) | (spread(y) << 1) | (spread(z) << 2);
+    ids[k] = (int32_t)k;
+}
+
+__global__ void k_build_vol(int64_t nvol, const double *__restrict__ xv0, const int32_t *__restrict__ perm,
+                            double4 *__restrict__ vol) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nvol) return;
+    const int64_t j = perm[k];
+    vol[k] = make_double4(xv0[3 * j], xv0[3 * j + 1], xv0[3 * j + 2], 0.0);
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static inline unsigned grid_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+static inline unsigned xcd_grid(int64_t n) {
+    const int64_t nb = (n + 255) / 256;
+    return (unsigned)(((nb + 7) / 8) * 8);
+}
+
+void launch_face_props(hipStream_t st, const DevSurface &S, const double *xs) {
+    if (S.nface) k_face_props<<<grid_for(S.nface, 256), 256, 0, st>>>(S.nface, S.facePtr, S.faceConn, S.xuInd, xs, S.fprops);
+}
+
+void launch_node_props(hipStream_t st, const DevSurface &S, const double *xs, int initial, int useRot) {
+    k_node_props<<<grid_for(S.n, 256), 256, 0, st>>>(S.n, initial, useRot, S.n2eStart, S.n2e, S.fprops, S.xuInd,
+                                                     S.corner, xs, S.ps, S.normals0, S.nprops, S.pd, S.pd32);
+}
+
+void launch_corners(hipStream_t st, const DevSurface &S, double threshold) {
+    k_corners<<<grid_for(S.n, 256), 256, 0, st>>>(S.n, threshold, S.n2eStart, S.n2e, S.fprops, S.normals0, S.corner);
+}
+
+void launch_cluster_data(hipStream_t st, const DevSurface &S) {
+    if (S.nInternal) k_cluster_data<<<S.nInternal, 256, 0, st>>>(S.internal, S.nodes, S.ps, S.pd, S.nd, S.nd32);
+}
+
+void launch_errors(hipStream_t st, const DevSurface &S, const WParams &P, const double *sphere60) {
+    (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(c_sphere), sphere60, sizeof(double) * 60, 0, hipMemcpyHostToDevice, st);
+    if (S.nInternal) k_errors<<<S.nInternal, 256, 0, st>>>(S.internal, S.nodes, S.ps, P);
+}
+
+void launch_den0(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P) {
+    k_den0<<<xcd_grid(V.n), 256, 0, st>>>((int)V.n, V.vol, S.nnodes, S.nodes, S.ps, P);
+}
+
+void launch_morton(hipStream_t st, int64_t n, const double *xv0, double lo[3], double scale[3], uint64_t *keys,
+                   int32_t *ids) {
+    k_morton<<<grid_for(n, 256), 256, 0, st>>>(n, xv0, make_double3(lo[0], lo[1], lo[2]),
+                                               make_double3(scale[0], scale[1], scale[2]), keys, ids);
+}
+
+void launch_build_vol(hipStream_t st, int64_t n, const double *xv0, const int32_t *perm, double4 *vol) {
+    k_build_vol<<<grid_for(n, 256), 256, 0, st>>>(n, xv0, perm, vol);
+}
+
+void launch_fill(hipStream_t st, double *a, int64_t n, double v) {
+    if (n > 0) k_fill<<<grid_for(n, 256), 256, 0, st>>>(n, a, v);
+}
+
+void launch_forward(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, int fast, int prec,
+                    double *xv) {
+    const unsigned g = xcd_grid(V.n);
+    if (!fast) {
+        if (prec == IDW_PREC_FP64)
+            k_exact_fwd<IDW_PREC_FP64><<<g, 256, 0, st>>>((int)V.n, V.vol, V.perm, S.n, S.ps, S.pd, S.pd32, P, xv, V.den);
+        else
+            k_exact_fwd<IDW_PREC_MIXED><<<g, 256, 0, st>>>((int)V.n, V.vol, V.perm, S.n, S.ps, S.pd, S.pd32, P, xv, V.den);
+    } else {
+        if (prec == IDW_PREC_FP64)
+            k_fast_fwd<IDW_PREC_FP64><<<g, 256, 0, st>>>((int)V.n, V.vol, V.perm, S.nnodes, S.nodes, S.nd, S.nd32, S.ps,
+                                                        S.pd, S.pd32, P, xv, V.den);
+        else
+            k_fast_fwd<IDW_PREC_MIXED><<<g, 256, 0, st>>>((int)V.n, V.vol, V.perm, S.nnodes, S.nodes, S.nd, S.nd32,
+                                                         S.ps, S.pd, S.pd32, P, xv, V.den);
+    }
+}
+
+void launch_prep_numb(hipStream_t st, const DevVolume &V, const double *dxv) {
+    k_prep_numb<<<grid_for(V.n, 256), 256, 0, st>>>((int)V.n, V.perm, dxv, V.den, V.numb);
+}
+
+void launch_reverse(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, int fast, int prec,
+                    double *partial, int nchunk) {
+    if (!fast) {
+        const int chunk = (int)((V.n + nchunk - 1) / nchunk);
+        dim3 g(grid_for(S.n, 256), nchunk);
+        if (prec == IDW_PREC_FP64)
+            k_exact_rev<IDW_PREC_FP64><<<g, 256, 0, st>>>((int)V.n, chunk, V.vol, V.numb, S.n, S.ps, P, partial);
+        else
+            k_exact_rev<IDW_PREC_MIXED><<<g, 256, 0, st>>>((int)V.n, chunk, V.vol, V.numb, S.n, S.ps, P, partial);
+        k_exact_rev_reduce<<<grid_for((int64_t)S.n * 12, 256), 256, 0, st>>>(S.n, nchunk, partial, S.padj);
+    } else {
+        launch_fill(st, S.padj, (int64_t)S.n * ADJ_STRIDE, 0.0);
+        launch_fill(st, S.nadj, (int64_t)S.nnodes * ADJ_STRIDE, 0.0);
+        const unsigned g = xcd_grid(V.n);
+        if (prec == IDW_PREC_FP64)
+            k_fast_rev<IDW_PREC_FP64><<<g, 256, 0, st>>>((int)V.n, V.vol, V.numb, S.nnodes, S.nodes, S.ps, P, S.nadj, S.padj);
+        else
+            k_fast_rev<IDW_PREC_MIXED><<<g, 256, 0, st>>>((int)V.n, V.vol, V.numb, S.nnodes, S.nodes, S.ps, P, S.nadj,
+                                                         S.padj);
+    }
+}
+
+void launch_nodal_reverse(hipStream_t st, const DevSurface &S, const double *xs, int fast, int useRot, double *dxs,
+                          int64_t npts) {
+    k_nodal_rev_a<<<grid_for(S.n, 256), 256, 0, st>>>(S.n, fast, useRot, S.nodes, S.nadj, S.padj, S.ps, S.nprops,
+                                                      S.normals0, S.corner, S.ntmp);
+    if (S.nface)
+        k_nodal_rev_face<<<grid_for(S.nface, 128), 128, 0, st>>>(S.nface, S.facePtr, S.faceConn, S.xuInd, xs, S.fprops,
+                                                                 S.ntmp, S.pb);
+    launch_fill(st, dxs, 3 * npts, 0.0);
+    k_nodal_rev_gather<<<grid_for(S.n, 256), 256, 0, st>>>(S.n, S.slotStart, S.slots, S.pb, S.ntmp, S.xuInd, dxs);
+}
+
+}  // namespace idw

@@ -1,0 +1,284 @@
+"""USMesh: IDWarp's unstructured-mesh API on the MI355X volume-warp library.
+
+Method and option names, argument meaning and error behaviour follow
+idwarp/UnstructuredMesh.py (class USMesh, :43).  The volume grid comes from
+memory (``gridFile`` may be an (N, 3) array or a ``.npy`` path) instead of a
+CGNS/PLOT3D/OpenFOAM reader, and two options are added: ``precision``
+("fp64" or "mixed") and ``nearWallFactor``.  All numerics run in HIP kernels
+through ``libidwarp_hip.so``; nothing here computes a warp on the CPU.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+
+class Error(Exception):
+    """Raised for user errors, as baseclasses.Error in the reference."""
+
+    def __init__(self, message):
+        super().__init__(message)
+        self.message = message
+
+
+_DEFAULTS = {
+    "gridFile": [(str, np.ndarray, type(None)), None],
+    "fileType": [str, ["memory", "NPY", "CGNS", "OpenFOAM", "PLOT3D"]],
+    "specifiedSurfaces": [(list, type(None)), None],
+    "symmetrySurfaces": [(list, type(None)), None],
+    "symmetryPlanes": [(list, type(None)), None],
+    "aExp": [float, 3.0],
+    "bExp": [float, 5.0],
+    "LdefFact": [float, 1.0],
+    "alpha": [float, 0.25],
+    "errTol": [float, 0.0005],
+    "evalMode": [str, ["fast", "exact"]],
+    "symmTol": [float, 1e-6],
+    "useRotations": [bool, True],
+    "zeroCornerRotations": [bool, True],
+    "cornerAngle": [float, 30.0],
+    "restartFile": [(str, type(None)), None],
+    "bucketSize": [int, 8],
+    # device-only
+    "precision": [str, ["fp64", "mixed"]],
+    "nearWallFactor": [float, 1e-3],
+    "device": [int, 0],
+}
+
+
+class USMesh:
+    """Volume mesh warping (UnstructuredMesh.py:43-1091)."""
+
+    def __init__(self, options=None, comm=None, debug=False):
+        if options is None:
+            raise Error("The 'options' keyword argument is *NOT* optional. An options dictionary must be passed upon "
+                        "creation of this object")
+        self.comm = comm
+        self.name = "IDWarp"
+        self.category = "Volume mesh warping"
+        self._opts = {}
+        for k, (typ, default) in _DEFAULTS.items():
+            self._opts[k.lower()] = (k, default[0] if isinstance(default, list) else default)
+        for k, v in options.items():
+            self.setOption(k, v)
+        if self.getOption("symmetryPlanes") or self.getOption("symmetrySurfaces"):
+            raise Error("symmetry planes are not supported by the HIP backend")
+        L = _lib.load()
+        self._L = L
+        self._ctx = ctypes.c_void_p(0)
+        _lib.check(L.idw_create(ctypes.byref(self._ctx), int(self.getOption("device")), ctypes.byref(self._copts())))
+        self.dtype = "d"
+        self.warpInitialized = False
+        self.nSurf = 0
+        self.nVol = 0
+        grid = self.getOption("gridFile")
+        if grid is not None:
+            if isinstance(grid, str):
+                if self.getOption("fileType") not in ("NPY", "memory") or not grid.endswith(".npy"):
+                    raise Error("Only in-memory volume grids (an array or a .npy file) are supported")
+                grid = np.load(grid)
+            self.setVolumeNodes(grid)
+
+    # ------------------------------------------------------------------
+    # options (BaseSolver subset)
+    # ------------------------------------------------------------------
+    def setOption(self, name, value):
+        key = name.lower()
+        if key not in self._opts:
+            raise Error("Option '%s' is not a valid IDWarp option" % name)
+        spec = _DEFAULTS[self._opts[key][0]]
+        if isinstance(spec[1], list) and value not in spec[1]:
+            raise Error("Option '%s' must be one of %s" % (name, spec[1]))
+        if spec[0] is float and isinstance(value, (int, np.integer)) and not isinstance(value, bool):
+            value = float(value)
+        self._opts[key] = (self._opts[key][0], value)
+
+    def getOption(self, name):
+        key = name.lower()
+        if key not in self._opts:
+            raise Error("Option '%s' is not a valid IDWarp option" % name)
+        return self._opts[key][1]
+
+    def printOptions(self):
+        for key in sorted(self._opts):
+            print("%-22s: %s" % (self._opts[key][0], self._opts[key][1]))
+
+    def _copts(self):
+        o = _lib.IdwOptions()
+        self._L.idw_default_options(ctypes.byref(o)) if hasattr(self, "_L") else _lib.load().idw_default_options(
+            ctypes.byref(o))
+        g = self.getOption
+        o.aExp, o.bExp, o.LdefFact, o.alpha = g("aExp"), g("bExp"), g("LdefFact"), g("alpha")
+        o.errTol, o.cornerAngle = g("errTol"), g("cornerAngle")
+        o.evalMode = _lib.EVAL_FAST if g("evalMode") == "fast" else _lib.EVAL_EXACT
+        o.useRotations, o.zeroCornerRotations = int(bool(g("useRotations"))), int(bool(g("zeroCornerRotations")))
+        o.bucketSize = int(g("bucketSize"))
+        o.precision = _lib.PREC_MIXED if g("precision") == "mixed" else _lib.PREC_FP64
+        o.nearWallFactor = g("nearWallFactor")
+        return o
+
+    def _setMeshOptions(self):
+        """UnstructuredMesh.py:1051-1069: push options before each warp."""
+        _lib.check(self._L.idw_set_options(self._ctx, ctypes.byref(self._copts())))
+
+    def __del__(self):
+        ctx = getattr(self, "_ctx", None)
+        if ctx is not None and ctx.value:
+            self._L.idw_destroy(ctx)
+            self._ctx = None
+
+    # ------------------------------------------------------------------
+    # grid and surface definition
+    # ------------------------------------------------------------------
+    def setVolumeNodes(self, xv0):
+        """Set the undeformed volume grid (the CGNS reader's createCommonGrid,
+        src/IO/createGrid.F90), an (N, 3) array."""
+        xv0 = _lib.f64(xv0).reshape(-1, 3)
+        _lib.check(self._L.idw_set_volume_nodes(self._ctx, _lib.ptr(xv0), xv0.size))
+        self.nVol = len(xv0)
+
+    def setSurfaceDefinition(self, pts, conn, faceSizes, cgnsBlockID=None):
+        """UnstructuredMesh.py:196-262: point-reduce the surface, sort the
+        unique points by (z, y, x), relink and initialise the warp."""
+        pts = _lib.f64(pts).reshape(-1, 3)
+        conn = _lib.i32(np.array(conn).flatten())
+        faceSizes = _lib.i32(faceSizes)
+        uniquePts, link, nUnique = _lib.point_reduce(pts, 1e-12)
+        ind = np.lexsort(uniquePts.T)
+        uniquePts = np.ascontiguousarray(uniquePts[ind])
+        inv = np.zeros_like(ind)
+        inv[ind] = np.arange(len(ind))
+        link = _lib.i32(inv[link - 1] + 1)
+        self._setMeshOptions()
+        _lib.check(self._L.idw_initialize_warping(self._ctx, _lib.ptr(pts), len(pts), _lib.ptr(uniquePts),
+                                                  len(uniquePts), _lib.ptr(link), _lib.ptr(faceSizes),
+                                                  len(faceSizes), _lib.ptr(conn), len(conn)))
+        self.nSurf = len(pts)
+        self.warpInitialized = True
+
+    def setExternalMeshIndices(self, ind):
+        """0-based per-DOF indices of the solver grid into the common grid."""
+        ind = _lib.i32(ind)
+        _lib.check(self._L.idw_set_external_mesh_indices(self._ctx, _lib.ptr(ind), ind.size))
+
+    # ------------------------------------------------------------------
+    # surface coordinates
+    # ------------------------------------------------------------------
+    def getSurfaceCoordinates(self):
+        coords = np.zeros((self.nSurf, 3), self.dtype)
+        _lib.check(self._L.idw_get_surface_coordinates(self._ctx, _lib.ptr(coords), coords.size))
+        return coords
+
+    def setSurfaceCoordinates(self, coordinates):
+        if len(coordinates) != self.nSurf:
+            raise Error("Incorrect length of coordinates supplied to setSurfaceCoordinates on proc %d. Expected "
+                        "aray of length %d, received an array of length %d." % (0, self.nSurf, len(coordinates)))
+        c = _lib.f64(coordinates).ravel()
+        _lib.check(self._L.idw_set_surface_coordinates(self._ctx, _lib.ptr(c), c.size))
+
+    # ------------------------------------------------------------------
+    # warp, grids, derivatives
+    # ------------------------------------------------------------------
+    def warpMesh(self):
+        """Warp the volume to the surface set by setSurfaceCoordinates()."""
+        self._setMeshOptions()
+        _lib.check(self._L.idw_warp_mesh(self._ctx))
+
+    def getWarpGrid(self):
+        xv = np.zeros(3 * self.nVol, self.dtype)
+        _lib.check(self._L.idw_get_volume_coordinates(self._ctx, _lib.ptr(xv), xv.size))
+        return xv
+
+    def getCommonGrid(self):
+        return self.getWarpGrid()
+
+    def _solverDOF(self):
+        n = ctypes.c_int64(0)
+        _lib.check(self._L.idw_get_solver_dof(self._ctx, ctypes.byref(n)))
+        return n.value
+
+    def getSolverGrid(self):
+        """Warped grid in the order of setExternalMeshIndices()."""
+        warpGrid = self.getWarpGrid()
+        solverGrid = np.zeros(self._solverDOF(), self.dtype)
+        _lib.check(self._L.idw_warp_to_solver_grid(self._ctx, _lib.ptr(warpGrid), warpGrid.size, _lib.ptr(solverGrid),
+                                                   solverGrid.size))
+        return solverGrid
+
+    def warpDeriv(self, dXv, solverVec=True):
+        """dXs = (dXv/dXs)^T dXv; the result is read with getdXs()."""
+        self._setMeshOptions()
+        dXv = _lib.f64(dXv).ravel()
+        if solverVec:
+            dXvWarp = np.zeros(3 * self.nVol, self.dtype)
+            _lib.check(self._L.idw_solver_to_warp_grid(self._ctx, _lib.ptr(dXv), dXv.size, _lib.ptr(dXvWarp),
+                                                       dXvWarp.size))
+        else:
+            dXvWarp = dXv
+        _lib.check(self._L.idw_warp_deriv(self._ctx, _lib.ptr(dXvWarp), dXvWarp.size))
+
+    def getdXs(self):
+        dXs = np.zeros((self.nSurf, 3), self.dtype)
+        _lib.check(self._L.idw_get_dxs(self._ctx, _lib.ptr(dXs), dXs.size))
+        return dXs
+
+    def verifyWarpDeriv(self, dXv=None, solverVec=True, dofStart=0, dofEnd=10, h=1e-6, randomSeed=314):
+        """Central-FD check of warpDeriv on surface DOFs [dofStart, dofEnd)
+        (src/warp/verifyWarpDeriv.F90:43-148).  Prints and returns the
+        relative errors (%)."""
+        if dXv is None:
+            np.random.seed(randomSeed)
+            dXvWarp = np.random.random(3 * self.nVol)
+        elif solverVec:
+            dXv = _lib.f64(dXv).ravel()
+            dXvWarp = np.zeros(3 * self.nVol)
+            _lib.check(self._L.idw_solver_to_warp_grid(self._ctx, _lib.ptr(dXv), dXv.size, _lib.ptr(dXvWarp),
+                                                       dXvWarp.size))
+        else:
+            dXvWarp = _lib.f64(dXv).ravel()
+        xs0 = self.getSurfaceCoordinates().ravel()
+        self.warpMesh()
+        self.warpDeriv(dXvWarp, solverVec=False)
+        ad = self.getdXs().ravel()
+        errs = []
+        for dof in range(dofStart, min(dofEnd, xs0.size)):
+            xs = xs0.copy()
+            xs[dof] += h
+            self.setSurfaceCoordinates(xs.reshape(-1, 3))
+            self.warpMesh()
+            xp = self.getWarpGrid()
+            xs[dof] -= 2 * h
+            self.setSurfaceCoordinates(xs.reshape(-1, 3))
+            self.warpMesh()
+            xm = self.getWarpGrid()
+            fd = np.dot(dXvWarp, (xp - xm) / (2 * h))
+            err = (fd - ad[dof]) / (ad[dof] if ad[dof] != 0 else 1.0) * 100
+            print("Dof: %4d, FD: %18.10e, AD: %18.10e, Err(%%): %12.5e" % (dof, fd, ad[dof], err))
+            errs.append(err)
+        self.setSurfaceCoordinates(xs0.reshape(-1, 3))
+        self.warpMesh()
+        return np.array(errs)
+
+    # ------------------------------------------------------------------
+    # extras of the device library
+    # ------------------------------------------------------------------
+    def getLdef0(self):
+        v = ctypes.c_double(0.0)
+        _lib.check(self._L.idw_get_ldef0(self._ctx, ctypes.byref(v)))
+        return v.value
+
+    def getDenominator0(self):
+        d = np.zeros(self.nVol)
+        _lib.check(self._L.idw_get_denominator0(self._ctx, _lib.ptr(d), d.size))
+        return d
+
+    def getTimings(self):
+        """ms of the last warp/deriv: nodal+cluster, forward sum, reverse sum, nodal reverse."""
+        t = np.zeros(4)
+        _lib.check(self._L.idw_get_timings(self._ctx, _lib.ptr(t)))
+        return t
+
+    @property
+    def ctx(self):
+        return self._ctx

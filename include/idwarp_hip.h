@@ -1,0 +1,154 @@
+/*
+ * idwarp_hip.h -- C ABI of the MI355X volume-warp library (libidwarp_hip.so).
+ *
+ * The entry points replace the f2py module `libidwarp` that IDWarp's
+ * USMesh binds (src/f2py/libidwarp.pyf).  Each reference binding is cited
+ * beside its replacement.  All state lives in an opaque context (one per
+ * mesh instance) instead of Fortran module globals; this replaces the
+ * private shared-library copies of idwarp/MExt.py:29-63.
+ *
+ * Conventions
+ *  - Return 0 on success, a negative IDW_E* code on failure; the message of
+ *    the last failure on the calling thread is in idw_last_error().  No entry
+ *    point aborts the process (the reference's EChk calls MPI_Abort,
+ *    src/utils/EChk.F90:12-25).
+ *  - Coordinates are FP64, xyz-interleaved, flat (3*n values), as the
+ *    reference's numpy arrays.  Host buffers are caller-owned and copied.
+ *  - *_dev entry points take device pointers that are borrowed for the call
+ *    and enqueue on the given hipStream_t (NULL = default stream); they
+ *    return once the work is enqueued.
+ *  - A context is not re-entrant; distinct contexts may run concurrently on
+ *    distinct streams.
+ */
+#ifndef IDWARP_HIP_H
+#define IDWARP_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    IDW_OK = 0,
+    IDW_EINVAL = -1,  /* bad argument or size mismatch */
+    IDW_ESTATE = -2,  /* call out of order (e.g. warp before surface definition) */
+    IDW_EHIP = -3,    /* HIP runtime error */
+    IDW_ENOMEM = -4,  /* device or host allocation failed */
+    IDW_ENODEV = -5   /* no usable gfx950 device */
+};
+
+enum { IDW_EVAL_EXACT = 0, IDW_EVAL_FAST = 1 };       /* gridInput.f90:16-17 */
+enum { IDW_PREC_FP64 = 0, IDW_PREC_MIXED = 1 };
+
+/* Options of USMesh._getDefaultOptions (UnstructuredMesh.py:124-145) that
+ * reach the numerics, plus the device-only ones. */
+typedef struct idw_options {
+    double aExp;            /* 3.0 */
+    double bExp;            /* 5.0 */
+    double LdefFact;        /* 1.0 */
+    double alpha;           /* 0.25 */
+    double errTol;          /* 5e-4 */
+    double cornerAngle;     /* 30.0 degrees */
+    int32_t evalMode;       /* IDW_EVAL_FAST */
+    int32_t useRotations;   /* 1 */
+    int32_t zeroCornerRotations; /* 1 */
+    int32_t bucketSize;     /* 8 (kd_tree.F90:14 default 18 is overridden by USMesh) */
+    int32_t precision;      /* IDW_PREC_FP64 or IDW_PREC_MIXED (device only) */
+    int32_t reserved;
+    double nearWallFactor;  /* mixed mode: pairs closer than nearWallFactor*Ldef stay FP64 (1e-3) */
+} idw_options;
+
+typedef struct idw_ctx idw_ctx;
+
+/* Fill *opt with the reference defaults. */
+void idw_default_options(idw_options *opt);
+
+/* Message of the last failure on this thread ("" if none). */
+const char *idw_last_error(void);
+
+/* Create a context on HIP device `device`.  Replaces USMesh.__init__'s
+ * MExt load + initpetsc + _setMeshOptions (UnstructuredMesh.py:50-106). */
+int idw_create(idw_ctx **out, int device, const idw_options *opt);
+int idw_destroy(idw_ctx *ctx);
+
+/* _setMeshOptions (UnstructuredMesh.py:1051-1069): gridInput values used by
+ * the next warp.  As in the reference, the tree exponents, error tables and
+ * denominator estimate keep their values from idw_initialize_warping. */
+int idw_set_options(idw_ctx *ctx, const idw_options *opt);
+
+/* createCommonGrid (src/IO/createGrid.F90:1-77, reached from readcgns,
+ * libidwarp.pyf:19-21): the undeformed volume nodes, 3*n values.  Must be
+ * set before idw_initialize_warping. */
+int idw_set_volume_nodes(idw_ctx *ctx, const double *xv0, int64_t ndof);
+
+/* pointreduce (libidwarp.pyf:36-43, src/utils/pointReduce.F90:1-83):
+ * merge points within tol.  pts and unique_pts are 3*n; link (n) is 1-based
+ * into unique_pts.  Context-free. */
+int idw_point_reduce(const double *pts, int64_t n, double tol, double *unique_pts, int32_t *link,
+                     int64_t *n_unique);
+
+/* initializewarping (libidwarp.pyf:55-67, src/warp/initializeWarping.F90):
+ * pts 3*n_pts initial surface (local order), unique_pts 3*n_unique (sorted
+ * by z, y, x), link 1-based (n_pts), face_sizes (n_faces), face_conn
+ * 0-based into pts.  Builds the kd-tree, nodal properties, error tables and
+ * the denominator estimate on the device. */
+int idw_initialize_warping(idw_ctx *ctx, const double *pts, int64_t n_pts, const double *unique_pts,
+                           int64_t n_unique, const int32_t *link, const int32_t *face_sizes, int64_t n_faces,
+                           const int32_t *face_conn, int64_t n_conn);
+
+/* setsurfacecoordinates / getsurfacecoordinates (libidwarp.pyf:78-86). */
+int idw_set_surface_coordinates(idw_ctx *ctx, const double *xs, int64_t ndof);
+int idw_get_surface_coordinates(idw_ctx *ctx, double *xs, int64_t ndof);
+
+/* warpmesh (libidwarp.pyf:75-76, src/warp/warpMesh.F90): warp the volume
+ * for the current surface coordinates; result stays on the device. */
+int idw_warp_mesh(idw_ctx *ctx);
+
+/* getvolumecoordinates (libidwarp.pyf:45-48): copy the warped grid out. */
+int idw_get_volume_coordinates(idw_ctx *ctx, double *xv, int64_t ndof);
+
+/* setexternalmeshindices (libidwarp.pyf:23-26, setExternalMeshIndices.F90):
+ * 0-based per-DOF indices into the common grid, n = solver DOF count. */
+int idw_set_external_mesh_indices(idw_ctx *ctx, const int32_t *ind, int64_t n);
+int idw_get_solver_dof(idw_ctx *ctx, int64_t *n);
+
+/* warp_to_solver_grid / solver_to_warp_grid (libidwarp.pyf:88-100,
+ * gridTransform.F90:4-99).  The reverse scatter sums duplicated indices. */
+int idw_warp_to_solver_grid(idw_ctx *ctx, const double *warp_grid, int64_t wdof, double *solver_grid, int64_t sdof);
+int idw_solver_to_warp_grid(idw_ctx *ctx, const double *solver_grid, int64_t sdof, double *warp_grid, int64_t wdof);
+
+/* warpderiv (libidwarp.pyf:102-105, src/warp/warpDeriv.F90): dXs =
+ * (dXv/dXs)^T dxv for the current surface; dxv is 3*n_vol in warp order.
+ * Uses the denominator of the last warp at the same surface and options,
+ * recomputing it if there was none. */
+int idw_warp_deriv(idw_ctx *ctx, const double *dxv, int64_t ndof);
+
+/* getdxs (libidwarp.pyf:114-117): 3*n_pts result of the last warp_deriv. */
+int idw_get_dxs(idw_ctx *ctx, double *dxs, int64_t ndof);
+
+/* Device-resident forms of warpMesh and warpDeriv for frameworks that keep
+ * coordinates on the GPU (torch autograd wrapper).  d_xs: 3*n_pts surface,
+ * d_xv: 3*n_vol output, d_dxv: 3*n_vol seed, d_dxs: 3*n_pts output. */
+int idw_warp_mesh_dev(idw_ctx *ctx, const double *d_xs, double *d_xv, void *hip_stream);
+int idw_warp_deriv_dev(idw_ctx *ctx, const double *d_xs, const double *d_dxv, double *d_dxs, void *hip_stream);
+
+/* Sizes and derived scalars. */
+int idw_get_sizes(idw_ctx *ctx, int64_t *n_vol, int64_t *n_pts, int64_t *n_unique, int64_t *n_tree_nodes);
+int idw_get_ldef0(idw_ctx *ctx, double *ldef0);
+
+/* Denominator estimate of the fast mode (initializeWarping.F90:240-247),
+ * 3 -> 1 per volume node, warp order; n = n_vol. */
+int idw_get_denominator0(idw_ctx *ctx, double *den0, int64_t n);
+
+/* Per-kernel timing of the last warp / warp_deriv, measured with HIP events
+ * on the context's stream (milliseconds): [0] nodal properties + cluster
+ * data, [1] volume sum (forward), [2] volume sum (reverse), [3] nodal
+ * reverse. */
+int idw_get_timings(idw_ctx *ctx, double *ms4);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

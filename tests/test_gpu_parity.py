@@ -1,0 +1,149 @@
+"""HIP warp vs the CPU oracle on identical inputs (needs an MI355X).
+
+Tolerances: FP64 mode max|dXv| <= 1e-12 Ldef and adjoint
+relative error <= 1e-10 (the oracle and the device differ only in pow vs
+repeated products, FMA contraction and summation order); mixed mode
+max|dXv| <= 1e-6 Ldef and adjoint relative error <= 1e-6.
+"""
+import numpy as np
+import pytest
+
+from idwarp_amd import USMesh, synthetic
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+MIXED_TOL = 1e-6
+
+
+def _meshes():
+    return {
+        "sphere": synthetic.sphere_omesh(13, 9, 9),
+        "ellipsoid": synthetic.ellipsoid_omesh(33, 17, 13),
+        "star": synthetic.star_shells(n=9, nshell=6),
+    }
+
+
+MESHES = _meshes()
+
+
+def _pair(m, **opts):
+    ora = O.OracleMesh(m["pts"], m["conn"], m["faceSizes"], m["xv0"], **{k: v for k, v in opts.items()
+                                                                         if k not in ("precision", "nearWallFactor")})
+    gpu = USMesh(options=dict(gridFile=m["xv0"], fileType="memory", **opts))
+    gpu.setSurfaceDefinition(m["pts"], m["conn"], m["faceSizes"])
+    return ora, gpu
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+@pytest.mark.parametrize("name", sorted(MESHES))
+@pytest.mark.parametrize("mode", ["exact", "fast"])
+def test_forward_and_adjoint_fp64(name, mode):
+    m = MESHES[name]
+    print("case", name, mode, flush=True)
+    ora, gpu = _pair(m, evalMode=mode, errTol=1e-4)
+    ldef = ora.Ldef0
+    assert abs(gpu.getLdef0() - ldef) <= 1e-15 * ldef
+    np.testing.assert_allclose(gpu.getDenominator0(), ora.den0, rtol=1e-12)
+    xs = m["perturb"](m["pts"])
+    ora.setSurfaceCoordinates(xs)
+    gpu.setSurfaceCoordinates(xs)
+    ref = ora.warpMesh()
+    gpu.warpMesh()
+    got = gpu.getWarpGrid()
+    err = np.abs(got - ref).max() / ldef
+    print("max |dXv|/Ldef", err)
+    assert err <= FP64_TOL
+    dxv = synthetic.random_dxv(len(m["xv0"]), seed=5, frac=0.3)
+    dref = ora.warpDeriv(dxv)
+    gpu.warpDeriv(dxv, solverVec=False)
+    dgot = gpu.getdXs().ravel()
+    r = _rel(dgot, dref)
+    print("adjoint rel err", r)
+    assert r <= 1e-10
+
+
+@pytest.mark.parametrize("name", ["sphere", "star"])
+@pytest.mark.parametrize("mode", ["exact", "fast"])
+def test_mixed_precision(name, mode):
+    m = MESHES[name]
+    ora, gpu = _pair(m, evalMode=mode, errTol=1e-4, precision="mixed", nearWallFactor=1e-3)
+    xs = m["perturb"](m["pts"])
+    ora.setSurfaceCoordinates(xs)
+    gpu.setSurfaceCoordinates(xs)
+    ref = ora.warpMesh()
+    gpu.warpMesh()
+    err = np.abs(gpu.getWarpGrid() - ref).max() / ora.Ldef0
+    print("mixed max |dXv|/Ldef", err)
+    assert err <= MIXED_TOL
+    dxv = synthetic.random_dxv(len(m["xv0"]), seed=7, frac=0.5)
+    dref = ora.warpDeriv(dxv)
+    gpu.warpDeriv(dxv, solverVec=False)
+    r = _rel(gpu.getdXs().ravel(), dref)
+    print("mixed adjoint rel err", r)
+    assert r <= 1e-6
+
+
+def test_nondefault_options():
+    """Options of the reference's second ONERA M6 instance (test_MultiUSMesh.py:161-172)."""
+    m = MESHES["sphere"]
+    opts = dict(aExp=4.0, bExp=6.0, LdefFact=100.0, alpha=0.3, errTol=1e-5, evalMode="fast", useRotations=False,
+                zeroCornerRotations=False, cornerAngle=45.0, bucketSize=4)
+    ora, gpu = _pair(m, **opts)
+    xs = m["perturb"](m["pts"])
+    ora.setSurfaceCoordinates(xs)
+    gpu.setSurfaceCoordinates(xs)
+    ref = ora.warpMesh()
+    gpu.warpMesh()
+    assert np.abs(gpu.getWarpGrid() - ref).max() <= FP64_TOL * ora.Ldef0
+    dxv = synthetic.random_dxv(len(m["xv0"]), seed=11, frac=1.0)
+    dref = ora.warpDeriv(dxv)
+    gpu.warpDeriv(dxv, solverVec=False)
+    assert _rel(gpu.getdXs().ravel(), dref) <= 1e-10
+
+
+@pytest.mark.parametrize("mode", ["exact", "fast"])
+def test_closed_forms(mode):
+    """Zero perturbation keeps the grid bit-exact; a rigid translation moves
+    every volume node by the same vector."""
+    m = MESHES["star"]
+    gpu = USMesh(options=dict(gridFile=m["xv0"], evalMode=mode, errTol=1e-4))
+    gpu.setSurfaceDefinition(m["pts"], m["conn"], m["faceSizes"])
+    gpu.setSurfaceCoordinates(m["pts"])
+    gpu.warpMesh()
+    assert np.array_equal(gpu.getWarpGrid(), m["xv0"].ravel())
+    t = np.array([0.013, -0.02, 0.031])
+    gpu.setSurfaceCoordinates(m["pts"] + t)
+    gpu.warpMesh()
+    d = gpu.getWarpGrid().reshape(-1, 3) - m["xv0"]
+    assert np.abs(d - t).max() <= 1e-14
+
+
+def test_solver_grid_and_errors():
+    m = MESHES["sphere"]
+    gpu = USMesh(options=dict(gridFile=m["xv0"], evalMode="fast"))
+    gpu.setSurfaceDefinition(m["pts"], m["conn"], m["faceSizes"])
+    nvol = len(m["xv0"])
+    rng = np.random.default_rng(0)
+    ind = rng.integers(0, 3 * nvol, size=3 * nvol + 17).astype(np.int32)  # duplicates included
+    gpu.setExternalMeshIndices(ind)
+    gpu.setSurfaceCoordinates(m["perturb"](m["pts"]))
+    gpu.warpMesh()
+    warp = gpu.getWarpGrid()
+    np.testing.assert_array_equal(gpu.getSolverGrid(), warp[ind])
+    dsol = rng.standard_normal(len(ind))
+    gpu.warpDeriv(dsol, solverVec=True)
+    a = gpu.getdXs().copy()
+    dwarp = np.zeros(3 * nvol)
+    np.add.at(dwarp, ind, dsol)
+    gpu.warpDeriv(dwarp, solverVec=False)
+    # the fast reverse sums cluster/point adjoints with FP64 atomics, so two
+    # runs agree to rounding (in norm), not bit for bit
+    assert _rel(gpu.getdXs().ravel(), a.ravel()) <= 1e-10
+    from idwarp_amd import Error
+    with pytest.raises(Error):
+        gpu.setSurfaceCoordinates(m["pts"][:-1])
