@@ -1,0 +1,218 @@
+#!/usr/bin/env python3
+"""Volume-warp benchmark on one MI355X: warpMesh + warpDeriv, fast (kd-tree)
+mode at errTol=1e-4, on a star-shaped body with 80 stretched shells of volume
+nodes (cube-sphere n=205: 19,975,840 volume nodes, 249,698 unique surface
+nodes).  A step is one warpMesh plus one warpDeriv (one adjoint seed, N(0,1)
+on a random 10% of the volume nodes) with every input resident in HBM.
+
+Prints one JSON line: ms per step in mixed precision (FP32 far field, FP64
+near wall and accumulators) as ``value``, the FP64 kernel and the CPU
+oracle (timed on a bounded sample of the same workload, scaled to the full
+mesh) beside it, the mixed-vs-FP64 error, and the roofline of the dominant
+kernel (algorithmic flops from the device work count / its mean HIP-event
+duration over the timed steps).
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector (= FP64 matrix) peak
+
+# algorithmic flops per unit of fast-mode work (FMA = 2, sqrt/div = 1):
+# forward: leaf pair 42, accepted cluster 32, internal visit 10, error check 7
+# reverse: leaf pair 41, accepted cluster 31, internal visit 10, error check 7
+FWD_FLOPS = (10.0, 32.0, 42.0, 7.0)  # per (V, A, L, E)
+REV_FLOPS = (10.0, 31.0, 41.0, 7.0)
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=205, help="cube-sphere nodes per edge")
+    ap.add_argument("--shells", type=int, default=80)
+    ap.add_argument("--errtol", type=float, default=1e-4)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU work of the baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", choices=["both", "fp64", "mixed"], default="both")
+    return ap.parse_args()
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(m, xs, dxv, errtol, target_s):
+    """The oracle (C restatement of the reference loops) on a random sample
+    of the volume nodes, on all host cores; set-up time is added once and the
+    volume loops are scaled by N / sample."""
+    from oracle import oracle as O
+
+    cores = len(os.sched_getaffinity(0))
+    O.set_threads(cores)
+    nvol = len(m["xv0"])
+    rng = np.random.default_rng(7)
+
+    def run(k):
+        idx = np.sort(rng.choice(nvol, size=k, replace=False))
+        om = O.OracleMesh(m["pts"], m["conn"], m["faceSizes"], m["xv0"][idx], evalMode="fast", errTol=errtol)
+        om.setSurfaceCoordinates(xs)
+        om.warpMesh()
+        sw, lw = O.last_times()
+        om.warpDeriv(dxv.reshape(-1, 3)[idx].ravel())
+        sd, ld = O.last_times()
+        return sw + sd, lw + ld
+
+    k = min(nvol, 2000)
+    setup, loop = run(k)
+    if loop > 0 and k < nvol:
+        k = int(min(nvol, max(k, k * target_s / loop)))
+        setup, loop = run(k)
+    ms = 1e3 * (setup + loop * nvol / k)
+    return dict(value=round(ms, 1), unit="ms per warpMesh+warpDeriv (full mesh, extrapolated)", cores=cores,
+                kind="port",
+                sample="%d of %d volume nodes (seed 7), fast mode errTol=%g; volume loops %.2fs scaled x%.0f, "
+                       "surface set-up %.2fs added once" % (k, nvol, errtol, loop, nvol / k, setup))
+
+
+def main():
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (the volume-node sharding is described "
+              "in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from idwarp_amd import USMesh, _lib, synthetic
+
+    t0 = time.time()
+    m = synthetic.star_shells(n=args.n, nshell=args.shells)
+    xs = m["perturb"](m["pts"])
+    nvol, npts = len(m["xv0"]), len(m["pts"])
+    dxv = synthetic.random_dxv(nvol, seed=5, frac=0.1)
+    log("mesh %s: %d volume nodes, %d surface points (%.1fs)" % (m["name"], nvol, npts, time.time() - t0))
+
+    t0 = time.time()
+    mesh = USMesh(options=dict(gridFile=m["xv0"], evalMode="fast", errTol=args.errtol, precision="fp64"))
+    mesh.setSurfaceDefinition(m["pts"], m["conn"], m["faceSizes"])
+    ldef = mesh.getLdef0()
+    log("init (tree, error tables, denominator estimate): %.1fs" % (time.time() - t0))
+    counts = mesh.getFastCounts()  # V, A, L, E over all volume nodes
+    fwd_flops = float(np.dot(FWD_FLOPS, counts))
+    rev_flops = float(np.dot(REV_FLOPS, counts))
+    log("work: visits %.3g accepts %.3g leaf pairs %.3g checks %.3g" % tuple(counts))
+
+    dev = torch.device("cuda:0")
+    d_xs = torch.tensor(xs.ravel(), dtype=torch.float64, device=dev)
+    d_dxv = torch.tensor(dxv, dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    L, ctx = mesh._L, mesh.ctx
+
+    def vp(t):
+        return ctypes.c_void_p(t.data_ptr())
+
+    out = {}
+    precs = ["fp64", "mixed"] if args.only == "both" else [args.only]
+    for prec in precs:
+        mesh.setOption("precision", prec)
+        mesh._setMeshOptions()
+        d_xv = torch.empty(3 * nvol, dtype=torch.float64, device=dev)
+        d_dxs = torch.empty(3 * npts, dtype=torch.float64, device=dev)
+        sp = ctypes.c_void_p(stream.cuda_stream)
+
+        def step():
+            _lib.check(L.idw_warp_mesh_dev(ctx, vp(d_xs), vp(d_xv), sp))
+            _lib.check(L.idw_warp_deriv_dev(ctx, vp(d_xs), vp(d_dxv), vp(d_dxs), sp))
+
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        phases = np.zeros(6)
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+            phases += mesh.getTimings()  # HIP events on the context stream (synchronises it)
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t1) * 1e3 / max(args.steps, 1)
+        phases /= max(args.steps, 1)
+        out[prec] = dict(ms=wall, phases=phases, xv=d_xv, dxs=d_dxs)
+        log("%s: %.3f ms/step  [nodal %.3f | fwd %.3f | rev setup %.3f | rev %.3f | nodal rev %.3f]" %
+            ((prec, wall) + tuple(phases[:5])))
+
+    head = out.get("mixed", out.get("fp64"))
+    head_prec = "mixed" if "mixed" in out else "fp64"
+    res = {
+        "metric": "warp+warpDeriv ms at 20M nodes: mixed vs FP64 GPU vs CPU, err<=1e-6*Ldef",
+        "value": round(head["ms"], 3),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["ms"], 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if head_prec == "mixed" else "f64",
+        "data": "synthetic",
+        "config": {"workload": "star-shaped body, cube-sphere n=%d, %d shells: %d volume nodes, %d unique surface "
+                               "nodes; fast mode errTol=%g; warpMesh+warpDeriv, device-resident" %
+                               (args.n, args.shells, nvol, _unique(mesh), args.errtol),
+                   "precision": head_prec, "n_vol": nvol, "n_surface_points": npts},
+    }
+    if "fp64" in out:
+        res["fp64_ms_per_step"] = round(out["fp64"]["ms"], 3)
+    if "fp64" in out and "mixed" in out:
+        a, b = out["mixed"], out["fp64"]
+        res["mixed_speedup_vs_fp64"] = round(b["ms"] / a["ms"], 3)
+        res["mixed_max_err_over_Ldef"] = float((a["xv"] - b["xv"]).abs().max().item() / ldef)
+        res["mixed_adjoint_rel_err"] = float(((a["dxs"] - b["dxs"]).norm() / b["dxs"].norm()).item())
+    # roofline of the dominant volume kernel of the headline precision
+    ph = head["phases"]
+    if ph[3] >= ph[1]:
+        kname, kms, kflops = "k_fast_rev<%s>" % head_prec, ph[3], rev_flops
+    else:
+        kname, kms, kflops = "k_fast_fwd<%s>" % head_prec, ph[1], fwd_flops
+    achieved = kflops / (kms * 1e-3) / 1e12
+    res["roofline"] = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                       "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": _traffic(kname),
+                       "kernel": kname, "kernel_ms": round(float(kms), 3),
+                       "algorithmic_flops_per_launch": kflops,
+                       "compute": "VALU FP64/FP32 (no MFMA): the 78.6 TFLOP/s FP64 vector roof equals the FP64 "
+                                  "matrix roof and non-packed FP32 issues at the same rate"}
+    res["phases_ms"] = {p: [round(float(x), 3) for x in out[p]["phases"][:5]] for p in out}
+    if not args.no_cpu:
+        t0 = time.time()
+        res["cpu_baseline"] = cpu_baseline(m, xs, dxv, args.errtol, args.cpu_seconds)
+        res["cpu_baseline"]["speedup_gpu_vs_cpu"] = round(res["cpu_baseline"]["value"] / head["ms"], 1)
+        log("cpu baseline: %s (%.1fs)" % (res["cpu_baseline"], time.time() - t0))
+    print(json.dumps(res), flush=True)
+
+
+def _unique(mesh):
+    n = ctypes.c_int64(0)
+    mesh._L.idw_get_sizes(mesh.ctx, None, None, ctypes.byref(n), None)
+    return n.value
+
+
+def _traffic(kname):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE +
+    WRITE_SIZE, scaled by a copy-kernel calibration), if recorded."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get(kname)
+    except (OSError, ValueError):
+        return None
+
+
+if __name__ == "__main__":
+    main()

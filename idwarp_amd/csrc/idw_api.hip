@@ -70,8 +70,9 @@ struct idw_ctx {
     int nchunk = 0;
     std::vector<void *> allocs;
     std::vector<int32_t> solverInd;
-    hipEvent_t ev[6] = {};
-    double ms[4] = {0, 0, 0, 0};
+    hipEvent_t ev[7] = {};
+    double ms[6] = {0, 0, 0, 0, 0, 0};
+    bool fwdTimed = false, revTimed = false;
 };
 
 namespace {
@@ -161,6 +162,7 @@ int run_warp(idw_ctx *c, const double *xs, double *xv) {
     HIPCHK(hipEventRecord(c->ev[2], c->st));
     c->denOpt = c->opt;
     c->denValid = true;
+    c->fwdTimed = true;
     return IDW_OK;
 }
 
@@ -184,30 +186,31 @@ int run_deriv(idw_ctx *c, const double *xs, const double *dxv, double *dxs) {
     if (!fast) TRY(ensure_partial(c));
     HIPCHK(hipEventRecord(c->ev[3], c->st));
     TRY(surface_pass(c, xs, false));
-    if (c->V.n) {
-        idw::launch_prep_numb(c->st, c->V, dxv);
-        idw::launch_reverse(c->st, c->S, c->V, make_params(c, false), fast, c->opt.precision, c->partial, c->nchunk);
-    } else {
-        idw::launch_fill(c->st, c->S.padj, (int64_t)c->S.n * idw::ADJ_STRIDE, 0.0);
-        idw::launch_fill(c->st, c->S.nadj, (int64_t)c->S.nnodes * idw::ADJ_STRIDE, 0.0);
-    }
+    idw::launch_reverse_prep(c->st, c->S, fast || c->V.n == 0);
+    if (c->V.n) idw::launch_prep_numb(c->st, c->V, dxv);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c->ev[4], c->st));
-    idw::launch_nodal_reverse(c->st, c->S, xs, fast, c->opt.useRotations, dxs, c->npts);
+    if (c->V.n)
+        idw::launch_reverse(c->st, c->S, c->V, make_params(c, false), fast, c->opt.precision, c->partial, c->nchunk);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c->ev[5], c->st));
+    idw::launch_nodal_reverse(c->st, c->S, xs, fast, c->opt.useRotations, dxs, c->npts);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c->ev[6], c->st));
+    c->revTimed = true;
     return IDW_OK;
 }
 
-void collect_timings(idw_ctx *c, bool fwd, bool rev) {
+void collect_timings(idw_ctx *c) {
     float t;
-    if (fwd) {
+    if (c->fwdTimed) {
         if (hipEventElapsedTime(&t, c->ev[0], c->ev[1]) == hipSuccess) c->ms[0] = t;
         if (hipEventElapsedTime(&t, c->ev[1], c->ev[2]) == hipSuccess) c->ms[1] = t;
     }
-    if (rev) {
+    if (c->revTimed) {
         if (hipEventElapsedTime(&t, c->ev[3], c->ev[4]) == hipSuccess) c->ms[2] = t;
         if (hipEventElapsedTime(&t, c->ev[4], c->ev[5]) == hipSuccess) c->ms[3] = t;
+        if (hipEventElapsedTime(&t, c->ev[5], c->ev[6]) == hipSuccess) c->ms[4] = t;
     }
 }
 
@@ -272,7 +275,7 @@ int idw_create(idw_ctx **out, int device, const idw_options *opt) {
     c->opt = *opt;
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
-    for (int k = 0; k < 6 && e == hipSuccess; ++k) e = hipEventCreate(&c->ev[k]);
+    for (int k = 0; k < 7 && e == hipSuccess; ++k) e = hipEventCreate(&c->ev[k]);
     if (e != hipSuccess) {
         idw_destroy(c);
         return fail(IDW_EHIP, "stream/event creation: %s", hipGetErrorString(e));
@@ -484,7 +487,7 @@ int idw_warp_mesh(idw_ctx *c) {
     TRY(ensure_device(c));
     TRY(run_warp(c, c->xs, c->xv));
     HIPCHK(hipStreamSynchronize(c->st));
-    collect_timings(c, true, false);
+    collect_timings(c);
     return IDW_OK;
 }
 
@@ -538,7 +541,7 @@ int idw_warp_deriv(idw_ctx *c, const double *dxv, int64_t ndof) {
     if (ndof) HIPCHK(hipMemcpyAsync(c->dxv, dxv, sizeof(double) * ndof, hipMemcpyHostToDevice, c->st));
     TRY(run_deriv(c, c->xs, c->dxv, c->dxs));
     HIPCHK(hipStreamSynchronize(c->st));
-    collect_timings(c, false, true);
+    collect_timings(c);
     c->haveDxs = true;
     return IDW_OK;
 }
@@ -620,9 +623,29 @@ int idw_get_denominator0(idw_ctx *c, double *den0, int64_t n) {
     return IDW_OK;
 }
 
-int idw_get_timings(idw_ctx *c, double *ms4) {
-    if (!c || !ms4) return fail(IDW_EINVAL, "null argument");
-    for (int k = 0; k < 4; ++k) ms4[k] = c->ms[k];
+int idw_get_timings(idw_ctx *c, double *ms6) {
+    if (!c || !ms6) return fail(IDW_EINVAL, "null argument");
+    TRY(ensure_device(c));
+    HIPCHK(hipStreamSynchronize(c->st));
+    collect_timings(c);
+    for (int k = 0; k < 6; ++k) ms6[k] = c->ms[k];
+    return IDW_OK;
+}
+
+int idw_fast_counts(idw_ctx *c, double *counts4) {
+    if (!c || !counts4) return fail(IDW_EINVAL, "null argument");
+    if (!c->volSet || !c->surfInit) return fail(IDW_ESTATE, "not initialized");
+    TRY(ensure_device(c));
+    unsigned long long *d = nullptr;
+    TRY(dalloc(c, &d, 4));
+    HIPCHK(hipMemsetAsync(d, 0, 4 * sizeof(unsigned long long), c->st));
+    if (c->V.n) idw::launch_fast_count(c->st, c->S, c->V, c->opt.errTol, d);
+    unsigned long long h[4] = {0, 0, 0, 0};
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    dfree(c, d);
+    for (int k = 0; k < 4; ++k) counts4[k] = (double)h[k];
     return IDW_OK;
 }
 

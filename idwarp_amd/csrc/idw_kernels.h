@@ -62,6 +62,9 @@ void launch_forward(hipStream_t st, const DevSurface &S, const DevVolume &V, con
 void launch_prep_numb(hipStream_t st, const DevVolume &V, const double *dxv);
 void launch_reverse(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, int fast, int prec,
                     double *partial, int nchunk);
+void launch_reverse_prep(hipStream_t st, const DevSurface &S, int fast);
+void launch_fast_count(hipStream_t st, const DevSurface &S, const DevVolume &V, double errTol,
+                       unsigned long long *out);
 void launch_nodal_reverse(hipStream_t st, const DevSurface &S, const double *xs, int fast, int useRot, double *dxs,
                           int64_t npts);
 // Sort keys/ids pairs (Morton order); temp storage allocated inside.

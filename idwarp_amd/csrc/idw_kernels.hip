@@ -695,6 +695,55 @@ __global__ void __launch_bounds__(256) k_fast_rev(int nvol, const double4 *__res
     }
 }
 
+// Per-lane work of the fast walk, summed over all volume nodes: internal
+// node visits V, accepted clusters A, leaf point pairs L and error checks E
+// (the quantities of dryRun's cost model, kd_tree.F90:789-830).
+__global__ void __launch_bounds__(256) k_fast_count(int nvol, const double4 *__restrict__ vol, int nnodes,
+                                                    const NodeRec *__restrict__ nodes, double errTol,
+                                                    unsigned long long *__restrict__ out) {
+    __shared__ unsigned long long red[4];
+    if (threadIdx.x < 4) red[threadIdx.x] = 0;
+    __syncthreads();
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    const bool valid = k < nvol;
+    const double4 v = valid ? vol[k] : make_double4(0.0, 0.0, 0.0, 1.0);
+    const double tol = errTol * v.w;
+    int stop = valid ? INT32_MAX : -1;
+    unsigned long long cV = 0, cA = 0, cL = 0, cE = 0;
+    int node = 0;
+    while (node < nnodes) {
+        const NodeRec *r = nodes + node;
+        const int depth = uniform(r->depth);
+        const bool active = depth <= stop;
+        if (uniform(r->leaf)) {
+            if (active) cL += r->n;
+            node = uniform(r->skip);
+            continue;
+        }
+        const double cx = v.x - r->X[0], cy = v.y - r->X[1], cz = v.z - r->X[2];
+        const double dist = sqrt(cx * cx + cy * cy + cz * cz);
+        bool descend = false;
+        if (active) {
+            ++cV;
+            const double dOvrR = dist * r->orad;
+            if (dOvrR < 2.0) descend = true;
+            else {
+                ++cE;
+                if (get_error(r, dOvrR) < tol) ++cA;
+                else descend = true;
+            }
+            stop = descend ? INT32_MAX : depth;
+        }
+        node = __any(descend) ? node + 1 : uniform(r->skip);
+    }
+    atomicAdd(&red[0], cV);
+    atomicAdd(&red[1], cA);
+    atomicAdd(&red[2], cL);
+    atomicAdd(&red[3], cE);
+    __syncthreads();
+    if (threadIdx.x < 4) atomicAdd(out + threadIdx.x, red[threadIdx.x]);
+}
+
 // ---------------------------------------------------------------------------
 // Nodal reverse (setData_b + COMPUTENODALPROPERTIES_B, kd_tree.F90:1033-1162)
 // ---------------------------------------------------------------------------
@@ -1076,8 +1125,6 @@ void launch_reverse(hipStream_t st, const DevSurface &S, const DevVolume &V, con
             k_exact_rev<IDW_PREC_MIXED><<<g, 256, 0, st>>>((int)V.n, chunk, V.vol, V.numb, S.n, S.ps, P, partial);
         k_exact_rev_reduce<<<grid_for((int64_t)S.n * 12, 256), 256, 0, st>>>(S.n, nchunk, partial, S.padj);
     } else {
-        launch_fill(st, S.padj, (int64_t)S.n * ADJ_STRIDE, 0.0);
-        launch_fill(st, S.nadj, (int64_t)S.nnodes * ADJ_STRIDE, 0.0);
         const unsigned g = xcd_grid(V.n);
         if (prec == IDW_PREC_FP64)
             k_fast_rev<IDW_PREC_FP64><<<g, 256, 0, st>>>((int)V.n, V.vol, V.numb, S.nnodes, S.nodes, S.ps, P, S.nadj, S.padj);
@@ -1085,6 +1132,18 @@ void launch_reverse(hipStream_t st, const DevSurface &S, const DevVolume &V, con
             k_fast_rev<IDW_PREC_MIXED><<<g, 256, 0, st>>>((int)V.n, V.vol, V.numb, S.nnodes, S.nodes, S.ps, P, S.nadj,
                                                          S.padj);
     }
+}
+
+void launch_reverse_prep(hipStream_t st, const DevSurface &S, int fast) {
+    if (fast) {
+        launch_fill(st, S.padj, (int64_t)S.n * ADJ_STRIDE, 0.0);
+        launch_fill(st, S.nadj, (int64_t)S.nnodes * ADJ_STRIDE, 0.0);
+    }
+}
+
+void launch_fast_count(hipStream_t st, const DevSurface &S, const DevVolume &V, double errTol,
+                       unsigned long long *out) {
+    k_fast_count<<<grid_for(V.n, 256), 256, 0, st>>>((int)V.n, V.vol, S.nnodes, S.nodes, errTol, out);
 }
 
 void launch_nodal_reverse(hipStream_t st, const DevSurface &S, const double *xs, int fast, int useRot, double *dxs,

@@ -274,10 +274,18 @@ class USMesh:
         return d
 
     def getTimings(self):
-        """ms of the last warp/deriv: nodal+cluster, forward sum, reverse sum, nodal reverse."""
-        t = np.zeros(4)
+        """ms of the last warp/deriv phases: [nodal+cluster, forward kernel,
+        reverse set-up, reverse kernel, nodal reverse, unused]."""
+        t = np.zeros(6)
         _lib.check(self._L.idw_get_timings(self._ctx, _lib.ptr(t)))
         return t
+
+    def getFastCounts(self):
+        """Fast-mode work over all volume nodes: visits, accepts, leaf pairs, error checks."""
+        c = np.zeros(4)
+        self._setMeshOptions()
+        _lib.check(self._L.idw_fast_counts(self._ctx, _lib.ptr(c)))
+        return c
 
     @property
     def ctx(self):

@@ -141,11 +141,20 @@ int idw_get_ldef0(idw_ctx *ctx, double *ldef0);
  * 3 -> 1 per volume node, warp order; n = n_vol. */
 int idw_get_denominator0(idw_ctx *ctx, double *den0, int64_t n);
 
-/* Per-kernel timing of the last warp / warp_deriv, measured with HIP events
- * on the context's stream (milliseconds): [0] nodal properties + cluster
- * data, [1] volume sum (forward), [2] volume sum (reverse), [3] nodal
- * reverse. */
-int idw_get_timings(idw_ctx *ctx, double *ms4);
+/* Per-phase timing of the last warp / warp_deriv (host or _dev form),
+ * measured with HIP events on the context's stream, in milliseconds:
+ * [0] nodal properties + cluster data, [1] forward volume kernel,
+ * [2] reverse set-up (nodal properties, seed, zero fill), [3] reverse
+ * volume kernel(s), [4] nodal reverse, [5] unused.  Synchronises the
+ * context's stream.  No counterpart in the reference (which has no timers). */
+int idw_get_timings(idw_ctx *ctx, double *ms6);
+
+/* Work of the fast mode summed over all volume nodes for the current
+ * options: [0] internal-node visits, [1] accepted clusters, [2] leaf point
+ * pairs, [3] error-table checks (the terms of dryRun's cost model,
+ * kd_tree.F90:789-830).  Used to express kernel times as algorithmic
+ * flop rates. */
+int idw_fast_counts(idw_ctx *ctx, double *counts4);
 
 #ifdef __cplusplus
 }

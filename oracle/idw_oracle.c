@@ -458,8 +458,9 @@ static inline double wfun(const octx *t, double ldefodist) {
     return pow(ldefodist, t->aExp) + t->alphaToBexp * pow(ldefodist, t->bExp);
 }
 
-/* computeErrors (kd_tree.F90:832-880) */
-static void compute_errors(octx *t, int id) {
+/* computeErrors (kd_tree.F90:832-880) for one internal node; the
+ * reference recurses over the tree, each node's table is independent. */
+static void compute_errors_node(octx *t, int id) {
     onode *np = &t->nodes[id];
     if (np->dnum == 0) return;
     double vpts[60];
@@ -485,8 +486,13 @@ static void compute_errors(octx *t, int id) {
         }
         np->err[j] = sqrt(np->err[j] / 20);
     }
-    compute_errors(t, np->left);
-    compute_errors(t, t->nodes[id].right);
+}
+
+static int g_threads = 1;
+
+static void compute_errors(octx *t) {
+#pragma omp parallel for schedule(dynamic, 1) num_threads(g_threads)
+    for (int id = 0; id < t->nnodes; id++) compute_errors_node(t, id);
 }
 
 /* ------------------------------------------------------------------ */
@@ -801,7 +807,6 @@ static void compute_nodal_properties_b(octx *t, const double *xs, double *xsb) {
 /* ------------------------------------------------------------------ */
 /* Public entry points                                                 */
 /* ------------------------------------------------------------------ */
-static int g_threads = 1;
 void orc_set_threads(int n) { g_threads = n > 0 ? n : 1; }
 
 /* _setMeshOptions (UnstructuredMesh.py:1051-1069): only gridInput changes;
@@ -891,7 +896,7 @@ void *orc_create(const orc_options *opt, const double *pts, int npts, const doub
     t->bExp = opt->bExp;
     t->errTol = opt->errTol;
     for (int i = 0; i < NERR - 1; i++) t->orstar[i] = 1.0 / (RSTAR[i + 1] - RSTAR[i]);
-    compute_errors(t, 0);
+    compute_errors(t);
     return t;
 }
 
@@ -914,10 +919,25 @@ static void warp_setup(octx *t, const double *xs) {
     set_data(t, 0);
 }
 
+/* Wall seconds of the last orc_warp / orc_warp_deriv: [0] surface set-up
+ * (nodal properties, setData, and for the reverse setData_b + nodal
+ * reverse), [1] the loop over volume nodes. */
+static double g_times[2];
+void orc_last_times(double *out) { out[0] = g_times[0]; out[1] = g_times[1]; }
+static double wall(void) {
+#ifdef _OPENMP
+    return omp_get_wtime();
+#else
+    return 0.0;
+#endif
+}
+
 /* warpMesh (warpMesh.F90:38-70): writes xv and the denominator. */
 void orc_warp(void *h, const double *xs, const double *xv0, long nvol, const double *den0, double *xv, double *den) {
     octx *t = h;
+    double t0 = wall();
     warp_setup(t, xs);
+    double t1 = wall();
     int fast = t->opt.evalMode == 1;
 #pragma omp parallel for schedule(dynamic, 256) num_threads(g_threads)
     for (long j = 0; j < nvol; j++) {
@@ -929,6 +949,8 @@ void orc_warp(void *h, const double *xs, const double *xv0, long nvol, const dou
         for (int k = 0; k < 3; k++) xv[3 * j + k] = r[k] + num[k] * oden;
         den[j] = d;
     }
+    g_times[0] = t1 - t0;
+    g_times[1] = wall() - t1;
 }
 
 /* Per-node work counts for the fast mode: V internal visits, A accepted
@@ -950,7 +972,9 @@ void orc_counts(void *h, const double *xs, const double *xv0, long nvol, const d
 void orc_warp_deriv(void *h, const double *xs, const double *xv0, long nvol, const double *den0, const double *den,
                     const double *dxv, double *dxs) {
     octx *t = h;
+    double t0 = wall();
     warp_setup(t, xs);
+    double t1 = wall();
     int n = t->n, nn = t->nnodes, fast = t->opt.evalMode == 1;
     int nth = g_threads;
     double **pM = malloc(sizeof(double *) * nth), **pB = malloc(sizeof(double *) * nth);
@@ -991,9 +1015,12 @@ void orc_warp_deriv(void *h, const double *xs, const double *xv0, long nvol, con
     }
     for (int k = 0; k < nth; k++) { free(pM[k]); free(pB[k]); free(qM[k]); free(qB[k]); }
     free(pM); free(pB); free(qM); free(qB);
+    double t2 = wall();
     set_data_b(t, 0);
     memset(dxs, 0, sizeof(double) * 3 * t->nlink);
     compute_nodal_properties_b(t, xs, dxs);
+    g_times[0] = (t1 - t0) + (wall() - t2);
+    g_times[1] = t2 - t1;
 }
 
 /* ------------------------------------------------------------------ */

@@ -79,12 +79,20 @@ def lib():
         L.orc_rotation_matrix.argtypes = [dp, dp, dp]
         L.orc_rotation_matrix_b.argtypes = [dp, dp, dp, dp]
         L.orc_sphere_pts.argtypes = [dp, D, dp, dp]
+        L.orc_last_times.argtypes = [dp]
         _LIB = L
     return _LIB
 
 
 def set_threads(n):
     lib().orc_set_threads(int(n))
+
+
+def last_times():
+    """(set-up seconds, volume-loop seconds) of the last warp or warpDeriv."""
+    t = np.zeros(2)
+    lib().orc_last_times(t)
+    return t[0], t[1]
 
 
 def point_reduce(pts, tol=1e-12):
