@@ -106,10 +106,14 @@ def main():
     mesh.setSurfaceDefinition(m["pts"], m["conn"], m["faceSizes"])
     ldef = mesh.getLdef0()
     log("init (tree, error tables, denominator estimate): %.1fs" % (time.time() - t0))
-    counts = mesh.getFastCounts()  # V, A, L, E over all volume nodes
+    allc = mesh.getFastCounts()
+    counts = allc[:4]  # V, A, L, E summed over volume nodes
     fwd_flops = float(np.dot(FWD_FLOPS, counts))
     rev_flops = float(np.dot(REV_FLOPS, counts))
-    log("work: visits %.3g accepts %.3g leaf pairs %.3g checks %.3g" % tuple(counts))
+    util = {"internal_visits": counts[0] / (64 * allc[4]), "accepts": counts[1] / (64 * allc[5]),
+            "bucket_pairs": counts[2] / (64 * allc[7])}
+    log("work: visits %.3g accepts %.3g leaf pairs %.3g checks %.3g; wave-level %.3g %.3g %.3g %.3g" % tuple(allc))
+    log("SIMD lane utilisation of the walk: %s" % {k: round(v, 3) for k, v in util.items()})
 
     dev = torch.device("cuda:0")
     d_xs = torch.tensor(xs.ravel(), dtype=torch.float64, device=dev)
@@ -189,6 +193,7 @@ def main():
                        "compute": "VALU FP64/FP32 (no MFMA): the 78.6 TFLOP/s FP64 vector roof equals the FP64 "
                                   "matrix roof and non-packed FP32 issues at the same rate"}
     res["phases_ms"] = {p: [round(float(x), 3) for x in out[p]["phases"][:5]] for p in out}
+    res["walk_lane_utilisation"] = {k: round(float(v), 3) for k, v in util.items()}
     if not args.no_cpu:
         t0 = time.time()
         res["cpu_baseline"] = cpu_baseline(m, xs, dxv, args.errtol, args.cpu_seconds)

@@ -632,20 +632,20 @@ int idw_get_timings(idw_ctx *c, double *ms6) {
     return IDW_OK;
 }
 
-int idw_fast_counts(idw_ctx *c, double *counts4) {
-    if (!c || !counts4) return fail(IDW_EINVAL, "null argument");
+int idw_fast_counts(idw_ctx *c, double *counts8) {
+    if (!c || !counts8) return fail(IDW_EINVAL, "null argument");
     if (!c->volSet || !c->surfInit) return fail(IDW_ESTATE, "not initialized");
     TRY(ensure_device(c));
     unsigned long long *d = nullptr;
-    TRY(dalloc(c, &d, 4));
-    HIPCHK(hipMemsetAsync(d, 0, 4 * sizeof(unsigned long long), c->st));
+    TRY(dalloc(c, &d, 8));
+    HIPCHK(hipMemsetAsync(d, 0, 8 * sizeof(unsigned long long), c->st));
     if (c->V.n) idw::launch_fast_count(c->st, c->S, c->V, c->opt.errTol, d);
-    unsigned long long h[4] = {0, 0, 0, 0};
+    unsigned long long h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     dfree(c, d);
-    for (int k = 0; k < 4; ++k) counts4[k] = (double)h[k];
+    for (int k = 0; k < 8; ++k) counts8[k] = (double)h[k];
     return IDW_OK;
 }
 

@@ -701,8 +701,8 @@ __global__ void __launch_bounds__(256) k_fast_rev(int nvol, const double4 *__res
 __global__ void __launch_bounds__(256) k_fast_count(int nvol, const double4 *__restrict__ vol, int nnodes,
                                                     const NodeRec *__restrict__ nodes, double errTol,
                                                     unsigned long long *__restrict__ out) {
-    __shared__ unsigned long long red[4];
-    if (threadIdx.x < 4) red[threadIdx.x] = 0;
+    __shared__ unsigned long long red[8];
+    if (threadIdx.x < 8) red[threadIdx.x] = 0;
     __syncthreads();
     const int k = blockIdx.x * 256 + threadIdx.x;
     const bool valid = k < nvol;
@@ -710,6 +710,7 @@ __global__ void __launch_bounds__(256) k_fast_count(int nvol, const double4 *__r
     const double tol = errTol * v.w;
     int stop = valid ? INT32_MAX : -1;
     unsigned long long cV = 0, cA = 0, cL = 0, cE = 0;
+    unsigned long long wV = 0, wA = 0, wL = 0, wP = 0;  // per wave (uniform)
     int node = 0;
     while (node < nnodes) {
         const NodeRec *r = nodes + node;
@@ -717,9 +718,12 @@ __global__ void __launch_bounds__(256) k_fast_count(int nvol, const double4 *__r
         const bool active = depth <= stop;
         if (uniform(r->leaf)) {
             if (active) cL += r->n;
+            ++wL;
+            wP += uniform(r->n);
             node = uniform(r->skip);
             continue;
         }
+        ++wV;
         const double cx = v.x - r->X[0], cy = v.y - r->X[1], cz = v.z - r->X[2];
         const double dist = sqrt(cx * cx + cy * cy + cz * cz);
         bool descend = false;
@@ -734,14 +738,21 @@ __global__ void __launch_bounds__(256) k_fast_count(int nvol, const double4 *__r
             }
             stop = descend ? INT32_MAX : depth;
         }
+        if (__any(active && !descend && dist * r->orad >= 2.0)) ++wA;
         node = __any(descend) ? node + 1 : uniform(r->skip);
     }
     atomicAdd(&red[0], cV);
     atomicAdd(&red[1], cA);
     atomicAdd(&red[2], cL);
     atomicAdd(&red[3], cE);
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&red[4], wV);
+        atomicAdd(&red[5], wA);
+        atomicAdd(&red[6], wL);
+        atomicAdd(&red[7], wP);
+    }
     __syncthreads();
-    if (threadIdx.x < 4) atomicAdd(out + threadIdx.x, red[threadIdx.x]);
+    if (threadIdx.x < 8) atomicAdd(out + threadIdx.x, red[threadIdx.x]);
 }
 
 // ---------------------------------------------------------------------------

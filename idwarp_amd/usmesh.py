@@ -281,8 +281,10 @@ class USMesh:
         return t
 
     def getFastCounts(self):
-        """Fast-mode work over all volume nodes: visits, accepts, leaf pairs, error checks."""
-        c = np.zeros(4)
+        """Fast-mode work: per-node sums of visits, accepts, leaf pairs, error
+        checks, then per-wave sums of visits, accepting visits, bucket
+        visits, bucket point evaluations."""
+        c = np.zeros(8)
         self._setMeshOptions()
         _lib.check(self._L.idw_fast_counts(self._ctx, _lib.ptr(c)))
         return c

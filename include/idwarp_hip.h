@@ -149,12 +149,15 @@ int idw_get_denominator0(idw_ctx *ctx, double *den0, int64_t n);
  * context's stream.  No counterpart in the reference (which has no timers). */
 int idw_get_timings(idw_ctx *ctx, double *ms6);
 
-/* Work of the fast mode summed over all volume nodes for the current
- * options: [0] internal-node visits, [1] accepted clusters, [2] leaf point
- * pairs, [3] error-table checks (the terms of dryRun's cost model,
- * kd_tree.F90:789-830).  Used to express kernel times as algorithmic
- * flop rates. */
-int idw_fast_counts(idw_ctx *ctx, double *counts4);
+/* Work of the fast mode for the current options.  Per volume node, summed
+ * over all nodes: [0] internal-node visits, [1] accepted clusters, [2] leaf
+ * point pairs, [3] error-table checks (the terms of dryRun's cost model,
+ * kd_tree.F90:789-830).  Per 64-node wave of the device walk, summed over
+ * waves: [4] internal-node visits, [5] visits with an accepting lane,
+ * [6] bucket visits, [7] bucket point evaluations.  [0..3] give the
+ * algorithmic flops; [0..3] / (64 [4..7]) the SIMD lane utilisation.
+ * No counterpart in the reference. */
+int idw_fast_counts(idw_ctx *ctx, double *counts8);
 
 #ifdef __cplusplus
 }

@@ -122,19 +122,20 @@ void orc_rotation_matrix(const double *v1, const double *v2, double *M) {
 static void getmag_b(const double *v, double *vb, double magb) {
     double s = v[0] * v[0] + v[1] * v[1] + v[2] * v[2] + 1e-30;
     double tempb = (s == 0.0) ? 0.0 : magb / (2.0 * sqrt(s));
-    vb[0] += 2 * v[0] * tempb;
-    vb[1] += 2 * v[1] * tempb;
-    vb[2] += 2 * v[2] * tempb;
+    vb[0] = vb[0] + 2 * v[0] * tempb;
+    vb[1] = vb[1] + 2 * v[1] * tempb;
+    vb[2] = vb[2] + 2 * v[2] * tempb;
 }
 
 /* CROSS_PRODUCT_3D_B (vectorUtils_b.f90:155-180) */
 static void cross3_b(const double *v1, double *v1b, const double *v2, double *v2b, double *cb) {
-    v1b[0] += v2[1] * cb[2] - v2[2] * cb[1];
-    v2b[1] += v1[0] * cb[2] - v1[2] * cb[0];
-    v1b[1] += v2[2] * cb[0] - v2[0] * cb[2];
-    v2b[0] += v1[2] * cb[1] - v1[1] * cb[2];
-    v1b[2] += v2[0] * cb[1] - v2[1] * cb[0];
-    v2b[2] += v1[1] * cb[0] - v1[0] * cb[1];
+    /* (a + b) - c, as the Fortran statements associate */
+    v1b[0] = v1b[0] + v2[1] * cb[2] - v2[2] * cb[1];
+    v2b[1] = v2b[1] + v1[0] * cb[2] - v1[2] * cb[0];
+    v1b[1] = v1b[1] + v2[2] * cb[0] - v2[0] * cb[2];
+    v2b[0] = v2b[0] + v1[2] * cb[1] - v1[1] * cb[2];
+    v1b[2] = v1b[2] + v2[0] * cb[1] - v2[1] * cb[0];
+    v2b[2] = v2b[2] + v1[1] * cb[0] - v1[0] * cb[1];
     cb[0] = cb[1] = cb[2] = 0.0;
 }
 
