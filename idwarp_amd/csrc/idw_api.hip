@@ -64,6 +64,8 @@ struct idw_ctx {
     idw::HostSurface hs;
     idw::DevSurface S;
     idw::DevVolume V;
+    idw::DevPlan plan;
+    bool planValid = false;  // plan written by the last forward at denOpt
     int64_t npts = 0;
     double *xv0w = nullptr, *xv = nullptr, *xs = nullptr, *dxs = nullptr, *dxv = nullptr;
     double *partial = nullptr;
@@ -141,6 +143,62 @@ int compute_den0(idw_ctx *c) {
     return IDW_OK;
 }
 
+void drop_plan(idw_ctx *c) {
+    dfree(c, c->plan.entries);
+    dfree(c, c->plan.waveOff);
+    dfree(c, c->plan.waveCount);
+    dfree(c, c->plan.mismatch);
+    c->plan = idw::DevPlan();
+    c->planValid = false;
+}
+
+// Size the per-wave plan for the current errTol: one counting walk and an
+// exclusive scan, redone only when errTol, the volume nodes or the surface
+// change (like the denominator estimate, the counts depend on init-time data
+// only).
+int size_plan(idw_ctx *c) {
+    idw::DevPlan &P = c->plan;
+    if (P.sized && P.errTol == c->opt.errTol) return IDW_OK;
+    const int64_t nwave = ((c->V.n + 255) / 256) * 4;
+    if (!P.waveOff || P.nwave != nwave) {
+        dfree(c, P.waveOff);
+        dfree(c, P.waveCount);
+        if (!P.mismatch) TRY(dalloc(c, &P.mismatch, 1));
+        TRY(dalloc(c, &P.waveOff, nwave + 1));
+        TRY(dalloc(c, &P.waveCount, nwave + 1));
+        P.nwave = nwave;
+    }
+    HIPCHK(hipMemsetAsync(P.waveCount, 0, sizeof(int64_t) * (nwave + 1), c->st));
+    if (c->V.n) idw::launch_plan_count(c->st, c->S, c->V, c->opt.errTol, P.waveCount);
+    HIPCHK(hipGetLastError());
+    HIPCHK(idw::exclusive_scan(c->st, P.waveCount, P.waveOff, nwave + 1));
+    int64_t total = 0;
+    HIPCHK(hipMemcpyAsync(&total, P.waveOff + nwave, sizeof(int64_t), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (total > P.capacity) {
+        dfree(c, P.entries);
+        TRY(dalloc(c, &P.entries, (size_t)total));
+        P.capacity = total;
+    }
+    P.total = total;
+    P.errTol = c->opt.errTol;
+    P.sized = true;
+    return IDW_OK;
+}
+
+int check_plan(idw_ctx *c) {
+    if (!c->plan.mismatch) return IDW_OK;
+    unsigned int mm = 0;
+    HIPCHK(hipMemcpyAsync(&mm, c->plan.mismatch, sizeof(mm), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (mm) {
+        c->planValid = false;
+        c->plan.sized = false;
+        return fail(IDW_EHIP, "fast-mode plan of %u waves disagreed with its counting pass", mm);
+    }
+    return IDW_OK;
+}
+
 // Surface nodal properties and, in fast mode, the cluster data.
 int surface_pass(idw_ctx *c, const double *xs, bool clusters) {
     idw::launch_face_props(c->st, c->S, xs);
@@ -154,14 +212,21 @@ int run_warp(idw_ctx *c, const double *xs, double *xv) {
     if (!c->surfInit) return fail(IDW_ESTATE, "warp before initialize_warping");
     if (!c->volSet) return fail(IDW_ESTATE, "no volume nodes set");
     const int fast = c->opt.evalMode == IDW_EVAL_FAST;
+    if (fast) {
+        TRY(size_plan(c));
+        HIPCHK(hipMemsetAsync(c->plan.mismatch, 0, sizeof(unsigned int), c->st));
+    }
     HIPCHK(hipEventRecord(c->ev[0], c->st));
     TRY(surface_pass(c, xs, fast));
     HIPCHK(hipEventRecord(c->ev[1], c->st));
-    if (c->V.n) idw::launch_forward(c->st, c->S, c->V, make_params(c, false), fast, c->opt.precision, xv);
+    if (c->V.n)
+        idw::launch_forward(c->st, c->S, c->V, make_params(c, false), fast, c->opt.precision, xv,
+                            fast ? &c->plan : nullptr);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c->ev[2], c->st));
     c->denOpt = c->opt;
     c->denValid = true;
+    c->planValid = fast;
     c->fwdTimed = true;
     return IDW_OK;
 }
@@ -181,17 +246,18 @@ int run_deriv(idw_ctx *c, const double *xs, const double *dxv, double *dxs) {
     if (!c->surfInit) return fail(IDW_ESTATE, "warp_deriv before initialize_warping");
     if (!c->volSet) return fail(IDW_ESTATE, "no volume nodes set");
     const int fast = c->opt.evalMode == IDW_EVAL_FAST;
-    // The denominator depends on the options, not on the surface coordinates.
-    if (!c->denValid || !same_numeric_options(c->denOpt, c->opt)) TRY(run_warp(c, xs, c->xv));
+    // The denominator (and the plan) depend on the options, not on the surface coordinates.
+    if (!c->denValid || !same_numeric_options(c->denOpt, c->opt) || (fast && !c->planValid))
+        TRY(run_warp(c, xs, c->xv));
     if (!fast) TRY(ensure_partial(c));
     HIPCHK(hipEventRecord(c->ev[3], c->st));
     TRY(surface_pass(c, xs, false));
     idw::launch_reverse_prep(c->st, c->S, fast || c->V.n == 0);
-    if (c->V.n) idw::launch_prep_numb(c->st, c->V, dxv);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c->ev[4], c->st));
     if (c->V.n)
-        idw::launch_reverse(c->st, c->S, c->V, make_params(c, false), fast, c->opt.precision, c->partial, c->nchunk);
+        idw::launch_reverse(c->st, c->S, c->V, make_params(c, false), fast, c->opt.precision, c->partial, c->nchunk,
+                            dxv, &c->plan);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c->ev[5], c->st));
     idw::launch_nodal_reverse(c->st, c->S, xs, fast, c->opt.useRotations, dxs, c->npts);
@@ -231,6 +297,21 @@ hipError_t sort_pairs(hipStream_t st, int64_t n, uint64_t *keys, int32_t *ids, u
     e = hipMalloc(&tmp, bytes);
     if (e != hipSuccess) return e;
     e = hipcub::DeviceRadixSort::SortPairs(tmp, bytes, keys, keys_out, ids, ids_out, (int)n, 0, 63, st);
+    hipError_t e2 = hipStreamSynchronize(st);
+    (void)hipFree(tmp);
+    return e != hipSuccess ? e : e2;
+}
+}  // namespace idw
+
+namespace idw {
+hipError_t exclusive_scan(hipStream_t st, const int64_t *in, int64_t *out, int64_t n) {
+    size_t bytes = 0;
+    hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, (int)n, st);
+    if (e != hipSuccess) return e;
+    void *tmp = nullptr;
+    e = hipMalloc(&tmp, bytes);
+    if (e != hipSuccess) return e;
+    e = hipcub::DeviceScan::ExclusiveSum(tmp, bytes, in, out, (int)n, st);
     hipError_t e2 = hipStreamSynchronize(st);
     (void)hipFree(tmp);
     return e != hipSuccess ? e : e2;
@@ -314,6 +395,7 @@ int idw_set_volume_nodes(idw_ctx *c, const double *xv0, int64_t ndof) {
     dfree(c, c->partial);
     c->volSet = false;
     c->denValid = false;
+    drop_plan(c);
     c->V.n = n;
     TRY(dalloc(c, &c->xv0w, 3 * n));
     TRY(dalloc(c, &c->xv, 3 * n));
@@ -387,6 +469,7 @@ int idw_initialize_warping(idw_ctx *c, const double *pts, int64_t n_pts, const d
     c->xs = c->dxs = c->partial = nullptr;
     c->surfInit = false;
     c->denValid = false;
+    drop_plan(c);
     c->hs = std::move(hs);
     c->initOpt = c->opt;
     c->npts = n_pts;
@@ -488,7 +571,7 @@ int idw_warp_mesh(idw_ctx *c) {
     TRY(run_warp(c, c->xs, c->xv));
     HIPCHK(hipStreamSynchronize(c->st));
     collect_timings(c);
-    return IDW_OK;
+    return check_plan(c);
 }
 
 int idw_get_volume_coordinates(idw_ctx *c, double *xv, int64_t ndof) {
@@ -542,6 +625,7 @@ int idw_warp_deriv(idw_ctx *c, const double *dxv, int64_t ndof) {
     TRY(run_deriv(c, c->xs, c->dxv, c->dxs));
     HIPCHK(hipStreamSynchronize(c->st));
     collect_timings(c);
+    TRY(check_plan(c));
     c->haveDxs = true;
     return IDW_OK;
 }
@@ -629,7 +713,7 @@ int idw_get_timings(idw_ctx *c, double *ms6) {
     HIPCHK(hipStreamSynchronize(c->st));
     collect_timings(c);
     for (int k = 0; k < 6; ++k) ms6[k] = c->ms[k];
-    return IDW_OK;
+    return check_plan(c);
 }
 
 int idw_fast_counts(idw_ctx *c, double *counts8) {

@@ -12,18 +12,28 @@ constexpr int NERR = 12;
 // kd-tree node in depth-first pre-order (parent, left subtree, right subtree)
 // as created by build_tree (kd_tree.F90:140-182).  One record per node; the
 // traversal kernels read it with wave-uniform (scalar) loads.
+// The first 64 bytes (NodeHdr) are what the walk needs at every visit and
+// are fetched with one scalar load.
 struct alignas(16) NodeRec {
     double X[3];      // centroid of the node's points (setCenters, kd_tree.F90:558-593)
     double orad;      // 1 / max distance from X to a point of the node
-    double Ai;        // summed area weights (setAi, kd_tree.F90:536-556)
-    double err[NERR]; // RMS denominator error at rstar radii (computeErrors, :832-880)
     int32_t l, u;     // points [l, u] (0-based, inclusive) in tree order
     int32_t skip;     // index of the next node after this subtree (== node count at the end)
     int32_t depth;    // root = 0
     int32_t leaf;     // 1 for a bucket
     int32_t n;        // u - l + 1
+    double Ai;        // summed area weights (setAi, kd_tree.F90:536-556)
+    double err[NERR]; // RMS denominator error at rstar radii (computeErrors, :832-880)
 };
 static_assert(sizeof(NodeRec) == 160, "NodeRec layout");
+
+struct alignas(16) NodeHdr {
+    double X[3];
+    double orad;
+    int32_t l, u, skip, depth, leaf, n;
+    double Ai;
+};
+static_assert(sizeof(NodeHdr) == 64, "NodeHdr layout");
 
 // Per-call cluster data of an internal node (setData, kd_tree.F90:595-617),
 // stored as Kbar = mean(M_i) - I and e = Kbar X + mean(b_i), so that the

@@ -39,6 +39,25 @@ struct DevSurface {
     double *padj = nullptr, *nadj = nullptr, *ntmp = nullptr, *pb = nullptr;
 };
 
+// One record of the forward walk: a tree node and the lanes of the wave it
+// served (accepted cluster or evaluated bucket).
+struct alignas(16) PlanEntry {
+    unsigned long long mask;
+    int32_t node;
+    int32_t pad;
+};
+
+// Per-wave plan of the fast mode (valid for one volume ordering, tree and errTol).
+struct DevPlan {
+    PlanEntry *entries = nullptr;
+    int64_t *waveOff = nullptr;  // nwave + 1 offsets
+    int64_t *waveCount = nullptr;
+    unsigned int *mismatch = nullptr;
+    int64_t nwave = 0, total = 0, capacity = 0;
+    double errTol = -1.0;
+    bool sized = false;
+};
+
 struct DevVolume {
     int64_t n = 0;
     double4 *vol = nullptr;   // x, y, z, denominator0 in traversal (Morton) order
@@ -58,10 +77,12 @@ void launch_morton(hipStream_t st, int64_t n, const double *xv0, double lo[3], d
 void launch_build_vol(hipStream_t st, int64_t n, const double *xv0, const int32_t *perm, double4 *vol);
 void launch_fill(hipStream_t st, double *a, int64_t n, double v);
 void launch_forward(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, int fast, int prec,
-                    double *xv);
+                    double *xv, const DevPlan *plan);
 void launch_prep_numb(hipStream_t st, const DevVolume &V, const double *dxv);
+void launch_plan_count(hipStream_t st, const DevSurface &S, const DevVolume &V, double errTol, int64_t *waveCount);
 void launch_reverse(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, int fast, int prec,
-                    double *partial, int nchunk);
+                    double *partial, int nchunk, const double *dxv, const DevPlan *plan);
+hipError_t exclusive_scan(hipStream_t st, const int64_t *in, int64_t *out, int64_t n);
 void launch_reverse_prep(hipStream_t st, const DevSurface &S, int fast);
 void launch_fast_count(hipStream_t st, const DevSurface &S, const DevVolume &V, double errTol,
                        unsigned long long *out);

@@ -56,10 +56,12 @@ __device__ __forceinline__ T ipow_u(T q, int e) {
     return r;
 }
 
-// A_i * (q^aExp + alpha^bExp q^bExp) without the A factor.
-template <typename T>
+// A_i * (q^aExp + alpha^bExp q^bExp) without the A factor.  EXPM = EXP_35
+// compiles the default exponents (3, 5) as products; otherwise the mode is
+// read from P at run time.
+template <int EXPM = EXP_INT, typename T>
 __device__ __forceinline__ T wfun(T q, const WParams &P) {
-    if (P.expMode == EXP_35) {
+    if (EXPM == EXP_35 || P.expMode == EXP_35) {
         const T q2 = q * q, q3 = q2 * q, q5 = q3 * q2;
         return q3 + T(P.a5) * q5;
     } else if (P.expMode == EXP_INT) {
@@ -261,7 +263,7 @@ __global__ void __launch_bounds__(256) k_cluster_data(const int32_t *__restrict_
 // ---------------------------------------------------------------------------
 // Exact mode (evalNodeExact on the root, kd_tree.F90:664-690)
 // ---------------------------------------------------------------------------
-template <int PREC>
+template <int PREC, int EXPM>
 __global__ void __launch_bounds__(256) k_exact_fwd(int nvol, const double4 *__restrict__ vol,
                                                    const int32_t *__restrict__ perm, int n,
                                                    const double4 *__restrict__ ps, const PointDyn *__restrict__ pd,
@@ -280,7 +282,7 @@ __global__ void __launch_bounds__(256) k_exact_fwd(int nvol, const double4 *__re
             const PointDyn &q = pd[i];
             const double rx = v.x - s.x, ry = v.y - s.y, rz = v.z - s.z;
             const double lo = P.Ldef / sqrt(rx * rx + ry * ry + rz * rz + EPS_W);
-            const double w = s.w * wfun(lo, P);
+            const double w = s.w * wfun<EXPM>(lo, P);
             n0 += w * (q.K[0] * rx + q.K[1] * ry + q.K[2] * rz + q.d[0]);
             n1 += w * (q.K[3] * rx + q.K[4] * ry + q.K[5] * rz + q.d[1]);
             n2 += w * (q.K[6] * rx + q.K[7] * ry + q.K[8] * rz + q.d[2]);
@@ -302,7 +304,7 @@ __global__ void __launch_bounds__(256) k_exact_fwd(int nvol, const double4 *__re
                 if (__any(valid && d2 < T2)) {
                     const PointDyn &q = pd[i];
                     const double lo = P.Ldef / sqrt(d2 + EPS_W);
-                    const double w = s.w * wfun(lo, P);
+                    const double w = s.w * wfun<EXPM>(lo, P);
                     n0 += w * (q.K[0] * rx + q.K[1] * ry + q.K[2] * rz + q.d[0]);
                     n1 += w * (q.K[3] * rx + q.K[4] * ry + q.K[5] * rz + q.d[1]);
                     n2 += w * (q.K[6] * rx + q.K[7] * ry + q.K[8] * rz + q.d[2]);
@@ -311,7 +313,7 @@ __global__ void __launch_bounds__(256) k_exact_fwd(int nvol, const double4 *__re
                     const PointDyn32 &q = pd32[i];
                     const float fx = (float)rx, fy = (float)ry, fz = (float)rz;
                     const float lo = Lf * __frsqrt_rn(fx * fx + fy * fy + fz * fz);
-                    const float w = (float)s.w * wfun(lo, P);
+                    const float w = (float)s.w * wfun<EXPM>(lo, P);
                     a0 += w * (q.K[0] * fx + q.K[1] * fy + q.K[2] * fz + q.d[0]);
                     a1 += w * (q.K[3] * fx + q.K[4] * fy + q.K[5] * fz + q.d[1]);
                     a2 += w * (q.K[6] * fx + q.K[7] * fy + q.K[8] * fz + q.d[2]);
@@ -345,7 +347,7 @@ __global__ void k_prep_numb(int nvol, const int32_t *__restrict__ perm, const do
 // node, a chunk of volume nodes per block row.  Accumulates
 // Mb'(r,c) = sum s_r (x_c - Xu0_c) (the reverse's Mib - Xu0 (x) Bib of
 // COMPUTENODALPROPERTIES_B, kd_tree.F90:1115-1118) and Bb = sum s.
-template <int PREC>
+template <int PREC, int EXPM>
 __global__ void __launch_bounds__(256) k_exact_rev(int nvol, int chunk, const double4 *__restrict__ vol,
                                                    const double4 *__restrict__ numb, int n,
                                                    const double4 *__restrict__ ps, WParams P,
@@ -364,16 +366,16 @@ __global__ void __launch_bounds__(256) k_exact_rev(int nvol, int chunk, const do
         double w;
         if (PREC == IDW_PREC_FP64) {
             const double lo = P.Ldef / sqrt(rx * rx + ry * ry + rz * rz + EPS_W);
-            w = s.w * wfun(lo, P);
+            w = s.w * wfun<EXPM>(lo, P);
         } else {
             const double d2 = rx * rx + ry * ry + rz * rz;
             if (__any(valid && d2 < P.nearT2)) {
                 const double lo = P.Ldef / sqrt(d2 + EPS_W);
-                w = s.w * wfun(lo, P);
+                w = s.w * wfun<EXPM>(lo, P);
             } else {
                 const float fx = (float)rx, fy = (float)ry, fz = (float)rz;
                 const float lo = (float)P.Ldef * __frsqrt_rn(fx * fx + fy * fy + fz * fz);
-                w = (double)((float)s.w * wfun(lo, P));
+                w = (double)((float)s.w * wfun<EXPM>(lo, P));
             }
         }
         const double s0 = w * nb.x, s1 = w * nb.y, s2 = w * nb.z;
@@ -416,14 +418,92 @@ __device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirs
 
 // Forward walk (evalNode, kd_tree.F90:619-662; leaves evalNodeExact,
 // accepted clusters evalNodeApprox, :692-713).
-template <int PREC>
+// Wave-level record of the walk: every bucket the wave evaluated and every
+// cluster some lane accepted, with the mask of lanes concerned, in depth-first
+// order.  Slots [off, end) of the wave come from the counting pass; writes
+// beyond them are dropped and counted in *mismatch.
+struct PlanWriter {
+    PlanEntry *plan;
+    int64_t at, end;
+    unsigned int *mismatch;
+    __device__ __forceinline__ void put(int node, unsigned long long mask) {
+        if (!plan) return;
+        if (at < end) {
+            if ((threadIdx.x & 63) == 0) {
+                PlanEntry e;
+                e.mask = mask;
+                e.node = node;
+                e.pad = 0;
+                plan[at] = e;
+            }
+        } else if ((threadIdx.x & 63) == 0) {
+            atomicAdd(mismatch, 1u);
+        }
+        ++at;
+    }
+    __device__ __forceinline__ void finish() {
+        if (plan && at != end && (threadIdx.x & 63) == 0) atomicAdd(mismatch, 1u);
+    }
+};
+
+__device__ __forceinline__ NodeHdr load_hdr(const NodeRec *nodes, int node) {
+    return *reinterpret_cast<const NodeHdr *>(nodes + node);
+}
+
+// Bucket contribution (evalNodeExact over the bucket's points) to one lane.
+template <int PREC, int EXPM>
+__device__ __forceinline__ void bucket_fwd(const double4 &v, bool active, int l, int u,
+                                           const double4 *__restrict__ ps, const PointDyn *__restrict__ pd,
+                                           const PointDyn32 *__restrict__ pd32, const WParams &P, double &n0,
+                                           double &n1, double &n2, double &dd, bool fp64) {
+    if (PREC == IDW_PREC_FP64 || fp64) {
+        double4 s = ps[l];
+        for (int i = l; i <= u; ++i) {
+            const double4 sn = ps[min(i + 1, u)];  // next point, in flight during this one
+            const PointDyn &q = pd[i];
+            const double rx = v.x - s.x, ry = v.y - s.y, rz = v.z - s.z;
+            const double lo = P.Ldef / sqrt(rx * rx + ry * ry + rz * rz + EPS_W);
+            const double w = active ? s.w * wfun<EXPM>(lo, P) : 0.0;
+            n0 += w * (q.K[0] * rx + q.K[1] * ry + q.K[2] * rz + q.d[0]);
+            n1 += w * (q.K[3] * rx + q.K[4] * ry + q.K[5] * rz + q.d[1]);
+            n2 += w * (q.K[6] * rx + q.K[7] * ry + q.K[8] * rz + q.d[2]);
+            dd += w;
+            s = sn;
+        }
+    } else {
+        float a0 = 0.f, a1 = 0.f, a2 = 0.f, ad = 0.f;
+        const float Lf = (float)P.Ldef;
+        double4 s = ps[l];
+        for (int i = l; i <= u; ++i) {
+            const double4 sn = ps[min(i + 1, u)];
+            const PointDyn32 &q = pd32[i];
+            const float fx = (float)(v.x - s.x), fy = (float)(v.y - s.y), fz = (float)(v.z - s.z);
+            const float lo = Lf * __frsqrt_rn(fx * fx + fy * fy + fz * fz);
+            const float w = active ? (float)s.w * wfun<EXPM>(lo, P) : 0.f;
+            a0 += w * (q.K[0] * fx + q.K[1] * fy + q.K[2] * fz + q.d[0]);
+            a1 += w * (q.K[3] * fx + q.K[4] * fy + q.K[5] * fz + q.d[1]);
+            a2 += w * (q.K[6] * fx + q.K[7] * fy + q.K[8] * fz + q.d[2]);
+            ad += w;
+            s = sn;
+        }
+        n0 += a0; n1 += a1; n2 += a2; dd += ad;
+    }
+}
+
+// Forward walk (evalNode, kd_tree.F90:619-662; leaves evalNodeExact,
+// accepted clusters evalNodeApprox, :692-713).  The headers of both
+// possible successors (node+1 when a lane descends, skip otherwise) are
+// requested before the current node is evaluated.
+template <int PREC, int EXPM>
 __global__ void __launch_bounds__(256) k_fast_fwd(int nvol, const double4 *__restrict__ vol,
                                                   const int32_t *__restrict__ perm, int nnodes,
                                                   const NodeRec *__restrict__ nodes, const NodeDyn *__restrict__ nd,
                                                   const NodeDyn32 *__restrict__ nd32,
                                                   const double4 *__restrict__ ps, const PointDyn *__restrict__ pd,
                                                   const PointDyn32 *__restrict__ pd32, WParams P,
-                                                  double *__restrict__ xv, double *__restrict__ den) {
+                                                  double *__restrict__ xv, double *__restrict__ den,
+                                                  PlanEntry *__restrict__ plan, const int64_t *__restrict__ waveOff,
+                                                  unsigned int *__restrict__ mismatch) {
     const int nb = (nvol + 255) / 256;
     const int blk = xcd_block(nb);
     if (blk >= nb) return;
@@ -433,95 +513,68 @@ __global__ void __launch_bounds__(256) k_fast_fwd(int nvol, const double4 *__res
     const double tol = P.errTol * v.w;  // errTol * denominator0
     int stop = valid ? INT32_MAX : -1;  // depth at which this lane stopped descending
     double n0 = 0.0, n1 = 0.0, n2 = 0.0, dd = 0.0;
+    const int64_t wave = (int64_t)blk * 4 + (threadIdx.x >> 6);
+    PlanWriter pw{plan, plan ? waveOff[wave] : 0, plan ? waveOff[wave + 1] : 0, mismatch};
     int node = 0;
-    while (node < nnodes) {
-        const NodeRec *r = nodes + node;
-        const int depth = uniform(r->depth);
-        const bool active = depth <= stop;
-        if (uniform(r->leaf)) {
-            const int l = uniform(r->l), u = uniform(r->u);
-            if (PREC == IDW_PREC_FP64) {
-                for (int i = l; i <= u; ++i) {
-                    const double4 s = ps[i];
-                    const PointDyn &q = pd[i];
-                    const double rx = v.x - s.x, ry = v.y - s.y, rz = v.z - s.z;
-                    const double lo = P.Ldef / sqrt(rx * rx + ry * ry + rz * rz + EPS_W);
-                    const double w = active ? s.w * wfun(lo, P) : 0.0;
-                    n0 += w * (q.K[0] * rx + q.K[1] * ry + q.K[2] * rz + q.d[0]);
-                    n1 += w * (q.K[3] * rx + q.K[4] * ry + q.K[5] * rz + q.d[1]);
-                    n2 += w * (q.K[6] * rx + q.K[7] * ry + q.K[8] * rz + q.d[2]);
-                    dd += w;
-                }
-            } else {
+    NodeHdr h = load_hdr(nodes, 0);
+    for (;;) {
+        const NodeHdr hA = load_hdr(nodes, min(node + 1, nnodes - 1));
+        const NodeHdr hB = load_hdr(nodes, min(h.skip, nnodes - 1));
+        const bool active = h.depth <= stop;
+        int next;
+        if (h.leaf) {
+            pw.put(node, __ballot(active));
+            bool fp64 = false;
+            if (PREC == IDW_PREC_MIXED) {
                 // FP32 bucket unless an active lane is within nearWallFactor*Ldef
-                // of the bucket's bounding sphere.
-                const double cx = v.x - r->X[0], cy = v.y - r->X[1], cz = v.z - r->X[2];
-                const double dc = sqrt(cx * cx + cy * cy + cz * cz) - 1.0 / r->orad;
-                if (__any(active && dc * fabs(dc) < P.nearT2)) {
-                    for (int i = l; i <= u; ++i) {
-                        const double4 s = ps[i];
-                        const PointDyn &q = pd[i];
-                        const double rx = v.x - s.x, ry = v.y - s.y, rz = v.z - s.z;
-                        const double lo = P.Ldef / sqrt(rx * rx + ry * ry + rz * rz + EPS_W);
-                        const double w = active ? s.w * wfun(lo, P) : 0.0;
-                        n0 += w * (q.K[0] * rx + q.K[1] * ry + q.K[2] * rz + q.d[0]);
-                        n1 += w * (q.K[3] * rx + q.K[4] * ry + q.K[5] * rz + q.d[1]);
-                        n2 += w * (q.K[6] * rx + q.K[7] * ry + q.K[8] * rz + q.d[2]);
+                // of the bucket's bounding sphere
+                const double cx = v.x - h.X[0], cy = v.y - h.X[1], cz = v.z - h.X[2];
+                const double dc = sqrt(cx * cx + cy * cy + cz * cz) - 1.0 / h.orad;
+                fp64 = __any(active && dc * fabs(dc) < P.nearT2);
+            }
+            bucket_fwd<PREC, EXPM>(v, active, h.l, h.u, ps, pd, pd32, P, n0, n1, n2, dd, fp64);
+            next = h.skip;
+        } else {
+            const double cx = v.x - h.X[0], cy = v.y - h.X[1], cz = v.z - h.X[2];
+            const double dist = sqrt(cx * cx + cy * cy + cz * cz);
+            bool descend = false, accept = false;
+            if (active) {
+                const double dOvrR = dist * h.orad;
+                if (dOvrR < 2.0) descend = true;
+                else if (get_error(nodes + node, dOvrR) < tol) accept = true;
+                else descend = true;
+                stop = descend ? INT32_MAX : h.depth;
+            }
+            const unsigned long long acc = __ballot(accept);
+            if (acc) {
+                pw.put(node, acc);
+                if (accept) {
+                    if (PREC == IDW_PREC_FP64) {
+                        const NodeDyn &q = nd[node];
+                        const double w = h.Ai * wfun<EXPM>(P.Ldef / dist, P);
+                        n0 += w * (q.K[0] * cx + q.K[1] * cy + q.K[2] * cz + q.e[0]);
+                        n1 += w * (q.K[3] * cx + q.K[4] * cy + q.K[5] * cz + q.e[1]);
+                        n2 += w * (q.K[6] * cx + q.K[7] * cy + q.K[8] * cz + q.e[2]);
                         dd += w;
+                    } else {
+                        const NodeDyn32 &q = nd32[node];
+                        const float fx = (float)cx, fy = (float)cy, fz = (float)cz;
+                        const float w = (float)h.Ai * wfun<EXPM>((float)P.Ldef * __frcp_rn((float)dist), P);
+                        n0 += (double)(w * (q.K[0] * fx + q.K[1] * fy + q.K[2] * fz + q.e[0]));
+                        n1 += (double)(w * (q.K[3] * fx + q.K[4] * fy + q.K[5] * fz + q.e[1]));
+                        n2 += (double)(w * (q.K[6] * fx + q.K[7] * fy + q.K[8] * fz + q.e[2]));
+                        dd += (double)w;
                     }
-                } else {
-                    float a0 = 0.f, a1 = 0.f, a2 = 0.f, ad = 0.f;
-                    const float Lf = (float)P.Ldef;
-                    for (int i = l; i <= u; ++i) {
-                        const double4 s = ps[i];
-                        const PointDyn32 &q = pd32[i];
-                        const float fx = (float)(v.x - s.x), fy = (float)(v.y - s.y), fz = (float)(v.z - s.z);
-                        const float lo = Lf * __frsqrt_rn(fx * fx + fy * fy + fz * fz);
-                        const float w = active ? (float)s.w * wfun(lo, P) : 0.f;
-                        a0 += w * (q.K[0] * fx + q.K[1] * fy + q.K[2] * fz + q.d[0]);
-                        a1 += w * (q.K[3] * fx + q.K[4] * fy + q.K[5] * fz + q.d[1]);
-                        a2 += w * (q.K[6] * fx + q.K[7] * fy + q.K[8] * fz + q.d[2]);
-                        ad += w;
-                    }
-                    n0 += a0; n1 += a1; n2 += a2; dd += ad;
                 }
             }
-            node = uniform(r->skip);
-            continue;
+            next = __any(descend) ? node + 1 : h.skip;
         }
-        const double cx = v.x - r->X[0], cy = v.y - r->X[1], cz = v.z - r->X[2];
-        const double dist = sqrt(cx * cx + cy * cy + cz * cz);
-        const double orad = r->orad;
-        bool descend = false;
-        if (active) {
-            const double dOvrR = dist * orad;
-            if (dOvrR < 2.0) {
-                descend = true;
-            } else if (get_error(r, dOvrR) < tol) {
-                const double Ai = r->Ai;
-                if (PREC == IDW_PREC_FP64) {
-                    const NodeDyn &q = nd[node];
-                    const double w = Ai * wfun(P.Ldef / dist, P);
-                    n0 += w * (q.K[0] * cx + q.K[1] * cy + q.K[2] * cz + q.e[0]);
-                    n1 += w * (q.K[3] * cx + q.K[4] * cy + q.K[5] * cz + q.e[1]);
-                    n2 += w * (q.K[6] * cx + q.K[7] * cy + q.K[8] * cz + q.e[2]);
-                    dd += w;
-                } else {
-                    const NodeDyn32 &q = nd32[node];
-                    const float fx = (float)cx, fy = (float)cy, fz = (float)cz;
-                    const float w = (float)Ai * wfun((float)P.Ldef * __frcp_rn((float)dist), P);
-                    n0 += (double)(w * (q.K[0] * fx + q.K[1] * fy + q.K[2] * fz + q.e[0]));
-                    n1 += (double)(w * (q.K[3] * fx + q.K[4] * fy + q.K[5] * fz + q.e[1]));
-                    n2 += (double)(w * (q.K[6] * fx + q.K[7] * fy + q.K[8] * fz + q.e[2]));
-                    dd += (double)w;
-                }
-            } else {
-                descend = true;
-            }
-            stop = descend ? INT32_MAX : depth;
-        }
-        node = __any(descend) ? node + 1 : uniform(r->skip);
+        next = uniform(next);
+        if (next >= nnodes) break;
+        h = (next == node + 1) ? hA : hB;
+        node = next;
     }
+    pw.finish();
     if (valid) {
         const int64_t j = perm[k];
         const double oden = 1.0 / dd;
@@ -532,167 +585,214 @@ __global__ void __launch_bounds__(256) k_fast_fwd(int nvol, const double4 *__res
     }
 }
 
-// Sum of 16 per-lane values over the wave; afterwards lane 4*q (q = 0..15)
-// holds the total of value q.  Transpose-reduce: at each of the first four
-// steps a lane keeps half of its values and receives the partner's copy of
-// them, so the exchange volume halves every step.
-__device__ __forceinline__ double xchg_dpp(double x, int ctrl) {
-    const int lo = __double2loint(x), hi = __double2hiint(x);
-    int rl, rh;
-    switch (ctrl) {
-        case 0x140:
-            rl = __builtin_amdgcn_update_dpp(0, lo, 0x140, 0xf, 0xf, false);
-            rh = __builtin_amdgcn_update_dpp(0, hi, 0x140, 0xf, 0xf, false);
-            break;
-        case 0x141:
-            rl = __builtin_amdgcn_update_dpp(0, lo, 0x141, 0xf, 0xf, false);
-            rh = __builtin_amdgcn_update_dpp(0, hi, 0x141, 0xf, 0xf, false);
-            break;
-        case 0x4e:  // quad_perm [2,3,0,1]
-            rl = __builtin_amdgcn_update_dpp(0, lo, 0x4e, 0xf, 0xf, false);
-            rh = __builtin_amdgcn_update_dpp(0, hi, 0x4e, 0xf, 0xf, false);
-            break;
-        default:  // 0xb1 quad_perm [1,0,3,2]
-            rl = __builtin_amdgcn_update_dpp(0, lo, 0xb1, 0xf, 0xf, false);
-            rh = __builtin_amdgcn_update_dpp(0, hi, 0xb1, 0xf, 0xf, false);
-            break;
+typedef double dvec4 __attribute__((ext_vector_type(4)));
+
+// wave-uniform copy of lane 0's value
+__device__ __forceinline__ double first_lane(double x) {
+    return __hiloint2double(__builtin_amdgcn_readfirstlane(__double2hiint(x)),
+                            __builtin_amdgcn_readfirstlane(__double2loint(x)));
+}
+__device__ __forceinline__ unsigned long long uniform64(unsigned long long x) {
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)(x & 0xffffffffu));
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(x >> 32));
+    return ((unsigned long long)hi << 32) | lo;
+}
+// LDS written by one lane and read by another lane of the same wave: the
+// wave's LDS operations execute in order, the barrier keeps the compiler
+// from moving them.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+constexpr int WROW = 65;  // padded stride of the 16 x 64 weight tile (doubles)
+constexpr int GROW = 17;  // padded stride of the 64 x 16 seed tile
+
+// Reduce 16 rows of per-lane weights against the per-lane 16-vectors G:
+// P(16x16) = W(16x64) G(64x16) as 16 v_mfma_f64_16x16x4f64 over 4-lane
+// chunks (A lane l = W[l%16][4q + l/16], B_q lane l = G_{4q+l/16}[l%16],
+// D lane l reg t = P[4t + l/16][l%16]).  Row r's 12 outputs are
+// Mb'(a,c) = P[3a+c] - P[9+a] * d_r[c] and Bb(a) = P[9+a], added to dst_r.
+__device__ __forceinline__ void flush_rows(const double *W, const double *B, double *const *dst,
+                                           const double (*dd)[4], int nrow) {
+    wave_sync();
+    const int lane = threadIdx.x & 63, i = lane & 15, g = lane >> 4;
+    dvec4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < 16; ++q) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(W[i * WROW + 4 * q + g], B[q], acc, 0, 0, 0);
+    const int j = i;
+    const int src = (lane & ~15) | (9 + (j < 9 ? j / 3 : 0));
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int row = 4 * t + g;
+        double val = acc[t];
+        const double pb = __shfl(val, src, 64);
+        if (j < 9) val -= pb * dd[row][j % 3];
+        if (row < nrow && j < 12 && val != 0.0) atomicAdd(dst[row] + j, val);
     }
-    return __hiloint2double(rh, rl);
+    wave_sync();
 }
 
-// a + partner's b across the 32-lane (sw32) or 16-lane (sw16) halves, where
-// lanes in the lower half keep a and lanes in the upper half keep b.
-__device__ __forceinline__ double swap_add32(double a, double b) {
-    const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)__double2loint(a), (unsigned)__double2loint(b), false, false);
-    const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)__double2hiint(a), (unsigned)__double2hiint(b), false, false);
-    return __hiloint2double((int)hi[0], (int)lo[0]) + __hiloint2double((int)hi[1], (int)lo[1]);
-}
-__device__ __forceinline__ double swap_add16(double a, double b) {
-    const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)__double2loint(a), (unsigned)__double2loint(b), false, false);
-    const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)__double2hiint(a), (unsigned)__double2hiint(b), false, false);
-    return __hiloint2double((int)hi[0], (int)lo[0]) + __hiloint2double((int)hi[1], (int)lo[1]);
-}
-
-__device__ __forceinline__ double wave_reduce16(double *v) {
-    const int lane = threadIdx.x & 63;
-    // bit 5: keep v[k] (lower 32 lanes) or v[k+8] (upper)
-    double a[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) a[k] = swap_add32(v[k], v[k + 8]);
-    // bit 4
-    double b[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) b[k] = swap_add16(a[k], a[k + 4]);
-    // bit 3: partner lane ^ 15 (row mirror)
-    const bool hi3 = (lane >> 3) & 1;
-    double c[2];
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-        const double keep = hi3 ? b[k + 2] : b[k], send = hi3 ? b[k] : b[k + 2];
-        c[k] = keep + xchg_dpp(send, 0x140);
-    }
-    // bit 2: partner lane ^ 7 (half-row mirror)
-    const bool hi2 = (lane >> 2) & 1;
-    const double keep = hi2 ? c[1] : c[0], send = hi2 ? c[0] : c[1];
-    double d = keep + xchg_dpp(send, 0x141);
-    d += xchg_dpp(d, 0x4e);
-    d += xchg_dpp(d, 0xb1);
-    return d;
-}
-
-// index of the value that lane 4q holds after wave_reduce16
-__device__ __forceinline__ int reduced_index(int lane) {
-    return ((lane >> 5) & 1) * 8 + ((lane >> 4) & 1) * 4 + ((lane >> 3) & 1) * 2 + ((lane >> 2) & 1);
-}
-
-__device__ __forceinline__ void reduce_add12(double *c, double *dst) {
-    double v[16];
-#pragma unroll
-    for (int k = 0; k < 12; ++k) v[k] = c[k];
-    v[12] = v[13] = v[14] = v[15] = 0.0;
-    const double s = wave_reduce16(v);
-    const int lane = threadIdx.x & 63;
-    const int q = reduced_index(lane);
-    if ((lane & 3) == 0 && q < 12 && s != 0.0) atomicAdd(dst + q, s);
-}
-
-// Reverse walk (evalNode_b, kd_tree.F90:942-1031): same decisions as the
-// forward; accepted clusters accumulate C(r,c) = sum s_r (x_c - X_c) and
-// B = sum s per node, leaf points Mb'(r,c) = sum s_r (x_c - Xu0_c) and
-// Bb = sum s, with s = w * numb.
-template <int PREC>
+// Reverse of the fast mode (evalNode_b / evalNodeApprox_b, kd_tree.F90:
+// 942-1031) replaying the forward's plan: for accepted cluster c,
+// C(a,b) += sum s_a (x_b - X_b), B(a) += sum s_a; for bucket point i,
+// Mb'(a,b) += sum s_a (x_b - Xu0_b), Bb(a) += sum s_a, with
+// s = w * dXv / den (warpDeriv.F90:76-78).  Plan entries are fetched two
+// ahead and node headers one ahead with scalar loads.
+template <int PREC, int EXPM>
 __global__ void __launch_bounds__(256) k_fast_rev(int nvol, const double4 *__restrict__ vol,
-                                                  const double4 *__restrict__ numb, int nnodes,
-                                                  const NodeRec *__restrict__ nodes, const double4 *__restrict__ ps,
-                                                  WParams P, double *__restrict__ nadj, double *__restrict__ padj) {
+                                                  const int32_t *__restrict__ perm, const double *__restrict__ dxv,
+                                                  const double *__restrict__ den, const PlanEntry *__restrict__ plan,
+                                                  const int64_t *__restrict__ waveOff, const NodeRec *__restrict__ nodes,
+                                                  const double4 *__restrict__ ps, WParams P, double *__restrict__ nadj,
+                                                  double *__restrict__ padj) {
+    __shared__ double sW[4][64 * GROW];
+    __shared__ double sD[4][16][4];
+    __shared__ double *sDst[4][16];
     const int nb = (nvol + 255) / 256;
     const int blk = xcd_block(nb);
     if (blk >= nb) return;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int k = blk * 256 + threadIdx.x;
     const bool valid = k < nvol;
+    const int wave = uniform(blk * 4 + wv);
     const double4 v = valid ? vol[k] : make_double4(0.0, 0.0, 0.0, 1.0);
-    const double4 s = valid ? numb[k] : make_double4(0.0, 0.0, 0.0, 0.0);
-    const double tol = P.errTol * v.w;
-    int stop = valid ? INT32_MAX : -1;
-    int node = 0;
-    double c[12];
-    while (node < nnodes) {
-        const NodeRec *r = nodes + node;
-        const int depth = uniform(r->depth);
-        const bool active = depth <= stop;
-        if (uniform(r->leaf)) {
-            const int l = uniform(r->l), u = uniform(r->u);
-            for (int i = l; i <= u; ++i) {
-                const double4 p = ps[i];
-                const double rx = v.x - p.x, ry = v.y - p.y, rz = v.z - p.z;
-                double w;
-                if (PREC == IDW_PREC_FP64) {
-                    w = active ? p.w * wfun(P.Ldef / sqrt(rx * rx + ry * ry + rz * rz + EPS_W), P) : 0.0;
-                } else {
-                    const double d2 = rx * rx + ry * ry + rz * rz;
-                    if (__any(active && d2 < P.nearT2)) {
-                        w = active ? p.w * wfun(P.Ldef / sqrt(d2 + EPS_W), P) : 0.0;
-                    } else {
-                        const float fx = (float)rx, fy = (float)ry, fz = (float)rz;
-                        const float lo = (float)P.Ldef * __frsqrt_rn(fx * fx + fy * fy + fz * fz);
-                        w = active ? (double)((float)p.w * wfun(lo, P)) : 0.0;
-                    }
-                }
-                const double s0 = w * s.x, s1 = w * s.y, s2 = w * s.z;
-                c[0] = s0 * rx; c[1] = s0 * ry; c[2] = s0 * rz;
-                c[3] = s1 * rx; c[4] = s1 * ry; c[5] = s1 * rz;
-                c[6] = s2 * rx; c[7] = s2 * ry; c[8] = s2 * rz;
-                c[9] = s0; c[10] = s1; c[11] = s2;
-                reduce_add12(c, padj + (int64_t)i * ADJ_STRIDE);
-            }
-            node = uniform(r->skip);
-            continue;
-        }
-        const double cx = v.x - r->X[0], cy = v.y - r->X[1], cz = v.z - r->X[2];
-        const double dist = sqrt(cx * cx + cy * cy + cz * cz);
-        bool descend = false, accept = false;
-        if (active) {
-            const double dOvrR = dist * r->orad;
-            if (dOvrR < 2.0) descend = true;
-            else if (get_error(r, dOvrR) < tol) accept = true;
-            else descend = true;
-            stop = descend ? INT32_MAX : depth;
-        }
-        if (__any(accept)) {
-            double w = 0.0;
-            if (accept) {
-                if (PREC == IDW_PREC_FP64) w = r->Ai * wfun(P.Ldef / dist, P);
-                else w = (double)((float)r->Ai * wfun((float)P.Ldef * __frcp_rn((float)dist), P));
-            }
-            const double s0 = w * s.x, s1 = w * s.y, s2 = w * s.z;
-            c[0] = s0 * cx; c[1] = s0 * cy; c[2] = s0 * cz;
-            c[3] = s1 * cx; c[4] = s1 * cy; c[5] = s1 * cz;
-            c[6] = s2 * cx; c[7] = s2 * cy; c[8] = s2 * cz;
-            c[9] = s0; c[10] = s1; c[11] = s2;
-            reduce_add12(c, nadj + (int64_t)node * ADJ_STRIDE);
-        }
-        node = __any(descend) ? node + 1 : uniform(r->skip);
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    if (valid) {
+        const int64_t j = perm[k];
+        const double oden = 1.0 / den[k];
+        s0 = dxv[3 * j] * oden;
+        s1 = dxv[3 * j + 1] * oden;
+        s2 = dxv[3 * j + 2] * oden;
     }
+    // wave origin: lane 0 is valid whenever any lane of the wave is
+    const double ox = first_lane(v.x), oy = first_lane(v.y), oz = first_lane(v.z);
+    const double gx = v.x - ox, gy = v.y - oy, gz = v.z - oz;
+    double *W = sW[wv];
+    {
+        const double G[12] = {s0 * gx, s0 * gy, s0 * gz, s1 * gx, s1 * gy, s1 * gz,
+                              s2 * gx, s2 * gy, s2 * gz, s0, s1, s2};
+#pragma unroll
+        for (int c = 0; c < 16; ++c) W[lane * GROW + c] = c < 12 ? G[c] : 0.0;
+    }
+    wave_sync();
+    double B[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) B[q] = W[(4 * q + (lane >> 4)) * GROW + (lane & 15)];
+    wave_sync();
+    const float Lf = (float)P.Ldef;
+    int nrow = 0;
+    const int64_t e0 = waveOff[wave], e1 = waveOff[wave + 1];
+    if (e0 < e1) {
+        PlanEntry pe = plan[e0];
+        PlanEntry pn = plan[min(e0 + 1, e1 - 1)];
+        NodeHdr h = load_hdr(nodes, pe.node);
+        for (int64_t e = e0; e < e1; ++e) {
+            const PlanEntry pnn = plan[min(e + 2, e1 - 1)];
+            const NodeHdr hn = load_hdr(nodes, pn.node);
+            const int node = pe.node;
+            const bool active = (pe.mask >> lane) & 1ull;
+            if (h.leaf) {
+                double4 p = ps[h.l];
+                for (int i = h.l; i <= h.u; ++i) {
+                    const double4 pnext = ps[min(i + 1, h.u)];
+                    const double rx = v.x - p.x, ry = v.y - p.y, rz = v.z - p.z;
+                    double w;
+                    if (PREC == IDW_PREC_FP64) {
+                        w = active ? p.w * wfun<EXPM>(P.Ldef / sqrt(rx * rx + ry * ry + rz * rz + EPS_W), P) : 0.0;
+                    } else {
+                        const double d2 = rx * rx + ry * ry + rz * rz;
+                        if (__any(active && d2 < P.nearT2)) {
+                            w = active ? p.w * wfun<EXPM>(P.Ldef / sqrt(d2 + EPS_W), P) : 0.0;
+                        } else {
+                            const float fx = (float)rx, fy = (float)ry, fz = (float)rz;
+                            w = active ? (double)((float)p.w * wfun<EXPM>(Lf * __frsqrt_rn(fx * fx + fy * fy + fz * fz), P))
+                                       : 0.0;
+                        }
+                    }
+                    W[nrow * WROW + lane] = w;
+                    if (lane == 0) {
+                        sDst[wv][nrow] = padj + (int64_t)i * ADJ_STRIDE;
+                        sD[wv][nrow][0] = p.x - ox;
+                        sD[wv][nrow][1] = p.y - oy;
+                        sD[wv][nrow][2] = p.z - oz;
+                    }
+                    if (++nrow == 16) {
+                        flush_rows(W, B, sDst[wv], sD[wv], 16);
+                        nrow = 0;
+                    }
+                    p = pnext;
+                }
+            } else {
+                const double cx = v.x - h.X[0], cy = v.y - h.X[1], cz = v.z - h.X[2];
+                const double dist = sqrt(cx * cx + cy * cy + cz * cz);
+                double w = 0.0;
+                if (active) {
+                    if (PREC == IDW_PREC_FP64) w = h.Ai * wfun<EXPM>(P.Ldef / dist, P);
+                    else w = (double)((float)h.Ai * wfun<EXPM>(Lf * __frcp_rn((float)dist), P));
+                }
+                W[nrow * WROW + lane] = w;
+                if (lane == 0) {
+                    sDst[wv][nrow] = nadj + (int64_t)node * ADJ_STRIDE;
+                    sD[wv][nrow][0] = h.X[0] - ox;
+                    sD[wv][nrow][1] = h.X[1] - oy;
+                    sD[wv][nrow][2] = h.X[2] - oz;
+                }
+                if (++nrow == 16) {
+                    flush_rows(W, B, sDst[wv], sD[wv], 16);
+                    nrow = 0;
+                }
+            }
+            pe = pn;
+            pn = pnn;
+            h = hn;
+        }
+    }
+    if (nrow) flush_rows(W, B, sDst[wv], sD[wv], nrow);
+}
+
+// Counting pass of the plan: entries per wave for the current errTol (same
+// walk and decisions as the forward).
+__global__ void __launch_bounds__(256) k_plan_count(int nvol, const double4 *__restrict__ vol, int nnodes,
+                                                    const NodeRec *__restrict__ nodes, double errTol,
+                                                    int64_t *__restrict__ waveCount) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    const bool valid = k < nvol;
+    const double4 v = valid ? vol[k] : make_double4(0.0, 0.0, 0.0, 1.0);
+    const double tol = errTol * v.w;
+    int stop = valid ? INT32_MAX : -1;
+    int64_t cnt = 0;
+    int node = 0;
+    NodeHdr h = load_hdr(nodes, 0);
+    for (;;) {
+        const NodeHdr hA = load_hdr(nodes, min(node + 1, nnodes - 1));
+        const NodeHdr hB = load_hdr(nodes, min(h.skip, nnodes - 1));
+        const bool active = h.depth <= stop;
+        int next;
+        if (h.leaf) {
+            ++cnt;
+            next = h.skip;
+        } else {
+            const double cx = v.x - h.X[0], cy = v.y - h.X[1], cz = v.z - h.X[2];
+            const double dist = sqrt(cx * cx + cy * cy + cz * cz);
+            bool descend = false, accept = false;
+            if (active) {
+                const double dOvrR = dist * h.orad;
+                if (dOvrR < 2.0) descend = true;
+                else if (get_error(nodes + node, dOvrR) < tol) accept = true;
+                else descend = true;
+                stop = descend ? INT32_MAX : h.depth;
+            }
+            if (__any(accept)) ++cnt;
+            next = __any(descend) ? node + 1 : h.skip;
+        }
+        next = uniform(next);
+        if (next >= nnodes) break;
+        h = (next == node + 1) ? hA : hB;
+        node = next;
+    }
+    if ((threadIdx.x & 63) == 0) waveCount[(int64_t)k >> 6] = cnt;
 }
 
 // Per-lane work of the fast walk, summed over all volume nodes: internal
@@ -1103,21 +1203,25 @@ void launch_fill(hipStream_t st, double *a, int64_t n, double v) {
     if (n > 0) k_fill<<<grid_for(n, 256), 256, 0, st>>>(n, a, v);
 }
 
-void launch_forward(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, int fast, int prec,
-                    double *xv) {
+template <int EXPM>
+static void forward_impl(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, int fast, int prec,
+                         double *xv, const DevPlan *plan) {
     const unsigned g = xcd_grid(V.n);
     if (!fast) {
         if (prec == IDW_PREC_FP64)
-            k_exact_fwd<IDW_PREC_FP64><<<g, 256, 0, st>>>((int)V.n, V.vol, V.perm, S.n, S.ps, S.pd, S.pd32, P, xv, V.den);
+            k_exact_fwd<IDW_PREC_FP64, EXPM><<<g, 256, 0, st>>>((int)V.n, V.vol, V.perm, S.n, S.ps, S.pd, S.pd32, P, xv, V.den);
         else
-            k_exact_fwd<IDW_PREC_MIXED><<<g, 256, 0, st>>>((int)V.n, V.vol, V.perm, S.n, S.ps, S.pd, S.pd32, P, xv, V.den);
+            k_exact_fwd<IDW_PREC_MIXED, EXPM><<<g, 256, 0, st>>>((int)V.n, V.vol, V.perm, S.n, S.ps, S.pd, S.pd32, P, xv, V.den);
     } else {
+        PlanEntry *pe = plan ? plan->entries : nullptr;
+        const int64_t *off = plan ? plan->waveOff : nullptr;
+        unsigned int *mm = plan ? plan->mismatch : nullptr;
         if (prec == IDW_PREC_FP64)
-            k_fast_fwd<IDW_PREC_FP64><<<g, 256, 0, st>>>((int)V.n, V.vol, V.perm, S.nnodes, S.nodes, S.nd, S.nd32, S.ps,
-                                                        S.pd, S.pd32, P, xv, V.den);
+            k_fast_fwd<IDW_PREC_FP64, EXPM><<<g, 256, 0, st>>>((int)V.n, V.vol, V.perm, S.nnodes, S.nodes, S.nd, S.nd32, S.ps,
+                                                        S.pd, S.pd32, P, xv, V.den, pe, off, mm);
         else
-            k_fast_fwd<IDW_PREC_MIXED><<<g, 256, 0, st>>>((int)V.n, V.vol, V.perm, S.nnodes, S.nodes, S.nd, S.nd32,
-                                                         S.ps, S.pd, S.pd32, P, xv, V.den);
+            k_fast_fwd<IDW_PREC_MIXED, EXPM><<<g, 256, 0, st>>>((int)V.n, V.vol, V.perm, S.nnodes, S.nodes, S.nd, S.nd32,
+                                                         S.ps, S.pd, S.pd32, P, xv, V.den, pe, off, mm);
     }
 }
 
@@ -1125,23 +1229,30 @@ void launch_prep_numb(hipStream_t st, const DevVolume &V, const double *dxv) {
     k_prep_numb<<<grid_for(V.n, 256), 256, 0, st>>>((int)V.n, V.perm, dxv, V.den, V.numb);
 }
 
-void launch_reverse(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, int fast, int prec,
-                    double *partial, int nchunk) {
+void launch_plan_count(hipStream_t st, const DevSurface &S, const DevVolume &V, double errTol, int64_t *waveCount) {
+    k_plan_count<<<grid_for(V.n, 256), 256, 0, st>>>((int)V.n, V.vol, S.nnodes, S.nodes, errTol, waveCount);
+}
+
+template <int EXPM>
+static void reverse_impl(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, int fast, int prec,
+                         double *partial, int nchunk, const double *dxv, const DevPlan *plan) {
     if (!fast) {
+        launch_prep_numb(st, V, dxv);
         const int chunk = (int)((V.n + nchunk - 1) / nchunk);
         dim3 g(grid_for(S.n, 256), nchunk);
         if (prec == IDW_PREC_FP64)
-            k_exact_rev<IDW_PREC_FP64><<<g, 256, 0, st>>>((int)V.n, chunk, V.vol, V.numb, S.n, S.ps, P, partial);
+            k_exact_rev<IDW_PREC_FP64, EXPM><<<g, 256, 0, st>>>((int)V.n, chunk, V.vol, V.numb, S.n, S.ps, P, partial);
         else
-            k_exact_rev<IDW_PREC_MIXED><<<g, 256, 0, st>>>((int)V.n, chunk, V.vol, V.numb, S.n, S.ps, P, partial);
+            k_exact_rev<IDW_PREC_MIXED, EXPM><<<g, 256, 0, st>>>((int)V.n, chunk, V.vol, V.numb, S.n, S.ps, P, partial);
         k_exact_rev_reduce<<<grid_for((int64_t)S.n * 12, 256), 256, 0, st>>>(S.n, nchunk, partial, S.padj);
     } else {
         const unsigned g = xcd_grid(V.n);
         if (prec == IDW_PREC_FP64)
-            k_fast_rev<IDW_PREC_FP64><<<g, 256, 0, st>>>((int)V.n, V.vol, V.numb, S.nnodes, S.nodes, S.ps, P, S.nadj, S.padj);
+            k_fast_rev<IDW_PREC_FP64, EXPM><<<g, 256, 0, st>>>((int)V.n, V.vol, V.perm, dxv, V.den, plan->entries,
+                                                        plan->waveOff, S.nodes, S.ps, P, S.nadj, S.padj);
         else
-            k_fast_rev<IDW_PREC_MIXED><<<g, 256, 0, st>>>((int)V.n, V.vol, V.numb, S.nnodes, S.nodes, S.ps, P, S.nadj,
-                                                         S.padj);
+            k_fast_rev<IDW_PREC_MIXED, EXPM><<<g, 256, 0, st>>>((int)V.n, V.vol, V.perm, dxv, V.den, plan->entries,
+                                                         plan->waveOff, S.nodes, S.ps, P, S.nadj, S.padj);
     }
 }
 
@@ -1155,6 +1266,18 @@ void launch_reverse_prep(hipStream_t st, const DevSurface &S, int fast) {
 void launch_fast_count(hipStream_t st, const DevSurface &S, const DevVolume &V, double errTol,
                        unsigned long long *out) {
     k_fast_count<<<grid_for(V.n, 256), 256, 0, st>>>((int)V.n, V.vol, S.nnodes, S.nodes, errTol, out);
+}
+
+void launch_forward(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, int fast, int prec,
+                    double *xv, const DevPlan *plan) {
+    if (P.expMode == EXP_35) forward_impl<EXP_35>(st, S, V, P, fast, prec, xv, plan);
+    else forward_impl<EXP_INT>(st, S, V, P, fast, prec, xv, plan);
+}
+
+void launch_reverse(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, int fast, int prec,
+                    double *partial, int nchunk, const double *dxv, const DevPlan *plan) {
+    if (P.expMode == EXP_35) reverse_impl<EXP_35>(st, S, V, P, fast, prec, partial, nchunk, dxv, plan);
+    else reverse_impl<EXP_INT>(st, S, V, P, fast, prec, partial, nchunk, dxv, plan);
 }
 
 void launch_nodal_reverse(hipStream_t st, const DevSurface &S, const double *xs, int fast, int useRot, double *dxs,
