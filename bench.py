@@ -26,11 +26,11 @@ sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector (= FP64 matrix) peak
 
-# algorithmic flops per unit of fast-mode work (FMA = 2, sqrt/div = 1):
-# forward: leaf pair 42, accepted cluster 32, internal visit 10, error check 7
-# reverse: leaf pair 41, accepted cluster 31, internal visit 10, error check 7
-FWD_FLOPS = (10.0, 32.0, 42.0, 7.0)  # per (V, A, L, E)
-REV_FLOPS = (10.0, 31.0, 41.0, 7.0)
+# algorithmic flops per unit of fast-mode work (FMA = 2, sqrt/div = 1), per
+# (V internal visit, A accepted cluster, L bucket point pair, E error check):
+WALK_FLOPS = (10.0, 0.0, 0.0, 7.0)   # distance test, error-table interpolation
+FWD_FLOPS = (0.0, 32.0, 42.0, 0.0)   # weights and displacement sums
+REV_FLOPS = (0.0, 31.0, 41.0, 0.0)   # weights and the 12 adjoint sums
 
 
 def parse():
@@ -44,7 +44,23 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU work of the baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["both", "fp64", "mixed"], default="both")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a device copy of 1 GiB (reference for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
+
+
+def copy_calibration():
+    """One device-to-device copy kernel of exactly 2**30 bytes read and
+    2**30 bytes written, run 3 times: the per-dispatch FETCH_SIZE and
+    WRITE_SIZE it reports convert the counters of the warp kernels to bytes."""
+    import torch
+
+    a = torch.ones(2 ** 27, dtype=torch.float64, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": 2 ** 30, "copy_bytes_written": 2 ** 30}))
 
 
 def log(*a):
@@ -86,6 +102,9 @@ def cpu_baseline(m, xs, dxv, errtol, target_s):
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the volume-node sharding is described "
@@ -108,8 +127,8 @@ def main():
     log("init (tree, error tables, denominator estimate): %.1fs" % (time.time() - t0))
     allc = mesh.getFastCounts()
     counts = allc[:4]  # V, A, L, E summed over volume nodes
-    fwd_flops = float(np.dot(FWD_FLOPS, counts))
-    rev_flops = float(np.dot(REV_FLOPS, counts))
+    kflops = {"walk": float(np.dot(WALK_FLOPS, counts)), "fwd": float(np.dot(FWD_FLOPS, counts)),
+              "rev": float(np.dot(REV_FLOPS, counts))}
     util = {"internal_visits": counts[0] / (64 * allc[4]), "accepts": counts[1] / (64 * allc[5]),
             "bucket_pairs": counts[2] / (64 * allc[7])}
     log("work: visits %.3g accepts %.3g leaf pairs %.3g checks %.3g; wave-level %.3g %.3g %.3g %.3g" % tuple(allc))
@@ -125,9 +144,11 @@ def main():
         return ctypes.c_void_p(t.data_ptr())
 
     out = {}
-    precs = ["fp64", "mixed"] if args.only == "both" else [args.only]
-    for prec in precs:
+    legs = {"both": ["fp64", "mixed", "mixed_reuse"], "fp64": ["fp64"], "mixed": ["mixed"]}[args.only]
+    for leg in legs:
+        prec = leg.split("_")[0]
         mesh.setOption("precision", prec)
+        mesh.setOption("reusePlan", leg.endswith("_reuse"))
         mesh._setMeshOptions()
         d_xv = torch.empty(3 * nvol, dtype=torch.float64, device=dev)
         d_dxs = torch.empty(3 * npts, dtype=torch.float64, device=dev)
@@ -148,9 +169,9 @@ def main():
         torch.cuda.synchronize()
         wall = (time.perf_counter() - t1) * 1e3 / max(args.steps, 1)
         phases /= max(args.steps, 1)
-        out[prec] = dict(ms=wall, phases=phases, xv=d_xv, dxs=d_dxs)
-        log("%s: %.3f ms/step  [nodal %.3f | fwd %.3f | rev setup %.3f | rev %.3f | nodal rev %.3f]" %
-            ((prec, wall) + tuple(phases[:5])))
+        out[leg] = dict(ms=wall, phases=phases, xv=d_xv, dxs=d_dxs)
+        log("%s: %.3f ms/step  [nodal %.3f | walk %.3f | fwd %.3f | rev setup %.3f | rev %.3f | nodal rev %.3f]" %
+            ((leg, wall) + tuple(phases)))
 
     head = out.get("mixed", out.get("fp64"))
     head_prec = "mixed" if "mixed" in out else "fp64"
@@ -174,25 +195,27 @@ def main():
     }
     if "fp64" in out:
         res["fp64_ms_per_step"] = round(out["fp64"]["ms"], 3)
+    if "mixed_reuse" in out:
+        res["mixed_plan_reused_ms_per_step"] = round(out["mixed_reuse"]["ms"], 3)
     if "fp64" in out and "mixed" in out:
         a, b = out["mixed"], out["fp64"]
         res["mixed_speedup_vs_fp64"] = round(b["ms"] / a["ms"], 3)
         res["mixed_max_err_over_Ldef"] = float((a["xv"] - b["xv"]).abs().max().item() / ldef)
         res["mixed_adjoint_rel_err"] = float(((a["dxs"] - b["dxs"]).norm() / b["dxs"].norm()).item())
-    # roofline of the dominant volume kernel of the headline precision
+    # roofline of the dominant volume kernel of the headline leg
     ph = head["phases"]
-    if ph[3] >= ph[1]:
-        kname, kms, kflops = "k_fast_rev<%s>" % head_prec, ph[3], rev_flops
-    else:
-        kname, kms, kflops = "k_fast_fwd<%s>" % head_prec, ph[1], fwd_flops
-    achieved = kflops / (kms * 1e-3) / 1e12
+    cand = [("k_plan_walk", ph[1], kflops["walk"]), ("k_fast_fwd<%s>" % head_prec, ph[2], kflops["fwd"]),
+            ("k_fast_rev<%s>" % head_prec, ph[4], kflops["rev"])]
+    kname, kms, kfl = max(cand, key=lambda t: t[1])
+    achieved = kfl / (kms * 1e-3) / 1e12
     res["roofline"] = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                        "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": _traffic(kname),
                        "kernel": kname, "kernel_ms": round(float(kms), 3),
-                       "algorithmic_flops_per_launch": kflops,
+                       "algorithmic_flops_per_launch": kfl,
                        "compute": "VALU FP64/FP32 (no MFMA): the 78.6 TFLOP/s FP64 vector roof equals the FP64 "
                                   "matrix roof and non-packed FP32 issues at the same rate"}
-    res["phases_ms"] = {p: [round(float(x), 3) for x in out[p]["phases"][:5]] for p in out}
+    res["phases_ms"] = {p: dict(zip(("nodal", "walk", "fwd", "rev_setup", "rev", "nodal_rev"),
+                                    [round(float(x), 3) for x in out[p]["phases"]])) for p in out}
     res["walk_lane_utilisation"] = {k: round(float(v), 3) for k, v in util.items()}
     if not args.no_cpu:
         t0 = time.time()

@@ -39,7 +39,7 @@ class IdwOptions(ctypes.Structure):
         ("zeroCornerRotations", ctypes.c_int32),
         ("bucketSize", ctypes.c_int32),
         ("precision", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("reusePlan", ctypes.c_int32),
         ("nearWallFactor", ctypes.c_double),
     ]
 

@@ -52,6 +52,8 @@ bool same_numeric_options(const idw_options &a, const idw_options &b) {
            a.evalMode == b.evalMode && a.precision == b.precision && a.nearWallFactor == b.nearWallFactor;
 }
 
+double plan_near_t2(const idw_ctx *c);
+
 }  // namespace
 
 struct idw_ctx {
@@ -70,9 +72,10 @@ struct idw_ctx {
     double *xv0w = nullptr, *xv = nullptr, *xs = nullptr, *dxs = nullptr, *dxv = nullptr;
     double *partial = nullptr;
     int nchunk = 0;
+    double bboxLo[3] = {0, 0, 0}, bboxScale[3] = {0, 0, 0};  // Morton quantisation of the volume box
     std::vector<void *> allocs;
     std::vector<int32_t> solverInd;
-    hipEvent_t ev[7] = {};
+    hipEvent_t ev[8] = {};
     double ms[6] = {0, 0, 0, 0, 0, 0};
     bool fwdTimed = false, revTimed = false;
 };
@@ -134,12 +137,15 @@ int ensure_device(idw_ctx *c) {
     return IDW_OK;
 }
 
+void drop_plan(idw_ctx *c);
+
 int compute_den0(idw_ctx *c) {
     if (!c->volSet || !c->surfInit || c->V.n == 0) return IDW_OK;
     idw::launch_den0(c->st, c->S, c->V, make_params(c, true));
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->st));
     c->denValid = false;
+    drop_plan(c);
     return IDW_OK;
 }
 
@@ -156,9 +162,56 @@ void drop_plan(idw_ctx *c) {
 // exclusive scan, redone only when errTol, the volume nodes or the surface
 // change (like the denominator estimate, the counts depend on init-time data
 // only).
+double plan_near_t2(const idw_ctx *c) {
+    if (c->opt.precision != IDW_PREC_MIXED) return 0.0;
+    const double t = c->opt.nearWallFactor * c->hs.ldef0 * c->opt.LdefFact;
+    return t * t;
+}
+
+// Mixed precision: move the volume nodes that need FP64 buckets behind the
+// others (Morton order kept within both groups), so that the other waves
+// run FP32 throughout.  Redone when errTol or the near-wall distance change.
+int order_near_wall(idw_ctx *c) {
+    idw::DevPlan &P = c->plan;
+    const double nt2 = plan_near_t2(c);
+    if (nt2 <= 0.0 || c->V.n == 0) return IDW_OK;
+    if (P.orderTol == c->opt.errTol && P.orderT2 == nt2) return IDW_OK;
+    const int64_t n = c->V.n;
+    uint64_t *keys = nullptr, *keys2 = nullptr;
+    int32_t *ids = nullptr, *order = nullptr, *perm2 = nullptr;
+    double4 *vol2 = nullptr;
+    TRY(dalloc(c, &keys, n));
+    TRY(dalloc(c, &keys2, n));
+    TRY(dalloc(c, &ids, n));
+    TRY(dalloc(c, &order, n));
+    idw::launch_near_keys(c->st, c->S, c->V, c->opt.errTol, nt2, c->bboxLo, c->bboxScale, keys, ids);
+    HIPCHK(hipGetLastError());
+    HIPCHK(idw::sort_pairs(c->st, n, keys, ids, keys2, order));
+    dfree(c, keys);
+    dfree(c, keys2);
+    dfree(c, ids);
+    TRY(dalloc(c, &vol2, n));
+    TRY(dalloc(c, &perm2, n));
+    idw::launch_permute_vol(c->st, n, order, c->V.vol, c->V.perm, vol2, perm2);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->st));
+    dfree(c, order);
+    dfree(c, c->V.vol);
+    dfree(c, c->V.perm);
+    c->V.vol = vol2;
+    c->V.perm = perm2;
+    c->denValid = false;  // den is stored in traversal order
+    P.orderTol = c->opt.errTol;
+    P.orderT2 = nt2;
+    P.sized = false;
+    return IDW_OK;
+}
+
 int size_plan(idw_ctx *c) {
     idw::DevPlan &P = c->plan;
+    TRY(order_near_wall(c));
     if (P.sized && P.errTol == c->opt.errTol) return IDW_OK;
+    P.filled = false;
     const int64_t nwave = ((c->V.n + 255) / 256) * 4;
     if (!P.waveOff || P.nwave != nwave) {
         dfree(c, P.waveOff);
@@ -212,18 +265,27 @@ int run_warp(idw_ctx *c, const double *xs, double *xv) {
     if (!c->surfInit) return fail(IDW_ESTATE, "warp before initialize_warping");
     if (!c->volSet) return fail(IDW_ESTATE, "no volume nodes set");
     const int fast = c->opt.evalMode == IDW_EVAL_FAST;
-    if (fast) {
-        TRY(size_plan(c));
-        HIPCHK(hipMemsetAsync(c->plan.mismatch, 0, sizeof(unsigned int), c->st));
-    }
+    const idw::WParams P = make_params(c, false);
+    if (fast) TRY(size_plan(c));
     HIPCHK(hipEventRecord(c->ev[0], c->st));
     TRY(surface_pass(c, xs, fast));
     HIPCHK(hipEventRecord(c->ev[1], c->st));
-    if (c->V.n)
-        idw::launch_forward(c->st, c->S, c->V, make_params(c, false), fast, c->opt.precision, xv,
-                            fast ? &c->plan : nullptr);
-    HIPCHK(hipGetLastError());
+    if (fast) {
+        idw::DevPlan &pl = c->plan;
+        const double nt2 = plan_near_t2(c);
+        if (!c->opt.reusePlan || !pl.filled || pl.errTol != c->opt.errTol || pl.nearT2 != nt2) {
+            HIPCHK(hipMemsetAsync(pl.mismatch, 0, sizeof(unsigned int), c->st));
+            if (c->V.n) idw::launch_plan_fill(c->st, c->S, c->V, c->opt.errTol, nt2, pl);
+            HIPCHK(hipGetLastError());
+            pl.filled = true;
+            pl.nearT2 = nt2;
+        }
+    }
     HIPCHK(hipEventRecord(c->ev[2], c->st));
+    if (c->V.n)
+        idw::launch_forward(c->st, c->S, c->V, P, fast, c->opt.precision, xv, fast ? &c->plan : nullptr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c->ev[3], c->st));
     c->denOpt = c->opt;
     c->denValid = true;
     c->planValid = fast;
@@ -250,19 +312,19 @@ int run_deriv(idw_ctx *c, const double *xs, const double *dxv, double *dxs) {
     if (!c->denValid || !same_numeric_options(c->denOpt, c->opt) || (fast && !c->planValid))
         TRY(run_warp(c, xs, c->xv));
     if (!fast) TRY(ensure_partial(c));
-    HIPCHK(hipEventRecord(c->ev[3], c->st));
+    HIPCHK(hipEventRecord(c->ev[4], c->st));
     TRY(surface_pass(c, xs, false));
     idw::launch_reverse_prep(c->st, c->S, fast || c->V.n == 0);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(c->ev[4], c->st));
+    HIPCHK(hipEventRecord(c->ev[5], c->st));
     if (c->V.n)
         idw::launch_reverse(c->st, c->S, c->V, make_params(c, false), fast, c->opt.precision, c->partial, c->nchunk,
                             dxv, &c->plan);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(c->ev[5], c->st));
+    HIPCHK(hipEventRecord(c->ev[6], c->st));
     idw::launch_nodal_reverse(c->st, c->S, xs, fast, c->opt.useRotations, dxs, c->npts);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(c->ev[6], c->st));
+    HIPCHK(hipEventRecord(c->ev[7], c->st));
     c->revTimed = true;
     return IDW_OK;
 }
@@ -270,18 +332,18 @@ int run_deriv(idw_ctx *c, const double *xs, const double *dxv, double *dxs) {
 void collect_timings(idw_ctx *c) {
     float t;
     if (c->fwdTimed) {
-        if (hipEventElapsedTime(&t, c->ev[0], c->ev[1]) == hipSuccess) c->ms[0] = t;
-        if (hipEventElapsedTime(&t, c->ev[1], c->ev[2]) == hipSuccess) c->ms[1] = t;
+        for (int k = 0; k < 3; ++k)
+            if (hipEventElapsedTime(&t, c->ev[k], c->ev[k + 1]) == hipSuccess) c->ms[k] = t;
     }
     if (c->revTimed) {
-        if (hipEventElapsedTime(&t, c->ev[3], c->ev[4]) == hipSuccess) c->ms[2] = t;
-        if (hipEventElapsedTime(&t, c->ev[4], c->ev[5]) == hipSuccess) c->ms[3] = t;
-        if (hipEventElapsedTime(&t, c->ev[5], c->ev[6]) == hipSuccess) c->ms[4] = t;
+        for (int k = 0; k < 3; ++k)
+            if (hipEventElapsedTime(&t, c->ev[4 + k], c->ev[5 + k]) == hipSuccess) c->ms[3 + k] = t;
     }
 }
 
 bool valid_opts(const idw_options *o) {
     return o && o->bucketSize >= 1 && (o->evalMode == 0 || o->evalMode == 1) && (o->precision == 0 || o->precision == 1) &&
+           (o->reusePlan == 0 || o->reusePlan == 1) &&
            std::isfinite(o->aExp) && std::isfinite(o->bExp) && o->LdefFact > 0 && o->errTol >= 0 &&
            o->nearWallFactor >= 0;
 }
@@ -334,7 +396,8 @@ void idw_default_options(idw_options *o) {
     o->zeroCornerRotations = 1;
     o->bucketSize = 8;
     o->precision = IDW_PREC_FP64;
-    o->nearWallFactor = 1e-3;
+    o->reusePlan = 0;
+    o->nearWallFactor = 1e-6;
 }
 
 const char *idw_last_error(void) { return t_err.c_str(); }
@@ -356,7 +419,7 @@ int idw_create(idw_ctx **out, int device, const idw_options *opt) {
     c->opt = *opt;
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
-    for (int k = 0; k < 7 && e == hipSuccess; ++k) e = hipEventCreate(&c->ev[k]);
+    for (int k = 0; k < 8 && e == hipSuccess; ++k) e = hipEventCreate(&c->ev[k]);
     if (e != hipSuccess) {
         idw_destroy(c);
         return fail(IDW_EHIP, "stream/event creation: %s", hipGetErrorString(e));
@@ -418,6 +481,10 @@ int idw_set_volume_nodes(idw_ctx *c, const double *xv0, int64_t ndof) {
         for (int k = 0; k < 3; ++k) span = std::max(span, hi[k] - lo[k]);
         double scale[3];
         for (int k = 0; k < 3; ++k) scale[k] = span > 0 ? 2097151.0 / span : 0.0;
+        for (int k = 0; k < 3; ++k) {
+            c->bboxLo[k] = lo[k];
+            c->bboxScale[k] = scale[k];
+        }
         uint64_t *keys = nullptr, *keys2 = nullptr;
         int32_t *ids = nullptr;
         TRY(dalloc(c, &keys, n));

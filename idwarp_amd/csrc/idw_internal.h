@@ -48,18 +48,24 @@ struct alignas(16) NodeDyn32 {
     float e[3];
 };
 
-// Per-call surface point data in tree order: K = M_i - I and d = Xu_i - Xu0_i,
-// so that M_i r + b_i - r == K (r - Xu0_i) + d (b_i = Xu_i - M_i Xu0_i,
-// kd_tree.F90:1648).
+// Per-call surface point record in tree order: position Xu0, area weight A,
+// K = M_i - I and d = Xu_i - Xu0_i, so that M_i r + b_i - r == K (r - Xu0_i) + d
+// (b_i = Xu_i - M_i Xu0_i, kd_tree.F90:1648).  One record per point so a
+// bucket point is one contiguous fetch.
 struct alignas(16) PointDyn {
+    double x0[3];
+    double A;
     double K[9];
     double d[3];
-    double pad[4];
 };
+static_assert(sizeof(PointDyn) == 128, "PointDyn layout");
 struct alignas(16) PointDyn32 {
+    double x0[3];  // positions stay FP64: r - Xu0 is formed before rounding
+    float A;
     float K[9];
     float d[3];
 };
+static_assert(sizeof(PointDyn32) == 80, "PointDyn32 layout");
 
 // Host-side result of the surface definition (myCreateTree, kd_tree.F90:1365-1545).
 struct HostSurface {

@@ -18,7 +18,8 @@ enum { EXP_35 = 0, EXP_INT = 1, EXP_REAL = 2 };
 // the tree at initialisation.
 struct WParams {
     double Ldef, a5, aExp, bExp, errTol, nearT2;
-    int32_t expMode, ia, ib, pad;
+    int32_t expMode, ia, ib;
+    int32_t zero;  // always 0: lane & zero turns a uniform index into a per-lane one (vector load)
 };
 
 struct DevSurface {
@@ -44,8 +45,9 @@ struct DevSurface {
 struct alignas(16) PlanEntry {
     unsigned long long mask;
     int32_t node;
-    int32_t pad;
+    int32_t flags;  // PLAN_FP64: bucket with a lane near the wall (mixed precision keeps FP64)
 };
+constexpr int PLAN_FP64 = 1;
 
 // Per-wave plan of the fast mode (valid for one volume ordering, tree and errTol).
 struct DevPlan {
@@ -54,8 +56,9 @@ struct DevPlan {
     int64_t *waveCount = nullptr;
     unsigned int *mismatch = nullptr;
     int64_t nwave = 0, total = 0, capacity = 0;
-    double errTol = -1.0;
-    bool sized = false;
+    double errTol = -1.0, nearT2 = -1.0;  // key of the filled plan
+    double orderTol = -1.0, orderT2 = -1.0;  // key of the near-wall ordering of the volume nodes
+    bool sized = false, filled = false;
 };
 
 struct DevVolume {
@@ -75,11 +78,17 @@ void launch_den0(hipStream_t st, const DevSurface &S, const DevVolume &V, const 
 void launch_morton(hipStream_t st, int64_t n, const double *xv0, double lo[3], double scale[3], uint64_t *keys,
                    int32_t *ids);
 void launch_build_vol(hipStream_t st, int64_t n, const double *xv0, const int32_t *perm, double4 *vol);
+void launch_near_keys(hipStream_t st, const DevSurface &S, const DevVolume &V, double errTol, double nearT2,
+                      const double lo[3], const double scale[3], uint64_t *keys, int32_t *ids);
+void launch_permute_vol(hipStream_t st, int64_t n, const int32_t *order, const double4 *vin, const int32_t *pin,
+                        double4 *vout, int32_t *pout);
 void launch_fill(hipStream_t st, double *a, int64_t n, double v);
 void launch_forward(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, int fast, int prec,
                     double *xv, const DevPlan *plan);
 void launch_prep_numb(hipStream_t st, const DevVolume &V, const double *dxv);
 void launch_plan_count(hipStream_t st, const DevSurface &S, const DevVolume &V, double errTol, int64_t *waveCount);
+void launch_plan_fill(hipStream_t st, const DevSurface &S, const DevVolume &V, double errTol, double nearT2,
+                      const DevPlan &plan);
 void launch_reverse(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, int fast, int prec,
                     double *partial, int nchunk, const double *dxv, const DevPlan *plan);
 hipError_t exclusive_scan(hipStream_t st, const int64_t *in, int64_t *out, int64_t n);

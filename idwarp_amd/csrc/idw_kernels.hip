@@ -45,6 +45,17 @@ __device__ __forceinline__ int xcd_block(int nblocks) {
     return (b & 7) * per + (b >> 3);
 }
 
+// 1/sqrt(x) for normal x > 0: hardware estimate (v_rsq_f64) refined by two
+// Newton steps y += y (1 - x y^2) / 2, within a few ulp of the rounded
+// value; replaces sqrt + IEEE division in the weights q = Ldef / d.
+__device__ __forceinline__ double rsqrt_nr(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    double e = fma(-x * y, y, 1.0);
+    y = fma(0.5 * y, e, y);
+    e = fma(-x * y, y, 1.0);
+    return fma(0.5 * y, e, y);
+}
+
 template <typename T>
 __device__ __forceinline__ T ipow_u(T q, int e) {
     T r = T(1);
@@ -175,6 +186,8 @@ __global__ void k_node_props(int n, int initial, int useRot, const int32_t *__re
     }
     const double4 p = ps[i];
     PointDyn out;
+    out.x0[0] = p.x; out.x0[1] = p.y; out.x0[2] = p.z;
+    out.A = p.w;
     double M[9] = {1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0};
     double xu[3];
     load_xu(xs, xuInd, i, xu);
@@ -189,9 +202,10 @@ __global__ void k_node_props(int n, int initial, int useRot, const int32_t *__re
         for (int k = 0; k < 9; ++k) out.K[k] = M[k] - ((k == 0 || k == 4 || k == 8) ? 1.0 : 0.0);
         out.d[0] = xu[0] - p.x; out.d[1] = xu[1] - p.y; out.d[2] = xu[2] - p.z;
     }
-    for (int k = 0; k < 4; ++k) out.pad[k] = 0.0;
     pd[i] = out;
     PointDyn32 o32;
+    o32.x0[0] = p.x; o32.x0[1] = p.y; o32.x0[2] = p.z;
+    o32.A = (float)p.w;
     for (int k = 0; k < 9; ++k) o32.K[k] = (float)out.K[k];
     for (int k = 0; k < 3; ++k) o32.d[k] = (float)out.d[k];
     pd32[i] = o32;
@@ -281,7 +295,7 @@ __global__ void __launch_bounds__(256) k_exact_fwd(int nvol, const double4 *__re
             const double4 s = ps[i];
             const PointDyn &q = pd[i];
             const double rx = v.x - s.x, ry = v.y - s.y, rz = v.z - s.z;
-            const double lo = P.Ldef / sqrt(rx * rx + ry * ry + rz * rz + EPS_W);
+            const double lo = P.Ldef * rsqrt_nr(rx * rx + ry * ry + rz * rz + EPS_W);
             const double w = s.w * wfun<EXPM>(lo, P);
             n0 += w * (q.K[0] * rx + q.K[1] * ry + q.K[2] * rz + q.d[0]);
             n1 += w * (q.K[3] * rx + q.K[4] * ry + q.K[5] * rz + q.d[1]);
@@ -303,7 +317,7 @@ __global__ void __launch_bounds__(256) k_exact_fwd(int nvol, const double4 *__re
                 const double d2 = rx * rx + ry * ry + rz * rz;
                 if (__any(valid && d2 < T2)) {
                     const PointDyn &q = pd[i];
-                    const double lo = P.Ldef / sqrt(d2 + EPS_W);
+                    const double lo = P.Ldef * rsqrt_nr(d2 + EPS_W);
                     const double w = s.w * wfun<EXPM>(lo, P);
                     n0 += w * (q.K[0] * rx + q.K[1] * ry + q.K[2] * rz + q.d[0]);
                     n1 += w * (q.K[3] * rx + q.K[4] * ry + q.K[5] * rz + q.d[1]);
@@ -365,12 +379,12 @@ __global__ void __launch_bounds__(256) k_exact_rev(int nvol, int chunk, const do
         const double rx = v.x - s.x, ry = v.y - s.y, rz = v.z - s.z;
         double w;
         if (PREC == IDW_PREC_FP64) {
-            const double lo = P.Ldef / sqrt(rx * rx + ry * ry + rz * rz + EPS_W);
+            const double lo = P.Ldef * rsqrt_nr(rx * rx + ry * ry + rz * rz + EPS_W);
             w = s.w * wfun<EXPM>(lo, P);
         } else {
             const double d2 = rx * rx + ry * ry + rz * rz;
             if (__any(valid && d2 < P.nearT2)) {
-                const double lo = P.Ldef / sqrt(d2 + EPS_W);
+                const double lo = P.Ldef * rsqrt_nr(d2 + EPS_W);
                 w = s.w * wfun<EXPM>(lo, P);
             } else {
                 const float fx = (float)rx, fy = (float)ry, fz = (float)rz;
@@ -416,8 +430,6 @@ __device__ __forceinline__ double get_error(const NodeRec *nd, double dOvrR) {
 
 __device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
 
-// Forward walk (evalNode, kd_tree.F90:619-662; leaves evalNodeExact,
-// accepted clusters evalNodeApprox, :692-713).
 // Wave-level record of the walk: every bucket the wave evaluated and every
 // cluster some lane accepted, with the mask of lanes concerned, in depth-first
 // order.  Slots [off, end) of the wave come from the counting pass; writes
@@ -426,14 +438,14 @@ struct PlanWriter {
     PlanEntry *plan;
     int64_t at, end;
     unsigned int *mismatch;
-    __device__ __forceinline__ void put(int node, unsigned long long mask) {
+    __device__ __forceinline__ void put(int node, unsigned long long mask, int flags) {
         if (!plan) return;
         if (at < end) {
             if ((threadIdx.x & 63) == 0) {
                 PlanEntry e;
                 e.mask = mask;
                 e.node = node;
-                e.pad = 0;
+                e.flags = flags;
                 plan[at] = e;
             }
         } else if ((threadIdx.x & 63) == 0) {
@@ -451,95 +463,98 @@ __device__ __forceinline__ NodeHdr load_hdr(const NodeRec *nodes, int node) {
 }
 
 // Bucket contribution (evalNodeExact over the bucket's points) to one lane.
+// Point records are fetched with vector loads of a wave-uniform address
+// (index + (lane & zero)): those return in order, so the next point's record
+// is in flight while the current one is evaluated without blocking the
+// wave's scalar loads.
 template <int PREC, int EXPM>
-__device__ __forceinline__ void bucket_fwd(const double4 &v, bool active, int l, int u,
-                                           const double4 *__restrict__ ps, const PointDyn *__restrict__ pd,
-                                           const PointDyn32 *__restrict__ pd32, const WParams &P, double &n0,
-                                           double &n1, double &n2, double &dd, bool fp64) {
+__device__ __forceinline__ void bucket_fwd(const double4 &v, bool active, int l, int u, int lz,
+                                           const PointDyn *__restrict__ pd, const PointDyn32 *__restrict__ pd32,
+                                           const WParams &P, double &n0, double &n1, double &n2, double &dd,
+                                           bool fp64) {
     if (PREC == IDW_PREC_FP64 || fp64) {
-        double4 s = ps[l];
+        PointDyn q = pd[l];
         for (int i = l; i <= u; ++i) {
-            const double4 sn = ps[min(i + 1, u)];  // next point, in flight during this one
-            const PointDyn &q = pd[i];
-            const double rx = v.x - s.x, ry = v.y - s.y, rz = v.z - s.z;
-            const double lo = P.Ldef / sqrt(rx * rx + ry * ry + rz * rz + EPS_W);
-            const double w = active ? s.w * wfun<EXPM>(lo, P) : 0.0;
+            const PointDyn qn = pd[min(i + 1, u)];
+            const double rx = v.x - q.x0[0], ry = v.y - q.x0[1], rz = v.z - q.x0[2];
+            const double lo = P.Ldef * rsqrt_nr(rx * rx + ry * ry + rz * rz + EPS_W);
+            const double w = active ? q.A * wfun<EXPM>(lo, P) : 0.0;
             n0 += w * (q.K[0] * rx + q.K[1] * ry + q.K[2] * rz + q.d[0]);
             n1 += w * (q.K[3] * rx + q.K[4] * ry + q.K[5] * rz + q.d[1]);
             n2 += w * (q.K[6] * rx + q.K[7] * ry + q.K[8] * rz + q.d[2]);
             dd += w;
-            s = sn;
+            q = qn;
         }
     } else {
         float a0 = 0.f, a1 = 0.f, a2 = 0.f, ad = 0.f;
         const float Lf = (float)P.Ldef;
-        double4 s = ps[l];
+        PointDyn32 q = pd32[l];
         for (int i = l; i <= u; ++i) {
-            const double4 sn = ps[min(i + 1, u)];
-            const PointDyn32 &q = pd32[i];
-            const float fx = (float)(v.x - s.x), fy = (float)(v.y - s.y), fz = (float)(v.z - s.z);
+            const PointDyn32 qn = pd32[min(i + 1, u)];
+            const float fx = (float)(v.x - q.x0[0]), fy = (float)(v.y - q.x0[1]), fz = (float)(v.z - q.x0[2]);
             const float lo = Lf * __frsqrt_rn(fx * fx + fy * fy + fz * fz);
-            const float w = active ? (float)s.w * wfun<EXPM>(lo, P) : 0.f;
+            const float w = active ? q.A * wfun<EXPM>(lo, P) : 0.f;
             a0 += w * (q.K[0] * fx + q.K[1] * fy + q.K[2] * fz + q.d[0]);
             a1 += w * (q.K[3] * fx + q.K[4] * fy + q.K[5] * fz + q.d[1]);
             a2 += w * (q.K[6] * fx + q.K[7] * fy + q.K[8] * fz + q.d[2]);
             ad += w;
-            s = sn;
+            q = qn;
         }
         n0 += a0; n1 += a1; n2 += a2; dd += ad;
     }
 }
 
-// Forward walk (evalNode, kd_tree.F90:619-662; leaves evalNodeExact,
-// accepted clusters evalNodeApprox, :692-713).  The headers of both
-// possible successors (node+1 when a lane descends, skip otherwise) are
-// requested before the current node is evaluated.
-template <int PREC, int EXPM>
-__global__ void __launch_bounds__(256) k_fast_fwd(int nvol, const double4 *__restrict__ vol,
-                                                  const int32_t *__restrict__ perm, int nnodes,
-                                                  const NodeRec *__restrict__ nodes, const NodeDyn *__restrict__ nd,
-                                                  const NodeDyn32 *__restrict__ nd32,
-                                                  const double4 *__restrict__ ps, const PointDyn *__restrict__ pd,
-                                                  const PointDyn32 *__restrict__ pd32, WParams P,
-                                                  double *__restrict__ xv, double *__restrict__ den,
-                                                  PlanEntry *__restrict__ plan, const int64_t *__restrict__ waveOff,
-                                                  unsigned int *__restrict__ mismatch) {
+// The walk of evalNode (kd_tree.F90:619-662) for 64 volume nodes per wave:
+// records every bucket the wave evaluates and every cluster some lane
+// accepts (evalNodeApprox, :692-713), with the lane masks, in depth-first
+// order.  The decisions use only init-time data (centroids, radii, error
+// tables, denominator0) and errTol.  COUNT: entries per wave only.
+// Bucket entries carry PLAN_FP64 when an active lane lies within
+// sqrt(nearT2) of the bucket's bounding sphere (mixed precision keeps those
+// pairs in FP64).  The headers of both possible successors (node+1 when a
+// lane descends, skip otherwise) are requested before the current node is
+// evaluated.
+template <bool COUNT>
+__global__ void __launch_bounds__(256) k_plan_walk(int nvol, const double4 *__restrict__ vol, int nnodes,
+                                                   const NodeRec *__restrict__ nodes, double errTol, double nearT2,
+                                                   int64_t *__restrict__ waveCount, PlanEntry *__restrict__ plan,
+                                                   const int64_t *__restrict__ waveOff,
+                                                   unsigned int *__restrict__ mismatch) {
     const int nb = (nvol + 255) / 256;
-    const int blk = xcd_block(nb);
+    const int blk = COUNT ? (int)blockIdx.x : xcd_block(nb);
     if (blk >= nb) return;
     const int k = blk * 256 + threadIdx.x;
     const bool valid = k < nvol;
     const double4 v = valid ? vol[k] : make_double4(0.0, 0.0, 0.0, 1.0);
-    const double tol = P.errTol * v.w;  // errTol * denominator0
+    const double tol = errTol * v.w;  // errTol * denominator0
     int stop = valid ? INT32_MAX : -1;  // depth at which this lane stopped descending
-    double n0 = 0.0, n1 = 0.0, n2 = 0.0, dd = 0.0;
-    const int64_t wave = (int64_t)blk * 4 + (threadIdx.x >> 6);
-    PlanWriter pw{plan, plan ? waveOff[wave] : 0, plan ? waveOff[wave + 1] : 0, mismatch};
+    const int wave = uniform(blk * 4 + (threadIdx.x >> 6));
+    PlanWriter pw{COUNT ? nullptr : plan, COUNT ? 0 : waveOff[wave], COUNT ? 0 : waveOff[wave + 1], mismatch};
+    int64_t cnt = 0;
     int node = 0;
     NodeHdr h = load_hdr(nodes, 0);
     for (;;) {
         const NodeHdr hA = load_hdr(nodes, min(node + 1, nnodes - 1));
         const NodeHdr hB = load_hdr(nodes, min(h.skip, nnodes - 1));
         const bool active = h.depth <= stop;
+        const double cx = v.x - h.X[0], cy = v.y - h.X[1], cz = v.z - h.X[2];
         int next;
         if (h.leaf) {
-            pw.put(node, __ballot(active));
-            bool fp64 = false;
-            if (PREC == IDW_PREC_MIXED) {
-                // FP32 bucket unless an active lane is within nearWallFactor*Ldef
-                // of the bucket's bounding sphere
-                const double cx = v.x - h.X[0], cy = v.y - h.X[1], cz = v.z - h.X[2];
-                const double dc = sqrt(cx * cx + cy * cy + cz * cz) - 1.0 / h.orad;
-                fp64 = __any(active && dc * fabs(dc) < P.nearT2);
+            if (COUNT) {
+                ++cnt;
+            } else {
+                int flags = 0;
+                if (nearT2 > 0.0) {
+                    const double dc = sqrt(cx * cx + cy * cy + cz * cz) - 1.0 / h.orad;
+                    flags = __any(active && dc * fabs(dc) < nearT2) ? PLAN_FP64 : 0;
+                }
+                pw.put(node, __ballot(active), flags);
             }
-            bucket_fwd<PREC, EXPM>(v, active, h.l, h.u, ps, pd, pd32, P, n0, n1, n2, dd, fp64);
             next = h.skip;
         } else {
-            const double cx = v.x - h.X[0], cy = v.y - h.X[1], cz = v.z - h.X[2];
-            const double dist = sqrt(cx * cx + cy * cy + cz * cz);
             bool descend = false, accept = false;
             if (active) {
-                const double dOvrR = dist * h.orad;
+                const double dOvrR = sqrt(cx * cx + cy * cy + cz * cz) * h.orad;
                 if (dOvrR < 2.0) descend = true;
                 else if (get_error(nodes + node, dOvrR) < tol) accept = true;
                 else descend = true;
@@ -547,25 +562,8 @@ __global__ void __launch_bounds__(256) k_fast_fwd(int nvol, const double4 *__res
             }
             const unsigned long long acc = __ballot(accept);
             if (acc) {
-                pw.put(node, acc);
-                if (accept) {
-                    if (PREC == IDW_PREC_FP64) {
-                        const NodeDyn &q = nd[node];
-                        const double w = h.Ai * wfun<EXPM>(P.Ldef / dist, P);
-                        n0 += w * (q.K[0] * cx + q.K[1] * cy + q.K[2] * cz + q.e[0]);
-                        n1 += w * (q.K[3] * cx + q.K[4] * cy + q.K[5] * cz + q.e[1]);
-                        n2 += w * (q.K[6] * cx + q.K[7] * cy + q.K[8] * cz + q.e[2]);
-                        dd += w;
-                    } else {
-                        const NodeDyn32 &q = nd32[node];
-                        const float fx = (float)cx, fy = (float)cy, fz = (float)cz;
-                        const float w = (float)h.Ai * wfun<EXPM>((float)P.Ldef * __frcp_rn((float)dist), P);
-                        n0 += (double)(w * (q.K[0] * fx + q.K[1] * fy + q.K[2] * fz + q.e[0]));
-                        n1 += (double)(w * (q.K[3] * fx + q.K[4] * fy + q.K[5] * fz + q.e[1]));
-                        n2 += (double)(w * (q.K[6] * fx + q.K[7] * fy + q.K[8] * fz + q.e[2]));
-                        dd += (double)w;
-                    }
-                }
+                if (COUNT) ++cnt;
+                else pw.put(node, acc, 0);
             }
             next = __any(descend) ? node + 1 : h.skip;
         }
@@ -574,7 +572,145 @@ __global__ void __launch_bounds__(256) k_fast_fwd(int nvol, const double4 *__res
         h = (next == node + 1) ? hA : hB;
         node = next;
     }
-    pw.finish();
+    if (COUNT) {
+        if ((threadIdx.x & 63) == 0) waveCount[wave] = cnt;
+    } else {
+        pw.finish();
+    }
+}
+
+// Mixed precision keeps FP64 for buckets that an active lane approaches
+// within sqrt(nearT2).  This walk flags those lanes (the wall layer and
+// anything closer than nearT) so that they can be given waves of their
+// own: key = near bit above the 63-bit Morton code of the node.
+__global__ void __launch_bounds__(256) k_near_keys(int nvol, const double4 *__restrict__ vol, int nnodes,
+                                                   const NodeRec *__restrict__ nodes, double errTol, double nearT2,
+                                                   double3 lo, double3 scale, uint64_t *__restrict__ keys,
+                                                   int32_t *__restrict__ ids) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    const bool valid = k < nvol;
+    const double4 v = valid ? vol[k] : make_double4(0.0, 0.0, 0.0, 1.0);
+    const double tol = errTol * v.w;
+    int stop = valid ? INT32_MAX : -1;
+    bool near = false;
+    int node = 0;
+    NodeHdr h = load_hdr(nodes, 0);
+    for (;;) {
+        const NodeHdr hA = load_hdr(nodes, min(node + 1, nnodes - 1));
+        const NodeHdr hB = load_hdr(nodes, min(h.skip, nnodes - 1));
+        const bool active = h.depth <= stop;
+        const double cx = v.x - h.X[0], cy = v.y - h.X[1], cz = v.z - h.X[2];
+        int next;
+        if (h.leaf) {
+            const double dc = sqrt(cx * cx + cy * cy + cz * cz) - 1.0 / h.orad;
+            near = near || (active && dc * fabs(dc) < nearT2);
+            next = h.skip;
+        } else {
+            bool descend = false;
+            if (active) {
+                const double dOvrR = sqrt(cx * cx + cy * cy + cz * cz) * h.orad;
+                if (dOvrR < 2.0) descend = true;
+                else if (!(get_error(nodes + node, dOvrR) < tol)) descend = true;
+                stop = descend ? INT32_MAX : h.depth;
+            }
+            next = __any(descend) ? node + 1 : h.skip;
+        }
+        next = uniform(next);
+        if (next >= nnodes) break;
+        h = (next == node + 1) ? hA : hB;
+        node = next;
+    }
+    if (!valid) return;
+    auto spread = [](uint64_t x) {
+        x &= 0x1fffff;
+        x = (x | x << 32) & 0x1f00000000ffffull;
+        x = (x | x << 16) & 0x1f0000ff0000ffull;
+        x = (x | x << 8) & 0x100f00f00f00f00full;
+        x = (x | x << 4) & 0x10c30c30c30c30c3ull;
+        x = (x | x << 2) & 0x1249249249249249ull;
+        return x;
+    };
+    auto q = [](double t) { return (uint64_t)fmin(fmax(t, 0.0), 2097151.0); };
+    keys[k] = ((uint64_t)near << 63) | spread(q((v.x - lo.x) * scale.x)) | (spread(q((v.y - lo.y) * scale.y)) << 1) |
+              (spread(q((v.z - lo.z) * scale.z)) << 2);
+    ids[k] = k;
+}
+
+__global__ void k_permute_vol(int64_t nvol, const int32_t *__restrict__ order, const double4 *__restrict__ vin,
+                              const int32_t *__restrict__ pin, double4 *__restrict__ vout, int32_t *__restrict__ pout) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nvol) return;
+    const int32_t o = order[k];
+    vout[k] = vin[o];
+    pout[k] = pin[o];
+}
+
+// Forward sums replaying the plan: accepted clusters add
+// Ai w(dist) (Kbar (x - X) + e) (evalNodeApprox), buckets the exact terms
+// of their points (evalNodeExact); then Xv = Xv0 + num / den
+// (warpMesh.F90:65-70).  Entries are fetched two ahead, headers one ahead.
+template <int PREC, int EXPM>
+__global__ void __launch_bounds__(256) k_fast_fwd(int nvol, const double4 *__restrict__ vol,
+                                                  const int32_t *__restrict__ perm, const PlanEntry *__restrict__ plan,
+                                                  const int64_t *__restrict__ waveOff, const NodeRec *__restrict__ nodes,
+                                                  const NodeDyn *__restrict__ nd, const NodeDyn32 *__restrict__ nd32,
+                                                  const double4 *__restrict__ ps, const PointDyn *__restrict__ pd,
+                                                  const PointDyn32 *__restrict__ pd32, WParams P,
+                                                  double *__restrict__ xv, double *__restrict__ den) {
+    const int nb = (nvol + 255) / 256;
+    const int blk = xcd_block(nb);
+    if (blk >= nb) return;
+    const int k = blk * 256 + threadIdx.x;
+    const bool valid = k < nvol;
+    const int wave = uniform(blk * 4 + (threadIdx.x >> 6));
+    const double4 v = valid ? vol[k] : make_double4(0.0, 0.0, 0.0, 1.0);
+    const int lane = threadIdx.x & 63, lz = lane & P.zero;
+    double n0 = 0.0, n1 = 0.0, n2 = 0.0, dd = 0.0;
+    const int64_t e0 = waveOff[wave], e1 = waveOff[wave + 1];
+    if (e0 < e1) {
+        PlanEntry pe = plan[e0];
+        PlanEntry pn = plan[min(e0 + 1, e1 - 1)];
+        NodeHdr h = load_hdr(nodes, pe.node);
+        NodeDyn32 c32 = nd32[pe.node];  // cluster record (FP32 path), one entry ahead
+        for (int64_t e = e0; e < e1; ++e) {
+            const PlanEntry pnn = plan[min(e + 2, e1 - 1)];
+            const NodeHdr hn = load_hdr(nodes, pn.node);
+            NodeDyn32 c32n;
+            if (PREC == IDW_PREC_MIXED) c32n = nd32[pn.node];
+            const bool active = (pe.mask >> lane) & 1ull;
+            if (h.leaf) {
+                bucket_fwd<PREC, EXPM>(v, active, h.l, h.u, lz, pd, pd32, P, n0, n1, n2, dd,
+                                       (pe.flags & PLAN_FP64) != 0);
+            } else {
+                const double cx = v.x - h.X[0], cy = v.y - h.X[1], cz = v.z - h.X[2];
+                const double d2 = cx * cx + cy * cy + cz * cz;
+                if (PREC == IDW_PREC_FP64) {
+                    const NodeDyn q = nd[pe.node];
+                    if (active) {
+                        const double w = h.Ai * wfun<EXPM>(P.Ldef * rsqrt_nr(d2), P);
+                        n0 += w * (q.K[0] * cx + q.K[1] * cy + q.K[2] * cz + q.e[0]);
+                        n1 += w * (q.K[3] * cx + q.K[4] * cy + q.K[5] * cz + q.e[1]);
+                        n2 += w * (q.K[6] * cx + q.K[7] * cy + q.K[8] * cz + q.e[2]);
+                        dd += w;
+                    }
+                } else {
+                    const NodeDyn32 &q = c32;
+                    if (active) {
+                        const float fx = (float)cx, fy = (float)cy, fz = (float)cz;
+                        const float w = (float)h.Ai * wfun<EXPM>((float)P.Ldef * __frsqrt_rn((float)d2), P);
+                        n0 += (double)(w * (q.K[0] * fx + q.K[1] * fy + q.K[2] * fz + q.e[0]));
+                        n1 += (double)(w * (q.K[3] * fx + q.K[4] * fy + q.K[5] * fz + q.e[1]));
+                        n2 += (double)(w * (q.K[6] * fx + q.K[7] * fy + q.K[8] * fz + q.e[2]));
+                        dd += (double)w;
+                    }
+                }
+            }
+            pe = pn;
+            pn = pnn;
+            h = hn;
+            if (PREC == IDW_PREC_MIXED) c32 = c32n;
+        }
+    }
     if (valid) {
         const int64_t j = perm[k];
         const double oden = 1.0 / dd;
@@ -618,9 +754,16 @@ __device__ __forceinline__ void flush_rows(const double *W, const double *B, dou
                                            const double (*dd)[4], int nrow) {
     wave_sync();
     const int lane = threadIdx.x & 63, i = lane & 15, g = lane >> 4;
-    dvec4 acc = {0.0, 0.0, 0.0, 0.0};
+    // four independent accumulation chains keep the matrix pipe busy
+    dvec4 a0 = {0.0, 0.0, 0.0, 0.0}, a1 = a0, a2 = a0, a3 = a0;
 #pragma unroll
-    for (int q = 0; q < 16; ++q) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(W[i * WROW + 4 * q + g], B[q], acc, 0, 0, 0);
+    for (int q = 0; q < 16; q += 4) {
+        a0 = __builtin_amdgcn_mfma_f64_16x16x4f64(W[i * WROW + 4 * q + g], B[q], a0, 0, 0, 0);
+        a1 = __builtin_amdgcn_mfma_f64_16x16x4f64(W[i * WROW + 4 * (q + 1) + g], B[q + 1], a1, 0, 0, 0);
+        a2 = __builtin_amdgcn_mfma_f64_16x16x4f64(W[i * WROW + 4 * (q + 2) + g], B[q + 2], a2, 0, 0, 0);
+        a3 = __builtin_amdgcn_mfma_f64_16x16x4f64(W[i * WROW + 4 * (q + 3) + g], B[q + 3], a3, 0, 0, 0);
+    }
+    const dvec4 acc = (a0 + a1) + (a2 + a3);
     const int j = i;
     const int src = (lane & ~15) | (9 + (j < 9 ? j / 3 : 0));
 #pragma unroll
@@ -682,6 +825,7 @@ __global__ void __launch_bounds__(256) k_fast_rev(int nvol, const double4 *__res
     for (int q = 0; q < 16; ++q) B[q] = W[(4 * q + (lane >> 4)) * GROW + (lane & 15)];
     wave_sync();
     const float Lf = (float)P.Ldef;
+    const int lz = lane & P.zero;
     int nrow = 0;
     const int64_t e0 = waveOff[wave], e1 = waveOff[wave + 1];
     if (e0 < e1) {
@@ -699,17 +843,12 @@ __global__ void __launch_bounds__(256) k_fast_rev(int nvol, const double4 *__res
                     const double4 pnext = ps[min(i + 1, h.u)];
                     const double rx = v.x - p.x, ry = v.y - p.y, rz = v.z - p.z;
                     double w;
-                    if (PREC == IDW_PREC_FP64) {
-                        w = active ? p.w * wfun<EXPM>(P.Ldef / sqrt(rx * rx + ry * ry + rz * rz + EPS_W), P) : 0.0;
+                    if (PREC == IDW_PREC_FP64 || (pe.flags & PLAN_FP64)) {
+                        w = active ? p.w * wfun<EXPM>(P.Ldef * rsqrt_nr(rx * rx + ry * ry + rz * rz + EPS_W), P) : 0.0;
                     } else {
-                        const double d2 = rx * rx + ry * ry + rz * rz;
-                        if (__any(active && d2 < P.nearT2)) {
-                            w = active ? p.w * wfun<EXPM>(P.Ldef / sqrt(d2 + EPS_W), P) : 0.0;
-                        } else {
-                            const float fx = (float)rx, fy = (float)ry, fz = (float)rz;
-                            w = active ? (double)((float)p.w * wfun<EXPM>(Lf * __frsqrt_rn(fx * fx + fy * fy + fz * fz), P))
-                                       : 0.0;
-                        }
+                        const float fx = (float)rx, fy = (float)ry, fz = (float)rz;
+                        w = active ? (double)((float)p.w * wfun<EXPM>(Lf * __frsqrt_rn(fx * fx + fy * fy + fz * fz), P))
+                                   : 0.0;
                     }
                     W[nrow * WROW + lane] = w;
                     if (lane == 0) {
@@ -726,11 +865,11 @@ __global__ void __launch_bounds__(256) k_fast_rev(int nvol, const double4 *__res
                 }
             } else {
                 const double cx = v.x - h.X[0], cy = v.y - h.X[1], cz = v.z - h.X[2];
-                const double dist = sqrt(cx * cx + cy * cy + cz * cz);
+                const double d2 = cx * cx + cy * cy + cz * cz;
                 double w = 0.0;
                 if (active) {
-                    if (PREC == IDW_PREC_FP64) w = h.Ai * wfun<EXPM>(P.Ldef / dist, P);
-                    else w = (double)((float)h.Ai * wfun<EXPM>(Lf * __frcp_rn((float)dist), P));
+                    if (PREC == IDW_PREC_FP64) w = h.Ai * wfun<EXPM>(P.Ldef * rsqrt_nr(d2), P);
+                    else w = (double)((float)h.Ai * wfun<EXPM>(Lf * __frsqrt_rn((float)d2), P));
                 }
                 W[nrow * WROW + lane] = w;
                 if (lane == 0) {
@@ -750,49 +889,6 @@ __global__ void __launch_bounds__(256) k_fast_rev(int nvol, const double4 *__res
         }
     }
     if (nrow) flush_rows(W, B, sDst[wv], sD[wv], nrow);
-}
-
-// Counting pass of the plan: entries per wave for the current errTol (same
-// walk and decisions as the forward).
-__global__ void __launch_bounds__(256) k_plan_count(int nvol, const double4 *__restrict__ vol, int nnodes,
-                                                    const NodeRec *__restrict__ nodes, double errTol,
-                                                    int64_t *__restrict__ waveCount) {
-    const int k = blockIdx.x * 256 + threadIdx.x;
-    const bool valid = k < nvol;
-    const double4 v = valid ? vol[k] : make_double4(0.0, 0.0, 0.0, 1.0);
-    const double tol = errTol * v.w;
-    int stop = valid ? INT32_MAX : -1;
-    int64_t cnt = 0;
-    int node = 0;
-    NodeHdr h = load_hdr(nodes, 0);
-    for (;;) {
-        const NodeHdr hA = load_hdr(nodes, min(node + 1, nnodes - 1));
-        const NodeHdr hB = load_hdr(nodes, min(h.skip, nnodes - 1));
-        const bool active = h.depth <= stop;
-        int next;
-        if (h.leaf) {
-            ++cnt;
-            next = h.skip;
-        } else {
-            const double cx = v.x - h.X[0], cy = v.y - h.X[1], cz = v.z - h.X[2];
-            const double dist = sqrt(cx * cx + cy * cy + cz * cz);
-            bool descend = false, accept = false;
-            if (active) {
-                const double dOvrR = dist * h.orad;
-                if (dOvrR < 2.0) descend = true;
-                else if (get_error(nodes + node, dOvrR) < tol) accept = true;
-                else descend = true;
-                stop = descend ? INT32_MAX : h.depth;
-            }
-            if (__any(accept)) ++cnt;
-            next = __any(descend) ? node + 1 : h.skip;
-        }
-        next = uniform(next);
-        if (next >= nnodes) break;
-        h = (next == node + 1) ? hA : hB;
-        node = next;
-    }
-    if ((threadIdx.x & 63) == 0) waveCount[(int64_t)k >> 6] = cnt;
 }
 
 // Per-lane work of the fast walk, summed over all volume nodes: internal
@@ -1195,6 +1291,18 @@ void launch_morton(hipStream_t st, int64_t n, const double *xv0, double lo[3], d
                                                make_double3(scale[0], scale[1], scale[2]), keys, ids);
 }
 
+void launch_near_keys(hipStream_t st, const DevSurface &S, const DevVolume &V, double errTol, double nearT2,
+                      const double lo[3], const double scale[3], uint64_t *keys, int32_t *ids) {
+    k_near_keys<<<grid_for(V.n, 256), 256, 0, st>>>((int)V.n, V.vol, S.nnodes, S.nodes, errTol, nearT2,
+                                                    make_double3(lo[0], lo[1], lo[2]),
+                                                    make_double3(scale[0], scale[1], scale[2]), keys, ids);
+}
+
+void launch_permute_vol(hipStream_t st, int64_t n, const int32_t *order, const double4 *vin, const int32_t *pin,
+                        double4 *vout, int32_t *pout) {
+    k_permute_vol<<<grid_for(n, 256), 256, 0, st>>>(n, order, vin, pin, vout, pout);
+}
+
 void launch_build_vol(hipStream_t st, int64_t n, const double *xv0, const int32_t *perm, double4 *vol) {
     k_build_vol<<<grid_for(n, 256), 256, 0, st>>>(n, xv0, perm, vol);
 }
@@ -1213,15 +1321,12 @@ static void forward_impl(hipStream_t st, const DevSurface &S, const DevVolume &V
         else
             k_exact_fwd<IDW_PREC_MIXED, EXPM><<<g, 256, 0, st>>>((int)V.n, V.vol, V.perm, S.n, S.ps, S.pd, S.pd32, P, xv, V.den);
     } else {
-        PlanEntry *pe = plan ? plan->entries : nullptr;
-        const int64_t *off = plan ? plan->waveOff : nullptr;
-        unsigned int *mm = plan ? plan->mismatch : nullptr;
         if (prec == IDW_PREC_FP64)
-            k_fast_fwd<IDW_PREC_FP64, EXPM><<<g, 256, 0, st>>>((int)V.n, V.vol, V.perm, S.nnodes, S.nodes, S.nd, S.nd32, S.ps,
-                                                        S.pd, S.pd32, P, xv, V.den, pe, off, mm);
+            k_fast_fwd<IDW_PREC_FP64, EXPM><<<g, 256, 0, st>>>((int)V.n, V.vol, V.perm, plan->entries, plan->waveOff,
+                                                              S.nodes, S.nd, S.nd32, S.ps, S.pd, S.pd32, P, xv, V.den);
         else
-            k_fast_fwd<IDW_PREC_MIXED, EXPM><<<g, 256, 0, st>>>((int)V.n, V.vol, V.perm, S.nnodes, S.nodes, S.nd, S.nd32,
-                                                         S.ps, S.pd, S.pd32, P, xv, V.den, pe, off, mm);
+            k_fast_fwd<IDW_PREC_MIXED, EXPM><<<g, 256, 0, st>>>((int)V.n, V.vol, V.perm, plan->entries, plan->waveOff,
+                                                               S.nodes, S.nd, S.nd32, S.ps, S.pd, S.pd32, P, xv, V.den);
     }
 }
 
@@ -1230,7 +1335,14 @@ void launch_prep_numb(hipStream_t st, const DevVolume &V, const double *dxv) {
 }
 
 void launch_plan_count(hipStream_t st, const DevSurface &S, const DevVolume &V, double errTol, int64_t *waveCount) {
-    k_plan_count<<<grid_for(V.n, 256), 256, 0, st>>>((int)V.n, V.vol, S.nnodes, S.nodes, errTol, waveCount);
+    k_plan_walk<true><<<grid_for(V.n, 256), 256, 0, st>>>((int)V.n, V.vol, S.nnodes, S.nodes, errTol, 0.0, waveCount,
+                                                          nullptr, nullptr, nullptr);
+}
+
+void launch_plan_fill(hipStream_t st, const DevSurface &S, const DevVolume &V, double errTol, double nearT2,
+                      const DevPlan &plan) {
+    k_plan_walk<false><<<xcd_grid(V.n), 256, 0, st>>>((int)V.n, V.vol, S.nnodes, S.nodes, errTol, nearT2, nullptr,
+                                                      plan.entries, plan.waveOff, plan.mismatch);
 }
 
 template <int EXPM>

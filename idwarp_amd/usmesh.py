@@ -42,7 +42,8 @@ _DEFAULTS = {
     "bucketSize": [int, 8],
     # device-only
     "precision": [str, ["fp64", "mixed"]],
-    "nearWallFactor": [float, 1e-3],
+    "nearWallFactor": [float, 1e-6],
+    "reusePlan": [bool, False],
     "device": [int, 0],
 }
 
@@ -116,6 +117,7 @@ class USMesh:
         o.bucketSize = int(g("bucketSize"))
         o.precision = _lib.PREC_MIXED if g("precision") == "mixed" else _lib.PREC_FP64
         o.nearWallFactor = g("nearWallFactor")
+        o.reusePlan = int(bool(g("reusePlan")))
         return o
 
     def _setMeshOptions(self):
@@ -274,8 +276,8 @@ class USMesh:
         return d
 
     def getTimings(self):
-        """ms of the last warp/deriv phases: [nodal+cluster, forward kernel,
-        reverse set-up, reverse kernel, nodal reverse, unused]."""
+        """ms of the last warp/deriv phases: [nodal+cluster, tree walk,
+        forward kernel, reverse set-up, reverse kernel, nodal reverse]."""
         t = np.zeros(6)
         _lib.check(self._L.idw_get_timings(self._ctx, _lib.ptr(t)))
         return t

@@ -55,8 +55,11 @@ typedef struct idw_options {
     int32_t zeroCornerRotations; /* 1 */
     int32_t bucketSize;     /* 8 (kd_tree.F90:14 default 18 is overridden by USMesh) */
     int32_t precision;      /* IDW_PREC_FP64 or IDW_PREC_MIXED (device only) */
-    int32_t reserved;
-    double nearWallFactor;  /* mixed mode: pairs closer than nearWallFactor*Ldef stay FP64 (1e-3) */
+    int32_t reusePlan;      /* fast mode: 0 = walk the tree in every warp (as the reference does);
+                               1 = keep the walk's plan while errTol, the grid and the surface
+                               definition are unchanged (its decisions depend on nothing else) */
+    double nearWallFactor;  /* mixed mode: buckets with a volume node closer than
+                               nearWallFactor*Ldef stay FP64 (1e-6: keeps q^5 < 1e30 in FP32) */
 } idw_options;
 
 typedef struct idw_ctx idw_ctx;
@@ -143,10 +146,11 @@ int idw_get_denominator0(idw_ctx *ctx, double *den0, int64_t n);
 
 /* Per-phase timing of the last warp / warp_deriv (host or _dev form),
  * measured with HIP events on the context's stream, in milliseconds:
- * [0] nodal properties + cluster data, [1] forward volume kernel,
- * [2] reverse set-up (nodal properties, seed, zero fill), [3] reverse
- * volume kernel(s), [4] nodal reverse, [5] unused.  Synchronises the
- * context's stream.  No counterpart in the reference (which has no timers). */
+ * [0] nodal properties + cluster data, [1] tree walk recording the plan
+ * (fast mode; 0 when reused or exact), [2] forward volume kernel,
+ * [3] reverse set-up (nodal properties, zero fill), [4] reverse volume
+ * kernel(s), [5] nodal reverse.  Synchronises the context's stream.  No
+ * counterpart in the reference (which has no timers). */
 int idw_get_timings(idw_ctx *ctx, double *ms6);
 
 /* Work of the fast mode for the current options.  Per volume node, summed

@@ -147,3 +147,25 @@ def test_solver_grid_and_errors():
     from idwarp_amd import Error
     with pytest.raises(Error):
         gpu.setSurfaceCoordinates(m["pts"][:-1])
+
+
+def test_plan_reuse_and_errtol_change():
+    """reusePlan keeps the walk's plan only while errTol is unchanged; results
+    equal the per-call walk and the oracle at each tolerance."""
+    m = MESHES["star"]
+    ora, gpu = _pair(m, evalMode="fast", errTol=1e-4, reusePlan=True)
+    xs = m["perturb"](m["pts"])
+    for tol in (1e-4, 1e-3, 1e-5, 1e-4):
+        ora.setOptions(errTol=tol)
+        gpu.setOption("errTol", tol)
+        for shift in (0.0, 0.01):
+            x = xs + shift
+            ora.setSurfaceCoordinates(x)
+            gpu.setSurfaceCoordinates(x)
+            ref = ora.warpMesh()
+            gpu.warpMesh()
+            assert np.abs(gpu.getWarpGrid() - ref).max() <= FP64_TOL * ora.Ldef0, (tol, shift)
+            dxv = synthetic.random_dxv(len(m["xv0"]), seed=3, frac=0.5)
+            dref = ora.warpDeriv(dxv)
+            gpu.warpDeriv(dxv, solverVec=False)
+            assert _rel(gpu.getdXs().ravel(), dref) <= 1e-10, (tol, shift)
