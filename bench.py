@@ -88,10 +88,14 @@ def cpu_baseline(m, xs, dxv, errtol, target_s):
         sd, ld = O.last_times()
         return sw + sd, lw + ld
 
+    # grow the sample until its volume loops take about target_s (small
+    # samples overstate the per-node cost: thread start-up, cold caches)
     k = min(nvol, 2000)
     setup, loop = run(k)
-    if loop > 0 and k < nvol:
-        k = int(min(nvol, max(k, k * target_s / loop)))
+    for _ in range(4):
+        if loop >= 0.6 * target_s or k >= nvol:
+            break
+        k = int(min(nvol, max(2 * k, k * target_s / max(loop, 1e-3))))
         setup, loop = run(k)
     ms = 1e3 * (setup + loop * nvol / k)
     return dict(value=round(ms, 1), unit="ms per warpMesh+warpDeriv (full mesh, extrapolated)", cores=cores,
@@ -204,16 +208,25 @@ def main():
         res["mixed_adjoint_rel_err"] = float(((a["dxs"] - b["dxs"]).norm() / b["dxs"].norm()).item())
     # roofline of the dominant volume kernel of the headline leg
     ph = head["phases"]
-    cand = [("k_plan_walk", ph[1], kflops["walk"]), ("k_fast_fwd<%s>" % head_prec, ph[2], kflops["fwd"]),
-            ("k_fast_rev<%s>" % head_prec, ph[4], kflops["rev"])]
+    # kernel names as rocprofv3 reports them: <precision, exponent mode>,
+    # exponent mode 0 = the default exponents (3, 5) compiled as products
+    pi = 1 if head_prec == "mixed" else 0
+    cand = [("idw::k_plan_walk<false>", ph[1], kflops["walk"]), ("idw::k_fast_fwd<%d, 0>" % pi, ph[2], kflops["fwd"]),
+            ("idw::k_fast_rev<%d, 0>" % pi, ph[4], kflops["rev"])]
     kname, kms, kfl = max(cand, key=lambda t: t[1])
     achieved = kfl / (kms * 1e-3) / 1e12
     res["roofline"] = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                       "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": _traffic(kname),
+                       "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+                       "traffic": _traffic_record(kname)["bytes"] if _traffic_record(kname) else None,
                        "kernel": kname, "kernel_ms": round(float(kms), 3),
                        "algorithmic_flops_per_launch": kfl,
                        "compute": "VALU FP64/FP32 (no MFMA): the 78.6 TFLOP/s FP64 vector roof equals the FP64 "
                                   "matrix roof and non-packed FP32 issues at the same rate"}
+    tr = _traffic_record(kname)
+    if tr:
+        res["roofline"]["traffic_detail"] = {"fetch_bytes": tr["fetch_bytes"], "write_bytes": tr["write_bytes"],
+                                             "hbm_GBps": round(tr["bytes"] / (kms * 1e-3) / 1e9, 1),
+                                             "source": "profiles/traffic.json"}
     res["phases_ms"] = {p: dict(zip(("nodal", "walk", "fwd", "rev_setup", "rev", "nodal_rev"),
                                     [round(float(x), 3) for x in out[p]["phases"]])) for p in out}
     res["walk_lane_utilisation"] = {k: round(float(v), 3) for k, v in util.items()}
@@ -231,15 +244,17 @@ def _unique(mesh):
     return n.value
 
 
-def _traffic(kname):
-    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE +
-    WRITE_SIZE, scaled by a copy-kernel calibration), if recorded."""
+def _traffic_record(kname):
+    """HBM bytes per launch of a kernel measured with rocprofv3 --pmc
+    (FETCH_SIZE, then WRITE_SIZE, each scaled by what the counter reads for a
+    1 GiB device copy), as recorded in profiles/traffic.json, or None."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
-            return json.load(f).get(kname)
+            rec = json.load(f).get(kname)
     except (OSError, ValueError):
         return None
+    return rec if isinstance(rec, dict) and "bytes" in rec else None
 
 
 if __name__ == "__main__":

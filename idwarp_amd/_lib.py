@@ -60,6 +60,16 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError("%s is missing: build it with idwarp_amd.build.build_library()" % LIB_PATH)
+    # torch ships its own HIP runtime (torch/lib/libamdhip64.so, soname
+    # libamdhip64.so.7) and links it by the file name libamdhip64.so.  If this
+    # library were loaded first, /opt/rocm's runtime would be in the process
+    # and torch would load a second one, which then finds no device.  With
+    # torch loaded first this library binds to torch's runtime by soname and
+    # the process has a single HIP runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     vp, i64, i32, d = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_double
     P = ctypes.POINTER

@@ -140,8 +140,9 @@ int idw_warp_deriv_dev(idw_ctx *ctx, const double *d_xs, const double *d_dxv, do
 int idw_get_sizes(idw_ctx *ctx, int64_t *n_vol, int64_t *n_pts, int64_t *n_unique, int64_t *n_tree_nodes);
 int idw_get_ldef0(idw_ctx *ctx, double *ldef0);
 
-/* Denominator estimate of the fast mode (initializeWarping.F90:240-247),
- * 3 -> 1 per volume node, warp order; n = n_vol. */
+/* Denominator estimate of the fast mode (initializeWarping.F90:240-247,
+ * getWiEstimate, kd_tree.F90:751-787): one value per volume node, warp
+ * order; n = n_vol. */
 int idw_get_denominator0(idw_ctx *ctx, double *den0, int64_t n);
 
 /* Per-phase timing of the last warp / warp_deriv (host or _dev form),
