@@ -51,6 +51,7 @@ typedef struct {
     int left, right;
     double Ai, X[3], oradius, err[NERR];
     double Mi[9], Bi[3], Mib[9], Bib[3];
+    double Mid[9], Bid[3]; /* tangent cluster means (setData_d) */
 } onode;
 
 typedef struct {
@@ -64,6 +65,7 @@ typedef struct {
     int *indexes;   /* tree permutation, 1-based into the_data */
     double *the_data, *Xu0, *Xu, *Ai, *normals0, *normals, *Mi, *Bi;
     double *Mib, *Bib, *Xub, *normalsb;
+    double *Mid, *Bid, *Xud; /* tangent of Mi, Bi, Xu (COMPUTENODALPROPERTIES_D) */
     int *isCorner;
     double Ldef0, Ldef, alphaToBexp, aExp, bExp, errTol;
     double orstar[NERR - 1];
@@ -213,6 +215,82 @@ void orc_rotation_matrix_b(const double *v1, const double *v2, const double *Mb,
     getmag_b(v2, v2b, magv2b);
 }
 
+/* CROSS_PRODUCT_3D_D (vectorUtils_d.f90:138-151), statement order kept */
+static void cross3_d(const double *v1, const double *v1d, const double *v2, const double *v2d, double *c,
+                     double *cd) {
+    cd[0] = v2[2] * v1d[1] + v1[1] * v2d[2] - v2[1] * v1d[2] - v1[2] * v2d[1];
+    c[0] = v1[1] * v2[2] - v1[2] * v2[1];
+    cd[1] = v2[0] * v1d[2] + v1[2] * v2d[0] - v2[2] * v1d[0] - v1[0] * v2d[2];
+    c[1] = v1[2] * v2[0] - v1[0] * v2[2];
+    cd[2] = v2[1] * v1d[0] + v1[0] * v2d[1] - v2[0] * v1d[1] - v1[1] * v2d[0];
+    c[2] = v1[0] * v2[1] - v1[1] * v2[0];
+}
+
+/* GETMAG_D (vectorUtils_d.f90:155-176) */
+static double getmag_d(const double *v, const double *vd, double *magd) {
+    double argd = 2 * v[0] * vd[0] + 2 * v[1] * vd[1] + 2 * v[2] * vd[2];
+    double arg = v[0] * v[0] + v[1] * v[1] + v[2] * v[2] + 1e-30;
+    double t = sqrt(arg);
+    *magd = (arg == 0.0) ? 0.0 : argd / (2.0 * t);
+    return t;
+}
+
+/* GETROTATIONMATRIX3D_D (vectorUtils_d.f90:7-129): M and its tangent Md
+ * for a tangent v2d of v2 (v1 fixed); both column-major. */
+void orc_rotation_matrix_d(const double *v1, const double *v2, const double *v2d, double *M, double *Md) {
+    double magv1 = getmag(v1), magv2d, magv2 = getmag_d(v2, v2d, &magv2d);
+    double v1d[3] = {0, 0, 0}, axis[3], axisd[3], axismagd;
+    cross3_d(v1, v1d, v2, v2d, axis, axisd);
+    double axismag = getmag_d(axis, axisd, &axismagd);
+    double angle, angled;
+    if (axismag < ROT_TOL) {
+        angle = 0.0;
+        axis[0] = 1.0; axis[1] = 0.0; axis[2] = 0.0;
+        axisd[0] = axisd[1] = axisd[2] = 0.0;
+        angled = 0.0;
+    } else {
+        double vv1[3], vv2[3], vv2d[3], arg, argd;
+        for (int k = 0; k < 3; k++) axisd[k] = (axisd[k] - axis[k] * axismagd / axismag) / axismag;
+        for (int k = 0; k < 3; k++) axis[k] = axis[k] / axismag;
+        for (int k = 0; k < 3; k++) vv1[k] = v1[k] / magv1;
+        for (int k = 0; k < 3; k++) vv2d[k] = (v2d[k] - v2[k] * magv2d / magv2) / magv2;
+        for (int k = 0; k < 3; k++) vv2[k] = v2[k] / magv2;
+        if (1.0 > vv1[0] * vv2[0] + vv1[1] * vv2[1] + vv1[2] * vv2[2]) {
+            argd = vv1[0] * vv2d[0] + vv1[1] * vv2d[1] + vv1[2] * vv2d[2];
+            arg = vv1[0] * vv2[0] + vv1[1] * vv2[1] + vv1[2] * vv2[2];
+        } else {
+            arg = 1.0;
+            argd = 0.0;
+        }
+        if (arg == 1.0 || arg == -1.0) angled = 0.0;
+        else angled = -(argd / sqrt(1.0 - arg * arg));
+        angle = acos(arg);
+    }
+    double A[3][3], C[3][3], a[3][3];
+    rot_AC(axis, A, C);
+    /* skew matrix of the axis tangent; its square's tangent as the
+     * reference's statements write it (1-based (r,c) -> [r-1][c-1]) */
+    a[0][0] = 0.0;       a[0][1] = -axisd[2]; a[0][2] = axisd[1];
+    a[1][0] = axisd[2];  a[1][1] = 0.0;       a[1][2] = -axisd[0];
+    a[2][0] = -axisd[1]; a[2][1] = axisd[0];  a[2][2] = 0.0;
+    double c[3][3];
+    c[0][0] = 2 * A[0][0] * a[0][0] + A[1][0] * a[0][1] + A[0][1] * a[1][0] + A[2][0] * a[0][2] + A[0][2] * a[2][0];
+    c[0][1] = A[0][1] * a[0][0] + (A[0][0] + A[1][1]) * a[0][1] + A[0][1] * a[1][1] + A[2][1] * a[0][2] + A[0][2] * a[2][1];
+    c[0][2] = A[0][2] * a[0][0] + (A[0][0] + A[2][2]) * a[0][2] + A[1][2] * a[0][1] + A[0][1] * a[1][2] + A[0][2] * a[2][2];
+    c[1][0] = (A[0][0] + A[1][1]) * a[1][0] + A[1][0] * a[0][0] + A[1][0] * a[1][1] + A[2][0] * a[1][2] + A[1][2] * a[2][0];
+    c[1][1] = A[0][1] * a[1][0] + A[1][0] * a[0][1] + 2 * A[1][1] * a[1][1] + A[2][1] * a[1][2] + A[1][2] * a[2][1];
+    c[1][2] = A[0][2] * a[1][0] + A[1][0] * a[0][2] + A[1][2] * a[1][1] + (A[1][1] + A[2][2]) * a[1][2] + A[1][2] * a[2][2];
+    c[2][0] = (A[0][0] + A[2][2]) * a[2][0] + A[2][0] * a[0][0] + A[1][0] * a[2][1] + A[2][1] * a[1][0] + A[2][0] * a[2][2];
+    c[2][1] = A[0][1] * a[2][0] + A[2][0] * a[0][1] + (A[1][1] + A[2][2]) * a[2][1] + A[2][1] * a[1][1] + A[2][1] * a[2][2];
+    c[2][2] = A[0][2] * a[2][0] + A[2][0] * a[0][2] + A[1][2] * a[2][1] + A[2][1] * a[1][2] + 2 * A[2][2] * a[2][2];
+    double sn = sin(angle), oc = 1.0 - cos(angle), cs = cos(angle);
+    for (int r = 0; r < 3; r++)
+        for (int q = 0; q < 3; q++) {
+            Md[r + 3 * q] = (A[r][q] * cs + C[r][q] * sn) * angled + sn * a[r][q] + oc * c[r][q];
+            M[r + 3 * q] = (r == q ? 1.0 : 0.0) + sn * A[r][q] + oc * C[r][q];
+        }
+}
+
 /* ------------------------------------------------------------------ */
 /* getElementProps.F90:1-46 and GETELEMENTPROPS_B (getElementProps_b.f90) */
 /* points: 3*npts, xyz per point                                      */
@@ -276,6 +354,47 @@ void orc_element_props_b(const double *pts, int npts, double areab, const double
     for (int k = 0; k < 3; k++) centerb[k] = centerb[k] / npts;
     for (int i = npts - 1; i >= 0; i--)
         for (int k = 0; k < 3; k++) pointsb[3 * i + k] = pointsb[3 * i + k] + centerb[k];
+}
+
+/* GETELEMENTPROPS_D (getElementProps_d.f90:7-94): area, unit normal and
+ * their tangents for point tangents ptsd (3*npts). */
+void orc_element_props_d(const double *pts, const double *ptsd, int npts, double *area, double *aread,
+                         double *normal, double *normald) {
+    double center[3] = {0, 0, 0}, centerd[3] = {0, 0, 0};
+    double rv[MAXFACEPTS][3], rvd[MAXFACEPTS][3], cr[3], crd[3];
+    for (int i = 0; i < npts; i++)
+        for (int k = 0; k < 3; k++) {
+            centerd[k] = centerd[k] + ptsd[3 * i + k];
+            center[k] = center[k] + pts[3 * i + k];
+        }
+    for (int k = 0; k < 3; k++) { centerd[k] = centerd[k] / npts; center[k] = center[k] / npts; }
+    for (int i = 0; i < npts; i++)
+        for (int k = 0; k < 3; k++) {
+            rvd[i][k] = ptsd[3 * i + k] - centerd[k];
+            rv[i][k] = pts[3 * i + k] - center[k];
+        }
+    for (int k = 0; k < 3; k++) normal[k] = normald[k] = 0.0;
+    *area = *aread = 0.0;
+    for (int i = 0; i < npts; i++) {
+        int i2 = i < npts - 1 ? i + 1 : 0;
+        cross3_d(rv[i], rvd[i], rv[i2], rvd[i2], cr, crd);
+        double arg1d = 2 * cr[0] * crd[0] + 2 * cr[1] * crd[1] + 2 * cr[2] * crd[2];
+        double arg1 = cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2] + 1e-15;
+        double temp = sqrt(arg1);
+        double cnd = (arg1 == 0.0) ? 0.0 : arg1d / (2.0 * temp);
+        double cn = temp;
+        *aread = *aread + 0.5 * cnd;
+        *area = *area + 0.5 * cn;
+        for (int k = 0; k < 3; k++) normald[k] = normald[k] + (crd[k] - cr[k] * cnd / cn) / cn;
+        for (int k = 0; k < 3; k++) normal[k] = normal[k] + cr[k] / cn;
+    }
+    double arg1d = 2 * normal[0] * normald[0] + 2 * normal[1] * normald[1] + 2 * normal[2] * normald[2];
+    double arg1 = normal[0] * normal[0] + normal[1] * normal[1] + normal[2] * normal[2];
+    double temp = sqrt(arg1);
+    double r1d = (arg1 == 0.0) ? 0.0 : arg1d / (2.0 * temp);
+    double r1 = temp;
+    for (int k = 0; k < 3; k++) normald[k] = (normald[k] - normal[k] * r1d / r1) / r1;
+    for (int k = 0; k < 3; k++) normal[k] = normal[k] / r1;
 }
 
 /* getSpherePts.F90:1-71; vpts 3x20, fpts 3x12 (xyz per point) */
@@ -805,6 +924,111 @@ static void compute_nodal_properties_b(octx *t, const double *xs, double *xsb) {
     }
 }
 
+/* COMPUTENODALPROPERTIES_D (kd_tree.F90:1275-1360) for the non-initial
+ * point: tangents Mid, Bid of Mi, Bi for a tangent xsd (3*npts) of the
+ * surface; Xud = xsd at the first duplicate of each unique node. */
+static void compute_nodal_properties_d(octx *t, const double *xs, const double *xsd) {
+    for (int i = 0; i < t->n; i++) {
+        int j = t->XuInd[i];
+        for (int k = 0; k < 3; k++) {
+            t->Xud[3 * i + k] = xsd[3 * (j - 1) + k];
+            t->Xu[3 * i + k] = xs[3 * (j - 1) + k];
+        }
+    }
+    double pts[3 * MAXFACEPTS], ptsd[3 * MAXFACEPTS];
+    for (int i = 0; i < t->n; i++) {
+        double sumarea = 0.0, sumnormal[3] = {0, 0, 0}, sumaread = 0.0, sumnormald[3] = {0, 0, 0};
+        for (int e = t->n2eStart[i]; e < t->n2eStart[i + 1]; e++) {
+            int ind = t->n2e[e];
+            int npts = t->facePtr[ind] - t->facePtr[ind - 1];
+            for (int kk = 0; kk < npts; kk++) {
+                int node = t->faceConn[t->facePtr[ind - 1] + kk];
+                for (int k = 0; k < 3; k++) {
+                    ptsd[3 * kk + k] = t->Xud[3 * (node - 1) + k];
+                    pts[3 * kk + k] = t->Xu[3 * (node - 1) + k];
+                }
+            }
+            double fa, fad, fn[3], fnd[3];
+            orc_element_props_d(pts, ptsd, npts, &fa, &fad, fn, fnd);
+            double dad = fad / npts, da = fa / npts;
+            sumaread = sumaread + dad;
+            sumarea = sumarea + da;
+            for (int k = 0; k < 3; k++) sumnormald[k] = sumnormald[k] + dad * fn[k] + da * fnd[k];
+            for (int k = 0; k < 3; k++) sumnormal[k] = sumnormal[k] + da * fn[k];
+        }
+        double nd[3], nrm[3];
+        for (int k = 0; k < 3; k++) nd[k] = (sumnormald[k] * sumarea - sumnormal[k] * sumaread) / (sumarea * sumarea);
+        for (int k = 0; k < 3; k++) nrm[k] = sumnormal[k] / sumarea;
+        double M[9], *Md = t->Mid + 9 * i;
+        if (t->opt.useRotations && !t->isCorner[i]) {
+            orc_rotation_matrix_d(t->normals0 + 3 * i, nrm, nd, M, Md);
+        } else {
+            memset(Md, 0, sizeof(double) * 9);
+        }
+        const double *x0 = t->Xu0 + 3 * i;
+        for (int r = 0; r < 3; r++)
+            t->Bid[3 * i + r] = t->Xud[3 * i + r] - x0[0] * Md[r] - x0[1] * Md[r + 3] - x0[2] * Md[r + 6];
+    }
+}
+
+/* setData_d (kd_tree.F90:1247-1267) */
+static void set_data_d(octx *t, int id) {
+    onode *np = &t->nodes[id];
+    if (np->dnum == 0) return;
+    for (int k = 0; k < 3; k++) np->Bid[k] = 0.0;
+    for (int k = 0; k < 9; k++) np->Mid[k] = 0.0;
+    for (int i = np->l; i <= np->u; i++) {
+        for (int k = 0; k < 3; k++) np->Bid[k] = np->Bid[k] + t->Bid[3 * (i - 1) + k];
+        for (int k = 0; k < 9; k++) np->Mid[k] = np->Mid[k] + t->Mid[9 * (i - 1) + k];
+    }
+    for (int k = 0; k < 3; k++) np->Bid[k] = np->Bid[k] / np->n;
+    for (int k = 0; k < 9; k++) np->Mid[k] = np->Mid[k] / np->n;
+    set_data_d(t, np->left);
+    set_data_d(t, t->nodes[id].right);
+}
+
+/* evalNodeExact_d (kd_tree.F90:1205-1228) */
+static void eval_exact_d(const octx *t, const onode *np, const double *r, double *numd) {
+    for (int i = np->l; i <= np->u; i++) {
+        const double *x0 = t->Xu0 + 3 * (i - 1), *M = t->Mid + 9 * (i - 1), *b = t->Bid + 3 * (i - 1);
+        double rr0 = r[0] - x0[0], rr1 = r[1] - x0[1], rr2 = r[2] - x0[2];
+        double lo = t->Ldef / sqrt(rr0 * rr0 + rr1 * rr1 + rr2 * rr2 + C_EPS);
+        double Wi = t->Ai[i - 1] * wfun(t, lo);
+        for (int k = 0; k < 3; k++) {
+            double Si = M[k] * r[0] + M[k + 3] * r[1] + M[k + 6] * r[2] + b[k];
+            numd[k] = numd[k] + Wi * Si;
+        }
+    }
+}
+
+/* evalNode_d + evalNodeApprox_d (kd_tree.F90:1164-1203, 1230-1245) */
+static void eval_node_d(const octx *t, int id, const double *r, double *numd, double approxDen) {
+    const onode *np = &t->nodes[id];
+    if (np->dnum == 0) {
+        eval_exact_d(t, np, r, numd);
+        return;
+    }
+    double d0 = r[0] - np->X[0], d1 = r[1] - np->X[1], d2 = r[2] - np->X[2];
+    double dist = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+    if (dist * np->oradius < 2.0) {
+        eval_node_d(t, np->left, r, numd, approxDen);
+        eval_node_d(t, np->right, r, numd, approxDen);
+        return;
+    }
+    double err = get_error(t, np, dist);
+    if (err < t->errTol * approxDen) {
+        double lo = t->Ldef / dist;
+        double Wi = np->Ai * wfun(t, lo);
+        for (int k = 0; k < 3; k++) {
+            double Si = np->Mid[k] * r[0] + np->Mid[k + 3] * r[1] + np->Mid[k + 6] * r[2] + np->Bid[k];
+            numd[k] = numd[k] + Wi * Si;
+        }
+    } else {
+        eval_node_d(t, np->left, r, numd, approxDen);
+        eval_node_d(t, np->right, r, numd, approxDen);
+    }
+}
+
 /* ------------------------------------------------------------------ */
 /* Public entry points                                                 */
 /* ------------------------------------------------------------------ */
@@ -820,6 +1044,7 @@ void orc_free(void *h) {
     free(t->facePtr); free(t->faceConn); free(t->n2eStart); free(t->n2e); free(t->XuInd); free(t->indexes);
     free(t->the_data); free(t->Xu0); free(t->Xu); free(t->Ai); free(t->normals0); free(t->normals);
     free(t->Mi); free(t->Bi); free(t->Mib); free(t->Bib); free(t->Xub); free(t->normalsb); free(t->isCorner);
+    free(t->Mid); free(t->Bid); free(t->Xud);
     free(t->nodes); free(t);
 }
 
@@ -888,6 +1113,9 @@ void *orc_create(const orc_options *opt, const double *pts, int npts, const doub
     t->Bib = calloc(3 * (size_t)n, sizeof(double));
     t->Xub = calloc(3 * (size_t)n, sizeof(double));
     t->normalsb = calloc(3 * (size_t)n, sizeof(double));
+    t->Mid = calloc(9 * (size_t)n, sizeof(double));
+    t->Bid = calloc(3 * (size_t)n, sizeof(double));
+    t->Xud = calloc(3 * (size_t)n, sizeof(double));
     compute_nodal_properties(t, pts, 1);
     determine_corners(t);
     set_ai(t, 0);
@@ -1022,6 +1250,28 @@ void orc_warp_deriv(void *h, const double *xs, const double *xv0, long nvol, con
     compute_nodal_properties_b(t, xs, dxs);
     g_times[0] = (t1 - t0) + (wall() - t2);
     g_times[1] = t2 - t1;
+}
+
+/* warpDerivFwd (warpDerivFwd.F90:62-99): xvd (3*nvol) = (dXv/dXs) xsd.
+ * den is the denominator of the last warp. */
+void orc_warp_deriv_fwd(void *h, const double *xs, const double *xsd, const double *xv0, long nvol,
+                        const double *den0, const double *den, double *xvd) {
+    octx *t = h;
+    compute_nodal_properties_d(t, xs, xsd);
+    t->Ldef = t->Ldef0 * t->opt.LdefFact;
+    t->alphaToBexp = pow(t->opt.alpha, t->opt.bExp);
+    t->errTol = t->opt.errTol;
+    set_data_d(t, 0);
+    int fast = t->opt.evalMode == 1;
+#pragma omp parallel for schedule(dynamic, 256) num_threads(g_threads)
+    for (long j = 0; j < nvol; j++) {
+        const double *r = xv0 + 3 * j;
+        double numd[3] = {0, 0, 0};
+        if (!fast) eval_exact_d(t, &t->nodes[0], r, numd);
+        else eval_node_d(t, 0, r, numd, den0[j]);
+        double oden = 1.0 / den[j];
+        for (int k = 0; k < 3; k++) xvd[3 * j + k] = numd[k] * oden;
+    }
 }
 
 /* ------------------------------------------------------------------ */

@@ -63,6 +63,7 @@ def lib():
         L.orc_den0.argtypes = [P, dp, L64, dp]
         L.orc_warp.argtypes = [P, dp, dp, L64, dp, dp, dp]
         L.orc_warp_deriv.argtypes = [P, dp, dp, L64, dp, dp, dp, dp]
+        L.orc_warp_deriv_fwd.argtypes = [P, dp, dp, dp, L64, dp, dp, dp]
         L.orc_counts.argtypes = [P, dp, dp, L64, dp, dp]
         L.orc_point_reduce.restype = I
         L.orc_point_reduce.argtypes = [dp, I, D, dp, ip]
@@ -78,6 +79,8 @@ def lib():
         L.orc_element_props_b.argtypes = [dp, I, D, dp, dp]
         L.orc_rotation_matrix.argtypes = [dp, dp, dp]
         L.orc_rotation_matrix_b.argtypes = [dp, dp, dp, dp]
+        L.orc_rotation_matrix_d.argtypes = [dp, dp, dp, dp, dp]
+        L.orc_element_props_d.argtypes = [dp, dp, I, dp, dp, dp, dp]
         L.orc_sphere_pts.argtypes = [dp, D, dp, dp]
         L.orc_last_times.argtypes = [dp]
         _LIB = L
@@ -171,6 +174,16 @@ class OracleMesh:
                              np.ascontiguousarray(dxv, np.float64).ravel(), dxs)
         return dxs
 
+    def warpDerivFwd(self, dxs):
+        """warpDerivFwd.F90: dXv = (dXv/dXs) dxs (3*npts), warp order, with
+        the denominator of the last warp."""
+        if self.den is None:
+            raise RuntimeError("warpDerivFwd before warpMesh")
+        dxv = np.zeros(3 * self.nvol)
+        lib().orc_warp_deriv_fwd(self.h, self.xs, np.ascontiguousarray(dxs, np.float64).ravel(), self.xv0.ravel(),
+                                 self.nvol, self.den0, self.den, dxv)
+        return dxv
+
     def counts(self, xv0=None, den0=None):
         xv0 = self.xv0 if xv0 is None else np.ascontiguousarray(xv0, np.float64).reshape(-1, 3)
         den0 = self.den0 if den0 is None else np.ascontiguousarray(den0, np.float64)
@@ -219,6 +232,23 @@ def rotation_matrix_b(v1, v2, Mb):
     lib().orc_rotation_matrix_b(np.ascontiguousarray(v1, np.float64), np.ascontiguousarray(v2, np.float64),
                                 np.ascontiguousarray(np.asarray(Mb, np.float64).T).ravel(), v2b)
     return v2b
+
+
+def element_props_d(points, pointsd):
+    """GETELEMENTPROPS_D: (area, normal, area tangent, normal tangent)."""
+    p = np.ascontiguousarray(points, np.float64).reshape(-1, 3)
+    pd = np.ascontiguousarray(pointsd, np.float64).reshape(-1, 3)
+    a = np.zeros(1); ad = np.zeros(1); nrm = np.zeros(3); nd = np.zeros(3)
+    lib().orc_element_props_d(p.ravel(), pd.ravel(), len(p), a, ad, nrm, nd)
+    return a[0], nrm, ad[0], nd
+
+
+def rotation_matrix_d(v1, v2, v2d):
+    """GETROTATIONMATRIX3D_D: (M, Md) as [r, c] arrays."""
+    M = np.zeros(9); Md = np.zeros(9)
+    lib().orc_rotation_matrix_d(np.ascontiguousarray(v1, np.float64), np.ascontiguousarray(v2, np.float64),
+                                np.ascontiguousarray(v2d, np.float64), M, Md)
+    return M.reshape(3, 3).T, Md.reshape(3, 3).T
 
 
 def sphere_pts(x0, radius):

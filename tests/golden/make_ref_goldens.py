@@ -2,9 +2,10 @@
 
 The routines come from oracle/_ref/libidwref.so, which oracle/Makefile
 (`make -C oracle ref`) compiles in place from the reference sources
-(getElementProps.F90, vectorUtils.f90, getSpherePts.F90 and the Tapenade
-reverse routines getElementProps_b.f90 / vectorUtils_b.f90, with the
-reference's -fdefault-real-8).  Inputs are seeded random faces and vector
+(getElementProps.F90, vectorUtils.f90, getSpherePts.F90, the Tapenade
+reverse routines getElementProps_b.f90 / vectorUtils_b.f90 and the tangent
+routines getElementProps_d.f90 / vectorUtils_d.f90, with the reference's
+-fdefault-real-8).  Inputs are seeded random faces and vector
 pairs plus the branch edge cases of the rotation (no rotation below the
 1.49e-8 axis tolerance, acos argument clipped at 1, opposite normals).
 
@@ -58,6 +59,21 @@ class Ref:
         self.L.getrotationmatrix3d_b_(_p(np.array(v1, np.float64)), _p(np.array(v2, np.float64)), _p(v2b),
                                       _p(np.asfortranarray(M)), _p(Mbf))
         return v2b
+
+    def element_props_d(self, pts, ptsd):
+        p = np.asfortranarray(np.asarray(pts, np.float64).T)
+        pdot = np.asfortranarray(np.asarray(ptsd, np.float64).T)
+        n = ctypes.c_int32(p.shape[1])
+        a, ad, nrm, nd = np.zeros(1), np.zeros(1), np.zeros(3), np.zeros(3)
+        self.L.getelementprops_d_(_p(p), _p(pdot), ctypes.byref(n), _p(a), _p(ad), _p(nrm), _p(nd))
+        return a[0], nrm, ad[0], nd
+
+    def rotation_d(self, v1, v2, v2d):
+        M = np.zeros((3, 3), order="F")
+        Md = np.zeros((3, 3), order="F")
+        self.L.getrotationmatrix3d_d_(_p(np.array(v1, np.float64)), _p(np.array(v2, np.float64)),
+                                      _p(np.array(v2d, np.float64)), _p(M), _p(Md))
+        return np.array(M), np.array(Md)
 
     def sphere(self, x0, radius):
         v = np.zeros((3, 20), order="F")
@@ -125,6 +141,21 @@ def main():
         out["sph%d_r" % k] = np.array([r])
         out["sph%d_v" % k] = v
         out["sph%d_f" % k] = f
+    # tangents (forward mode) from a separate stream so the arrays above do
+    # not depend on them
+    rd = np.random.default_rng(4321)
+    for k, p in enumerate(faces):
+        pd = rd.standard_normal(p.shape)
+        a, n, ad, nd = R.element_props_d(p, pd)
+        out["face%d_ptsd" % k] = pd
+        out["face%d_aread" % k] = np.array([ad])
+        out["face%d_normald" % k] = nd
+    for k, (v1, v2) in enumerate(vecs):
+        v2d = rd.standard_normal(3)
+        M, Md = R.rotation_d(v1, v2, v2d)
+        out["rot%d_v2d" % k] = v2d
+        out["rot%d_Md" % k] = Md
+        out["rot%d_Mfwd" % k] = M
     np.savez_compressed(os.path.join(HERE, "ref_elements.npz"), nfaces=len(faces), nrot=len(vecs), nsph=4, **out)
     print("wrote %d arrays" % len(out))
 

@@ -220,3 +220,45 @@ def test_point_reduce_semantics():
     assert nu == 40
     assert np.array_equal(uniq, base)
     assert list(link[40:]) == [4, 8, 8, 21, 6]
+
+
+@pytest.mark.parametrize("mode", ["exact", "fast"])
+def test_tangent_fd_and_dot_product(sphere, mode):
+    """warpDerivFwd (warpDerivFwd.F90:62-99) against central differences of
+    warpMesh, and the dot-product identity with warpDeriv,
+    <v, J p> == <J^T v, p> (examples/structured/dotprod.py:38-49)."""
+    m = sphere
+    om = O.OracleMesh(m["pts"], m["conn"], m["faceSizes"], m["xv0"], evalMode=mode, errTol=1e-4)
+    xs = m["perturb"](m["pts"]).ravel()
+    om.setSurfaceCoordinates(xs)
+    om.warpMesh()
+    rng = np.random.default_rng(8)
+    h = 1e-6
+    v = rng.standard_normal(3 * len(m["xv0"]))
+    jtv = om.warpDeriv(v)
+    for _ in range(3):
+        p = rng.standard_normal(xs.size)
+        jp = om.warpDerivFwd(p)
+        om.setSurfaceCoordinates(xs + h * p)
+        xp = om.warpMesh()
+        om.setSurfaceCoordinates(xs - h * p)
+        xm = om.warpMesh()
+        om.setSurfaceCoordinates(xs)
+        om.warpMesh()
+        fd = (xp - xm) / (2 * h)
+        assert np.linalg.norm(jp - fd) <= 1e-6 * np.linalg.norm(fd)
+        # both sides sum terms like K r and K Xu0 that cancel: rounding ~1e-11
+        assert abs(v @ jp - jtv @ p) <= 1e-9 * abs(v @ jp)
+
+
+def test_tangent_of_rigid_translation():
+    """A translation tangent t at every surface point moves every volume
+    node by t (M and the normals do not change)."""
+    pts, conn, fs, V = _plate()
+    for mode in ("exact", "fast"):
+        om = O.OracleMesh(pts, conn, fs, V, evalMode=mode, errTol=1e-4)
+        om.setSurfaceCoordinates(pts)
+        om.warpMesh()
+        t = np.array([0.3, -0.2, 0.7])
+        jp = om.warpDerivFwd(np.tile(t, len(pts))).reshape(-1, 3)
+        assert np.abs(jp - t).max() < 1e-13

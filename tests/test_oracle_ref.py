@@ -2,8 +2,9 @@
 
 Fixtures in tests/golden/ref_elements.npz were written by
 tests/golden/make_ref_goldens.py from oracle/_ref/libidwref.so (the
-reference's getElementProps.F90, vectorUtils.f90, getSpherePts.F90 and
-Tapenade getElementProps_b.f90 / vectorUtils_b.f90 compiled in place).  The
+reference's getElementProps.F90, vectorUtils.f90, getSpherePts.F90, the
+Tapenade reverse getElementProps_b.f90 / vectorUtils_b.f90 and tangent
+getElementProps_d.f90 / vectorUtils_d.f90, compiled in place).  The
 oracle restates those routines with the same operation order, so the
 comparison is bit for bit.  Where the reference build is present, fresh
 random cases are compared as well.
@@ -46,6 +47,41 @@ def test_rotation_reverse_bitwise():
     for k in range(int(GOLD["nrot"])):
         v2b = O.rotation_matrix_b(GOLD["rot%d_v1" % k], GOLD["rot%d_v2" % k], GOLD["rot%d_Mb" % k])
         assert np.array_equal(v2b, GOLD["rot%d_v2b" % k]), k
+
+
+def test_element_props_tangent_bitwise():
+    for k in range(int(GOLD["nfaces"])):
+        a, n, ad, nd = O.element_props_d(GOLD["face%d_pts" % k], GOLD["face%d_ptsd" % k])
+        assert a == GOLD["face%d_area" % k][0] and np.array_equal(n, GOLD["face%d_normal" % k]), k
+        assert ad == GOLD["face%d_aread" % k][0], k
+        assert np.array_equal(nd, GOLD["face%d_normald" % k]), k
+
+
+def test_rotation_tangent_bitwise():
+    """Same branch cases as the reverse: no rotation, clipped acos argument,
+    opposite and non-unit normals."""
+    for k in range(int(GOLD["nrot"])):
+        M, Md = O.rotation_matrix_d(GOLD["rot%d_v1" % k], GOLD["rot%d_v2" % k], GOLD["rot%d_v2d" % k])
+        assert np.array_equal(M, GOLD["rot%d_Mfwd" % k]), k
+        assert np.array_equal(Md, GOLD["rot%d_Md" % k]), k
+
+
+def test_tangent_and_reverse_are_transposes():
+    """<J_d v2d, Mb> == <v2d, J_b Mb> for the rotation and the element
+    routines: the tangent and the reverse restatements are consistent."""
+    rng = np.random.default_rng(11)
+    for _ in range(20):
+        v1 = rng.standard_normal(3)
+        v2 = v1 + rng.standard_normal(3)
+        v2d, Mb = rng.standard_normal(3), rng.standard_normal((3, 3))
+        _, Md = O.rotation_matrix_d(v1, v2, v2d)
+        assert abs(np.sum(Md * Mb) - O.rotation_matrix_b(v1, v2, Mb) @ v2d) <= 1e-12 * (1 + np.abs(Md).sum())
+    for npts in (3, 4, 5, 8):
+        p, pd = rng.standard_normal((npts, 3)), rng.standard_normal((npts, 3))
+        ab, nb = rng.standard_normal(), rng.standard_normal(3)
+        _, _, ad, nd = O.element_props_d(p, pd)
+        lhs = ab * ad + nb @ nd
+        assert abs(lhs - np.sum(O.element_props_b(p, ab, nb) * pd)) <= 1e-12 * (1 + abs(lhs))
 
 
 def test_sphere_points_bitwise():
@@ -111,3 +147,12 @@ def test_fresh_cases_against_reference_build():
         Mb = rng.standard_normal((3, 3))
         assert np.array_equal(R.rotation(v1, v2), O.rotation_matrix(v1, v2))
         assert np.array_equal(R.rotation_b(v1, v2, Mb), O.rotation_matrix_b(v1, v2, Mb))
+        v2d = rng.standard_normal(3)
+        Mr, Mdr = R.rotation_d(v1, v2, v2d)
+        Mo, Mdo = O.rotation_matrix_d(v1, v2, v2d)
+        assert np.array_equal(Mr, Mo) and np.array_equal(Mdr, Mdo)
+    for p in faces:
+        pd = rng.standard_normal(p.shape)
+        r = R.element_props_d(p, pd)
+        o = O.element_props_d(p, pd)
+        assert r[0] == o[0] and r[2] == o[2] and np.array_equal(r[1], o[1]) and np.array_equal(r[3], o[3])
