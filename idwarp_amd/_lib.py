@@ -22,7 +22,7 @@ EXPORTS = (
     "idw_get_surface_coordinates", "idw_warp_mesh", "idw_get_volume_coordinates",
     "idw_set_external_mesh_indices", "idw_get_solver_dof", "idw_warp_to_solver_grid", "idw_solver_to_warp_grid",
     "idw_warp_deriv", "idw_get_dxs", "idw_warp_mesh_dev", "idw_warp_deriv_dev", "idw_get_sizes", "idw_get_ldef0",
-    "idw_get_denominator0", "idw_get_timings", "idw_fast_counts",
+    "idw_get_denominator0", "idw_get_timings", "idw_fast_counts", "idw_warp_deriv_fwd", "idw_warp_deriv_fwd_dev",
 )
 
 
@@ -94,6 +94,8 @@ def load():
         "idw_get_dxs": (i32, [vp, vp, i64]),
         "idw_warp_mesh_dev": (i32, [vp, vp, vp, vp]),
         "idw_warp_deriv_dev": (i32, [vp, vp, vp, vp, vp]),
+        "idw_warp_deriv_fwd": (i32, [vp, vp, i64, vp, i64]),
+        "idw_warp_deriv_fwd_dev": (i32, [vp, vp, vp, vp, vp]),
         "idw_get_sizes": (i32, [vp, P(i64), P(i64), P(i64), P(i64)]),
         "idw_get_ldef0": (i32, [vp, P(d)]),
         "idw_get_denominator0": (i32, [vp, vp, i64]),

@@ -5,9 +5,11 @@
 volume grid (3 * nVol, warp ordering).  Its backward is the reverse product
 dXs = (dXv/dXs)^T dXv of warpDeriv (src/warp/warpDeriv.F90), computed by
 the library's reverse kernels, so ``xv.backward(g)`` fills ``xs.grad`` with
-what USMesh.warpDeriv(g) / getdXs() returns.  Both directions run on
-torch's current stream through idw_warp_mesh_dev / idw_warp_deriv_dev;
-torch only provides the device memory and the stream.
+what USMesh.warpDeriv(g) / getdXs() returns.  Forward-mode AD (torch.func.jvp,
+torch.autograd.forward_ad) uses the tangent of warpDerivFwd.F90.  All
+directions run on torch's current stream through idw_warp_mesh_dev,
+idw_warp_deriv_dev and idw_warp_deriv_fwd_dev; torch only provides the
+device memory and the stream.
 """
 import ctypes
 
@@ -28,7 +30,8 @@ def _check_surface(mesh, xs):
 
 
 class WarpFunction(torch.autograd.Function):
-    """Xv = warpMesh(Xs); backward: dXs = warpDeriv(dXv)."""
+    """Xv = warpMesh(Xs); backward: dXs = warpDeriv(dXv); jvp: dXv =
+    warpDerivFwd(dXs)."""
 
     @staticmethod
     def forward(ctx, xs, mesh):
@@ -40,7 +43,18 @@ class WarpFunction(torch.autograd.Function):
         _lib.check(mesh._L.idw_warp_mesh_dev(mesh.ctx, _vp(xs), _vp(xv), stream))
         ctx.mesh = mesh
         ctx.save_for_backward(xs)
+        ctx.save_for_forward(xs)
         return xv
+
+    @staticmethod
+    def jvp(ctx, dxs, _mesh_tangent):
+        (xs,) = ctx.saved_tensors
+        mesh = ctx.mesh
+        dxs = dxs.to(torch.float64).contiguous()
+        dxv = torch.empty(3 * mesh.nVol, dtype=torch.float64, device=xs.device)
+        stream = ctypes.c_void_p(torch.cuda.current_stream(xs.device).cuda_stream)
+        _lib.check(mesh._L.idw_warp_deriv_fwd_dev(mesh.ctx, _vp(xs), _vp(dxs), _vp(dxv), stream))
+        return dxv
 
     @staticmethod
     def backward(ctx, gxv):

@@ -70,6 +70,7 @@ struct idw_ctx {
     bool planValid = false;  // plan written by the last forward at denOpt
     int64_t npts = 0;
     double *xv0w = nullptr, *xv = nullptr, *xs = nullptr, *dxs = nullptr, *dxv = nullptr;
+    double *dxsIn = nullptr;  // host-API tangent seed (3 * npts)
     double *partial = nullptr;
     int nchunk = 0;
     double bboxLo[3] = {0, 0, 0}, bboxScale[3] = {0, 0, 0};  // Morton quantisation of the volume box
@@ -329,6 +330,31 @@ int run_deriv(idw_ctx *c, const double *xs, const double *dxv, double *dxs) {
     return IDW_OK;
 }
 
+// warpDerivFwd (warpDerivFwd.F90:62-99): dxv = (dXv/dXs) dxs.  The tangent
+// point and cluster records replace the per-call records of the last warp
+// (every warp and reverse product rebuilds them), and the forward kernels
+// replay the fast mode's plan with them; the denominator is recomputed with
+// the same weights in the same order as the forward's.
+int run_deriv_fwd(idw_ctx *c, const double *xs, const double *dxs, double *dxv) {
+    if (!c->surfInit) return fail(IDW_ESTATE, "warp_deriv_fwd before initialize_warping");
+    if (!c->volSet) return fail(IDW_ESTATE, "no volume nodes set");
+    const int fast = c->opt.evalMode == IDW_EVAL_FAST;
+    if (fast && (!c->planValid || !same_numeric_options(c->denOpt, c->opt))) TRY(run_warp(c, xs, c->xv));
+    idw::WParams P = make_params(c, false);
+    P.tangent = 1;
+    HIPCHK(hipEventRecord(c->ev[0], c->st));
+    idw::launch_props_tangent(c->st, c->S, xs, dxs, c->opt.useRotations);
+    if (fast) idw::launch_cluster_data(c->st, c->S);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c->ev[1], c->st));
+    HIPCHK(hipEventRecord(c->ev[2], c->st));
+    if (c->V.n) idw::launch_forward(c->st, c->S, c->V, P, fast, c->opt.precision, dxv, fast ? &c->plan : nullptr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c->ev[3], c->st));
+    c->fwdTimed = true;
+    return IDW_OK;
+}
+
 void collect_timings(idw_ctx *c) {
     float t;
     if (c->fwdTimed) {
@@ -563,12 +589,14 @@ int idw_initialize_warping(idw_ctx *c, const double *pts, int64_t n_pts, const d
     TRY(dalloc(c, &S.slotStart, S.n + 1));
     TRY(dalloc(c, &S.slots, H.nconn));
     TRY(dalloc(c, &S.fprops, H.nface));
+    TRY(dalloc(c, &S.fpropsd, H.nface));
     TRY(dalloc(c, &S.padj, (size_t)S.n * idw::ADJ_STRIDE));
     TRY(dalloc(c, &S.nadj, (size_t)S.nnodes * idw::ADJ_STRIDE));
     TRY(dalloc(c, &S.ntmp, (size_t)S.n * 8));
     TRY(dalloc(c, &S.pb, 3 * (size_t)H.nconn));
     TRY(dalloc(c, &c->xs, 3 * (size_t)n_pts));
     TRY(dalloc(c, &c->dxs, 3 * (size_t)n_pts));
+    TRY(dalloc(c, &c->dxsIn, 3 * (size_t)n_pts));
     std::vector<int32_t> internal;
     for (int k = 0; k < S.nnodes; ++k)
         if (!H.nodes[k].leaf) internal.push_back(k);
@@ -734,6 +762,39 @@ int idw_warp_deriv_dev(idw_ctx *c, const double *d_xs, const double *d_dxv, doub
     HIPCHK(hipEventRecord(e, user));
     HIPCHK(hipStreamWaitEvent(c->st, e, 0));
     int r = run_deriv(c, d_xs, d_dxv, d_dxs);
+    if (r == IDW_OK) {
+        HIPCHK(hipEventRecord(e, c->st));
+        HIPCHK(hipStreamWaitEvent(user, e, 0));
+    }
+    (void)hipEventDestroy(e);
+    return r;
+}
+
+int idw_warp_deriv_fwd(idw_ctx *c, const double *dxs, int64_t ndof_s, double *dxv, int64_t ndof_v) {
+    if (!c || (!dxs && ndof_s) || (!dxv && ndof_v)) return fail(IDW_EINVAL, "null argument");
+    if (!c->surfInit) return fail(IDW_ESTATE, "warp_deriv_fwd before initialize_warping");
+    if (!c->volSet) return fail(IDW_ESTATE, "no volume nodes set");
+    if (ndof_s != 3 * c->npts) return fail(IDW_EINVAL, "dxs: expected %lld values", (long long)(3 * c->npts));
+    if (ndof_v != 3 * c->V.n) return fail(IDW_EINVAL, "dxv: expected %lld values", (long long)(3 * c->V.n));
+    TRY(ensure_device(c));
+    HIPCHK(hipMemcpyAsync(c->dxsIn, dxs, sizeof(double) * ndof_s, hipMemcpyHostToDevice, c->st));
+    TRY(run_deriv_fwd(c, c->xs, c->dxsIn, c->dxv));
+    if (ndof_v) HIPCHK(hipMemcpyAsync(dxv, c->dxv, sizeof(double) * ndof_v, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    collect_timings(c);
+    TRY(check_plan(c));
+    return IDW_OK;
+}
+
+int idw_warp_deriv_fwd_dev(idw_ctx *c, const double *d_xs, const double *d_dxs, double *d_dxv, void *stream) {
+    if (!c || !d_xs || !d_dxs || !d_dxv) return fail(IDW_EINVAL, "null argument");
+    TRY(ensure_device(c));
+    hipStream_t user = static_cast<hipStream_t>(stream);
+    hipEvent_t e;
+    HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    HIPCHK(hipEventRecord(e, user));
+    HIPCHK(hipStreamWaitEvent(c->st, e, 0));
+    int r = run_deriv_fwd(c, d_xs, d_dxs, d_dxv);
     if (r == IDW_OK) {
         HIPCHK(hipEventRecord(e, c->st));
         HIPCHK(hipStreamWaitEvent(user, e, 0));

@@ -20,6 +20,7 @@ struct WParams {
     double Ldef, a5, aExp, bExp, errTol, nearT2;
     int32_t expMode, ia, ib;
     int32_t zero;  // always 0: lane & zero turns a uniform index into a per-lane one (vector load)
+    int32_t tangent;  // 1: the point/cluster records hold tangents; write num/den only (warpDerivFwd)
 };
 
 struct DevSurface {
@@ -37,6 +38,7 @@ struct DevSurface {
     int32_t *facePtr = nullptr, *faceConn = nullptr, *n2eStart = nullptr, *n2e = nullptr;
     int32_t *slotStart = nullptr, *slots = nullptr;
     double4 *fprops = nullptr;  // dA = area/nPts, unit normal
+    double4 *fpropsd = nullptr; // their tangents (warpDerivFwd)
     double *padj = nullptr, *nadj = nullptr, *ntmp = nullptr, *pb = nullptr;
 };
 
@@ -71,6 +73,9 @@ struct DevVolume {
 
 void launch_face_props(hipStream_t st, const DevSurface &S, const double *xs);
 void launch_node_props(hipStream_t st, const DevSurface &S, const double *xs, int initial, int useRot);
+// face and node property tangents for a surface tangent dxs; writes tangent
+// point records into S.pd / S.pd32
+void launch_props_tangent(hipStream_t st, const DevSurface &S, const double *xs, const double *dxs, int useRot);
 void launch_corners(hipStream_t st, const DevSurface &S, double threshold);
 void launch_cluster_data(hipStream_t st, const DevSurface &S);
 void launch_errors(hipStream_t st, const DevSurface &S, const WParams &P, const double *sphere60);

@@ -211,6 +211,168 @@ __global__ void k_node_props(int n, int initial, int useRot, const int32_t *__re
     pd32[i] = o32;
 }
 
+// ---------------------------------------------------------------------------
+// Tangent of the nodal properties (COMPUTENODALPROPERTIES_D,
+// kd_tree.F90:1275-1360) for a surface tangent dxs
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double dot3(const double *a, const double *b) {
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+// Face area / point count and unit normal with their tangents
+// (GETELEMENTPROPS_D, getElementProps_d.f90:7-94).
+__global__ void k_face_props_d(int nface, const int32_t *__restrict__ facePtr, const int32_t *__restrict__ faceConn,
+                               const int32_t *__restrict__ xuInd, const double *__restrict__ xs,
+                               const double *__restrict__ dxs, double4 *__restrict__ fprops,
+                               double4 *__restrict__ fpropsd) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nface) return;
+    const int p0 = facePtr[f], np = facePtr[f + 1] - p0;
+    double c[3] = {0.0, 0.0, 0.0}, cd[3] = {0.0, 0.0, 0.0}, p[3], pd[3];
+    for (int i = 0; i < np; ++i) {
+        load_xu(xs, xuInd, faceConn[p0 + i], p);
+        load_xu(dxs, xuInd, faceConn[p0 + i], pd);
+        for (int k = 0; k < 3; ++k) { c[k] += p[k]; cd[k] += pd[k]; }
+    }
+    for (int k = 0; k < 3; ++k) { c[k] /= np; cd[k] /= np; }
+    double first[3], firstd[3], prev[3], prevd[3];
+    load_xu(xs, xuInd, faceConn[p0], first);
+    load_xu(dxs, xuInd, faceConn[p0], firstd);
+    for (int k = 0; k < 3; ++k) {
+        first[k] -= c[k]; firstd[k] -= cd[k];
+        prev[k] = first[k]; prevd[k] = firstd[k];
+    }
+    double nrm[3] = {0.0, 0.0, 0.0}, nrmd[3] = {0.0, 0.0, 0.0}, area = 0.0, aread = 0.0;
+    for (int i = 0; i < np; ++i) {
+        double nx[3], nxd[3];
+        if (i + 1 < np) {
+            load_xu(xs, xuInd, faceConn[p0 + i + 1], nx);
+            load_xu(dxs, xuInd, faceConn[p0 + i + 1], nxd);
+            for (int k = 0; k < 3; ++k) { nx[k] -= c[k]; nxd[k] -= cd[k]; }
+        } else {
+            for (int k = 0; k < 3; ++k) { nx[k] = first[k]; nxd[k] = firstd[k]; }
+        }
+        double cr[3], t1[3], t2[3], crd[3];
+        cross3(prev, nx, cr);
+        cross3(prevd, nx, t1);
+        cross3(prev, nxd, t2);
+        for (int k = 0; k < 3; ++k) crd[k] = t1[k] + t2[k];
+        const double cn = sqrt(dot3(cr, cr) + 1e-15);
+        const double cnd = dot3(cr, crd) / cn;
+        area += 0.5 * cn;
+        aread += 0.5 * cnd;
+        for (int k = 0; k < 3; ++k) {
+            nrmd[k] += (crd[k] - cr[k] * cnd / cn) / cn;
+            nrm[k] += cr[k] / cn;
+        }
+        for (int k = 0; k < 3; ++k) { prev[k] = nx[k]; prevd[k] = nxd[k]; }
+    }
+    const double nn = sqrt(dot3(nrm, nrm));
+    const double nnd = dot3(nrm, nrmd) / nn;
+    fprops[f] = make_double4(area / np, nrm[0] / nn, nrm[1] / nn, nrm[2] / nn);
+    fpropsd[f] = make_double4(aread / np, (nrmd[0] - nrm[0] * nnd / nn) / nn, (nrmd[1] - nrm[1] * nnd / nn) / nn,
+                              (nrmd[2] - nrm[2] * nnd / nn) / nn);
+}
+
+// Tangent of the rotation taking v1 to v2 for a tangent v2d of v2
+// (GETROTATIONMATRIX3D_D, vectorUtils_d.f90:7-129), row-major: with
+// M = I + sin(a) A + (1 - cos(a)) A^2, Md = (cos(a) A + sin(a) A^2) a' +
+// sin(a) A' + (1 - cos(a)) (A' A + A A').  Zero below the axis tolerance.
+__device__ void rodrigues_d(const double *v1, const double *v2, const double *v2d, double *Md) {
+    for (int k = 0; k < 9; ++k) Md[k] = 0.0;
+    double ax[3], axd[3];
+    cross3(v1, v2, ax);
+    cross3(v1, v2d, axd);
+    const double am = sqrt(dot3(ax, ax) + 1e-30);
+    if (am < ROT_TOL) return;
+    const double amd = dot3(ax, axd) / am;
+    double u[3], ud[3];
+    for (int k = 0; k < 3; ++k) {
+        ud[k] = (axd[k] - ax[k] * amd / am) / am;
+        u[k] = ax[k] / am;
+    }
+    const double m1 = sqrt(dot3(v1, v1) + 1e-30), m2 = sqrt(dot3(v2, v2) + 1e-30);
+    const double m2d = dot3(v2, v2d) / m2;
+    double vv1[3], vv2[3], vv2d[3];
+    for (int k = 0; k < 3; ++k) {
+        vv1[k] = v1[k] / m1;
+        vv2[k] = v2[k] / m2;
+        vv2d[k] = (v2d[k] - v2[k] * m2d / m2) / m2;
+    }
+    const double dot = dot3(vv1, vv2);
+    double arg = 1.0, argd = 0.0;
+    if (1.0 > dot) {
+        arg = dot;
+        argd = dot3(vv1, vv2d);
+    }
+    const double angd = (arg == 1.0 || arg == -1.0) ? 0.0 : -(argd / sqrt(1.0 - arg * arg));
+    const double ang = acos(arg), sn = sin(ang), cs = cos(ang), oc = 1.0 - cs;
+    const double A[9] = {0.0, -u[2], u[1], u[2], 0.0, -u[0], -u[1], u[0], 0.0};
+    const double Ad[9] = {0.0, -ud[2], ud[1], ud[2], 0.0, -ud[0], -ud[1], ud[0], 0.0};
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+            double C = 0.0, Cd = 0.0;
+            for (int q = 0; q < 3; ++q) {
+                C += A[3 * r + q] * A[3 * q + c];
+                Cd += Ad[3 * r + q] * A[3 * q + c] + A[3 * r + q] * Ad[3 * q + c];
+            }
+            Md[3 * r + c] = (A[3 * r + c] * cs + C * sn) * angd + sn * Ad[3 * r + c] + oc * Cd;
+        }
+}
+
+// Tangent point records: the forward kernels evaluate
+// sum w (K (r - Xu0) + d) / sum w, and the tangent of the displacement
+// M r + b - r with b = Xu - M Xu0 is Md (r - Xu0) + Xud, so the record holds
+// K = Md and d = Xud (evalNodeExact_d, kd_tree.F90:1205-1228).  Zero-area
+// nodes carry zero weight; their tangent record is set to zero.
+__global__ void k_node_props_d(int n, int useRot, const int32_t *__restrict__ n2eStart,
+                               const int32_t *__restrict__ n2e, const double4 *__restrict__ fprops,
+                               const double4 *__restrict__ fpropsd, const int32_t *__restrict__ xuInd,
+                               const int32_t *__restrict__ corner, const double *__restrict__ dxs,
+                               const double4 *__restrict__ ps, const double *__restrict__ normals0,
+                               PointDyn *__restrict__ pd, PointDyn32 *__restrict__ pd32) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double sa = 0.0, sad = 0.0, sn[3] = {0.0, 0.0, 0.0}, snd[3] = {0.0, 0.0, 0.0};
+    for (int e = n2eStart[i]; e < n2eStart[i + 1]; ++e) {
+        const double4 fp = fprops[n2e[e]], fd = fpropsd[n2e[e]];
+        sa += fp.x;
+        sad += fd.x;
+        const double fn[3] = {fp.y, fp.z, fp.w}, fnd[3] = {fd.y, fd.z, fd.w};
+        for (int k = 0; k < 3; ++k) {
+            snd[k] += fd.x * fn[k] + fp.x * fnd[k];
+            sn[k] += fp.x * fn[k];
+        }
+    }
+    const double4 p = ps[i];
+    PointDyn out;
+    out.x0[0] = p.x; out.x0[1] = p.y; out.x0[2] = p.z;
+    out.A = p.w;
+    double Md[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double xud[3];
+    load_xu(dxs, xuInd, i, xud);
+    if (p.w == 0.0 || sa == 0.0) {
+        xud[0] = xud[1] = xud[2] = 0.0;
+    } else if (useRot && !corner[i]) {
+        double nrm[3], nrmd[3];
+        for (int k = 0; k < 3; ++k) {
+            nrm[k] = sn[k] / sa;
+            nrmd[k] = (snd[k] * sa - sn[k] * sad) / (sa * sa);
+        }
+        const double n0[3] = {normals0[3 * i], normals0[3 * i + 1], normals0[3 * i + 2]};
+        rodrigues_d(n0, nrm, nrmd, Md);
+    }
+    for (int k = 0; k < 9; ++k) out.K[k] = Md[k];
+    for (int k = 0; k < 3; ++k) out.d[k] = xud[k];
+    pd[i] = out;
+    PointDyn32 o32;
+    o32.x0[0] = p.x; o32.x0[1] = p.y; o32.x0[2] = p.z;
+    o32.A = (float)p.w;
+    for (int k = 0; k < 9; ++k) o32.K[k] = (float)out.K[k];
+    for (int k = 0; k < 3; ++k) o32.d[k] = (float)out.d[k];
+    pd32[i] = o32;
+}
+
 // determineCorners (kd_tree.F90:1666-1715) against the initial surface.
 __global__ void k_corners(int n, double threshold, const int32_t *__restrict__ n2eStart, const int32_t *__restrict__ n2e,
                           const double4 *__restrict__ fprops, const double *__restrict__ normals0,
@@ -338,12 +500,14 @@ __global__ void __launch_bounds__(256) k_exact_fwd(int nvol, const double4 *__re
         }
     }
     if (valid) {
+        // tangent (warpDerivFwd.F90:92-97): numerator tangent / den, no base
         const int64_t j = perm[k];
         const double oden = 1.0 / dd;
-        xv[3 * j] = v.x + n0 * oden;
-        xv[3 * j + 1] = v.y + n1 * oden;
-        xv[3 * j + 2] = v.z + n2 * oden;
-        den[k] = dd;
+        const bool tg = P.tangent != 0;
+        xv[3 * j] = (tg ? 0.0 : v.x) + n0 * oden;
+        xv[3 * j + 1] = (tg ? 0.0 : v.y) + n1 * oden;
+        xv[3 * j + 2] = (tg ? 0.0 : v.z) + n2 * oden;
+        if (!tg) den[k] = dd;
     }
 }
 
@@ -712,12 +876,14 @@ __global__ void __launch_bounds__(256) k_fast_fwd(int nvol, const double4 *__res
         }
     }
     if (valid) {
+        // tangent (warpDerivFwd.F90:92-97): numerator tangent / den, no base
         const int64_t j = perm[k];
         const double oden = 1.0 / dd;
-        xv[3 * j] = v.x + n0 * oden;
-        xv[3 * j + 1] = v.y + n1 * oden;
-        xv[3 * j + 2] = v.z + n2 * oden;
-        den[k] = dd;
+        const bool tg = P.tangent != 0;
+        xv[3 * j] = (tg ? 0.0 : v.x) + n0 * oden;
+        xv[3 * j + 1] = (tg ? 0.0 : v.y) + n1 * oden;
+        xv[3 * j + 2] = (tg ? 0.0 : v.z) + n2 * oden;
+        if (!tg) den[k] = dd;
     }
 }
 
@@ -1261,6 +1427,14 @@ static inline unsigned xcd_grid(int64_t n) {
 
 void launch_face_props(hipStream_t st, const DevSurface &S, const double *xs) {
     if (S.nface) k_face_props<<<grid_for(S.nface, 256), 256, 0, st>>>(S.nface, S.facePtr, S.faceConn, S.xuInd, xs, S.fprops);
+}
+
+void launch_props_tangent(hipStream_t st, const DevSurface &S, const double *xs, const double *dxs, int useRot) {
+    if (S.nface)
+        k_face_props_d<<<grid_for(S.nface, 256), 256, 0, st>>>(S.nface, S.facePtr, S.faceConn, S.xuInd, xs, dxs,
+                                                               S.fprops, S.fpropsd);
+    k_node_props_d<<<grid_for(S.n, 256), 256, 0, st>>>(S.n, useRot, S.n2eStart, S.n2e, S.fprops, S.fpropsd, S.xuInd,
+                                                       S.corner, dxs, S.ps, S.normals0, S.pd, S.pd32);
 }
 
 void launch_node_props(hipStream_t st, const DevSurface &S, const double *xs, int initial, int useRot) {

@@ -149,6 +149,13 @@ class LibIdwarp:
         d = _lib.f64(dxv_f)
         _lib.check(self._L.idw_warp_deriv(self._ensure(), _lib.ptr(d), d.size))
 
+    def warpderivfwd(self, xsdot, outvec):
+        """libidwarp.pyf:107-112: outvec (intent inout, 3*n_vol) = (dXv/dXs) xsdot."""
+        d = _lib.f64(xsdot)
+        out = np.zeros(np.asarray(outvec).size)
+        _lib.check(self._L.idw_warp_deriv_fwd(self._ensure(), _lib.ptr(d), d.size, _lib.ptr(out), out.size))
+        outvec[...] = out.reshape(np.shape(outvec))
+
     def getdxs(self, output):
         """intent(inout): fills ``output`` (flat, 3*n)."""
         out = np.zeros(np.asarray(output).size)

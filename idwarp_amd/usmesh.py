@@ -220,6 +220,24 @@ class USMesh:
             dXvWarp = dXv
         _lib.check(self._L.idw_warp_deriv(self._ctx, _lib.ptr(dXvWarp), dXvWarp.size))
 
+    def warpDerivFwd(self, dXs, solverVec=True):
+        """Forward-mode product dXv = (dXv/dXs) dXs (UnstructuredMesh.py:428-459,
+        warpDerivFwd.F90).  dXs has the shape of getSurfaceCoordinates();
+        the result is in solver ordering when solverVec, else warp ordering.
+        As the reference does, the mesh is warped first."""
+        dXs = _lib.f64(dXs).ravel()
+        if dXs.size != 3 * self.nSurf:
+            raise Error("Incorrect dimensions of dXs: expected %d values, received %d" % (3 * self.nSurf, dXs.size))
+        dXvWarp = np.zeros(3 * self.nVol, self.dtype)
+        self.warpMesh()
+        _lib.check(self._L.idw_warp_deriv_fwd(self._ctx, _lib.ptr(dXs), dXs.size, _lib.ptr(dXvWarp), dXvWarp.size))
+        if solverVec:
+            dXv = np.zeros(self._solverDOF(), self.dtype)
+            _lib.check(self._L.idw_warp_to_solver_grid(self._ctx, _lib.ptr(dXvWarp), dXvWarp.size, _lib.ptr(dXv),
+                                                       dXv.size))
+            return dXv
+        return dXvWarp
+
     def getdXs(self):
         dXs = np.zeros((self.nSurf, 3), self.dtype)
         _lib.check(self._L.idw_get_dxs(self._ctx, _lib.ptr(dXs), dXs.size))

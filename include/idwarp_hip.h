@@ -80,18 +80,18 @@ int idw_destroy(idw_ctx *ctx);
  * denominator estimate keep their values from idw_initialize_warping. */
 int idw_set_options(idw_ctx *ctx, const idw_options *opt);
 
-/* createCommonGrid (src/IO/createGrid.F90:1-77, reached from readcgns,
- * libidwarp.pyf:19-21): the undeformed volume nodes, 3*n values.  Must be
+/* createcommongrid (libidwarp.pyf:14-17, src/IO/createGrid.F90): the
+ * undeformed volume nodes, 3*n values.  Must be
  * set before idw_initialize_warping. */
 int idw_set_volume_nodes(idw_ctx *ctx, const double *xv0, int64_t ndof);
 
-/* pointreduce (libidwarp.pyf:36-43, src/utils/pointReduce.F90:1-83):
+/* pointreduce (libidwarp.pyf:195-202, src/utils/pointReduce.F90:1-83):
  * merge points within tol.  pts and unique_pts are 3*n; link (n) is 1-based
  * into unique_pts.  Context-free. */
 int idw_point_reduce(const double *pts, int64_t n, double tol, double *unique_pts, int32_t *link,
                      int64_t *n_unique);
 
-/* initializewarping (libidwarp.pyf:55-67, src/warp/initializeWarping.F90):
+/* initializewarping (libidwarp.pyf:55-68, src/warp/initializeWarping.F90):
  * pts 3*n_pts initial surface (local order), unique_pts 3*n_unique (sorted
  * by z, y, x), link 1-based (n_pts), face_sizes (n_faces), face_conn
  * 0-based into pts.  Builds the kd-tree, nodal properties, error tables and
@@ -100,7 +100,7 @@ int idw_initialize_warping(idw_ctx *ctx, const double *pts, int64_t n_pts, const
                            int64_t n_unique, const int32_t *link, const int32_t *face_sizes, int64_t n_faces,
                            const int32_t *face_conn, int64_t n_conn);
 
-/* setsurfacecoordinates / getsurfacecoordinates (libidwarp.pyf:78-86). */
+/* setsurfacecoordinates / getsurfacecoordinates (libidwarp.pyf:78-87). */
 int idw_set_surface_coordinates(idw_ctx *ctx, const double *xs, int64_t ndof);
 int idw_get_surface_coordinates(idw_ctx *ctx, double *xs, int64_t ndof);
 
@@ -111,7 +111,7 @@ int idw_warp_mesh(idw_ctx *ctx);
 /* getvolumecoordinates (libidwarp.pyf:45-48): copy the warped grid out. */
 int idw_get_volume_coordinates(idw_ctx *ctx, double *xv, int64_t ndof);
 
-/* setexternalmeshindices (libidwarp.pyf:23-26, setExternalMeshIndices.F90):
+/* setexternalmeshindices (libidwarp.pyf:40-43, setExternalMeshIndices.F90):
  * 0-based per-DOF indices into the common grid, n = solver DOF count. */
 int idw_set_external_mesh_indices(idw_ctx *ctx, const int32_t *ind, int64_t n);
 int idw_get_solver_dof(idw_ctx *ctx, int64_t *n);
@@ -130,11 +130,20 @@ int idw_warp_deriv(idw_ctx *ctx, const double *dxv, int64_t ndof);
 /* getdxs (libidwarp.pyf:114-117): 3*n_pts result of the last warp_deriv. */
 int idw_get_dxs(idw_ctx *ctx, double *dxs, int64_t ndof);
 
+/* warpderivfwd (libidwarp.pyf:107-112, src/warp/warpDerivFwd.F90:62-99):
+ * dxv (3*n_vol, warp order) = (dXv/dXs) dxs for the current surface and
+ * options; dxs is 3*n_pts.  The fast mode replays the accept/recurse
+ * decisions of the last warp (a warp runs first if there is none for the
+ * current options). */
+int idw_warp_deriv_fwd(idw_ctx *ctx, const double *dxs, int64_t ndof_s, double *dxv, int64_t ndof_v);
+
 /* Device-resident forms of warpMesh and warpDeriv for frameworks that keep
  * coordinates on the GPU (torch autograd wrapper).  d_xs: 3*n_pts surface,
  * d_xv: 3*n_vol output, d_dxv: 3*n_vol seed, d_dxs: 3*n_pts output. */
 int idw_warp_mesh_dev(idw_ctx *ctx, const double *d_xs, double *d_xv, void *hip_stream);
 int idw_warp_deriv_dev(idw_ctx *ctx, const double *d_xs, const double *d_dxv, double *d_dxs, void *hip_stream);
+/* Device-resident warpDerivFwd: d_dxs 3*n_pts tangent, d_dxv 3*n_vol output. */
+int idw_warp_deriv_fwd_dev(idw_ctx *ctx, const double *d_xs, const double *d_dxs, double *d_dxv, void *hip_stream);
 
 /* Sizes and derived scalars. */
 int idw_get_sizes(idw_ctx *ctx, int64_t *n_vol, int64_t *n_pts, int64_t *n_unique, int64_t *n_tree_nodes);

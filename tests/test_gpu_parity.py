@@ -253,3 +253,67 @@ def test_libidwarp_compat_follows_reference_call_sequence():
     assert _rel(dxs, ora.warpDeriv(dxv)) <= 1e-10
     with pytest.raises(NotImplementedError):
         w.setsymmetryplanes(np.zeros((3, 1)), np.ones((3, 1)))
+
+
+@pytest.mark.parametrize("name", ["sphere", "star"])
+@pytest.mark.parametrize("mode", ["exact", "fast"])
+def test_tangent_fp64_and_dot_product(name, mode):
+    """warpDerivFwd against the oracle's (warpDerivFwd.F90:62-99), and the
+    device dot-product identity <v, J p> == <J^T v, p> between the tangent
+    and the reverse kernels."""
+    m = MESHES[name]
+    print("case", name, mode, flush=True)
+    ora, gpu = _pair(m, evalMode=mode, errTol=1e-4)
+    xs = m["perturb"](m["pts"])
+    ora.setSurfaceCoordinates(xs)
+    gpu.setSurfaceCoordinates(xs)
+    ora.warpMesh()
+    rng = np.random.default_rng(12)
+    p = rng.standard_normal(xs.size)
+    ref = ora.warpDerivFwd(p)
+    got = gpu.warpDerivFwd(p.reshape(-1, 3), solverVec=False)
+    r = _rel(got, ref)
+    print("tangent rel err", r, "max", np.abs(got - ref).max() / np.abs(ref).max())
+    assert r <= 1e-10
+    assert np.abs(got - ref).max() <= 1e-10 * np.abs(ref).max()
+    v = rng.standard_normal(got.size)
+    gpu.warpDeriv(v, solverVec=False)
+    jtv = gpu.getdXs().ravel()
+    assert abs(v @ got - jtv @ p) <= 1e-9 * abs(v @ got)
+
+
+@pytest.mark.parametrize("mode", ["exact", "fast"])
+def test_tangent_mixed(mode):
+    m = MESHES["star"]
+    ora, gpu = _pair(m, evalMode=mode, errTol=1e-4, precision="mixed", nearWallFactor=1e-3)
+    xs = m["perturb"](m["pts"])
+    ora.setSurfaceCoordinates(xs)
+    gpu.setSurfaceCoordinates(xs)
+    ora.warpMesh()
+    p = np.random.default_rng(13).standard_normal(xs.size)
+    ref = ora.warpDerivFwd(p)
+    got = gpu.warpDerivFwd(p, solverVec=False)
+    assert _rel(got, ref) <= 1e-6
+
+
+def test_autograd_forward_mode():
+    """torch forward-mode AD through WarpFunction.jvp equals warpDerivFwd."""
+    import torch
+    import torch.autograd.forward_ad as fwAD
+
+    from idwarp_amd.autograd import warp
+
+    m = MESHES["ellipsoid"]
+    gpu = USMesh(options=dict(gridFile=m["xv0"], fileType="memory", evalMode="fast", errTol=1e-4))
+    gpu.setSurfaceDefinition(m["pts"], m["conn"], m["faceSizes"])
+    xs_np = m["perturb"](m["pts"])
+    gpu.setSurfaceCoordinates(xs_np)
+    p = np.random.default_rng(14).standard_normal(xs_np.size)
+    ref = gpu.warpDerivFwd(p, solverVec=False)
+    xs = torch.tensor(xs_np.ravel(), dtype=torch.float64, device="cuda")
+    with fwAD.dual_level():
+        dual = fwAD.make_dual(xs, torch.tensor(p, dtype=torch.float64, device="cuda"))
+        out = warp(gpu, dual)
+        primal, tangent = fwAD.unpack_dual(out)
+    np.testing.assert_array_equal(primal.cpu().numpy(), gpu.getWarpGrid())
+    assert _rel(tangent.cpu().numpy(), ref) <= 1e-12
