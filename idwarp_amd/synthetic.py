@@ -210,3 +210,34 @@ def random_dxv(nvol, seed=5, frac=0.1):
     d = rng.standard_normal((nvol, 3))
     d[rng.random(nvol) >= frac] = 0.0
     return d.ravel()
+
+
+def shifted(m, offset):
+    """The mesh m moved rigidly by offset (volume, surface and its
+    perturbation)."""
+    off = np.asarray(offset, np.float64)
+    base = m["perturb"]
+    return dict(m, xv0=m["xv0"] + off, pts=m["pts"] + off, perturb=lambda p: base(p - off) + off,
+                name=m["name"] + "_shifted")
+
+
+def reflect(x, point, normal):
+    """Reflection of points x (N, 3) about the plane (point, unit normal)."""
+    n = np.asarray(normal, np.float64)
+    return x - 2.0 * np.outer((x - np.asarray(point, np.float64)) @ n, n)
+
+
+def mirrored_body(m, point, normal):
+    """Surface of m plus its mirror image about the plane (same face lists,
+    offset), with the perturbation mirrored too; volume nodes are m's.  A
+    warp of this surface without symmetry planes equals the warp of m with
+    the plane, when m does not touch the plane."""
+    npts = len(m["pts"])
+    base = m["perturb"]
+
+    def perturb(p):
+        return np.concatenate([base(p[:npts]), reflect(base(reflect(p[npts:], point, normal)), point, normal)])
+
+    return dict(xv0=m["xv0"], pts=np.concatenate([m["pts"], reflect(m["pts"], point, normal)]),
+                conn=np.concatenate([m["conn"], m["conn"] + npts]), faceSizes=np.concatenate([m["faceSizes"]] * 2),
+                perturb=perturb, name=m["name"] + "_mirrored")

@@ -20,7 +20,9 @@
  *    denominator estimate are pinned by closed forms (rigid translation and
  *    rotation reproduce exactly), brute force on tiny meshes, the fast==exact
  *    identity at errTol=0, central finite differences and the dot-product
- *    identity; kd_tree.F90 itself needs PETSc and cannot be built here.
+ *    identity; kd_tree.F90 itself needs PETSc and cannot be built here;
+ *  - symmetry planes by the mirrored-body identity (a body plus its mirror
+ *    image without planes equals the body with the plane, exact mode).
  */
 #include <math.h>
 #include <stdint.h>
@@ -71,7 +73,58 @@ typedef struct {
     double orstar[NERR - 1];
     onode *nodes;
     int nnodes, cap, bucket;
+    int nplanes; /* symmetry planes (setSymmetryPlanes); nLoop = 2^nplanes */
+    double symmPts[6][3], symmNormals[6][3];
 } octx;
+
+/* ------------------------------------------------------------------ */
+/* Symmetry planes (initializeWarping.F90:511-659).  Loop index kk is  */
+/* 1-based as in the reference; plane j (1-based) is applied when      */
+/* mod(kk-1, 2^j) >= 2^(j-1).                                          */
+/* ------------------------------------------------------------------ */
+static int mirror_on(int kk, int j) { return (kk - 1) % (1 << j) >= (1 << (j - 1)); }
+
+/* getMirrorPt (initializeWarping.F90:533-577) */
+static void mirror_pt(const octx *t, const double *pt, double *r, int kk) {
+    r[0] = pt[0]; r[1] = pt[1]; r[2] = pt[2];
+    for (int j = 1; j <= t->nplanes; j++) {
+        if (!mirror_on(kk, j)) continue;
+        const double *p = t->symmPts[j - 1], *n = t->symmNormals[j - 1];
+        double d[3] = {r[0] - p[0], r[1] - p[1], r[2] - p[2]};
+        double dp = d[0] * n[0] + d[1] * n[1] + d[2] * n[2];
+        for (int k = 0; k < 3; k++) r[k] = r[k] - 2 * n[k] * dp;
+    }
+}
+
+/* getMirrorNumerator (:579-600) */
+static void mirror_num(const octx *t, double *num, int kk) {
+    for (int j = 1; j <= t->nplanes; j++) {
+        if (!mirror_on(kk, j)) continue;
+        const double *n = t->symmNormals[j - 1];
+        double dp = num[0] * n[0] + num[1] * n[1] + num[2] * n[2];
+        for (int k = 0; k < 3; k++) num[k] = num[k] - 2.0 * dp * n[k];
+    }
+}
+
+/* getMirrorNumerator_b (:602-634) */
+static void mirror_num_b(const octx *t, double *numb, int kk) {
+    for (int j = t->nplanes; j >= 1; j--) {
+        if (!mirror_on(kk, j)) continue;
+        const double *n = t->symmNormals[j - 1];
+        double dpb = -(n[0] * 2.0 * numb[0] + n[1] * 2.0 * numb[1] + n[2] * 2.0 * numb[2]);
+        for (int k = 0; k < 3; k++) numb[k] = numb[k] + n[k] * dpb;
+    }
+}
+
+/* getMirrorNumerator_d (:636-659) */
+static void mirror_num_d(const octx *t, double *numd, int kk) {
+    for (int j = 1; j <= t->nplanes; j++) {
+        if (!mirror_on(kk, j)) continue;
+        const double *n = t->symmNormals[j - 1];
+        double dpd = n[0] * numd[0] + n[1] * numd[1] + n[2] * numd[2];
+        for (int k = 0; k < 3; k++) numd[k] = numd[k] - 2.0 * n[k] * dpd;
+    }
+}
 
 /* ------------------------------------------------------------------ */
 /* vectorUtils.f90:7-103                                              */
@@ -1129,13 +1182,31 @@ void *orc_create(const orc_options *opt, const double *pts, int npts, const doub
     return t;
 }
 
-/* denominator0 (initializeWarping.F90:240-247), nLoop = 1 */
+/* setSymmetryPlanes (initializeWarping.F90:511-531): n <= 6 planes given
+ * by a point and a normal (used as given, as the reference does). */
+int orc_set_symmetry_planes(void *h, const double *pts, const double *normals, int n) {
+    octx *t = h;
+    if (n < 0 || n > 6) return -1;
+    t->nplanes = n;
+    for (int j = 0; j < n; j++)
+        for (int k = 0; k < 3; k++) {
+            t->symmPts[j][k] = pts[3 * j + k];
+            t->symmNormals[j][k] = normals[3 * j + k];
+        }
+    return 0;
+}
+
+/* denominator0 (initializeWarping.F90:240-247), summed over the mirror
+ * passes into one running accumulator */
 void orc_den0(void *h, const double *xv0, long nvol, double *den0) {
     octx *t = h;
 #pragma omp parallel for schedule(dynamic, 256) num_threads(g_threads)
     for (long j = 0; j < nvol; j++) {
-        double d = 0.0;
-        wi_estimate(t, 0, xv0 + 3 * j, &d);
+        double d = 0.0, r[3];
+        for (int kk = 1; kk <= (1 << t->nplanes); kk++) {
+            mirror_pt(t, xv0 + 3 * j, r, kk);
+            wi_estimate(t, 0, r, &d);
+        }
         den0[j] = d;
     }
 }
@@ -1170,13 +1241,20 @@ void orc_warp(void *h, const double *xs, const double *xv0, long nvol, const dou
     int fast = t->opt.evalMode == 1;
 #pragma omp parallel for schedule(dynamic, 256) num_threads(g_threads)
     for (long j = 0; j < nvol; j++) {
-        const double *r = xv0 + 3 * j;
-        double num[3] = {0, 0, 0}, d = 0.0;
-        if (!fast) eval_exact(t, &t->nodes[0], r, num, &d);
-        else eval_node(t, 0, r, num, &d, den0[j], NULL);
-        double oden = 1.0 / d;
-        for (int k = 0; k < 3; k++) xv[3 * j + k] = r[k] + num[k] * oden;
-        den[j] = d;
+        const double *r0 = xv0 + 3 * j;
+        double numer[3] = {0, 0, 0}, dsum = 0.0;
+        for (int kk = 1; kk <= (1 << t->nplanes); kk++) {
+            double r[3], num[3] = {0, 0, 0}, d = 0.0;
+            mirror_pt(t, r0, r, kk);
+            if (!fast) eval_exact(t, &t->nodes[0], r, num, &d);
+            else eval_node(t, 0, r, num, &d, den0[j], NULL);
+            mirror_num(t, num, kk);
+            for (int k = 0; k < 3; k++) numer[k] = numer[k] + num[k];
+            dsum = dsum + d;
+        }
+        double oden = 1.0 / dsum;
+        for (int k = 0; k < 3; k++) xv[3 * j + k] = r0[k] + numer[k] * oden;
+        den[j] = dsum;
     }
     g_times[0] = t1 - t0;
     g_times[1] = wall() - t1;
@@ -1189,9 +1267,12 @@ void orc_counts(void *h, const double *xs, const double *xv0, long nvol, const d
     warp_setup(t, xs);
 #pragma omp parallel for schedule(dynamic, 256) num_threads(g_threads)
     for (long j = 0; j < nvol; j++) {
-        double num[3] = {0, 0, 0}, d = 0.0;
         ocount c = {0, 0, 0, 0};
-        eval_node(t, 0, xv0 + 3 * j, num, &d, den0[j], &c);
+        for (int kk = 1; kk <= (1 << t->nplanes); kk++) {
+            double r[3], num[3] = {0, 0, 0}, d = 0.0;
+            mirror_pt(t, xv0 + 3 * j, r, kk);
+            eval_node(t, 0, r, num, &d, den0[j], &c);
+        }
         out[4 * j] = c.V; out[4 * j + 1] = c.A; out[4 * j + 2] = c.L; out[4 * j + 3] = c.E;
     }
 }
@@ -1220,11 +1301,15 @@ void orc_warp_deriv(void *h, const double *xs, const double *xv0, long nvol, con
 #endif
         long chunk = (nvol + nth - 1) / nth, j0 = tid * chunk, j1 = j0 + chunk < nvol ? j0 + chunk : nvol;
         for (long j = j0; j < j1; j++) {
-            const double *r = xv0 + 3 * j;
-            double oden = 1.0 / den[j];
-            double numb[3] = {dxv[3 * j] * oden, dxv[3 * j + 1] * oden, dxv[3 * j + 2] * oden};
-            if (!fast) eval_exact_b(t, &t->nodes[0], r, numb, pM[tid], pB[tid]);
-            else eval_node_b(t, 0, r, numb, den0[j], pM[tid], pB[tid], qM[tid], qB[tid]);
+            for (int kk = 1; kk <= (1 << t->nplanes); kk++) {
+                double r[3];
+                mirror_pt(t, xv0 + 3 * j, r, kk);
+                double oden = 1.0 / den[j];
+                double numb[3] = {dxv[3 * j] * oden, dxv[3 * j + 1] * oden, dxv[3 * j + 2] * oden};
+                mirror_num_b(t, numb, kk);
+                if (!fast) eval_exact_b(t, &t->nodes[0], r, numb, pM[tid], pB[tid]);
+                else eval_node_b(t, 0, r, numb, den0[j], pM[tid], pB[tid], qM[tid], qB[tid]);
+            }
         }
     }
     memset(t->Mib, 0, sizeof(double) * 9 * n);
@@ -1265,12 +1350,17 @@ void orc_warp_deriv_fwd(void *h, const double *xs, const double *xsd, const doub
     int fast = t->opt.evalMode == 1;
 #pragma omp parallel for schedule(dynamic, 256) num_threads(g_threads)
     for (long j = 0; j < nvol; j++) {
-        const double *r = xv0 + 3 * j;
-        double numd[3] = {0, 0, 0};
-        if (!fast) eval_exact_d(t, &t->nodes[0], r, numd);
-        else eval_node_d(t, 0, r, numd, den0[j]);
+        double numer[3] = {0, 0, 0};
+        for (int kk = 1; kk <= (1 << t->nplanes); kk++) {
+            double r[3], numd[3] = {0, 0, 0};
+            mirror_pt(t, xv0 + 3 * j, r, kk);
+            if (!fast) eval_exact_d(t, &t->nodes[0], r, numd);
+            else eval_node_d(t, 0, r, numd, den0[j]);
+            mirror_num_d(t, numd, kk);
+            for (int k = 0; k < 3; k++) numer[k] = numer[k] + numd[k];
+        }
         double oden = 1.0 / den[j];
-        for (int k = 0; k < 3; k++) xvd[3 * j + k] = numd[k] * oden;
+        for (int k = 0; k < 3; k++) xvd[3 * j + k] = numer[k] * oden;
     }
 }
 

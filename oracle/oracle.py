@@ -65,6 +65,8 @@ def lib():
         L.orc_warp_deriv.argtypes = [P, dp, dp, L64, dp, dp, dp, dp]
         L.orc_warp_deriv_fwd.argtypes = [P, dp, dp, dp, L64, dp, dp, dp]
         L.orc_counts.argtypes = [P, dp, dp, L64, dp, dp]
+        L.orc_set_symmetry_planes.restype = I
+        L.orc_set_symmetry_planes.argtypes = [P, dp, dp, I]
         L.orc_point_reduce.restype = I
         L.orc_point_reduce.argtypes = [dp, I, D, dp, ip]
         L.orc_num_nodes.restype = I
@@ -121,7 +123,7 @@ def surface_definition(pts, conn, faceSizes):
 class OracleMesh:
     """One surface + volume mesh instance on the CPU oracle."""
 
-    def __init__(self, pts, conn, faceSizes, xv0, **opts):
+    def __init__(self, pts, conn, faceSizes, xv0, symmetryPlanes=None, **opts):
         L = lib()
         self.opts = dict(DEFAULTS)
         self.opts.update(opts)
@@ -135,6 +137,11 @@ class OracleMesh:
         self.h = L.orc_create(ctypes.byref(o), pts.ravel(), len(pts), uniq.ravel(), len(uniq), link, faceSizes,
                               len(faceSizes), conn, len(conn))
         self.n = L.orc_num_unique(self.h)
+        if symmetryPlanes:  # [[point, normal], ...] as USMesh's 'symmetryPlanes' option
+            sp = np.ascontiguousarray([p for p, _ in symmetryPlanes], np.float64).ravel()
+            sn = np.ascontiguousarray([n for _, n in symmetryPlanes], np.float64).ravel()
+            if L.orc_set_symmetry_planes(self.h, sp, sn, len(symmetryPlanes)) != 0:
+                raise ValueError("at most 6 symmetry planes")
         self.xv0 = np.ascontiguousarray(xv0, np.float64).reshape(-1, 3)
         self.nvol = len(self.xv0)
         self.den0 = np.zeros(self.nvol)

@@ -262,3 +262,72 @@ def test_tangent_of_rigid_translation():
         t = np.array([0.3, -0.2, 0.7])
         jp = om.warpDerivFwd(np.tile(t, len(pts))).reshape(-1, 3)
         assert np.abs(jp - t).max() < 1e-13
+
+
+def _symmetric_case():
+    half = synthetic.shifted(synthetic.sphere_omesh(13, 9, 5), (0.3, 2.5, -0.2))
+    plane = ([0.0, 0.0, 0.0], [0.0, 1.0, 0.0])
+    return half, plane, synthetic.mirrored_body(half, *plane)
+
+
+def test_symmetry_plane_equals_mirrored_body():
+    """getMirrorPt / getMirrorNumerator (initializeWarping.F90:533-600): the
+    body with a symmetry plane warps the volume as the body plus its mirror
+    image does without one (exact mode, Ldef matched)."""
+    half, plane, full = _symmetric_case()
+    oh = O.OracleMesh(half["pts"], half["conn"], half["faceSizes"], half["xv0"], evalMode="exact",
+                      symmetryPlanes=[plane])
+    of = O.OracleMesh(full["pts"], full["conn"], full["faceSizes"], full["xv0"], evalMode="exact")
+    of.setOptions(LdefFact=oh.Ldef0 / of.Ldef0)
+    xs = half["perturb"](half["pts"])
+    oh.setSurfaceCoordinates(xs)
+    of.setSurfaceCoordinates(full["perturb"](full["pts"]))
+    a, b = oh.warpMesh(), of.warpMesh()
+    assert np.abs(a - b).max() <= 1e-12 * oh.Ldef0
+    assert np.abs(a - half["xv0"].ravel()).max() > 1e-3  # the warp is not trivial
+    # the reverse: dXs of the half body equals the full body's restricted to
+    # its first copy plus the mirrored sensitivities of the second copy
+    v = synthetic.random_dxv(len(half["xv0"]), seed=3, frac=1.0)
+    ga = oh.warpDeriv(v).reshape(-1, 3)
+    gb = of.warpDeriv(v).reshape(-1, 3)
+    n = len(half["pts"])
+    R = np.eye(3) - 2.0 * np.outer(plane[1], plane[1])
+    assert np.abs(ga - (gb[:n] + gb[n:] @ R)).max() <= 1e-10 * np.abs(ga).max()
+
+
+@pytest.mark.parametrize("mode", ["exact", "fast"])
+def test_symmetry_reverse_and_tangent_fd(mode):
+    half, plane, _ = _symmetric_case()
+    om = O.OracleMesh(half["pts"], half["conn"], half["faceSizes"], half["xv0"], evalMode=mode, errTol=1e-4,
+                      symmetryPlanes=[plane, ([0.0, 0.0, -3.0], [0.0, 0.0, 1.0])])
+    xs = half["perturb"](half["pts"]).ravel()
+    om.setSurfaceCoordinates(xs)
+    om.warpMesh()
+    rng = np.random.default_rng(21)
+    v = rng.standard_normal(3 * len(half["xv0"]))
+    jtv = om.warpDeriv(v)
+    h = 1e-6
+    for _ in range(2):
+        p = rng.standard_normal(xs.size)
+        jp = om.warpDerivFwd(p)
+        om.setSurfaceCoordinates(xs + h * p)
+        xp = om.warpMesh()
+        om.setSurfaceCoordinates(xs - h * p)
+        xm = om.warpMesh()
+        om.setSurfaceCoordinates(xs)
+        om.warpMesh()
+        fd = (xp - xm) / (2 * h)
+        assert np.linalg.norm(jp - fd) <= 1e-6 * np.linalg.norm(fd)
+        assert abs(v @ fd - jtv @ p) <= 1e-6 * abs(v @ fd)
+        assert abs(v @ jp - jtv @ p) <= 1e-9 * abs(v @ jp)
+
+
+def test_symmetry_fast_with_zero_tolerance_is_exact():
+    half, plane, _ = _symmetric_case()
+    kw = dict(symmetryPlanes=[plane])
+    oe = O.OracleMesh(half["pts"], half["conn"], half["faceSizes"], half["xv0"], evalMode="exact", **kw)
+    of = O.OracleMesh(half["pts"], half["conn"], half["faceSizes"], half["xv0"], evalMode="fast", errTol=0.0, **kw)
+    xs = half["perturb"](half["pts"])
+    oe.setSurfaceCoordinates(xs)
+    of.setSurfaceCoordinates(xs)
+    assert np.array_equal(oe.warpMesh(), of.warpMesh())
