@@ -66,8 +66,11 @@ struct idw_ctx {
     idw::HostSurface hs;
     idw::DevSurface S;
     idw::DevVolume V;
-    idw::DevPlan plan;
-    bool planValid = false;  // plan written by the last forward at denOpt
+    std::vector<idw::DevPlan> plans;  // one per symmetry-plane pass
+    bool planValid = false;  // plans written by the last forward at denOpt
+    double orderTol = -1.0, orderT2 = -1.0;  // key of the near-wall ordering of the volume nodes
+    int nplanes = 0;  // symmetry planes; 2^nplanes passes (setSymmetryPlanes)
+    double symmPts[idw::MAX_PLANES][3] = {}, symmNormals[idw::MAX_PLANES][3] = {};
     int64_t npts = 0;
     double *xv0w = nullptr, *xv = nullptr, *xs = nullptr, *dxs = nullptr, *dxv = nullptr;
     double *dxsIn = nullptr;  // host-API tangent seed (3 * npts)
@@ -140,9 +143,37 @@ int ensure_device(idw_ctx *c) {
 
 void drop_plan(idw_ctx *c);
 
+int nloop(const idw_ctx *c) { return 1 << c->nplanes; }
+
+// Reflection of mirror pass kk (0-based; the reference's kk - 1).
+idw::Mirror mirror_of(const idw_ctx *c, int kk) {
+    idw::Mirror M;
+    M.n = c->nplanes;
+    M.mask = kk;
+    for (int j = 0; j < c->nplanes; ++j)
+        for (int k = 0; k < 3; ++k) {
+            M.p[j][k] = c->symmPts[j][k];
+            M.nrm[j][k] = c->symmNormals[j][k];
+        }
+    return M;
+}
+
+int pass_of(const idw_ctx *c, int kk) {
+    if (nloop(c) == 1) return idw::PASS_ONLY;
+    return kk == 0 ? idw::PASS_FIRST : (kk == nloop(c) - 1 ? idw::PASS_LAST : idw::PASS_MID);
+}
+
+// Accumulators of the mirror passes (only with symmetry planes).
+int ensure_mirror_buffers(idw_ctx *c) {
+    if (nloop(c) == 1 || c->V.acc) return IDW_OK;
+    TRY(dalloc(c, &c->V.acc, c->V.n));
+    TRY(dalloc(c, &c->V.volm, c->V.n));
+    return IDW_OK;
+}
+
 int compute_den0(idw_ctx *c) {
     if (!c->volSet || !c->surfInit || c->V.n == 0) return IDW_OK;
-    idw::launch_den0(c->st, c->S, c->V, make_params(c, true));
+    for (int kk = 0; kk < nloop(c); ++kk) idw::launch_den0(c->st, c->S, c->V, make_params(c, true), mirror_of(c, kk));
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->st));
     c->denValid = false;
@@ -151,11 +182,14 @@ int compute_den0(idw_ctx *c) {
 }
 
 void drop_plan(idw_ctx *c) {
-    dfree(c, c->plan.entries);
-    dfree(c, c->plan.waveOff);
-    dfree(c, c->plan.waveCount);
-    dfree(c, c->plan.mismatch);
-    c->plan = idw::DevPlan();
+    for (idw::DevPlan &P : c->plans) {
+        dfree(c, P.entries);
+        dfree(c, P.waveOff);
+        dfree(c, P.waveCount);
+        dfree(c, P.mismatch);
+    }
+    c->plans.assign(nloop(c), idw::DevPlan());
+    c->orderTol = c->orderT2 = -1.0;
     c->planValid = false;
 }
 
@@ -173,10 +207,9 @@ double plan_near_t2(const idw_ctx *c) {
 // others (Morton order kept within both groups), so that the other waves
 // run FP32 throughout.  Redone when errTol or the near-wall distance change.
 int order_near_wall(idw_ctx *c) {
-    idw::DevPlan &P = c->plan;
     const double nt2 = plan_near_t2(c);
     if (nt2 <= 0.0 || c->V.n == 0) return IDW_OK;
-    if (P.orderTol == c->opt.errTol && P.orderT2 == nt2) return IDW_OK;
+    if (c->orderTol == c->opt.errTol && c->orderT2 == nt2) return IDW_OK;
     const int64_t n = c->V.n;
     uint64_t *keys = nullptr, *keys2 = nullptr;
     int32_t *ids = nullptr, *order = nullptr, *perm2 = nullptr;
@@ -202,15 +235,23 @@ int order_near_wall(idw_ctx *c) {
     c->V.vol = vol2;
     c->V.perm = perm2;
     c->denValid = false;  // den is stored in traversal order
-    P.orderTol = c->opt.errTol;
-    P.orderT2 = nt2;
-    P.sized = false;
+    c->orderTol = c->opt.errTol;
+    c->orderT2 = nt2;
+    for (idw::DevPlan &P : c->plans) P.sized = P.filled = false;
     return IDW_OK;
 }
 
+int size_one_plan(idw_ctx *c, int kk);
+
 int size_plan(idw_ctx *c) {
-    idw::DevPlan &P = c->plan;
+    if ((int)c->plans.size() != nloop(c)) drop_plan(c);
     TRY(order_near_wall(c));
+    for (int kk = 0; kk < nloop(c); ++kk) TRY(size_one_plan(c, kk));
+    return IDW_OK;
+}
+
+int size_one_plan(idw_ctx *c, int kk) {
+    idw::DevPlan &P = c->plans[kk];
     if (P.sized && P.errTol == c->opt.errTol) return IDW_OK;
     P.filled = false;
     const int64_t nwave = ((c->V.n + 255) / 256) * 4;
@@ -223,7 +264,7 @@ int size_plan(idw_ctx *c) {
         P.nwave = nwave;
     }
     HIPCHK(hipMemsetAsync(P.waveCount, 0, sizeof(int64_t) * (nwave + 1), c->st));
-    if (c->V.n) idw::launch_plan_count(c->st, c->S, c->V, c->opt.errTol, P.waveCount);
+    if (c->V.n) idw::launch_plan_count(c->st, c->S, c->V, mirror_of(c, kk), c->opt.errTol, P.waveCount);
     HIPCHK(hipGetLastError());
     HIPCHK(idw::exclusive_scan(c->st, P.waveCount, P.waveOff, nwave + 1));
     int64_t total = 0;
@@ -241,14 +282,16 @@ int size_plan(idw_ctx *c) {
 }
 
 int check_plan(idw_ctx *c) {
-    if (!c->plan.mismatch) return IDW_OK;
-    unsigned int mm = 0;
-    HIPCHK(hipMemcpyAsync(&mm, c->plan.mismatch, sizeof(mm), hipMemcpyDeviceToHost, c->st));
-    HIPCHK(hipStreamSynchronize(c->st));
-    if (mm) {
-        c->planValid = false;
-        c->plan.sized = false;
-        return fail(IDW_EHIP, "fast-mode plan of %u waves disagreed with its counting pass", mm);
+    for (idw::DevPlan &P : c->plans) {
+        if (!P.mismatch) continue;
+        unsigned int mm = 0;
+        HIPCHK(hipMemcpyAsync(&mm, P.mismatch, sizeof(mm), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        if (mm) {
+            c->planValid = false;
+            P.sized = false;
+            return fail(IDW_EHIP, "fast-mode plan of %u waves disagreed with its counting pass", mm);
+        }
     }
     return IDW_OK;
 }
@@ -266,25 +309,31 @@ int run_warp(idw_ctx *c, const double *xs, double *xv) {
     if (!c->surfInit) return fail(IDW_ESTATE, "warp before initialize_warping");
     if (!c->volSet) return fail(IDW_ESTATE, "no volume nodes set");
     const int fast = c->opt.evalMode == IDW_EVAL_FAST;
-    const idw::WParams P = make_params(c, false);
+    idw::WParams P = make_params(c, false);
     if (fast) TRY(size_plan(c));
+    TRY(ensure_mirror_buffers(c));
     HIPCHK(hipEventRecord(c->ev[0], c->st));
     TRY(surface_pass(c, xs, fast));
     HIPCHK(hipEventRecord(c->ev[1], c->st));
     if (fast) {
-        idw::DevPlan &pl = c->plan;
         const double nt2 = plan_near_t2(c);
-        if (!c->opt.reusePlan || !pl.filled || pl.errTol != c->opt.errTol || pl.nearT2 != nt2) {
-            HIPCHK(hipMemsetAsync(pl.mismatch, 0, sizeof(unsigned int), c->st));
-            if (c->V.n) idw::launch_plan_fill(c->st, c->S, c->V, c->opt.errTol, nt2, pl);
-            HIPCHK(hipGetLastError());
-            pl.filled = true;
-            pl.nearT2 = nt2;
+        for (int kk = 0; kk < nloop(c); ++kk) {
+            idw::DevPlan &pl = c->plans[kk];
+            if (!c->opt.reusePlan || !pl.filled || pl.errTol != c->opt.errTol || pl.nearT2 != nt2) {
+                HIPCHK(hipMemsetAsync(pl.mismatch, 0, sizeof(unsigned int), c->st));
+                if (c->V.n) idw::launch_plan_fill(c->st, c->S, c->V, mirror_of(c, kk), c->opt.errTol, nt2, pl);
+                HIPCHK(hipGetLastError());
+                pl.filled = true;
+                pl.nearT2 = nt2;
+            }
         }
     }
     HIPCHK(hipEventRecord(c->ev[2], c->st));
-    if (c->V.n)
-        idw::launch_forward(c->st, c->S, c->V, P, fast, c->opt.precision, xv, fast ? &c->plan : nullptr);
+    for (int kk = 0; kk < nloop(c) && c->V.n; ++kk) {
+        P.pass = pass_of(c, kk);
+        idw::launch_forward(c->st, c->S, c->V, P, mirror_of(c, kk), fast, c->opt.precision, xv,
+                            fast ? &c->plans[kk] : nullptr);
+    }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c->ev[3], c->st));
     c->denOpt = c->opt;
@@ -313,14 +362,15 @@ int run_deriv(idw_ctx *c, const double *xs, const double *dxv, double *dxs) {
     if (!c->denValid || !same_numeric_options(c->denOpt, c->opt) || (fast && !c->planValid))
         TRY(run_warp(c, xs, c->xv));
     if (!fast) TRY(ensure_partial(c));
+    TRY(ensure_mirror_buffers(c));
     HIPCHK(hipEventRecord(c->ev[4], c->st));
     TRY(surface_pass(c, xs, false));
     idw::launch_reverse_prep(c->st, c->S, fast || c->V.n == 0);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c->ev[5], c->st));
-    if (c->V.n)
-        idw::launch_reverse(c->st, c->S, c->V, make_params(c, false), fast, c->opt.precision, c->partial, c->nchunk,
-                            dxv, &c->plan);
+    for (int kk = 0; kk < nloop(c) && c->V.n; ++kk)
+        idw::launch_reverse(c->st, c->S, c->V, make_params(c, false), mirror_of(c, kk), fast, c->opt.precision,
+                            c->partial, c->nchunk, dxv, fast ? &c->plans[kk] : nullptr);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c->ev[6], c->st));
     idw::launch_nodal_reverse(c->st, c->S, xs, fast, c->opt.useRotations, dxs, c->npts);
@@ -340,6 +390,7 @@ int run_deriv_fwd(idw_ctx *c, const double *xs, const double *dxs, double *dxv) 
     if (!c->volSet) return fail(IDW_ESTATE, "no volume nodes set");
     const int fast = c->opt.evalMode == IDW_EVAL_FAST;
     if (fast && (!c->planValid || !same_numeric_options(c->denOpt, c->opt))) TRY(run_warp(c, xs, c->xv));
+    TRY(ensure_mirror_buffers(c));
     idw::WParams P = make_params(c, false);
     P.tangent = 1;
     HIPCHK(hipEventRecord(c->ev[0], c->st));
@@ -348,7 +399,11 @@ int run_deriv_fwd(idw_ctx *c, const double *xs, const double *dxs, double *dxv) 
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c->ev[1], c->st));
     HIPCHK(hipEventRecord(c->ev[2], c->st));
-    if (c->V.n) idw::launch_forward(c->st, c->S, c->V, P, fast, c->opt.precision, dxv, fast ? &c->plan : nullptr);
+    for (int kk = 0; kk < nloop(c) && c->V.n; ++kk) {
+        P.pass = pass_of(c, kk);
+        idw::launch_forward(c->st, c->S, c->V, P, mirror_of(c, kk), fast, c->opt.precision, dxv,
+                            fast ? &c->plans[kk] : nullptr);
+    }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c->ev[3], c->st));
     c->fwdTimed = true;
@@ -481,6 +536,7 @@ int idw_set_volume_nodes(idw_ctx *c, const double *xv0, int64_t ndof) {
     TRY(ensure_device(c));
     dfree(c, c->xv0w); dfree(c, c->xv); dfree(c, c->dxv);
     dfree(c, c->V.vol); dfree(c, c->V.perm); dfree(c, c->V.den); dfree(c, c->V.numb);
+    dfree(c, c->V.acc); dfree(c, c->V.volm);
     dfree(c, c->partial);
     c->volSet = false;
     c->denValid = false;
@@ -530,6 +586,27 @@ int idw_set_volume_nodes(idw_ctx *c, const double *xv0, int64_t ndof) {
     return compute_den0(c);
 }
 
+int idw_set_symmetry_planes(idw_ctx *c, const double *pts, const double *normals, int64_t n) {
+    if (!c || (n && (!pts || !normals))) return fail(IDW_EINVAL, "null argument");
+    if (n < 0 || n > idw::MAX_PLANES) return fail(IDW_EINVAL, "%lld symmetry planes (at most %d)", (long long)n,
+                                                  idw::MAX_PLANES);
+    TRY(ensure_device(c));
+    HIPCHK(hipStreamSynchronize(c->st));
+    drop_plan(c);
+    dfree(c, c->V.acc);
+    dfree(c, c->V.volm);
+    c->nplanes = (int)n;
+    for (int j = 0; j < c->nplanes; ++j)
+        for (int k = 0; k < 3; ++k) {
+            c->symmPts[j][k] = pts[3 * j + k];
+            c->symmNormals[j][k] = normals[3 * j + k];
+        }
+    drop_plan(c);
+    c->denValid = false;
+    // the denominator estimate sums over the mirror passes
+    return compute_den0(c);
+}
+
 int idw_point_reduce(const double *pts, int64_t n, double tol, double *unique_pts, int32_t *link, int64_t *n_unique) {
     if (n < 0 || (n && (!pts || !unique_pts || !link)) || !n_unique) return fail(IDW_EINVAL, "bad argument");
     if (n >= INT32_MAX) return fail(IDW_EINVAL, "too many points");
@@ -553,13 +630,13 @@ int idw_initialize_warping(idw_ctx *c, const double *pts, int64_t n_pts, const d
     idw::DevSurface &S = c->S;
     void *olds[] = {S.nodes, S.nd, S.nd32, S.internal, S.ps, S.pd, S.pd32, S.normals0, S.nprops, S.corner, S.xuInd,
                     S.facePtr, S.faceConn, S.n2eStart, S.n2e, S.slotStart, S.slots, S.fprops, S.padj, S.nadj,
-                    S.ntmp, S.pb, c->xs, c->dxs, c->partial};
+                    S.ntmp, S.pb, S.fpropsd, c->xs, c->dxs, c->dxsIn, c->partial};
     for (void *p : olds) {
         void *q = p;
         dfree(c, q);
     }
     S = idw::DevSurface();
-    c->xs = c->dxs = c->partial = nullptr;
+    c->xs = c->dxs = c->dxsIn = c->partial = nullptr;
     c->surfInit = false;
     c->denValid = false;
     drop_plan(c);
@@ -851,7 +928,8 @@ int idw_fast_counts(idw_ctx *c, double *counts8) {
     unsigned long long *d = nullptr;
     TRY(dalloc(c, &d, 8));
     HIPCHK(hipMemsetAsync(d, 0, 8 * sizeof(unsigned long long), c->st));
-    if (c->V.n) idw::launch_fast_count(c->st, c->S, c->V, c->opt.errTol, d);
+    for (int kk = 0; kk < nloop(c) && c->V.n; ++kk)
+        idw::launch_fast_count(c->st, c->S, c->V, mirror_of(c, kk), c->opt.errTol, d);
     unsigned long long h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, c->st));

@@ -13,6 +13,22 @@ constexpr int ADJ_STRIDE = 16;  // doubles per adjoint record (12 used, 128 B al
 
 enum { EXP_35 = 0, EXP_INT = 1, EXP_REAL = 2 };
 
+constexpr int MAX_PLANES = 6;
+
+// One pass of the symmetry-plane loop (warpMesh.F90:46-64): plane j (0-based)
+// reflects the volume node when bit j of mask is set, i.e. the reference's
+// getMirrorPt test mod(kk-1, 2^(j+1)) >= 2^j with mask = kk-1.
+struct Mirror {
+    int32_t n = 0, mask = 0;
+    double p[MAX_PLANES][3] = {};
+    double nrm[MAX_PLANES][3] = {};
+};
+
+// How a forward launch combines its sums with those of the other mirror
+// passes: ONLY writes Xv; FIRST / MID store / add (numerator, denominator)
+// into the accumulator; LAST adds the accumulator and writes Xv.
+enum { PASS_ONLY = 0, PASS_FIRST = 1, PASS_MID = 2, PASS_LAST = 3 };
+
 // Scalars of one warp (warpMesh.F90:39-41): Ldef = Ldef0 * LdefFact and
 // alpha^bExp with the current options; the exponents are those frozen into
 // the tree at initialisation.
@@ -21,6 +37,7 @@ struct WParams {
     int32_t expMode, ia, ib;
     int32_t zero;  // always 0: lane & zero turns a uniform index into a per-lane one (vector load)
     int32_t tangent;  // 1: the point/cluster records hold tangents; write num/den only (warpDerivFwd)
+    int32_t pass;     // PASS_* of this launch
 };
 
 struct DevSurface {
@@ -69,6 +86,8 @@ struct DevVolume {
     int32_t *perm = nullptr;  // traversal order -> warp order
     double *den = nullptr;    // denominator of the last warp, traversal order
     double4 *numb = nullptr;  // reverse seed dXv/den, traversal order
+    double4 *acc = nullptr;   // numerator and denominator summed over mirror passes (symmetry planes only)
+    double4 *volm = nullptr;  // mirrored positions of one pass for the exact reverse (symmetry planes only)
 };
 
 void launch_face_props(hipStream_t st, const DevSurface &S, const double *xs);
@@ -79,7 +98,7 @@ void launch_props_tangent(hipStream_t st, const DevSurface &S, const double *xs,
 void launch_corners(hipStream_t st, const DevSurface &S, double threshold);
 void launch_cluster_data(hipStream_t st, const DevSurface &S);
 void launch_errors(hipStream_t st, const DevSurface &S, const WParams &P, const double *sphere60);
-void launch_den0(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P);
+void launch_den0(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, const Mirror &M);
 void launch_morton(hipStream_t st, int64_t n, const double *xv0, double lo[3], double scale[3], uint64_t *keys,
                    int32_t *ids);
 void launch_build_vol(hipStream_t st, int64_t n, const double *xv0, const int32_t *perm, double4 *vol);
@@ -88,17 +107,18 @@ void launch_near_keys(hipStream_t st, const DevSurface &S, const DevVolume &V, d
 void launch_permute_vol(hipStream_t st, int64_t n, const int32_t *order, const double4 *vin, const int32_t *pin,
                         double4 *vout, int32_t *pout);
 void launch_fill(hipStream_t st, double *a, int64_t n, double v);
-void launch_forward(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, int fast, int prec,
-                    double *xv, const DevPlan *plan);
-void launch_prep_numb(hipStream_t st, const DevVolume &V, const double *dxv);
-void launch_plan_count(hipStream_t st, const DevSurface &S, const DevVolume &V, double errTol, int64_t *waveCount);
-void launch_plan_fill(hipStream_t st, const DevSurface &S, const DevVolume &V, double errTol, double nearT2,
-                      const DevPlan &plan);
-void launch_reverse(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, int fast, int prec,
-                    double *partial, int nchunk, const double *dxv, const DevPlan *plan);
+void launch_forward(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, const Mirror &M,
+                    int fast, int prec, double *xv, const DevPlan *plan);
+void launch_prep_numb(hipStream_t st, const DevVolume &V, const Mirror &M, const double *dxv);
+void launch_plan_count(hipStream_t st, const DevSurface &S, const DevVolume &V, const Mirror &M, double errTol,
+                       int64_t *waveCount);
+void launch_plan_fill(hipStream_t st, const DevSurface &S, const DevVolume &V, const Mirror &M, double errTol,
+                      double nearT2, const DevPlan &plan);
+void launch_reverse(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, const Mirror &M,
+                    int fast, int prec, double *partial, int nchunk, const double *dxv, const DevPlan *plan);
 hipError_t exclusive_scan(hipStream_t st, const int64_t *in, int64_t *out, int64_t n);
 void launch_reverse_prep(hipStream_t st, const DevSurface &S, int fast);
-void launch_fast_count(hipStream_t st, const DevSurface &S, const DevVolume &V, double errTol,
+void launch_fast_count(hipStream_t st, const DevSurface &S, const DevVolume &V, const Mirror &M, double errTol,
                        unsigned long long *out);
 void launch_nodal_reverse(hipStream_t st, const DevSurface &S, const double *xs, int fast, int useRot, double *dxs,
                           int64_t npts);

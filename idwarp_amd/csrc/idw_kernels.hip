@@ -83,9 +83,85 @@ __device__ __forceinline__ T wfun(T q, const WParams &P) {
 }
 
 __device__ __forceinline__ void cross3(const double *a, const double *b, double *c) {
+#pragma clang fp contract(off)
     c[0] = a[1] * b[2] - a[2] * b[1];
     c[1] = a[2] * b[0] - a[0] * b[2];
     c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// Symmetry planes.  getMirrorPt (initializeWarping.F90:533-577): reflect the
+// volume node about each plane selected by the pass mask, in plane order.
+// Evaluated without FMA contraction, as the reference rounds it: a mirror
+// image that lands near the wall turns position ulps into weight errors
+// (q^5 at small distance).
+__device__ __forceinline__ void mirror_pt(const Mirror &M, double &x, double &y, double &z) {
+#pragma clang fp contract(off)
+    for (int j = 0; j < M.n; ++j) {
+        if (!((M.mask >> j) & 1)) continue;
+        const double *n = M.nrm[j], *p = M.p[j];
+        const double dp = (x - p[0]) * n[0] + (y - p[1]) * n[1] + (z - p[2]) * n[2];
+        x -= 2.0 * n[0] * dp;
+        y -= 2.0 * n[1] * dp;
+        z -= 2.0 * n[2] * dp;
+    }
+}
+
+// getMirrorNumerator (:579-600): reflect the pass's numerator back.
+__device__ __forceinline__ void mirror_vec(const Mirror &M, double &a, double &b, double &c) {
+#pragma clang fp contract(off)
+    for (int j = 0; j < M.n; ++j) {
+        if (!((M.mask >> j) & 1)) continue;
+        const double *n = M.nrm[j];
+        const double dp = a * n[0] + b * n[1] + c * n[2];
+        a -= 2.0 * dp * n[0];
+        b -= 2.0 * dp * n[1];
+        c -= 2.0 * dp * n[2];
+    }
+}
+
+// getMirrorNumerator_b (:602-634): the transpose, planes in reverse order.
+__device__ __forceinline__ void mirror_vec_b(const Mirror &M, double &a, double &b, double &c) {
+#pragma clang fp contract(off)
+    for (int j = M.n - 1; j >= 0; --j) {
+        if (!((M.mask >> j) & 1)) continue;
+        const double *n = M.nrm[j];
+        const double dpb = -2.0 * (a * n[0] + b * n[1] + c * n[2]);
+        a += n[0] * dpb;
+        b += n[1] * dpb;
+        c += n[2] * dpb;
+    }
+}
+
+// End of a forward launch (warpMesh.F90:56-70): reflect the pass's
+// numerator back, combine it with the other passes' (numerator and
+// denominator are summed over the passes in pass order), and on the last
+// pass write Xv = Xv0 + num / den, or num / den for the tangent
+// (warpDerivFwd.F90:92-97).
+__device__ __forceinline__ void finish_fwd(const WParams &P, const Mirror &M, int k, int64_t j, const double4 &v0,
+                                           double n0, double n1, double n2, double dd, double *__restrict__ xv,
+                                           double *__restrict__ den, double4 *__restrict__ acc) {
+    if (M.mask) mirror_vec(M, n0, n1, n2);
+    if (P.pass == PASS_FIRST) {
+        acc[k] = make_double4(n0, n1, n2, dd);
+        return;
+    }
+    if (P.pass != PASS_ONLY) {
+        const double4 a = acc[k];
+        n0 = a.x + n0;
+        n1 = a.y + n1;
+        n2 = a.z + n2;
+        dd = a.w + dd;
+        if (P.pass == PASS_MID) {
+            acc[k] = make_double4(n0, n1, n2, dd);
+            return;
+        }
+    }
+    const double oden = 1.0 / dd;
+    const bool tg = P.tangent != 0;
+    xv[3 * j] = (tg ? 0.0 : v0.x) + n0 * oden;
+    xv[3 * j + 1] = (tg ? 0.0 : v0.y) + n1 * oden;
+    xv[3 * j + 2] = (tg ? 0.0 : v0.z) + n2 * oden;
+    if (!tg) den[k] = dd;
 }
 
 // ---------------------------------------------------------------------------
@@ -98,10 +174,19 @@ __device__ __forceinline__ void load_xu(const double *xs, const int32_t *xuInd, 
     p[2] = xs[3 * j + 2];
 }
 
+// The surface kernels below (face properties, nodal normals, rotations,
+// corner flags) evaluate the reference's expressions in its operation order
+// without FMA contraction, so that they round as the reference does: the
+// rotation angle acos(n0 . n) is ill-conditioned for small rotations
+// (d angle = d dot / sin(angle)), and an ulp in the dot product of a nearly
+// unrotated node becomes a visible rotation difference that far volume
+// nodes multiply by their distance.
+
 // getElementProps.F90: fan of cross products about the face centroid.
 __global__ void k_face_props(int nface, const int32_t *__restrict__ facePtr, const int32_t *__restrict__ faceConn,
                              const int32_t *__restrict__ xuInd, const double *__restrict__ xs,
                              double4 *__restrict__ fprops) {
+#pragma clang fp contract(off)
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= nface) return;
     const int p0 = facePtr[f], np = facePtr[f + 1] - p0;
@@ -135,6 +220,7 @@ __global__ void k_face_props(int nface, const int32_t *__restrict__ facePtr, con
 
 // getRotationMatrix3d (vectorUtils.f90:31-103), row-major M.
 __device__ void rodrigues(const double *v1, const double *v2, double *M) {
+#pragma clang fp contract(off)
     double axis[3];
     cross3(v1, v2, axis);
     const double axisMag = sqrt(axis[0] * axis[0] + axis[1] * axis[1] + axis[2] * axis[2] + 1e-30);
@@ -167,6 +253,7 @@ __global__ void k_node_props(int n, int initial, int useRot, const int32_t *__re
                              const int32_t *__restrict__ xuInd, const int32_t *__restrict__ corner,
                              const double *__restrict__ xs, double4 *__restrict__ ps, double *__restrict__ normals0,
                              double4 *__restrict__ nprops, PointDyn *__restrict__ pd, PointDyn32 *__restrict__ pd32) {
+#pragma clang fp contract(off)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double sa = 0.0, sn[3] = {0.0, 0.0, 0.0};
@@ -377,6 +464,7 @@ __global__ void k_node_props_d(int n, int useRot, const int32_t *__restrict__ n2
 __global__ void k_corners(int n, double threshold, const int32_t *__restrict__ n2eStart, const int32_t *__restrict__ n2e,
                           const double4 *__restrict__ fprops, const double *__restrict__ normals0,
                           int32_t *__restrict__ corner) {
+#pragma clang fp contract(off)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int c = 0;
@@ -443,14 +531,17 @@ template <int PREC, int EXPM>
 __global__ void __launch_bounds__(256) k_exact_fwd(int nvol, const double4 *__restrict__ vol,
                                                    const int32_t *__restrict__ perm, int n,
                                                    const double4 *__restrict__ ps, const PointDyn *__restrict__ pd,
-                                                   const PointDyn32 *__restrict__ pd32, WParams P,
-                                                   double *__restrict__ xv, double *__restrict__ den) {
+                                                   const PointDyn32 *__restrict__ pd32, WParams P, Mirror M,
+                                                   double *__restrict__ xv, double *__restrict__ den,
+                                                   double4 *__restrict__ acc) {
     const int nb = (nvol + 255) / 256;
     const int blk = xcd_block(nb);
     if (blk >= nb) return;
     const int k = blk * 256 + threadIdx.x;
     const bool valid = k < nvol;
-    const double4 v = valid ? vol[k] : make_double4(0.0, 0.0, 0.0, 0.0);
+    const double4 v0 = valid ? vol[k] : make_double4(0.0, 0.0, 0.0, 0.0);
+    double4 v = v0;
+    if (M.mask) mirror_pt(M, v.x, v.y, v.z);
     double n0 = 0.0, n1 = 0.0, n2 = 0.0, dd = 0.0;
     if (PREC == IDW_PREC_FP64) {
         for (int i = 0; i < n; ++i) {
@@ -499,26 +590,27 @@ __global__ void __launch_bounds__(256) k_exact_fwd(int nvol, const double4 *__re
             n0 += a0; n1 += a1; n2 += a2; dd += ad;
         }
     }
-    if (valid) {
-        // tangent (warpDerivFwd.F90:92-97): numerator tangent / den, no base
-        const int64_t j = perm[k];
-        const double oden = 1.0 / dd;
-        const bool tg = P.tangent != 0;
-        xv[3 * j] = (tg ? 0.0 : v.x) + n0 * oden;
-        xv[3 * j + 1] = (tg ? 0.0 : v.y) + n1 * oden;
-        xv[3 * j + 2] = (tg ? 0.0 : v.z) + n2 * oden;
-        if (!tg) den[k] = dd;
-    }
+    if (valid) finish_fwd(P, M, k, perm[k], v0, n0, n1, n2, dd, xv, den, acc);
 }
 
-// Reverse seed numb = dXv / den (warpDeriv.F90:76-78), sorted order.
+// Reverse seed numb = dXv / den (warpDeriv.F90:76-78), sorted order; for a
+// mirror pass the seed is reflected (getMirrorNumerator_b) and the mirrored
+// positions are written to volm.
 __global__ void k_prep_numb(int nvol, const int32_t *__restrict__ perm, const double *__restrict__ dxv,
-                            const double *__restrict__ den, double4 *__restrict__ numb) {
+                            const double *__restrict__ den, Mirror M, const double4 *__restrict__ vol,
+                            double4 *__restrict__ numb, double4 *__restrict__ volm) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nvol) return;
     const int64_t j = perm[k];
     const double oden = 1.0 / den[k];
-    numb[k] = make_double4(dxv[3 * j] * oden, dxv[3 * j + 1] * oden, dxv[3 * j + 2] * oden, 0.0);
+    double a = dxv[3 * j] * oden, b = dxv[3 * j + 1] * oden, c = dxv[3 * j + 2] * oden;
+    if (M.mask) {
+        mirror_vec_b(M, a, b, c);
+        double4 v = vol[k];
+        mirror_pt(M, v.x, v.y, v.z);
+        volm[k] = v;
+    }
+    numb[k] = make_double4(a, b, c, 0.0);
 }
 
 // evalNodeExact_b (kd_tree.F90:984-1013) transposed: one lane per surface
@@ -568,13 +660,15 @@ __global__ void __launch_bounds__(256) k_exact_rev(int nvol, int chunk, const do
     }
 }
 
-__global__ void k_exact_rev_reduce(int n, int nchunk, const double *__restrict__ partial, double *__restrict__ padj) {
+__global__ void k_exact_rev_reduce(int n, int nchunk, const double *__restrict__ partial, double *__restrict__ padj,
+                                   int add) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n * 12) return;
     const int i = t / 12, k = t % 12;
     double s = 0.0;
     for (int c = 0; c < nchunk; ++c) s += partial[((int64_t)c * n + i) * 12 + k];
-    padj[(int64_t)i * ADJ_STRIDE + k] = s;
+    double *o = padj + (int64_t)i * ADJ_STRIDE + k;
+    *o = add ? *o + s : s;
 }
 
 // ---------------------------------------------------------------------------
@@ -683,13 +777,14 @@ __global__ void __launch_bounds__(256) k_plan_walk(int nvol, const double4 *__re
                                                    const NodeRec *__restrict__ nodes, double errTol, double nearT2,
                                                    int64_t *__restrict__ waveCount, PlanEntry *__restrict__ plan,
                                                    const int64_t *__restrict__ waveOff,
-                                                   unsigned int *__restrict__ mismatch) {
+                                                   unsigned int *__restrict__ mismatch, Mirror M) {
     const int nb = (nvol + 255) / 256;
     const int blk = COUNT ? (int)blockIdx.x : xcd_block(nb);
     if (blk >= nb) return;
     const int k = blk * 256 + threadIdx.x;
     const bool valid = k < nvol;
-    const double4 v = valid ? vol[k] : make_double4(0.0, 0.0, 0.0, 1.0);
+    double4 v = valid ? vol[k] : make_double4(0.0, 0.0, 0.0, 1.0);
+    if (M.mask) mirror_pt(M, v.x, v.y, v.z);
     const double tol = errTol * v.w;  // errTol * denominator0
     int stop = valid ? INT32_MAX : -1;  // depth at which this lane stopped descending
     const int wave = uniform(blk * 4 + (threadIdx.x >> 6));
@@ -819,15 +914,18 @@ __global__ void __launch_bounds__(256) k_fast_fwd(int nvol, const double4 *__res
                                                   const int64_t *__restrict__ waveOff, const NodeRec *__restrict__ nodes,
                                                   const NodeDyn *__restrict__ nd, const NodeDyn32 *__restrict__ nd32,
                                                   const double4 *__restrict__ ps, const PointDyn *__restrict__ pd,
-                                                  const PointDyn32 *__restrict__ pd32, WParams P,
-                                                  double *__restrict__ xv, double *__restrict__ den) {
+                                                  const PointDyn32 *__restrict__ pd32, WParams P, Mirror M,
+                                                  double *__restrict__ xv, double *__restrict__ den,
+                                                  double4 *__restrict__ acc) {
     const int nb = (nvol + 255) / 256;
     const int blk = xcd_block(nb);
     if (blk >= nb) return;
     const int k = blk * 256 + threadIdx.x;
     const bool valid = k < nvol;
     const int wave = uniform(blk * 4 + (threadIdx.x >> 6));
-    const double4 v = valid ? vol[k] : make_double4(0.0, 0.0, 0.0, 1.0);
+    const double4 v0 = valid ? vol[k] : make_double4(0.0, 0.0, 0.0, 1.0);
+    double4 v = v0;
+    if (M.mask) mirror_pt(M, v.x, v.y, v.z);
     const int lane = threadIdx.x & 63, lz = lane & P.zero;
     double n0 = 0.0, n1 = 0.0, n2 = 0.0, dd = 0.0;
     const int64_t e0 = waveOff[wave], e1 = waveOff[wave + 1];
@@ -875,16 +973,7 @@ __global__ void __launch_bounds__(256) k_fast_fwd(int nvol, const double4 *__res
             if (PREC == IDW_PREC_MIXED) c32 = c32n;
         }
     }
-    if (valid) {
-        // tangent (warpDerivFwd.F90:92-97): numerator tangent / den, no base
-        const int64_t j = perm[k];
-        const double oden = 1.0 / dd;
-        const bool tg = P.tangent != 0;
-        xv[3 * j] = (tg ? 0.0 : v.x) + n0 * oden;
-        xv[3 * j + 1] = (tg ? 0.0 : v.y) + n1 * oden;
-        xv[3 * j + 2] = (tg ? 0.0 : v.z) + n2 * oden;
-        if (!tg) den[k] = dd;
-    }
+    if (valid) finish_fwd(P, M, k, perm[k], v0, n0, n1, n2, dd, xv, den, acc);
 }
 
 typedef double dvec4 __attribute__((ext_vector_type(4)));
@@ -954,8 +1043,8 @@ __global__ void __launch_bounds__(256) k_fast_rev(int nvol, const double4 *__res
                                                   const int32_t *__restrict__ perm, const double *__restrict__ dxv,
                                                   const double *__restrict__ den, const PlanEntry *__restrict__ plan,
                                                   const int64_t *__restrict__ waveOff, const NodeRec *__restrict__ nodes,
-                                                  const double4 *__restrict__ ps, WParams P, double *__restrict__ nadj,
-                                                  double *__restrict__ padj) {
+                                                  const double4 *__restrict__ ps, WParams P, Mirror M,
+                                                  double *__restrict__ nadj, double *__restrict__ padj) {
     __shared__ double sW[4][64 * GROW];
     __shared__ double sD[4][16][4];
     __shared__ double *sDst[4][16];
@@ -966,7 +1055,7 @@ __global__ void __launch_bounds__(256) k_fast_rev(int nvol, const double4 *__res
     const int k = blk * 256 + threadIdx.x;
     const bool valid = k < nvol;
     const int wave = uniform(blk * 4 + wv);
-    const double4 v = valid ? vol[k] : make_double4(0.0, 0.0, 0.0, 1.0);
+    double4 v = valid ? vol[k] : make_double4(0.0, 0.0, 0.0, 1.0);
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
     if (valid) {
         const int64_t j = perm[k];
@@ -974,6 +1063,10 @@ __global__ void __launch_bounds__(256) k_fast_rev(int nvol, const double4 *__res
         s0 = dxv[3 * j] * oden;
         s1 = dxv[3 * j + 1] * oden;
         s2 = dxv[3 * j + 2] * oden;
+    }
+    if (M.mask) {
+        mirror_pt(M, v.x, v.y, v.z);
+        mirror_vec_b(M, s0, s1, s2);
     }
     // wave origin: lane 0 is valid whenever any lane of the wave is
     const double ox = first_lane(v.x), oy = first_lane(v.y), oz = first_lane(v.z);
@@ -1062,13 +1155,14 @@ __global__ void __launch_bounds__(256) k_fast_rev(int nvol, const double4 *__res
 // (the quantities of dryRun's cost model, kd_tree.F90:789-830).
 __global__ void __launch_bounds__(256) k_fast_count(int nvol, const double4 *__restrict__ vol, int nnodes,
                                                     const NodeRec *__restrict__ nodes, double errTol,
-                                                    unsigned long long *__restrict__ out) {
+                                                    unsigned long long *__restrict__ out, Mirror M) {
     __shared__ unsigned long long red[8];
     if (threadIdx.x < 8) red[threadIdx.x] = 0;
     __syncthreads();
     const int k = blockIdx.x * 256 + threadIdx.x;
     const bool valid = k < nvol;
-    const double4 v = valid ? vol[k] : make_double4(0.0, 0.0, 0.0, 1.0);
+    double4 v = valid ? vol[k] : make_double4(0.0, 0.0, 0.0, 1.0);
+    if (M.mask) mirror_pt(M, v.x, v.y, v.z);
     const double tol = errTol * v.w;
     int stop = valid ? INT32_MAX : -1;
     unsigned long long cV = 0, cA = 0, cL = 0, cE = 0;
@@ -1347,15 +1441,18 @@ __global__ void __launch_bounds__(256) k_errors(const int32_t *__restrict__ inte
 // dist*oradius > 5 with dist including 1e-16.  Writes vol[k].w.
 __global__ void __launch_bounds__(256) k_den0(int nvol, double4 *__restrict__ vol, int nnodes,
                                               const NodeRec *__restrict__ nodes, const double4 *__restrict__ ps,
-                                              WParams P) {
+                                              WParams P, Mirror M) {
     const int nb = (nvol + 255) / 256;
     const int blk = xcd_block(nb);
     if (blk >= nb) return;
     const int k = blk * 256 + threadIdx.x;
     const bool valid = k < nvol;
-    const double4 v = valid ? vol[k] : make_double4(0.0, 0.0, 0.0, 0.0);
+    double4 v = valid ? vol[k] : make_double4(0.0, 0.0, 0.0, 0.0);
     int stop = valid ? INT32_MAX : -1;
-    double dd = 0.0;
+    // mirror passes after the first add to the same running sum
+    // (initializeWarping.F90:240-247)
+    double dd = M.mask ? v.w : 0.0;
+    if (M.mask) mirror_pt(M, v.x, v.y, v.z);
     int node = 0;
     while (node < nnodes) {
         const NodeRec *r = nodes + node;
@@ -1455,8 +1552,8 @@ void launch_errors(hipStream_t st, const DevSurface &S, const WParams &P, const 
     if (S.nInternal) k_errors<<<S.nInternal, 256, 0, st>>>(S.internal, S.nodes, S.ps, P);
 }
 
-void launch_den0(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P) {
-    k_den0<<<xcd_grid(V.n), 256, 0, st>>>((int)V.n, V.vol, S.nnodes, S.nodes, S.ps, P);
+void launch_den0(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, const Mirror &M) {
+    k_den0<<<xcd_grid(V.n), 256, 0, st>>>((int)V.n, V.vol, S.nnodes, S.nodes, S.ps, P, M);
 }
 
 void launch_morton(hipStream_t st, int64_t n, const double *xv0, double lo[3], double scale[3], uint64_t *keys,
@@ -1486,59 +1583,67 @@ void launch_fill(hipStream_t st, double *a, int64_t n, double v) {
 }
 
 template <int EXPM>
-static void forward_impl(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, int fast, int prec,
-                         double *xv, const DevPlan *plan) {
+static void forward_impl(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, const Mirror &M,
+                         int fast, int prec, double *xv, const DevPlan *plan) {
     const unsigned g = xcd_grid(V.n);
     if (!fast) {
         if (prec == IDW_PREC_FP64)
-            k_exact_fwd<IDW_PREC_FP64, EXPM><<<g, 256, 0, st>>>((int)V.n, V.vol, V.perm, S.n, S.ps, S.pd, S.pd32, P, xv, V.den);
+            k_exact_fwd<IDW_PREC_FP64, EXPM><<<g, 256, 0, st>>>((int)V.n, V.vol, V.perm, S.n, S.ps, S.pd, S.pd32, P, M,
+                                                               xv, V.den, V.acc);
         else
-            k_exact_fwd<IDW_PREC_MIXED, EXPM><<<g, 256, 0, st>>>((int)V.n, V.vol, V.perm, S.n, S.ps, S.pd, S.pd32, P, xv, V.den);
+            k_exact_fwd<IDW_PREC_MIXED, EXPM><<<g, 256, 0, st>>>((int)V.n, V.vol, V.perm, S.n, S.ps, S.pd, S.pd32, P, M,
+                                                                xv, V.den, V.acc);
     } else {
         if (prec == IDW_PREC_FP64)
             k_fast_fwd<IDW_PREC_FP64, EXPM><<<g, 256, 0, st>>>((int)V.n, V.vol, V.perm, plan->entries, plan->waveOff,
-                                                              S.nodes, S.nd, S.nd32, S.ps, S.pd, S.pd32, P, xv, V.den);
+                                                              S.nodes, S.nd, S.nd32, S.ps, S.pd, S.pd32, P, M, xv,
+                                                              V.den, V.acc);
         else
             k_fast_fwd<IDW_PREC_MIXED, EXPM><<<g, 256, 0, st>>>((int)V.n, V.vol, V.perm, plan->entries, plan->waveOff,
-                                                               S.nodes, S.nd, S.nd32, S.ps, S.pd, S.pd32, P, xv, V.den);
+                                                               S.nodes, S.nd, S.nd32, S.ps, S.pd, S.pd32, P, M, xv,
+                                                               V.den, V.acc);
     }
 }
 
-void launch_prep_numb(hipStream_t st, const DevVolume &V, const double *dxv) {
-    k_prep_numb<<<grid_for(V.n, 256), 256, 0, st>>>((int)V.n, V.perm, dxv, V.den, V.numb);
+void launch_prep_numb(hipStream_t st, const DevVolume &V, const Mirror &M, const double *dxv) {
+    k_prep_numb<<<grid_for(V.n, 256), 256, 0, st>>>((int)V.n, V.perm, dxv, V.den, M, V.vol, V.numb, V.volm);
 }
 
-void launch_plan_count(hipStream_t st, const DevSurface &S, const DevVolume &V, double errTol, int64_t *waveCount) {
+void launch_plan_count(hipStream_t st, const DevSurface &S, const DevVolume &V, const Mirror &M, double errTol,
+                       int64_t *waveCount) {
     k_plan_walk<true><<<grid_for(V.n, 256), 256, 0, st>>>((int)V.n, V.vol, S.nnodes, S.nodes, errTol, 0.0, waveCount,
-                                                          nullptr, nullptr, nullptr);
+                                                          nullptr, nullptr, nullptr, M);
 }
 
-void launch_plan_fill(hipStream_t st, const DevSurface &S, const DevVolume &V, double errTol, double nearT2,
-                      const DevPlan &plan) {
+void launch_plan_fill(hipStream_t st, const DevSurface &S, const DevVolume &V, const Mirror &M, double errTol,
+                      double nearT2, const DevPlan &plan) {
     k_plan_walk<false><<<xcd_grid(V.n), 256, 0, st>>>((int)V.n, V.vol, S.nnodes, S.nodes, errTol, nearT2, nullptr,
-                                                      plan.entries, plan.waveOff, plan.mismatch);
+                                                      plan.entries, plan.waveOff, plan.mismatch, M);
 }
 
 template <int EXPM>
-static void reverse_impl(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, int fast, int prec,
-                         double *partial, int nchunk, const double *dxv, const DevPlan *plan) {
+static void reverse_impl(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, const Mirror &M,
+                         int fast, int prec, double *partial, int nchunk, const double *dxv, const DevPlan *plan) {
     if (!fast) {
-        launch_prep_numb(st, V, dxv);
+        launch_prep_numb(st, V, M, dxv);
+        const double4 *pos = M.mask ? V.volm : V.vol;
         const int chunk = (int)((V.n + nchunk - 1) / nchunk);
         dim3 g(grid_for(S.n, 256), nchunk);
         if (prec == IDW_PREC_FP64)
-            k_exact_rev<IDW_PREC_FP64, EXPM><<<g, 256, 0, st>>>((int)V.n, chunk, V.vol, V.numb, S.n, S.ps, P, partial);
+            k_exact_rev<IDW_PREC_FP64, EXPM><<<g, 256, 0, st>>>((int)V.n, chunk, pos, V.numb, S.n, S.ps, P, partial);
         else
-            k_exact_rev<IDW_PREC_MIXED, EXPM><<<g, 256, 0, st>>>((int)V.n, chunk, V.vol, V.numb, S.n, S.ps, P, partial);
-        k_exact_rev_reduce<<<grid_for((int64_t)S.n * 12, 256), 256, 0, st>>>(S.n, nchunk, partial, S.padj);
+            k_exact_rev<IDW_PREC_MIXED, EXPM><<<g, 256, 0, st>>>((int)V.n, chunk, pos, V.numb, S.n, S.ps, P, partial);
+        // the first pass stores, later passes add
+        k_exact_rev_reduce<<<grid_for((int64_t)S.n * 12, 256), 256, 0, st>>>(S.n, nchunk, partial, S.padj,
+                                                                              M.mask != 0);
     } else {
         const unsigned g = xcd_grid(V.n);
         if (prec == IDW_PREC_FP64)
             k_fast_rev<IDW_PREC_FP64, EXPM><<<g, 256, 0, st>>>((int)V.n, V.vol, V.perm, dxv, V.den, plan->entries,
-                                                        plan->waveOff, S.nodes, S.ps, P, S.nadj, S.padj);
+                                                        plan->waveOff, S.nodes, S.ps, P, M, S.nadj, S.padj);
         else
             k_fast_rev<IDW_PREC_MIXED, EXPM><<<g, 256, 0, st>>>((int)V.n, V.vol, V.perm, dxv, V.den, plan->entries,
-                                                         plan->waveOff, S.nodes, S.ps, P, S.nadj, S.padj);
+                                                         plan->waveOff, S.nodes, S.ps, P, M, S.nadj, S.padj);
     }
 }
 
@@ -1549,21 +1654,21 @@ void launch_reverse_prep(hipStream_t st, const DevSurface &S, int fast) {
     }
 }
 
-void launch_fast_count(hipStream_t st, const DevSurface &S, const DevVolume &V, double errTol,
+void launch_fast_count(hipStream_t st, const DevSurface &S, const DevVolume &V, const Mirror &M, double errTol,
                        unsigned long long *out) {
-    k_fast_count<<<grid_for(V.n, 256), 256, 0, st>>>((int)V.n, V.vol, S.nnodes, S.nodes, errTol, out);
+    k_fast_count<<<grid_for(V.n, 256), 256, 0, st>>>((int)V.n, V.vol, S.nnodes, S.nodes, errTol, out, M);
 }
 
-void launch_forward(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, int fast, int prec,
-                    double *xv, const DevPlan *plan) {
-    if (P.expMode == EXP_35) forward_impl<EXP_35>(st, S, V, P, fast, prec, xv, plan);
-    else forward_impl<EXP_INT>(st, S, V, P, fast, prec, xv, plan);
+void launch_forward(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, const Mirror &M,
+                    int fast, int prec, double *xv, const DevPlan *plan) {
+    if (P.expMode == EXP_35) forward_impl<EXP_35>(st, S, V, P, M, fast, prec, xv, plan);
+    else forward_impl<EXP_INT>(st, S, V, P, M, fast, prec, xv, plan);
 }
 
-void launch_reverse(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, int fast, int prec,
-                    double *partial, int nchunk, const double *dxv, const DevPlan *plan) {
-    if (P.expMode == EXP_35) reverse_impl<EXP_35>(st, S, V, P, fast, prec, partial, nchunk, dxv, plan);
-    else reverse_impl<EXP_INT>(st, S, V, P, fast, prec, partial, nchunk, dxv, plan);
+void launch_reverse(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, const Mirror &M,
+                    int fast, int prec, double *partial, int nchunk, const double *dxv, const DevPlan *plan) {
+    if (P.expMode == EXP_35) reverse_impl<EXP_35>(st, S, V, P, M, fast, prec, partial, nchunk, dxv, plan);
+    else reverse_impl<EXP_INT>(st, S, V, P, M, fast, prec, partial, nchunk, dxv, plan);
 }
 
 void launch_nodal_reverse(hipStream_t st, const DevSurface &S, const double *xs, int fast, int useRot, double *dxs,

@@ -89,9 +89,10 @@ class LibIdwarp:
         return out, link, nu
 
     def setsymmetryplanes(self, pts, normals):
-        """libidwarp.pyf:69-73.  Only the empty set of planes is supported."""
-        if np.asarray(pts).size:
-            raise NotImplementedError("symmetry planes are not supported by the HIP backend")
+        """libidwarp.pyf:69-73: pts and normals are (3, n)."""
+        p = np.ascontiguousarray(np.asarray(pts, dtype=np.float64).reshape(3, -1).T)
+        n = np.ascontiguousarray(np.asarray(normals, dtype=np.float64).reshape(3, -1).T)
+        _lib.check(self._L.idw_set_symmetry_planes(self._ensure(), _lib.ptr(p), _lib.ptr(n), len(p)))
 
     def initializewarping(self, pts, uniquepts, link, facesizes, faceconn, restartfile=""):
         """libidwarp.pyf:55-68: pts flat 3*n, uniquepts (3, nunique), link

@@ -63,8 +63,6 @@ class USMesh:
             self._opts[k.lower()] = (k, default[0] if isinstance(default, list) else default)
         for k, v in options.items():
             self.setOption(k, v)
-        if self.getOption("symmetryPlanes") or self.getOption("symmetrySurfaces"):
-            raise Error("symmetry planes are not supported by the HIP backend")
         L = _lib.load()
         self._L = L
         self._ctx = ctypes.c_void_p(0)
@@ -152,12 +150,28 @@ class USMesh:
         inv = np.zeros_like(ind)
         inv[ind] = np.arange(len(ind))
         link = _lib.i32(inv[link - 1] + 1)
+        self._setSymmetryConditions()
         self._setMeshOptions()
         _lib.check(self._L.idw_initialize_warping(self._ctx, _lib.ptr(pts), len(pts), _lib.ptr(uniquePts),
                                                   len(uniquePts), _lib.ptr(link), _lib.ptr(faceSizes),
                                                   len(faceSizes), _lib.ptr(conn), len(conn)))
         self.nSurf = len(pts)
         self.warpInitialized = True
+
+    def _setSymmetryConditions(self):
+        """UnstructuredMesh.py:752-962 for user-given planes: the
+        'symmetryPlanes' option [[point, normal], ...].  Planes found from
+        CGNS symmetry families ('symmetrySurfaces') need a CGNS reader, which
+        this backend does not have."""
+        planes = self.getOption("symmetryPlanes")
+        if planes is None:
+            if self.getOption("symmetrySurfaces") is not None:
+                raise Error("Automatic determination of symmetry surfaces needs a CGNS grid. Please specify the "
+                            "symmetry planes using the 'symmetryPlanes' option.")
+            planes = []
+        pts = np.ascontiguousarray([p for p, _ in planes], np.float64).reshape(-1, 3)
+        normals = np.ascontiguousarray([n for _, n in planes], np.float64).reshape(-1, 3)
+        _lib.check(self._L.idw_set_symmetry_planes(self._ctx, _lib.ptr(pts), _lib.ptr(normals), len(pts)))
 
     def setExternalMeshIndices(self, ind):
         """0-based per-DOF indices of the solver grid into the common grid."""

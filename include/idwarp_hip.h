@@ -85,6 +85,14 @@ int idw_set_options(idw_ctx *ctx, const idw_options *opt);
  * set before idw_initialize_warping. */
 int idw_set_volume_nodes(idw_ctx *ctx, const double *xv0, int64_t ndof);
 
+/* setsymmetryplanes (libidwarp.pyf:69-73, initializeWarping.F90:511-531):
+ * n <= 6 planes, each a point and a normal (3*n values each, used as given);
+ * every warp, reverse and tangent then sums over the 2^n mirror images of
+ * each volume node (getMirrorPt / getMirrorNumerator, :533-659).  n = 0
+ * removes them.  May be called before or after idw_initialize_warping (the
+ * denominator estimate is recomputed). */
+int idw_set_symmetry_planes(idw_ctx *ctx, const double *pts, const double *normals, int64_t n);
+
 /* pointreduce (libidwarp.pyf:195-202, src/utils/pointReduce.F90:1-83):
  * merge points within tol.  pts and unique_pts are 3*n; link (n) is 1-based
  * into unique_pts.  Context-free. */

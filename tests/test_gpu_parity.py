@@ -251,8 +251,6 @@ def test_libidwarp_compat_follows_reference_call_sequence():
     dxs = np.zeros(allPts.size)
     w.getdxs(dxs)
     assert _rel(dxs, ora.warpDeriv(dxv)) <= 1e-10
-    with pytest.raises(NotImplementedError):
-        w.setsymmetryplanes(np.zeros((3, 1)), np.ones((3, 1)))
 
 
 @pytest.mark.parametrize("name", ["sphere", "star"])
@@ -317,3 +315,67 @@ def test_autograd_forward_mode():
         primal, tangent = fwAD.unpack_dual(out)
     np.testing.assert_array_equal(primal.cpu().numpy(), gpu.getWarpGrid())
     assert _rel(tangent.cpu().numpy(), ref) <= 1e-12
+
+
+def _symmetric(nplanes):
+    half = synthetic.shifted(synthetic.sphere_omesh(13, 9, 9), (0.3, 2.5, 3.2))
+    planes = [([0.0, 0.0, 0.0], [0.0, 1.0, 0.0]), ([0.0, 0.0, 1.5], [0.0, 0.0, 1.0])][:nplanes]
+    return half, planes
+
+
+@pytest.mark.parametrize("nplanes", [1, 2])
+@pytest.mark.parametrize("mode", ["exact", "fast"])
+def test_symmetry_planes_fp64(mode, nplanes):
+    """Symmetry planes (initializeWarping.F90:511-659): denominator
+    estimate, warp, reverse and tangent over the 2^n mirror passes."""
+    m, planes = _symmetric(nplanes)
+    print("case", mode, nplanes, flush=True)
+    ora = O.OracleMesh(m["pts"], m["conn"], m["faceSizes"], m["xv0"], evalMode=mode, errTol=1e-4,
+                       symmetryPlanes=planes)
+    gpu = USMesh(options=dict(gridFile=m["xv0"], fileType="memory", evalMode=mode, errTol=1e-4,
+                              symmetryPlanes=[[list(p), list(n)] for p, n in planes]))
+    gpu.setSurfaceDefinition(m["pts"], m["conn"], m["faceSizes"])
+    np.testing.assert_allclose(gpu.getDenominator0(), ora.den0, rtol=1e-12)
+    xs = m["perturb"](m["pts"])
+    ora.setSurfaceCoordinates(xs)
+    gpu.setSurfaceCoordinates(xs)
+    ref = ora.warpMesh()
+    gpu.warpMesh()
+    err = np.abs(gpu.getWarpGrid() - ref).max() / ora.Ldef0
+    print("max |dXv|/Ldef", err)
+    assert err <= FP64_TOL
+    v = synthetic.random_dxv(len(m["xv0"]), seed=6, frac=0.5)
+    gpu.warpDeriv(v, solverVec=False)
+    r = _rel(gpu.getdXs().ravel(), ora.warpDeriv(v))
+    print("adjoint rel err", r)
+    assert r <= 1e-10
+    p = np.random.default_rng(15).standard_normal(xs.size)
+    t = _rel(gpu.warpDerivFwd(p, solverVec=False), ora.warpDerivFwd(p))
+    print("tangent rel err", t)
+    assert t <= 1e-10
+
+
+def test_symmetry_planes_mixed_and_removal():
+    m, planes = _symmetric(1)
+    ora = O.OracleMesh(m["pts"], m["conn"], m["faceSizes"], m["xv0"], evalMode="fast", errTol=1e-4,
+                       symmetryPlanes=planes)
+    gpu = USMesh(options=dict(gridFile=m["xv0"], fileType="memory", evalMode="fast", errTol=1e-4, precision="mixed",
+                              nearWallFactor=1e-3, symmetryPlanes=[[list(p), list(n)] for p, n in planes]))
+    gpu.setSurfaceDefinition(m["pts"], m["conn"], m["faceSizes"])
+    xs = m["perturb"](m["pts"])
+    ora.setSurfaceCoordinates(xs)
+    gpu.setSurfaceCoordinates(xs)
+    ref = ora.warpMesh()
+    gpu.warpMesh()
+    assert np.abs(gpu.getWarpGrid() - ref).max() <= MIXED_TOL * ora.Ldef0
+    # removing the planes through the C ABI gives the plain warp again
+    plain = O.OracleMesh(m["pts"], m["conn"], m["faceSizes"], m["xv0"], evalMode="fast", errTol=1e-4)
+    plain.setSurfaceCoordinates(xs)
+    ref0 = plain.warpMesh()
+    from idwarp_amd import _lib
+
+    _lib.check(gpu._L.idw_set_symmetry_planes(gpu.ctx, None, None, 0))
+    gpu.setOption("precision", "fp64")
+    gpu.warpMesh()
+    assert np.abs(gpu.getWarpGrid() - ref0).max() <= FP64_TOL * plain.Ldef0
+    np.testing.assert_allclose(gpu.getDenominator0(), plain.den0, rtol=1e-12)
