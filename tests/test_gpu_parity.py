@@ -13,7 +13,7 @@ from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-FP64_TOL = 1e-12
+FP64_TOL = 1e-13
 MIXED_TOL = 1e-6
 
 
