@@ -44,6 +44,10 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU work of the baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["both", "fp64", "mixed"], default="both")
+    ap.add_argument("--workload", choices=["star-fast", "ellipsoid-exact"], default="star-fast",
+                    help="star-fast: 20M-node star body, fast mode (the headline); ellipsoid-exact: 3.2M-node "
+                         "10:1:1 ellipsoid O-mesh, exact mode, interactions/s and adjoint checks")
+    ap.add_argument("--fd-directions", type=int, default=20, help="ellipsoid-exact: central-FD directions")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device copy of 1 GiB (reference for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
@@ -111,9 +115,11 @@ def main():
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (the volume-node sharding is described "
-              "in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the library has no multi-GPU path (volume nodes would be sharded across "
+              "GPUs, with one all-reduce of dXs); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "ellipsoid-exact":
+        return ellipsoid_exact(args)
     import torch
     from idwarp_amd import USMesh, _lib, synthetic
 
@@ -236,6 +242,158 @@ def main():
         res["cpu_baseline"]["speedup_gpu_vs_cpu"] = round(res["cpu_baseline"]["value"] / head["ms"], 1)
         log("cpu baseline: %s (%.1fs)" % (res["cpu_baseline"], time.time() - t0))
     print(json.dumps(res), flush=True)
+
+
+def ellipsoid_exact(args):
+    """Exact mode on the 10:1:1 ellipsoid O-mesh 257x129x97 (3,215,841 volume
+    nodes, 32,514 unique surface nodes): warpMesh + warpDeriv in FP64 and
+    mixed precision, interactions/s = N_vol * N_surf / t, the mixed error
+    against FP64 (mode for mode), the FP64 error against the oracle on a
+    sample of volume nodes, and the adjoint checks: dXv ~ N(0,1) on a random
+    10% of the nodes (seed 5), central differences (h = 1e-6) along random
+    unit surface directions (seed 6), and the oracle's reverse for a seed
+    carried by a 2000-node sample."""
+    import torch
+    from idwarp_amd import USMesh, _lib, synthetic
+    from oracle import oracle as O
+
+    t0 = time.time()
+    m = synthetic.ellipsoid_omesh(257, 129, 97)
+    xs = m["perturb"](m["pts"])
+    nvol, npts = len(m["xv0"]), len(m["pts"])
+    dxv = synthetic.random_dxv(nvol, seed=5, frac=0.1)
+    log("mesh %s: %d volume nodes, %d surface points (%.1fs)" % (m["name"], nvol, npts, time.time() - t0))
+    mesh = USMesh(options=dict(gridFile=m["xv0"], fileType="memory", evalMode="exact", precision="fp64"))
+    mesh.setSurfaceDefinition(m["pts"], m["conn"], m["faceSizes"])
+    nsurf = _unique(mesh)
+    ldef = mesh.getLdef0()
+    dev = torch.device("cuda:0")
+    d_xs = torch.tensor(xs.ravel(), dtype=torch.float64, device=dev)
+    d_dxv = torch.tensor(dxv, dtype=torch.float64, device=dev)
+    sp = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+    L, ctx = mesh._L, mesh.ctx
+
+    def vp(t):
+        return ctypes.c_void_p(t.data_ptr())
+
+    out = {}
+    for prec in ("fp64", "mixed"):
+        mesh.setOption("precision", prec)
+        mesh._setMeshOptions()
+        d_xv = torch.empty(3 * nvol, dtype=torch.float64, device=dev)
+        d_dxs = torch.empty(3 * npts, dtype=torch.float64, device=dev)
+
+        def step():
+            _lib.check(L.idw_warp_mesh_dev(ctx, vp(d_xs), vp(d_xv), sp))
+            _lib.check(L.idw_warp_deriv_dev(ctx, vp(d_xs), vp(d_dxv), vp(d_dxs), sp))
+
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        phases = np.zeros(6)
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+            phases += mesh.getTimings()
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t1) * 1e3 / max(args.steps, 1)
+        phases /= max(args.steps, 1)
+        out[prec] = dict(ms=wall, phases=phases, xv=d_xv.clone(), dxs=d_dxs.clone())
+        log("%s: %.3f ms/step [nodal %.3f | fwd %.3f | rev setup %.3f | rev %.3f | nodal rev %.3f]" %
+            ((prec, wall) + tuple(phases[[0, 2, 3, 4, 5]])))
+    pairs = float(nvol) * nsurf
+    a, b = out["mixed"], out["fp64"]
+    dx = (a["xv"] - b["xv"]).abs()
+    res = {
+        "metric": "exact-mode interactions/s (volume x surface node pairs per second, warpMesh + warpDeriv)",
+        "value": round(2 * pairs / (a["ms"] * 1e-3), 1),
+        "unit": "pairs/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(a["ms"], 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": "10:1:1 ellipsoid O-mesh 257x129x97: %d volume nodes, %d unique surface nodes; "
+                               "exact mode; warpMesh+warpDeriv, device-resident" % (nvol, nsurf),
+                   "n_vol": nvol, "n_surface_unique": nsurf},
+        "fp64_ms_per_step": round(b["ms"], 3),
+        "fwd_interactions_per_s": {p: round(pairs / (out[p]["phases"][2] * 1e-3), 1) for p in out},
+        "rev_interactions_per_s": {p: round(pairs / (out[p]["phases"][4] * 1e-3), 1) for p in out},
+        "mixed_speedup_vs_fp64": round(b["ms"] / a["ms"], 3),
+        "mixed_vs_fp64": {"max_err_over_Ldef": float(dx.max().item() / ldef),
+                          "rms_err_over_Ldef": float(dx.pow(2).mean().sqrt().item() / ldef),
+                          "adjoint_rel_err": float(((a["dxs"] - b["dxs"]).norm() / b["dxs"].norm()).item())},
+    }
+    kms = {p: (out[p]["phases"][2], out[p]["phases"][4]) for p in out}
+    fwd_tf = 42.0 * pairs / (kms["mixed"][0] * 1e-3) / 1e12
+    rev_tf = 41.0 * pairs / (kms["mixed"][1] * 1e-3) / 1e12
+    kname, kt = ("idw::k_exact_rev<1, 0>", rev_tf) if kms["mixed"][1] > kms["mixed"][0] else ("idw::k_exact_fwd<1, 0>",
+                                                                                                fwd_tf)
+    res["roofline"] = {"bound": "mfma", "achieved": round(kt, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                       "frac": round(kt / FP64_PEAK_TFLOPS, 4),
+                       "traffic": _traffic_record(kname)["bytes"] if _traffic_record(kname) else None,
+                       "kernel": kname, "algorithmic_flops_per_pair": {"fwd": 42, "rev": 41},
+                       "fp64_TFLOPs": {"fwd": round(42.0 * pairs / (kms["fp64"][0] * 1e-3) / 1e12, 3),
+                                       "rev": round(41.0 * pairs / (kms["fp64"][1] * 1e-3) / 1e12, 3)},
+                       "mixed_TFLOPs": {"fwd": round(fwd_tf, 3), "rev": round(rev_tf, 3)},
+                       "compute": "VALU (no MFMA): all-pairs weights and displacement sums"}
+    # FP64 against the oracle on a sample of volume nodes (forward) and for
+    # a seed carried by a sample (reverse)
+    t0 = time.time()
+    O.set_threads(len(os.sched_getaffinity(0)))
+    rng = np.random.default_rng(7)
+    idx = np.sort(rng.choice(nvol, size=2000, replace=False))
+    om = O.OracleMesh(m["pts"], m["conn"], m["faceSizes"], m["xv0"][idx], evalMode="exact")
+    om.setSurfaceCoordinates(xs)
+    ref = om.warpMesh().reshape(-1, 3)
+    got = b["xv"].view(-1, 3)[torch.as_tensor(idx, device=dev)].cpu().numpy()
+    res["fp64_vs_oracle"] = {"sample": "2000 volume nodes (seed 7)",
+                             "max_err_over_Ldef": float(np.abs(got - ref).max() / ldef)}
+    seed = np.zeros(3 * nvol)
+    g = np.random.default_rng(5).standard_normal((2000, 3))
+    seed.reshape(-1, 3)[idx] = g
+    mesh.setOption("precision", "fp64")
+    mesh.setSurfaceCoordinates(xs)
+    mesh.warpMesh()
+    mesh.warpDeriv(seed, solverVec=False)
+    res["fp64_vs_oracle"]["adjoint_rel_err"] = float(_rel(mesh.getdXs().ravel(), om.warpDeriv(g.ravel())))
+    log("oracle checks %.1fs: %s" % (time.time() - t0, res["fp64_vs_oracle"]))
+    # central differences on the device along unit surface directions
+    mesh.warpMesh()
+    mesh.warpDeriv(dxv, solverVec=False)
+    dxs = mesh.getdXs().ravel()
+    rng = np.random.default_rng(6)
+    worst = 0.0
+    for _ in range(args.fd_directions):
+        p = rng.standard_normal(xs.size)
+        p /= np.linalg.norm(p)
+        h = 1e-6
+        vals = []
+        for sgn in (1.0, -1.0):
+            mesh.setSurfaceCoordinates((xs.ravel() + sgn * h * p).reshape(-1, 3))
+            mesh.warpMesh()
+            vals.append(dxv @ mesh.getWarpGrid())
+        fd = (vals[0] - vals[1]) / (2 * h)
+        worst = max(worst, abs(fd - dxs @ p) / max(abs(fd), 1e-300))
+    res["adjoint_fd"] = {"directions": args.fd_directions, "h": 1e-6, "max_rel_diff": worst,
+                         "seed": "dXv ~ N(0,1) on a random 10% of the volume nodes (seed 5)"}
+    log("FD check: %s" % res["adjoint_fd"])
+    if not args.no_cpu:
+        t0 = time.time()
+        om.setSurfaceCoordinates(xs)
+        om.warpMesh()
+        sw, lw = O.last_times()
+        om.warpDeriv(np.ones(3 * len(idx)))
+        sd, ld = O.last_times()
+        ms = 1e3 * (sw + sd + (lw + ld) * nvol / len(idx))
+        res["cpu_baseline"] = {"value": round(ms, 1), "unit": "ms per warpMesh+warpDeriv (full mesh, extrapolated)",
+                               "cores": len(os.sched_getaffinity(0)), "kind": "port",
+                               "sample": "2000 of %d volume nodes (seed 7), exact mode; volume loops %.2fs scaled "
+                                         "x%.0f" % (nvol, lw + ld, nvol / len(idx))}
+        log("cpu baseline %.1fs" % (time.time() - t0))
+    print(json.dumps(res), flush=True)
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
 def _unique(mesh):
