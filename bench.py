@@ -226,7 +226,7 @@ def main():
                        "traffic": _traffic_record(kname)["bytes"] if _traffic_record(kname) else None,
                        "kernel": kname, "kernel_ms": round(float(kms), 3),
                        "algorithmic_flops_per_launch": kfl,
-                       "compute": "VALU FP64/FP32 (no MFMA): the 78.6 TFLOP/s FP64 vector roof equals the FP64 "
+                       "compute": "VALU FP64/FP32 weights (the reverse reduces its lane sums with FP64 MFMA): the 78.6 TFLOP/s FP64 vector roof equals the FP64 "
                                   "matrix roof and non-packed FP32 issues at the same rate"}
     tr = _traffic_record(kname)
     if tr:

@@ -23,7 +23,8 @@ EXPORTS = (
     "idw_set_external_mesh_indices", "idw_get_solver_dof", "idw_warp_to_solver_grid", "idw_solver_to_warp_grid",
     "idw_warp_deriv", "idw_get_dxs", "idw_warp_mesh_dev", "idw_warp_deriv_dev", "idw_get_sizes", "idw_get_ldef0",
     "idw_get_denominator0", "idw_get_timings", "idw_fast_counts", "idw_warp_deriv_fwd", "idw_warp_deriv_fwd_dev",
-    "idw_set_symmetry_planes",
+    "idw_set_symmetry_planes", "idw_batch_create", "idw_batch_destroy", "idw_batch_add_warp",
+    "idw_batch_add_warp_deriv", "idw_batch_run", "idw_batch_info",
 )
 
 
@@ -103,6 +104,12 @@ def load():
         "idw_get_timings": (i32, [vp, vp]),
         "idw_fast_counts": (i32, [vp, vp]),
         "idw_set_symmetry_planes": (i32, [vp, vp, vp, i64]),
+        "idw_batch_create": (i32, [P(vp), i32]),
+        "idw_batch_destroy": (i32, [vp]),
+        "idw_batch_add_warp": (i32, [vp, vp, vp, vp]),
+        "idw_batch_add_warp_deriv": (i32, [vp, vp, vp, vp, vp]),
+        "idw_batch_run": (i32, [vp, vp]),
+        "idw_batch_info": (i32, [vp, P(i32), vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -82,6 +82,8 @@ struct idw_ctx {
     hipEvent_t ev[8] = {};
     double ms[6] = {0, 0, 0, 0, 0, 0};
     bool fwdTimed = false, revTimed = false;
+    bool capturing = false;  // work is being recorded into a HIP graph: no events, syncs or allocations
+    uint64_t gen = 0;        // bumped whenever device buffers, plans or options change (recorded graphs check it)
 };
 
 namespace {
@@ -95,6 +97,7 @@ int dalloc(idw_ctx *c, T **p, size_t count) {
     if (e != hipSuccess) return fail(IDW_ENOMEM, "hipMalloc(%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
     c->allocs.push_back(q);
     *p = static_cast<T *>(q);
+    ++c->gen;
     return IDW_OK;
 }
 
@@ -105,7 +108,14 @@ void dfree(idw_ctx *c, T *&p) {
     if (it != c->allocs.end()) c->allocs.erase(it);
     (void)hipFree(p);
     p = nullptr;
+    ++c->gen;
 }
+
+// Timing events of a warp / reverse product (not while a graph is captured).
+#define REC(c, i)                                                          \
+    do {                                                                   \
+        if (!(c)->capturing) HIPCHK(hipEventRecord((c)->ev[i], (c)->st)); \
+    } while (0)
 
 #define TRY(x)                  \
     do {                        \
@@ -191,6 +201,7 @@ void drop_plan(idw_ctx *c) {
     c->plans.assign(nloop(c), idw::DevPlan());
     c->orderTol = c->orderT2 = -1.0;
     c->planValid = false;
+    ++c->gen;
 }
 
 // Size the per-wave plan for the current errTol: one counting walk and an
@@ -312,15 +323,17 @@ int run_warp(idw_ctx *c, const double *xs, double *xv) {
     idw::WParams P = make_params(c, false);
     if (fast) TRY(size_plan(c));
     TRY(ensure_mirror_buffers(c));
-    HIPCHK(hipEventRecord(c->ev[0], c->st));
+    REC(c, 0);
     TRY(surface_pass(c, xs, fast));
-    HIPCHK(hipEventRecord(c->ev[1], c->st));
+    REC(c, 1);
     if (fast) {
         const double nt2 = plan_near_t2(c);
         for (int kk = 0; kk < nloop(c); ++kk) {
             idw::DevPlan &pl = c->plans[kk];
             if (!c->opt.reusePlan || !pl.filled || pl.errTol != c->opt.errTol || pl.nearT2 != nt2) {
-                HIPCHK(hipMemsetAsync(pl.mismatch, 0, sizeof(unsigned int), c->st));
+                if (c->capturing && !(pl.filled && pl.errTol == c->opt.errTol && pl.nearT2 == nt2))
+                    return fail(IDW_ESTATE, "a captured warp needs the plan of a previous warp at the same options");
+                idw::launch_zero_u32(c->st, pl.mismatch, 1);
                 if (c->V.n) idw::launch_plan_fill(c->st, c->S, c->V, mirror_of(c, kk), c->opt.errTol, nt2, pl);
                 HIPCHK(hipGetLastError());
                 pl.filled = true;
@@ -328,18 +341,18 @@ int run_warp(idw_ctx *c, const double *xs, double *xv) {
             }
         }
     }
-    HIPCHK(hipEventRecord(c->ev[2], c->st));
+    REC(c, 2);
     for (int kk = 0; kk < nloop(c) && c->V.n; ++kk) {
         P.pass = pass_of(c, kk);
         idw::launch_forward(c->st, c->S, c->V, P, mirror_of(c, kk), fast, c->opt.precision, xv,
                             fast ? &c->plans[kk] : nullptr);
     }
     HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(c->ev[3], c->st));
+    REC(c, 3);
     c->denOpt = c->opt;
     c->denValid = true;
     c->planValid = fast;
-    c->fwdTimed = true;
+    c->fwdTimed = !c->capturing;
     return IDW_OK;
 }
 
@@ -363,20 +376,20 @@ int run_deriv(idw_ctx *c, const double *xs, const double *dxv, double *dxs) {
         TRY(run_warp(c, xs, c->xv));
     if (!fast) TRY(ensure_partial(c));
     TRY(ensure_mirror_buffers(c));
-    HIPCHK(hipEventRecord(c->ev[4], c->st));
+    REC(c, 4);
     TRY(surface_pass(c, xs, false));
     idw::launch_reverse_prep(c->st, c->S, fast || c->V.n == 0);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(c->ev[5], c->st));
+    REC(c, 5);
     for (int kk = 0; kk < nloop(c) && c->V.n; ++kk)
         idw::launch_reverse(c->st, c->S, c->V, make_params(c, false), mirror_of(c, kk), fast, c->opt.precision,
                             c->partial, c->nchunk, dxv, fast ? &c->plans[kk] : nullptr);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(c->ev[6], c->st));
+    REC(c, 6);
     idw::launch_nodal_reverse(c->st, c->S, xs, fast, c->opt.useRotations, dxs, c->npts);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(c->ev[7], c->st));
-    c->revTimed = true;
+    REC(c, 7);
+    c->revTimed = !c->capturing;
     return IDW_OK;
 }
 
@@ -393,20 +406,20 @@ int run_deriv_fwd(idw_ctx *c, const double *xs, const double *dxs, double *dxv) 
     TRY(ensure_mirror_buffers(c));
     idw::WParams P = make_params(c, false);
     P.tangent = 1;
-    HIPCHK(hipEventRecord(c->ev[0], c->st));
+    REC(c, 0);
     idw::launch_props_tangent(c->st, c->S, xs, dxs, c->opt.useRotations);
     if (fast) idw::launch_cluster_data(c->st, c->S);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(c->ev[1], c->st));
-    HIPCHK(hipEventRecord(c->ev[2], c->st));
+    REC(c, 1);
+    REC(c, 2);
     for (int kk = 0; kk < nloop(c) && c->V.n; ++kk) {
         P.pass = pass_of(c, kk);
         idw::launch_forward(c->st, c->S, c->V, P, mirror_of(c, kk), fast, c->opt.precision, dxv,
                             fast ? &c->plans[kk] : nullptr);
     }
     HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(c->ev[3], c->st));
-    c->fwdTimed = true;
+    REC(c, 3);
+    c->fwdTimed = !c->capturing;
     return IDW_OK;
 }
 
@@ -524,6 +537,7 @@ int idw_destroy(idw_ctx *c) {
 int idw_set_options(idw_ctx *c, const idw_options *opt) {
     if (!c) return fail(IDW_EINVAL, "null context");
     if (!valid_opts(opt)) return fail(IDW_EINVAL, "invalid options");
+    if (std::memcmp(&c->opt, opt, sizeof(idw_options)) != 0) ++c->gen;
     c->opt = *opt;
     return IDW_OK;
 }
@@ -936,6 +950,227 @@ int idw_fast_counts(idw_ctx *c, double *counts8) {
     HIPCHK(hipStreamSynchronize(c->st));
     dfree(c, d);
     for (int k = 0; k < 8; ++k) counts8[k] = (double)h[k];
+    return IDW_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batches: many warps and reverse products over several meshes, recorded
+// once into a HIP graph and replayed with one launch.
+// ---------------------------------------------------------------------------
+struct idw_batch {
+    struct Op {
+        idw_ctx *c;
+        int kind;  // 0 warp, 1 warp_deriv
+        const double *xs, *dxv;
+        double *out;
+    };
+    int device = 0;
+    std::vector<Op> ops;
+    hipStream_t origin = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int layout = -1;  // 0: one graph branch per mesh, 1: one chain
+    double ms[2] = {-1.0, -1.0};
+    std::vector<hipEvent_t> events;                      // events recorded into the graph
+    hipEvent_t in = nullptr, out = nullptr;              // ordering with the caller's stream
+    std::vector<std::pair<idw_ctx *, uint64_t>> sig;     // context generations at recording time
+};
+
+namespace {
+
+int batch_op(idw_batch::Op &o) {
+    return o.kind == 0 ? run_warp(o.c, o.xs, o.out) : run_deriv(o.c, o.xs, o.dxv, o.out);
+}
+
+std::vector<idw_ctx *> batch_contexts(const idw_batch *b) {
+    std::vector<idw_ctx *> cs;
+    for (const auto &o : b->ops)
+        if (std::find(cs.begin(), cs.end(), o.c) == cs.end()) cs.push_back(o.c);
+    return cs;
+}
+
+int batch_event(idw_batch *b, hipEvent_t *e) {
+    HIPCHK(hipEventCreateWithFlags(e, hipEventDisableTiming));
+    b->events.push_back(*e);
+    return IDW_OK;
+}
+
+void batch_drop_graph(idw_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    for (hipEvent_t e : b->events) (void)hipEventDestroy(e);
+    b->events.clear();
+}
+
+bool batch_current(const idw_batch *b) {
+    for (const auto &p : b->sig)
+        if (p.first->gen != p.second) return false;
+    return b->exec != nullptr;
+}
+
+// Record every op into a graph: layout 0 forks one branch per mesh (each on
+// its context's stream) and joins them; layout 1 chains the meshes.
+int batch_capture(idw_batch *b, int layout) {
+    batch_drop_graph(b);
+    const std::vector<idw_ctx *> cs = batch_contexts(b);
+    HIPCHK(hipStreamBeginCapture(b->origin, hipStreamCaptureModeGlobal));
+    hipEvent_t fork;
+    int r = batch_event(b, &fork);
+    if (r == IDW_OK && hipEventRecord(fork, b->origin) != hipSuccess) r = fail(IDW_EHIP, "event record in capture");
+    hipStream_t prev = b->origin;
+    for (size_t k = 0; r == IDW_OK && k < cs.size(); ++k) {
+        idw_ctx *c = cs[k];
+        hipEvent_t dep = fork;
+        if (layout == 1 && k > 0) {
+            r = batch_event(b, &dep);
+            if (r == IDW_OK && hipEventRecord(dep, prev) != hipSuccess) r = fail(IDW_EHIP, "event record in capture");
+        }
+        if (r == IDW_OK && hipStreamWaitEvent(c->st, dep, 0) != hipSuccess) r = fail(IDW_EHIP, "stream join in capture");
+        c->capturing = true;
+        for (auto &o : b->ops)
+            if (r == IDW_OK && o.c == c) r = batch_op(o);
+        c->capturing = false;
+        prev = c->st;
+    }
+    // join every branch back into the origin stream
+    for (size_t k = 0; r == IDW_OK && k < cs.size(); ++k) {
+        hipEvent_t j;
+        r = batch_event(b, &j);
+        if (r == IDW_OK && hipEventRecord(j, cs[k]->st) != hipSuccess) r = fail(IDW_EHIP, "event record in capture");
+        if (r == IDW_OK && hipStreamWaitEvent(b->origin, j, 0) != hipSuccess) r = fail(IDW_EHIP, "join in capture");
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t ee = hipStreamEndCapture(b->origin, &g);
+    if (r != IDW_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return r;
+    }
+    if (ee != hipSuccess) return fail(IDW_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(ee));
+    b->graph = g;
+    HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    b->layout = layout;
+    b->sig.clear();
+    for (idw_ctx *c : cs) b->sig.push_back({c, c->gen});
+    return IDW_OK;
+}
+
+int batch_time(idw_batch *b, float *ms) {
+    hipEvent_t t0, t1;
+    HIPCHK(hipEventCreate(&t0));
+    HIPCHK(hipEventCreate(&t1));
+    HIPCHK(hipEventRecord(t0, b->origin));
+    for (int k = 0; k < 2; ++k) HIPCHK(hipGraphLaunch(b->exec, b->origin));
+    HIPCHK(hipEventRecord(t1, b->origin));
+    HIPCHK(hipEventSynchronize(t1));
+    HIPCHK(hipEventElapsedTime(ms, t0, t1));
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    return IDW_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int idw_batch_create(idw_batch **out, int device) {
+    if (!out) return fail(IDW_EINVAL, "null argument");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return fail(IDW_ENODEV, "no HIP device %d", device);
+    HIPCHK(hipSetDevice(device));
+    idw_batch *b = new (std::nothrow) idw_batch();
+    if (!b) return fail(IDW_ENOMEM, "out of host memory");
+    b->device = device;
+    if (hipStreamCreateWithFlags(&b->origin, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&b->in, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->out, hipEventDisableTiming) != hipSuccess) {
+        delete b;
+        return fail(IDW_EHIP, "stream or event creation failed");
+    }
+    *out = b;
+    return IDW_OK;
+}
+
+int idw_batch_destroy(idw_batch *b) {
+    if (!b) return IDW_OK;
+    (void)hipSetDevice(b->device);
+    (void)hipStreamSynchronize(b->origin);
+    batch_drop_graph(b);
+    (void)hipEventDestroy(b->in);
+    (void)hipEventDestroy(b->out);
+    (void)hipStreamDestroy(b->origin);
+    delete b;
+    return IDW_OK;
+}
+
+static int batch_add(idw_batch *b, idw_ctx *c, int kind, const double *xs, const double *dxv, double *out) {
+    if (!b || !c || !xs || !out || (kind == 1 && !dxv)) return fail(IDW_EINVAL, "null argument");
+    if (c->device != b->device) return fail(IDW_EINVAL, "context on device %d, batch on device %d", c->device, b->device);
+    batch_drop_graph(b);  // recorded again at the next run
+    b->ops.push_back({c, kind, xs, dxv, out});
+    return IDW_OK;
+}
+
+int idw_batch_add_warp(idw_batch *b, idw_ctx *c, const double *d_xs, double *d_xv) {
+    return batch_add(b, c, 0, d_xs, nullptr, d_xv);
+}
+
+int idw_batch_add_warp_deriv(idw_batch *b, idw_ctx *c, const double *d_xs, const double *d_dxv, double *d_dxs) {
+    return batch_add(b, c, 1, d_xs, d_dxv, d_dxs);
+}
+
+int idw_batch_run(idw_batch *b, void *hip_stream) {
+    if (!b) return fail(IDW_EINVAL, "null batch");
+    if (b->ops.empty()) return IDW_OK;
+    HIPCHK(hipSetDevice(b->device));
+    hipStream_t user = static_cast<hipStream_t>(hip_stream);
+    if (!batch_current(b)) {
+        // run every op once in order: sizes and fills the fast-mode plans and
+        // allocates the per-context buffers, none of which may happen in a
+        // graph.  The plans are then kept: within a batch every warp of a mesh
+        // uses the same accept/recurse decisions (they depend on the mesh and
+        // errTol only).
+        batch_drop_graph(b);
+        std::vector<idw_ctx *> cs = batch_contexts(b);
+        HIPCHK(hipEventRecord(b->in, user));
+        for (idw_ctx *c : cs) {
+            HIPCHK(hipStreamWaitEvent(c->st, b->in, 0));
+            if (!c->opt.reusePlan) ++c->gen;
+            c->opt.reusePlan = 1;
+        }
+        for (auto &o : b->ops) TRY(batch_op(o));
+        for (idw_ctx *c : cs) {
+            HIPCHK(hipStreamSynchronize(c->st));
+            TRY(check_plan(c));
+        }
+        // choose the graph layout by timing both (two replays each)
+        float t = 0.f;
+        for (int layout = 0; layout < 2; ++layout) {
+            TRY(batch_capture(b, layout));
+            TRY(batch_time(b, &t));
+            b->ms[layout] = t / 2.0;
+        }
+        TRY(batch_capture(b, b->ms[0] <= b->ms[1] ? 0 : 1));
+    }
+    HIPCHK(hipEventRecord(b->in, user));
+    HIPCHK(hipStreamWaitEvent(b->origin, b->in, 0));
+    HIPCHK(hipGraphLaunch(b->exec, b->origin));
+    HIPCHK(hipEventRecord(b->out, b->origin));
+    HIPCHK(hipStreamWaitEvent(user, b->out, 0));
+    return IDW_OK;
+}
+
+int idw_batch_info(idw_batch *b, int *layout, double *ms2) {
+    if (!b) return fail(IDW_EINVAL, "null batch");
+    if (layout) *layout = b->layout;
+    if (ms2) {
+        ms2[0] = b->ms[0];
+        ms2[1] = b->ms[1];
+    }
     return IDW_OK;
 }
 

@@ -107,6 +107,8 @@ void launch_near_keys(hipStream_t st, const DevSurface &S, const DevVolume &V, d
 void launch_permute_vol(hipStream_t st, int64_t n, const int32_t *order, const double4 *vin, const int32_t *pin,
                         double4 *vout, int32_t *pout);
 void launch_fill(hipStream_t st, double *a, int64_t n, double v);
+// zero n counters with a kernel (safe inside graph capture, unlike a memset node)
+void launch_zero_u32(hipStream_t st, unsigned int *a, int64_t n);
 void launch_forward(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, const Mirror &M,
                     int fast, int prec, double *xv, const DevPlan *plan);
 void launch_prep_numb(hipStream_t st, const DevVolume &V, const Mirror &M, const double *dxv);

@@ -1578,6 +1578,15 @@ void launch_build_vol(hipStream_t st, int64_t n, const double *xv0, const int32_
     k_build_vol<<<grid_for(n, 256), 256, 0, st>>>(n, xv0, perm, vol);
 }
 
+__global__ void k_zero_u32(int64_t n, unsigned int *__restrict__ a) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) a[k] = 0u;
+}
+
+void launch_zero_u32(hipStream_t st, unsigned int *a, int64_t n) {
+    if (n > 0) k_zero_u32<<<grid_for(n, 256), 256, 0, st>>>(n, a);
+}
+
 void launch_fill(hipStream_t st, double *a, int64_t n, double v) {
     if (n > 0) k_fill<<<grid_for(n, 256), 256, 0, st>>>(n, a, v);
 }

@@ -181,6 +181,31 @@ int idw_get_timings(idw_ctx *ctx, double *ms6);
  * No counterpart in the reference. */
 int idw_fast_counts(idw_ctx *ctx, double *counts8);
 
+/* Batches (no counterpart in the reference, which warps one surface per
+ * call): many warps and reverse products over one or more contexts on one
+ * device, recorded at the first idw_batch_run into a HIP graph and replayed
+ * by every later run with a single launch.  Ops take device pointers (3*n_pts
+ * surface, 3*n_vol volume) that must stay valid while the batch exists.
+ * The first run executes every op once in order (it sizes and fills the
+ * fast-mode plans, which are then kept for the batch: within a batch each
+ * mesh's warps share their accept/recurse decisions), then records the ops
+ * twice, as one graph branch per context and as one chain, times both and
+ * keeps the faster.  A context that changes (options, buffers, surface
+ * definition) after recording is recorded again at the next run.  Destroy a
+ * batch before the contexts it uses. */
+typedef struct idw_batch idw_batch;
+int idw_batch_create(idw_batch **out, int device);
+int idw_batch_destroy(idw_batch *batch);
+int idw_batch_add_warp(idw_batch *batch, idw_ctx *ctx, const double *d_xs, double *d_xv);
+int idw_batch_add_warp_deriv(idw_batch *batch, idw_ctx *ctx, const double *d_xs, const double *d_dxv,
+                             double *d_dxs);
+/* Enqueue the whole batch after the work on hip_stream (NULL = default
+ * stream); later work on hip_stream waits for it. */
+int idw_batch_run(idw_batch *batch, void *hip_stream);
+/* The chosen layout (0: a branch per context, 1: a chain) and the measured
+ * ms of one replay of each layout. */
+int idw_batch_info(idw_batch *batch, int *layout, double *ms2);
+
 #ifdef __cplusplus
 }
 #endif

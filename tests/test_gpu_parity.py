@@ -379,3 +379,100 @@ def test_symmetry_planes_mixed_and_removal():
     gpu.warpMesh()
     assert np.abs(gpu.getWarpGrid() - ref0).max() <= FP64_TOL * plain.Ldef0
     np.testing.assert_allclose(gpu.getDenominator0(), plain.den0, rtol=1e-12)
+
+
+@pytest.mark.parametrize("mode", ["exact", "fast"])
+def test_c1_sphere_omesh_49x33x33(mode):
+    """The reference's CPU-runnable configuration: spherical O-mesh 49x33x33
+    (53,361 volume nodes, 1,617 wall points), normal bump plus 2 degree
+    rotation; exact, then fast with errTol 1e-4; FP64 and mixed against the
+    oracle."""
+    O.set_threads(8)
+    m = synthetic.sphere_omesh(49, 33, 33)
+    assert len(m["xv0"]) == 53361 and len(m["pts"]) == 1617
+    print("case C1", mode, flush=True)
+    ora, gpu = _pair(m, evalMode=mode, errTol=1e-4)
+    xs = m["perturb"](m["pts"])
+    ora.setSurfaceCoordinates(xs)
+    gpu.setSurfaceCoordinates(xs)
+    ref = ora.warpMesh()
+    gpu.warpMesh()
+    err = np.abs(gpu.getWarpGrid() - ref).max() / ora.Ldef0
+    print("fp64 max |dXv|/Ldef", err)
+    assert err <= FP64_TOL
+    dxv = synthetic.random_dxv(len(m["xv0"]), seed=5, frac=0.1)
+    dref = ora.warpDeriv(dxv)
+    gpu.warpDeriv(dxv, solverVec=False)
+    r = _rel(gpu.getdXs().ravel(), dref)
+    print("fp64 adjoint rel err", r)
+    assert r <= 1e-10
+    gpu.setOption("precision", "mixed")
+    gpu.warpMesh()
+    errm = np.abs(gpu.getWarpGrid() - ref).max() / ora.Ldef0
+    gpu.warpDeriv(dxv, solverVec=False)
+    rm = _rel(gpu.getdXs().ravel(), dref)
+    print("mixed max |dXv|/Ldef", errm, "adjoint", rm)
+    assert errm <= MIXED_TOL and rm <= MIXED_TOL
+
+
+def test_batch_graph_replays_match_eager():
+    """Graph-recorded batch over two meshes (several surfaces each, reverse
+    products): every replay, with eager work between replays, equals the
+    eager per-call results."""
+    import torch
+
+    from idwarp_amd.batch import Batch
+
+    dev = torch.device("cuda:0")
+    cases = []
+    for name, opts in (("sphere", dict(evalMode="fast", errTol=1e-4)),
+                       ("star", dict(evalMode="fast", errTol=1e-4, precision="mixed", nearWallFactor=1e-3))):
+        m = MESHES[name]
+        g = USMesh(options=dict(gridFile=m["xv0"], fileType="memory", **opts))
+        g.setSurfaceDefinition(m["pts"], m["conn"], m["faceSizes"])
+        base = m["pts"]
+        delta = m["perturb"](base) - base
+        surfaces = [base + a * delta for a in (0.5, 1.0, 1.5)]
+        seeds = [synthetic.random_dxv(len(m["xv0"]), seed=s, frac=0.2) for s in (1, 2)]
+        ref_xv, ref_dxs = [], []
+        for xs in surfaces:
+            g.setSurfaceCoordinates(xs)
+            g.warpMesh()
+            ref_xv.append(g.getWarpGrid())
+        for sd in seeds:
+            g.setSurfaceCoordinates(surfaces[1])
+            g.warpMesh()
+            g.warpDeriv(sd, solverVec=False)
+            ref_dxs.append(g.getdXs().ravel())
+        cases.append((g, surfaces, seeds, ref_xv, ref_dxs))
+    b = Batch()
+    outs = []
+    for g, surfaces, seeds, _, _ in cases:
+        d_xs = [torch.tensor(x.ravel(), dtype=torch.float64, device=dev) for x in surfaces]
+        xv = [torch.empty(3 * g.nVol, dtype=torch.float64, device=dev) for _ in surfaces]
+        dxs = [torch.empty(3 * g.nSurf, dtype=torch.float64, device=dev) for _ in seeds]
+        for s, o in zip(d_xs, xv):
+            b.add_warp(g, s, o)
+        for sd, o in zip(seeds, dxs):
+            b.add_warp_deriv(g, d_xs[1], torch.tensor(sd, dtype=torch.float64, device=dev), o)
+        outs.append((xv, dxs))
+    for rep in range(3):
+        for xv, dxs in outs:
+            for t in xv + dxs:
+                t.fill_(float("nan"))
+        b.run()
+        torch.cuda.synchronize()
+        for (g, _, _, ref_xv, ref_dxs), (xv, dxs) in zip(cases, outs):
+            for r, t in zip(ref_xv, xv):
+                assert np.array_equal(t.cpu().numpy(), r), rep
+            for r, t in zip(ref_dxs, dxs):
+                # FP64 atomics make the reverse's summation order vary run to run
+                assert _rel(t.cpu().numpy(), r) <= 1e-10, rep
+        # eager work between replays
+        g0 = cases[0][0]
+        g0.setSurfaceCoordinates(cases[0][1][0])
+        g0.warpMesh()
+        torch.cuda.synchronize()
+    layout, ms = b.info()
+    print("batch layout", layout, "replay ms", ms)
+    assert layout in (0, 1) and min(ms) > 0
