@@ -44,9 +44,10 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU work of the baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["both", "fp64", "mixed"], default="both")
-    ap.add_argument("--workload", choices=["star-fast", "ellipsoid-exact"], default="star-fast",
+    ap.add_argument("--workload", choices=["star-fast", "ellipsoid-exact", "batch"], default="star-fast",
                     help="star-fast: 20M-node star body, fast mode (the headline); ellipsoid-exact: 3.2M-node "
-                         "10:1:1 ellipsoid O-mesh, exact mode, interactions/s and adjoint checks")
+                         "10:1:1 ellipsoid O-mesh, exact mode, interactions/s and adjoint checks; batch: four "
+                         "meshes of 0.5M-20M nodes, 16 surfaces and 8 adjoint seeds each, one graph launch")
     ap.add_argument("--fd-directions", type=int, default=20, help="ellipsoid-exact: central-FD directions")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device copy of 1 GiB (reference for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -120,6 +121,8 @@ def main():
         sys.exit(2)
     if args.workload == "ellipsoid-exact":
         return ellipsoid_exact(args)
+    if args.workload == "batch":
+        return batched(args)
     import torch
     from idwarp_amd import USMesh, _lib, synthetic
 
@@ -389,6 +392,112 @@ def ellipsoid_exact(args):
                                "sample": "2000 of %d volume nodes (seed 7), exact mode; volume loops %.2fs scaled "
                                          "x%.0f" % (nvol, lw + ld, nvol / len(idx))}
         log("cpu baseline %.1fs" % (time.time() - t0))
+    print(json.dumps(res), flush=True)
+
+
+def batched(args):
+    """Four meshes of uneven size (ellipsoid O-meshes 129x65x61, 193x97x107,
+    257x129x151 and the 20M-node star body), fast mode errTol 1e-4, mixed
+    precision; per mesh 16 surfaces Xs0 + a_p (Xs1 - Xs0), a_p ~ U(0.5, 1.5)
+    (seeds 100+p), and 8 reverse products with dXv ~ N(0,1) on 10% of the
+    nodes (seeds 200+s).  All 96 solves are recorded into one HIP graph and
+    replayed; reported beside the same solves as sequential FP64 calls that
+    walk the tree every time."""
+    import torch
+    from idwarp_amd import USMesh, _lib, synthetic
+    from idwarp_amd.batch import Batch
+
+    dev = torch.device("cuda:0")
+    specs = [("ellipsoid 129x65x61", lambda: synthetic.ellipsoid_omesh(129, 65, 61)),
+             ("ellipsoid 193x97x107", lambda: synthetic.ellipsoid_omesh(193, 97, 107)),
+             ("ellipsoid 257x129x151", lambda: synthetic.ellipsoid_omesh(257, 129, 151)),
+             ("star n=205, 80 shells", lambda: synthetic.star_shells(205, 80))]
+    meshes = []
+    for name, make in specs:
+        t0 = time.time()
+        m = make()
+        g = USMesh(options=dict(gridFile=m["xv0"], fileType="memory", evalMode="fast", errTol=1e-4,
+                                precision="mixed"))
+        g.setSurfaceDefinition(m["pts"], m["conn"], m["faceSizes"])
+        base = m["pts"]
+        delta = m["perturb"](base) - base
+        xs = [torch.tensor((base + np.random.default_rng(100 + p).uniform(0.5, 1.5) * delta).ravel(),
+                           dtype=torch.float64, device=dev) for p in range(16)]
+        dxv = [torch.tensor(synthetic.random_dxv(len(m["xv0"]), seed=200 + q, frac=0.1), dtype=torch.float64,
+                            device=dev) for q in range(8)]
+        xv = [torch.empty(3 * g.nVol, dtype=torch.float64, device=dev) for _ in range(16)]
+        dxs = [torch.empty(3 * g.nSurf, dtype=torch.float64, device=dev) for _ in range(8)]
+        meshes.append(dict(name=name, g=g, xs=xs, dxv=dxv, xv=xv, dxs=dxs, nvol=g.nVol))
+        log("%s: %d volume nodes (%.1fs)" % (name, g.nVol, time.time() - t0))
+
+    def fill(b, ms):
+        for m in ms:
+            for x, o in zip(m["xs"], m["xv"]):
+                b.add_warp(m["g"], x, o)
+            for d, o in zip(m["dxv"], m["dxs"]):
+                b.add_warp_deriv(m["g"], m["xs"][0], d, o)
+
+    def timed(run, reps):
+        run()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(reps):
+            run()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t1) / reps
+
+    b = Batch()
+    fill(b, meshes)
+    t0 = time.time()
+    b.run()
+    torch.cuda.synchronize()
+    layout, lms = b.info()
+    log("recorded (warm-up, two layouts timed: %s ms, layout %d) in %.1fs" % (lms, layout, time.time() - t0))
+    reps = max(1, args.steps // 2)
+    t_all = timed(b.run, reps)
+    solves = sum(24 for _ in meshes)
+    per_mesh = {}
+    for m in meshes:
+        bm = Batch()
+        fill(bm, [m])
+        t_graph = timed(bm.run, reps)
+        # the same solves as sequential FP64 calls, walking the tree each time
+        g = m["g"]
+        g.setOption("precision", "fp64")
+        g.setOption("reusePlan", False)
+        g._setMeshOptions()
+        L, ctx = g._L, g.ctx
+        sp = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+
+        def seq():
+            for x, o in zip(m["xs"], m["xv"]):
+                _lib.check(L.idw_warp_mesh_dev(ctx, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(o.data_ptr()), sp))
+            for d, o in zip(m["dxv"], m["dxs"]):
+                _lib.check(L.idw_warp_deriv_dev(ctx, ctypes.c_void_p(m["xs"][0].data_ptr()),
+                                                ctypes.c_void_p(d.data_ptr()), ctypes.c_void_p(o.data_ptr()), sp))
+
+        t_seq = timed(seq, 1)
+        g.setOption("precision", "mixed")
+        g._setMeshOptions()
+        per_mesh[m["name"]] = {"n_vol": m["nvol"], "graph_ms": round(t_graph * 1e3, 2),
+                               "sequential_fp64_ms": round(t_seq * 1e3, 2),
+                               "speedup": round(t_seq / t_graph, 2)}
+        log("%s: graph %.1f ms, sequential FP64 %.1f ms" % (m["name"], t_graph * 1e3, t_seq * 1e3))
+        del bm
+    seq_total = sum(v["sequential_fp64_ms"] for v in per_mesh.values())
+    res = {
+        "metric": "batched solves/s: 4 meshes (0.5M-20M volume nodes) x (16 warps + 8 reverse products), one graph",
+        "value": round(solves / t_all, 2), "unit": "solves/s", "n_gpus": 1, "steps": reps, "warmup": 1,
+        "ms_per_step": round(t_all * 1e3, 2), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": "ellipsoid O-meshes 129x65x61, 193x97x107, 257x129x151 and the 20M-node star body; "
+                               "fast mode errTol=1e-4, mixed; 16 surfaces + 8 reverse products per mesh, recorded "
+                               "into one HIP graph",
+                   "graph_layout": ["branch per mesh", "chain"][layout], "autotune_replay_ms": lms},
+        "per_mesh": per_mesh,
+        "sequential_fp64_ms": round(seq_total, 2),
+        "speedup_vs_sequential_fp64": round(seq_total / (t_all * 1e3), 2),
+    }
     print(json.dumps(res), flush=True)
 
 
