@@ -24,7 +24,8 @@ EXPORTS = (
     "idw_warp_deriv", "idw_get_dxs", "idw_warp_mesh_dev", "idw_warp_deriv_dev", "idw_get_sizes", "idw_get_ldef0",
     "idw_get_denominator0", "idw_get_timings", "idw_fast_counts", "idw_warp_deriv_fwd", "idw_warp_deriv_fwd_dev",
     "idw_set_symmetry_planes", "idw_batch_create", "idw_batch_destroy", "idw_batch_add_warp",
-    "idw_batch_add_warp_deriv", "idw_batch_run", "idw_batch_info",
+    "idw_batch_add_warp_deriv", "idw_batch_run", "idw_batch_info", "idw_set_restart_file",
+    "idw_get_restart_info",
 )
 
 
@@ -110,6 +111,8 @@ def load():
         "idw_batch_add_warp_deriv": (i32, [vp, vp, vp, vp, vp]),
         "idw_batch_run": (i32, [vp, vp]),
         "idw_batch_info": (i32, [vp, P(i32), vp]),
+        "idw_set_restart_file": (i32, [vp, ctypes.c_char_p]),
+        "idw_get_restart_info": (i32, [vp, P(i32), vp, i64]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -84,6 +84,9 @@ struct idw_ctx {
     bool fwdTimed = false, revTimed = false;
     bool capturing = false;  // work is being recorded into a HIP graph: no events, syncs or allocations
     uint64_t gen = 0;        // bumped whenever device buffers, plans or options change (recorded graphs check it)
+    std::string restartFile;        // denominator estimate + costs file (initializeWarping.F90:98-312)
+    std::string restartStatus;      // "", "loaded", "saved"
+    std::vector<double> costs;      // load-balancing costs of the last estimate (warp order)
 };
 
 namespace {
@@ -181,13 +184,72 @@ int ensure_mirror_buffers(idw_ctx *c) {
     return IDW_OK;
 }
 
+int restart_file(idw_ctx *c);
+
 int compute_den0(idw_ctx *c) {
     if (!c->volSet || !c->surfInit || c->V.n == 0) return IDW_OK;
     for (int kk = 0; kk < nloop(c); ++kk) idw::launch_den0(c->st, c->S, c->V, make_params(c, true), mirror_of(c, kk));
     HIPCHK(hipGetLastError());
+    if (!c->restartFile.empty()) TRY(restart_file(c));
     HIPCHK(hipStreamSynchronize(c->st));
     c->denValid = false;
     drop_plan(c);
+    return IDW_OK;
+}
+
+// Restart file (initializeWarping.F90:98-312), one process: int32 node
+// count, then the denominator estimate and the load-balancing costs, each
+// nodes x float64 in the volume nodes' own order.  A file whose count matches
+// and whose estimate agrees with the device's at every 100th node (relative
+// 1e-10: the device's powers and sums round differently from the Fortran's)
+// is loaded; otherwise the estimate is kept, the costs are computed and the
+// file is written.
+int restart_file(idw_ctx *c) {
+    const int64_t n = c->V.n;
+    std::vector<double4> vol(n);
+    std::vector<int32_t> perm(n);
+    HIPCHK(hipMemcpyAsync(vol.data(), c->V.vol, sizeof(double4) * n, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(perm.data(), c->V.perm, sizeof(int32_t) * n, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    std::vector<double> den(n), fden(n), fcost(n);
+    for (int64_t k = 0; k < n; ++k) den[perm[k]] = vol[k].w;
+    bool good = false;
+    if (FILE *f = std::fopen(c->restartFile.c_str(), "rb")) {
+        int32_t cnt = -1;
+        good = std::fread(&cnt, sizeof(cnt), 1, f) == 1 && cnt == n &&
+               std::fread(fden.data(), sizeof(double), n, f) == (size_t)n &&
+               std::fread(fcost.data(), sizeof(double), n, f) == (size_t)n;
+        std::fclose(f);
+        for (int64_t j = 0; good && j < n; j += 100)
+            if (!(std::fabs(1.0 - den[j] / fden[j]) <= 1e-10)) good = false;
+    }
+    if (good) {
+        for (int64_t k = 0; k < n; ++k) vol[k].w = fden[perm[k]];
+        HIPCHK(hipMemcpyAsync(c->V.vol, vol.data(), sizeof(double4) * n, hipMemcpyHostToDevice, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        c->costs = std::move(fcost);
+        c->restartStatus = "loaded";
+        return IDW_OK;
+    }
+    double *d = nullptr;
+    TRY(dalloc(c, &d, n));
+    for (int kk = 0; kk < nloop(c); ++kk)
+        idw::launch_dry_run(c->st, c->S, c->V, mirror_of(c, kk), c->initOpt.errTol, d);
+    HIPCHK(hipGetLastError());
+    std::vector<double> cost(n);
+    HIPCHK(hipMemcpyAsync(cost.data(), d, sizeof(double) * n, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    dfree(c, d);
+    c->costs.assign(n, 0.0);
+    for (int64_t k = 0; k < n; ++k) c->costs[perm[k]] = cost[k];
+    FILE *f = std::fopen(c->restartFile.c_str(), "wb");
+    if (!f) return fail(IDW_EINVAL, "cannot write restart file %s", c->restartFile.c_str());
+    const int32_t cnt = (int32_t)n;
+    const bool ok = std::fwrite(&cnt, sizeof(cnt), 1, f) == 1 &&
+                    std::fwrite(den.data(), sizeof(double), n, f) == (size_t)n &&
+                    std::fwrite(c->costs.data(), sizeof(double), n, f) == (size_t)n;
+    if (std::fclose(f) != 0 || !ok) return fail(IDW_EINVAL, "cannot write restart file %s", c->restartFile.c_str());
+    c->restartStatus = "saved";
     return IDW_OK;
 }
 
@@ -619,6 +681,23 @@ int idw_set_symmetry_planes(idw_ctx *c, const double *pts, const double *normals
     c->denValid = false;
     // the denominator estimate sums over the mirror passes
     return compute_den0(c);
+}
+
+int idw_set_restart_file(idw_ctx *c, const char *path) {
+    if (!c) return fail(IDW_EINVAL, "null context");
+    c->restartFile = path ? path : "";
+    c->restartStatus.clear();
+    return IDW_OK;
+}
+
+int idw_get_restart_info(idw_ctx *c, int32_t *status, double *costs, int64_t n) {
+    if (!c) return fail(IDW_EINVAL, "null context");
+    if (status) *status = c->restartStatus == "loaded" ? 1 : (c->restartStatus == "saved" ? 2 : 0);
+    if (costs) {
+        if (n != (int64_t)c->costs.size()) return fail(IDW_EINVAL, "expected %lld costs", (long long)c->costs.size());
+        std::copy(c->costs.begin(), c->costs.end(), costs);
+    }
+    return IDW_OK;
 }
 
 int idw_point_reduce(const double *pts, int64_t n, double tol, double *unique_pts, int32_t *link, int64_t *n_unique) {

@@ -99,6 +99,9 @@ void launch_corners(hipStream_t st, const DevSurface &S, double threshold);
 void launch_cluster_data(hipStream_t st, const DevSurface &S);
 void launch_errors(hipStream_t st, const DevSurface &S, const WParams &P, const double *sphere60);
 void launch_den0(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, const Mirror &M);
+// load-balancing costs (dryRun) of one mirror pass, added into cost (traversal order)
+void launch_dry_run(hipStream_t st, const DevSurface &S, const DevVolume &V, const Mirror &M, double errTol,
+                    double *cost);
 void launch_morton(hipStream_t st, int64_t n, const double *xv0, double lo[3], double scale[3], uint64_t *keys,
                    int32_t *ids);
 void launch_build_vol(hipStream_t st, int64_t n, const double *xv0, const int32_t *perm, double4 *vol);

@@ -778,6 +778,7 @@ __global__ void __launch_bounds__(256) k_plan_walk(int nvol, const double4 *__re
                                                    int64_t *__restrict__ waveCount, PlanEntry *__restrict__ plan,
                                                    const int64_t *__restrict__ waveOff,
                                                    unsigned int *__restrict__ mismatch, Mirror M) {
+#pragma clang fp contract(off)
     const int nb = (nvol + 255) / 256;
     const int blk = COUNT ? (int)blockIdx.x : xcd_block(nb);
     if (blk >= nb) return;
@@ -846,6 +847,7 @@ __global__ void __launch_bounds__(256) k_near_keys(int nvol, const double4 *__re
                                                    const NodeRec *__restrict__ nodes, double errTol, double nearT2,
                                                    double3 lo, double3 scale, uint64_t *__restrict__ keys,
                                                    int32_t *__restrict__ ids) {
+#pragma clang fp contract(off)
     const int k = blockIdx.x * 256 + threadIdx.x;
     const bool valid = k < nvol;
     const double4 v = valid ? vol[k] : make_double4(0.0, 0.0, 0.0, 1.0);
@@ -1156,6 +1158,7 @@ __global__ void __launch_bounds__(256) k_fast_rev(int nvol, const double4 *__res
 __global__ void __launch_bounds__(256) k_fast_count(int nvol, const double4 *__restrict__ vol, int nnodes,
                                                     const NodeRec *__restrict__ nodes, double errTol,
                                                     unsigned long long *__restrict__ out, Mirror M) {
+#pragma clang fp contract(off)
     __shared__ unsigned long long red[8];
     if (threadIdx.x < 8) red[threadIdx.x] = 0;
     __syncthreads();
@@ -1442,6 +1445,7 @@ __global__ void __launch_bounds__(256) k_errors(const int32_t *__restrict__ inte
 __global__ void __launch_bounds__(256) k_den0(int nvol, double4 *__restrict__ vol, int nnodes,
                                               const NodeRec *__restrict__ nodes, const double4 *__restrict__ ps,
                                               WParams P, Mirror M) {
+#pragma clang fp contract(off)
     const int nb = (nvol + 255) / 256;
     const int blk = xcd_block(nb);
     if (blk >= nb) return;
@@ -1480,6 +1484,47 @@ __global__ void __launch_bounds__(256) k_den0(int nvol, double4 *__restrict__ vo
         node = __any(descend) ? node + 1 : uniform(r->skip);
     }
     if (valid) vol[k].w = dd;
+}
+
+// dryRun (kd_tree.F90:789-830) per volume node: n per bucket, 0.5 per
+// internal node visited and 1.1 per accepted cluster, with the dry run's own
+// distance eps; summed over the mirror passes after division by the unique
+// surface node count (initializeWarping.F90:258-265).  These are the
+// load-balancing costs the restart file stores.
+__global__ void __launch_bounds__(256) k_dry_run(int nvol, const double4 *__restrict__ vol, int nnodes,
+                                                 const NodeRec *__restrict__ nodes, double errTol, int nsurf,
+                                                 Mirror M, double *__restrict__ cost) {
+#pragma clang fp contract(off)
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    const bool valid = k < nvol;
+    double4 v = valid ? vol[k] : make_double4(0.0, 0.0, 0.0, 1.0);
+    if (M.mask) mirror_pt(M, v.x, v.y, v.z);
+    const double tol = errTol * v.w;
+    int stop = valid ? INT32_MAX : -1;
+    double c = 0.0;
+    int node = 0;
+    while (node < nnodes) {
+        const NodeRec *r = nodes + node;
+        const int depth = uniform(r->depth);
+        const bool active = depth <= stop;
+        if (uniform(r->leaf)) {
+            if (active) c += (double)uniform(r->n);
+            node = uniform(r->skip);
+            continue;
+        }
+        bool descend = false;
+        if (active) {
+            const double cx = v.x - r->X[0], cy = v.y - r->X[1], cz = v.z - r->X[2];
+            const double dOvrR = sqrt(cx * cx + cy * cy + cz * cz + 1e-16) * r->orad;
+            c += 0.5;
+            if (dOvrR < 2.0) descend = true;
+            else if (get_error(r, dOvrR) < tol) c += 1.1;
+            else descend = true;
+            stop = descend ? INT32_MAX : depth;
+        }
+        node = __any(descend) ? node + 1 : uniform(r->skip);
+    }
+    if (valid) cost[k] = (M.mask ? cost[k] : 0.0) + c / nsurf;
 }
 
 __global__ void k_morton(int64_t nvol, const double *__restrict__ xv0, double3 lo, double3 scale,
@@ -1554,6 +1599,11 @@ void launch_errors(hipStream_t st, const DevSurface &S, const WParams &P, const 
 
 void launch_den0(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, const Mirror &M) {
     k_den0<<<xcd_grid(V.n), 256, 0, st>>>((int)V.n, V.vol, S.nnodes, S.nodes, S.ps, P, M);
+}
+
+void launch_dry_run(hipStream_t st, const DevSurface &S, const DevVolume &V, const Mirror &M, double errTol,
+                    double *cost) {
+    k_dry_run<<<grid_for(V.n, 256), 256, 0, st>>>((int)V.n, V.vol, S.nnodes, S.nodes, errTol, S.n, M, cost);
 }
 
 void launch_morton(hipStream_t st, int64_t n, const double *xv0, double lo[3], double scale[3], uint64_t *keys,

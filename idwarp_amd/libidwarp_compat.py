@@ -100,6 +100,8 @@ class LibIdwarp:
         p = _lib.f64(pts)
         u = np.ascontiguousarray(np.asarray(uniquepts, dtype=np.float64).T)
         link, fs, fc = _lib.i32(link), _lib.i32(facesizes), _lib.i32(faceconn)
+        restart = str(restartfile).strip() if restartfile is not None else ""
+        _lib.check(self._L.idw_set_restart_file(self._ensure(), restart.encode() if restart else None))
         _lib.check(self._L.idw_initialize_warping(self._ensure(), _lib.ptr(p), p.size // 3, _lib.ptr(u), len(u),
                                                   _lib.ptr(link), _lib.ptr(fs), fs.size, _lib.ptr(fc), fc.size))
         self._npts = p.size // 3

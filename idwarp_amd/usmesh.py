@@ -152,6 +152,8 @@ class USMesh:
         link = _lib.i32(inv[link - 1] + 1)
         self._setSymmetryConditions()
         self._setMeshOptions()
+        restart = self.getOption("restartFile")
+        _lib.check(self._L.idw_set_restart_file(self._ctx, restart.encode() if restart else None))
         _lib.check(self._L.idw_initialize_warping(self._ctx, _lib.ptr(pts), len(pts), _lib.ptr(uniquePts),
                                                   len(uniquePts), _lib.ptr(link), _lib.ptr(faceSizes),
                                                   len(faceSizes), _lib.ptr(conn), len(conn)))
@@ -306,6 +308,16 @@ class USMesh:
         d = np.zeros(self.nVol)
         _lib.check(self._L.idw_get_denominator0(self._ctx, _lib.ptr(d), d.size))
         return d
+
+    def getRestartInfo(self):
+        """('none' | 'loaded' | 'saved', load-balancing costs per volume node)
+        of the restart file used by the last initialisation."""
+        st = ctypes.c_int32(0)
+        _lib.check(self._L.idw_get_restart_info(self._ctx, ctypes.byref(st), None, 0))
+        costs = np.zeros(self.nVol if st.value else 0)
+        if st.value:
+            _lib.check(self._L.idw_get_restart_info(self._ctx, ctypes.byref(st), _lib.ptr(costs), costs.size))
+        return ["none", "loaded", "saved"][st.value], costs
 
     def getTimings(self):
         """ms of the last warp/deriv phases: [nodal+cluster, tree walk,

@@ -93,6 +93,22 @@ int idw_set_volume_nodes(idw_ctx *ctx, const double *xv0, int64_t ndof);
  * denominator estimate is recomputed). */
 int idw_set_symmetry_planes(idw_ctx *ctx, const double *pts, const double *normals, int64_t n);
 
+/* The restartfile argument of initializewarping (libidwarp.pyf:55-68,
+ * initializeWarping.F90:98-312): the file used by the next
+ * idw_initialize_warping (and by idw_set_symmetry_planes after it) for the
+ * denominator estimate.  Same layout as the reference's single-process file:
+ * int32 node count, then n_vol float64 estimates and n_vol float64
+ * load-balancing costs (dryRun, kd_tree.F90:789-830), in volume-node order.
+ * A file with the right count whose estimates agree with the device's at
+ * every 100th node (relative 1e-10) is loaded; otherwise the file is
+ * (re)written.  NULL or "" = no file. */
+int idw_set_restart_file(idw_ctx *ctx, const char *path);
+
+/* Outcome of the last restart-file use (0 none, 1 loaded, 2 written) and
+ * the costs it holds (n = n_vol; costs may be NULL).  No counterpart in the
+ * reference. */
+int idw_get_restart_info(idw_ctx *ctx, int32_t *status, double *costs, int64_t n);
+
 /* pointreduce (libidwarp.pyf:195-202, src/utils/pointReduce.F90:1-83):
  * merge points within tol.  pts and unique_pts are 3*n; link (n) is 1-based
  * into unique_pts.  Context-free. */

@@ -829,6 +829,30 @@ static void eval_node(const octx *t, int id, const double *r, double *num, doubl
     }
 }
 
+/* dryRun (kd_tree.F90:789-830): operation-count estimate of one walk */
+static void dry_run(const octx *t, int id, const double *r, double approxDen, double *c) {
+    const onode *np = &t->nodes[id];
+    if (np->dnum == 0) {
+        *c = *c + (double)np->n;
+        return;
+    }
+    double rr0 = r[0] - np->X[0], rr1 = r[1] - np->X[1], rr2 = r[2] - np->X[2];
+    double dist = sqrt(rr0 * rr0 + rr1 * rr1 + rr2 * rr2 + 1e-16);
+    *c = *c + 0.5;
+    if (dist * np->oradius < 2.0) {
+        dry_run(t, np->left, r, approxDen, c);
+        dry_run(t, np->right, r, approxDen, c);
+        return;
+    }
+    double err = get_error(t, np, dist);
+    if (err < t->errTol * approxDen) {
+        *c = *c + 1.1;
+    } else {
+        dry_run(t, np->left, r, approxDen, c);
+        dry_run(t, np->right, r, approxDen, c);
+    }
+}
+
 /* getWiEstimate (kd_tree.F90:751-787) */
 static void wi_estimate(const octx *t, int id, const double *r, double *den) {
     const onode *np = &t->nodes[id];
@@ -1258,6 +1282,24 @@ void orc_warp(void *h, const double *xs, const double *xv0, long nvol, const dou
     }
     g_times[0] = t1 - t0;
     g_times[1] = wall() - t1;
+}
+
+/* Load-balancing costs of initializeWarping.F90:258-265: per volume node,
+ * the sum over mirror passes of dryRun / (unique surface nodes), with the
+ * init-time errTol. */
+void orc_costs(void *h, const double *xv0, long nvol, const double *den0, double *costs) {
+    octx *t = h;
+#pragma omp parallel for schedule(dynamic, 256) num_threads(g_threads)
+    for (long j = 0; j < nvol; j++) {
+        double cost = 0.0;
+        for (int kk = 1; kk <= (1 << t->nplanes); kk++) {
+            double r[3], c = 0.0;
+            mirror_pt(t, xv0 + 3 * j, r, kk);
+            dry_run(t, 0, r, den0[j], &c);
+            cost = cost + c / t->n;
+        }
+        costs[j] = cost;
+    }
 }
 
 /* Per-node work counts for the fast mode: V internal visits, A accepted

@@ -65,6 +65,7 @@ def lib():
         L.orc_warp_deriv.argtypes = [P, dp, dp, L64, dp, dp, dp, dp]
         L.orc_warp_deriv_fwd.argtypes = [P, dp, dp, dp, L64, dp, dp, dp]
         L.orc_counts.argtypes = [P, dp, dp, L64, dp, dp]
+        L.orc_costs.argtypes = [P, dp, L64, dp, dp]
         L.orc_set_symmetry_planes.restype = I
         L.orc_set_symmetry_planes.argtypes = [P, dp, dp, I]
         L.orc_point_reduce.restype = I
@@ -190,6 +191,12 @@ class OracleMesh:
         lib().orc_warp_deriv_fwd(self.h, self.xs, np.ascontiguousarray(dxs, np.float64).ravel(), self.xv0.ravel(),
                                  self.nvol, self.den0, self.den, dxv)
         return dxv
+
+    def costs(self):
+        """Load-balancing costs of the restart file (dryRun per volume node)."""
+        out = np.zeros(self.nvol)
+        lib().orc_costs(self.h, self.xv0.ravel(), self.nvol, self.den0, out)
+        return out
 
     def counts(self, xv0=None, den0=None):
         xv0 = self.xv0 if xv0 is None else np.ascontiguousarray(xv0, np.float64).reshape(-1, 3)

@@ -476,3 +476,49 @@ def test_batch_graph_replays_match_eager():
     layout, ms = b.info()
     print("batch layout", layout, "replay ms", ms)
     assert layout in (0, 1) and min(ms) > 0
+
+
+def test_restart_file_roundtrip(tmp_path):
+    """initializeWarping.F90:98-312 restart file: written when missing (count,
+    estimate, costs), loaded when it matches, rewritten when its count or its
+    estimate disagree."""
+    m = MESHES["star"]
+    path = str(tmp_path / "restart.bin")
+    opts = dict(gridFile=m["xv0"], fileType="memory", evalMode="fast", errTol=1e-4, restartFile=path)
+    ora = O.OracleMesh(m["pts"], m["conn"], m["faceSizes"], m["xv0"], evalMode="fast", errTol=1e-4)
+    g1 = USMesh(options=opts)
+    g1.setSurfaceDefinition(m["pts"], m["conn"], m["faceSizes"])
+    status, costs = g1.getRestartInfo()
+    n = len(m["xv0"])
+    raw = open(path, "rb").read()
+    assert status == "saved" and len(raw) == 4 + 16 * n
+    assert np.frombuffer(raw[:4], np.int32)[0] == n
+    fden = np.frombuffer(raw[4:4 + 8 * n], np.float64)
+    fcost = np.frombuffer(raw[4 + 8 * n:], np.float64)
+    assert np.array_equal(fden, g1.getDenominator0()) and np.array_equal(fcost, costs)
+    np.testing.assert_allclose(fcost, ora.costs(), rtol=1e-12)
+    xs = m["perturb"](m["pts"])
+    g1.setSurfaceCoordinates(xs)
+    g1.warpMesh()
+    # loaded
+    g2 = USMesh(options=opts)
+    g2.setSurfaceDefinition(m["pts"], m["conn"], m["faceSizes"])
+    assert g2.getRestartInfo()[0] == "loaded"
+    assert np.array_equal(g2.getDenominator0(), fden)
+    g2.setSurfaceCoordinates(xs)
+    g2.warpMesh()
+    assert np.array_equal(g2.getWarpGrid(), g1.getWarpGrid())
+    # an estimate off by 1e-6 is rejected and the file rewritten
+    bad = np.frombuffer(raw, np.uint8).copy()
+    den_bad = (fden * (1 + 1e-6)).astype(np.float64)
+    bad[4:4 + 8 * n] = np.frombuffer(den_bad.tobytes(), np.uint8)
+    open(path, "wb").write(bad.tobytes())
+    g3 = USMesh(options=opts)
+    g3.setSurfaceDefinition(m["pts"], m["conn"], m["faceSizes"])
+    assert g3.getRestartInfo()[0] == "saved"
+    assert np.array_equal(np.frombuffer(open(path, "rb").read()[4:4 + 8 * n], np.float64), fden)
+    # a file for another mesh size is rewritten
+    open(path, "wb").write(np.int32(n + 1).tobytes() + bytes(16 * (n + 1)))
+    g4 = USMesh(options=opts)
+    g4.setSurfaceDefinition(m["pts"], m["conn"], m["faceSizes"])
+    assert g4.getRestartInfo()[0] == "saved"

@@ -331,3 +331,17 @@ def test_symmetry_fast_with_zero_tolerance_is_exact():
     oe.setSurfaceCoordinates(xs)
     of.setSurfaceCoordinates(xs)
     assert np.array_equal(oe.warpMesh(), of.warpMesh())
+
+
+def test_load_balancing_costs_follow_the_work_counts(star):
+    """dryRun (kd_tree.F90:789-830) counts n per bucket, 0.5 per internal
+    visit and 1.1 per accepted cluster; its distance carries a 1e-16 eps that
+    the warp's does not, which may only move a decision at a tie."""
+    m = star
+    om = O.OracleMesh(m["pts"], m["conn"], m["faceSizes"], m["xv0"], evalMode="fast", errTol=1e-4)
+    costs = om.costs()
+    om.setSurfaceCoordinates(m["pts"])
+    c = om.counts()
+    expect = (c[:, 2] + 0.5 * c[:, 0] + 1.1 * c[:, 1]) / om.n
+    assert np.allclose(costs, expect, rtol=1e-12, atol=0)
+    assert costs.min() > 0
