@@ -303,6 +303,7 @@ __global__ void k_node_props(int n, int initial, int useRot, const int32_t *__re
 // kd_tree.F90:1275-1360) for a surface tangent dxs
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double dot3(const double *a, const double *b) {
+#pragma clang fp contract(off)
     return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
 }
 
@@ -312,6 +313,7 @@ __global__ void k_face_props_d(int nface, const int32_t *__restrict__ facePtr, c
                                const int32_t *__restrict__ xuInd, const double *__restrict__ xs,
                                const double *__restrict__ dxs, double4 *__restrict__ fprops,
                                double4 *__restrict__ fpropsd) {
+#pragma clang fp contract(off)
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= nface) return;
     const int p0 = facePtr[f], np = facePtr[f + 1] - p0;
@@ -366,6 +368,7 @@ __global__ void k_face_props_d(int nface, const int32_t *__restrict__ facePtr, c
 // M = I + sin(a) A + (1 - cos(a)) A^2, Md = (cos(a) A + sin(a) A^2) a' +
 // sin(a) A' + (1 - cos(a)) (A' A + A A').  Zero below the axis tolerance.
 __device__ void rodrigues_d(const double *v1, const double *v2, const double *v2d, double *Md) {
+#pragma clang fp contract(off)
     for (int k = 0; k < 9; ++k) Md[k] = 0.0;
     double ax[3], axd[3];
     cross3(v1, v2, ax);
@@ -418,6 +421,7 @@ __global__ void k_node_props_d(int n, int useRot, const int32_t *__restrict__ n2
                                const int32_t *__restrict__ corner, const double *__restrict__ dxs,
                                const double4 *__restrict__ ps, const double *__restrict__ normals0,
                                PointDyn *__restrict__ pd, PointDyn32 *__restrict__ pd32) {
+#pragma clang fp contract(off)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double sa = 0.0, sad = 0.0, sn[3] = {0.0, 0.0, 0.0}, snd[3] = {0.0, 0.0, 0.0};
@@ -1219,6 +1223,7 @@ __global__ void __launch_bounds__(256) k_fast_count(int nvol, const double4 *__r
 // ---------------------------------------------------------------------------
 // GETROTATIONMATRIX3D_B (vectorUtils_b.f90:7-154): d<Mb, M>/dv2, row-major Mb.
 __device__ void rodrigues_b(const double *v1, const double *v2, const double *Mb, double *v2b) {
+#pragma clang fp contract(off)
     double ax0[3];
     cross3(v1, v2, ax0);
     const double axmag = sqrt(ax0[0] * ax0[0] + ax0[1] * ax0[1] + ax0[2] * ax0[2] + 1e-30);
@@ -1277,6 +1282,7 @@ __global__ void k_nodal_rev_a(int n, int fast, int useRot, const NodeRec *__rest
                               const double4 *__restrict__ ps, const double4 *__restrict__ nprops,
                               const double *__restrict__ normals0, const int32_t *__restrict__ corner,
                               double *__restrict__ ntmp) {
+#pragma clang fp contract(off)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double Mb[9], Bb[3];
@@ -1323,6 +1329,7 @@ __global__ void k_nodal_rev_face(int nface, const int32_t *__restrict__ facePtr,
                                  const int32_t *__restrict__ xuInd, const double *__restrict__ xs,
                                  const double4 *__restrict__ fprops, const double *__restrict__ ntmp,
                                  double *__restrict__ pb) {
+#pragma clang fp contract(off)
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= nface) return;
     const int p0 = facePtr[f], np = facePtr[f + 1] - p0;
@@ -1384,6 +1391,7 @@ __global__ void k_nodal_rev_face(int nface, const int32_t *__restrict__ facePtr,
 __global__ void k_nodal_rev_gather(int n, const int32_t *__restrict__ slotStart, const int32_t *__restrict__ slots,
                                    const double *__restrict__ pb, const double *__restrict__ ntmp,
                                    const int32_t *__restrict__ xuInd, double *__restrict__ dxs) {
+#pragma clang fp contract(off)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double *t = ntmp + (int64_t)i * 8;

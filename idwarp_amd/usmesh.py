@@ -71,11 +71,20 @@ class USMesh:
         self.warpInitialized = False
         self.nSurf = 0
         self.nVol = 0
+        self.OFData = {}
+        self.fileType = self.getOption("fileType")
         grid = self.getOption("gridFile")
         if grid is not None:
-            if isinstance(grid, str):
-                if self.getOption("fileType") not in ("NPY", "memory") or not grid.endswith(".npy"):
-                    raise Error("Only in-memory volume grids (an array or a .npy file) are supported")
+            if self.fileType == "OpenFOAM":
+                # UnstructuredMesh.py:1004-1049 (_readOFGrid)
+                from . import openfoam
+
+                self.OFData = openfoam.read_polymesh(grid)
+                grid = self.OFData["x0"]
+            elif isinstance(grid, str):
+                if self.fileType not in ("NPY", "memory") or not grid.endswith(".npy"):
+                    raise Error("Grid files other than OpenFOAM cases and .npy arrays are not supported "
+                                "(CGNS and PLOT3D readers are not part of this library)")
                 grid = np.load(grid)
             self.setVolumeNodes(grid)
 
@@ -183,7 +192,27 @@ class USMesh:
     # ------------------------------------------------------------------
     # surface coordinates
     # ------------------------------------------------------------------
+    def _setInternalSurface(self):
+        """UnstructuredMesh.py:602-750: without setSurfaceDefinition(), an
+        OpenFOAM case is warped about all of its patch, wall and slip
+        boundary faces."""
+        if self.warpInitialized:
+            return
+        if self.fileType != "OpenFOAM" or not self.OFData:
+            raise Error("No surface defined: call setSurfaceDefinition() first")
+        import warnings
+
+        from . import openfoam
+
+        warnings.warn("Using internally generated IDWarp surfaces. If this mesh object is to be used with an "
+                      "external solver, ensure the mesh object is passed to the solver immediatedly after it is "
+                      "created. The external solver must then call 'setExternalMeshIndices()' and "
+                      "'setSurfaceDefinition()' routines.", stacklevel=2)
+        pts, conn, faceSizes = openfoam.warp_surface(self.OFData)
+        self.setSurfaceDefinition(pts, conn, faceSizes)
+
     def getSurfaceCoordinates(self):
+        self._setInternalSurface()
         coords = np.zeros((self.nSurf, 3), self.dtype)
         _lib.check(self._L.idw_get_surface_coordinates(self._ctx, _lib.ptr(coords), coords.size))
         return coords
@@ -200,6 +229,7 @@ class USMesh:
     # ------------------------------------------------------------------
     def warpMesh(self):
         """Warp the volume to the surface set by setSurfaceCoordinates()."""
+        self._setInternalSurface()
         self._setMeshOptions()
         _lib.check(self._L.idw_warp_mesh(self._ctx))
 

@@ -522,3 +522,44 @@ def test_restart_file_roundtrip(tmp_path):
     g4 = USMesh(options=opts)
     g4.setSurfaceDefinition(m["pts"], m["conn"], m["faceSizes"])
     assert g4.getRestartInfo()[0] == "saved"
+
+
+@pytest.mark.parametrize("mode", ["exact", "fast"])
+def test_ahmed_body_openfoam_with_symmetry_plane(tmp_path, mode):
+    """The reference's Ahmed-body example (examples/AhmedBodyCoarse: an
+    OpenFOAM case read through fileType='OpenFOAM', warped about its patch,
+    wall and slip faces with a y=0 symmetry plane and the example's options)
+    against the oracle."""
+    from tests.test_openfoam_cpu import ahmed_case
+
+    from idwarp_amd import openfoam
+
+    case = ahmed_case(tmp_path)
+    opts = dict(gridFile=case, fileType="OpenFOAM", symmetryPlanes=[[[0, 0, 0], [0, 1, 0]]], aExp=3.0, bExp=5.0,
+                alpha=1.0, LdefFact=0.20, evalMode=mode)
+    gpu = USMesh(options=opts)
+    coords0 = gpu.getSurfaceCoordinates()  # builds the internal OpenFOAM surface
+    of = openfoam.read_polymesh(case)
+    pts, conn, fs = openfoam.warp_surface(of)
+    assert np.array_equal(coords0, pts)
+    ora = O.OracleMesh(pts, conn, fs, of["x0"], evalMode=mode, aExp=3.0, bExp=5.0, alpha=1.0, LdefFact=0.20,
+                       symmetryPlanes=[([0.0, 0.0, 0.0], [0.0, 1.0, 0.0])])
+    lo, hi = of["x0"].min(0), of["x0"].max(0)
+    on_box = (np.isclose(pts[:, 0], lo[0]) | np.isclose(pts[:, 0], hi[0]) | np.isclose(pts[:, 2], lo[2]) |
+              np.isclose(pts[:, 2], hi[2]) | np.isclose(pts[:, 1], hi[1]))
+    xs = pts.copy()
+    xs[~on_box, 0] += 0.005 * np.sin(3 * pts[~on_box, 0] + 1)
+    xs[~on_box, 2] += 0.01 * np.cos(2 * pts[~on_box, 0])
+    ora.setSurfaceCoordinates(xs)
+    gpu.setSurfaceCoordinates(xs)
+    ref = ora.warpMesh()
+    gpu.warpMesh()
+    err = np.abs(gpu.getWarpGrid() - ref).max() / ora.Ldef0
+    print("Ahmed", mode, "max |dXv|/Ldef", err, "moved surface points", int((~on_box).sum()))
+    assert err <= FP64_TOL
+    assert np.abs(ref - of["x0"].ravel()).max() > 1e-3
+    v = synthetic.random_dxv(len(of["x0"]), seed=11, frac=0.5)
+    gpu.warpDeriv(v, solverVec=False)
+    r = _rel(gpu.getdXs().ravel(), ora.warpDeriv(v))
+    print("adjoint rel err", r)
+    assert r <= 1e-10
