@@ -12,6 +12,9 @@ HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-munsafe-fp-atomics",
     # keep packed-FP32 instructions out of the generated code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # MFMA accumulators in VGPRs: the fast-mode loops carry them across
+    # iterations, and the AGPR form copied them back and forth every step
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
 ]
 
 

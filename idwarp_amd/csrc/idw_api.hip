@@ -257,6 +257,7 @@ void drop_plan(idw_ctx *c) {
     for (idw::DevPlan &P : c->plans) {
         dfree(c, P.entries);
         dfree(c, P.waveOff);
+        dfree(c, P.waveSplit);
         dfree(c, P.waveCount);
         dfree(c, P.mismatch);
     }
@@ -330,9 +331,11 @@ int size_one_plan(idw_ctx *c, int kk) {
     const int64_t nwave = ((c->V.n + 255) / 256) * 4;
     if (!P.waveOff || P.nwave != nwave) {
         dfree(c, P.waveOff);
+        dfree(c, P.waveSplit);
         dfree(c, P.waveCount);
         if (!P.mismatch) TRY(dalloc(c, &P.mismatch, 1));
         TRY(dalloc(c, &P.waveOff, nwave + 1));
+        TRY(dalloc(c, &P.waveSplit, nwave));
         TRY(dalloc(c, &P.waveCount, nwave + 1));
         P.nwave = nwave;
     }
@@ -721,8 +724,8 @@ int idw_initialize_warping(idw_ctx *c, const double *pts, int64_t n_pts, const d
         return fail(IDW_EINVAL, "%s", msg.c_str());
     // free a previous surface
     idw::DevSurface &S = c->S;
-    void *olds[] = {S.nodes, S.nd, S.nd32, S.internal, S.ps, S.pd, S.pd32, S.normals0, S.nprops, S.corner, S.xuInd,
-                    S.facePtr, S.faceConn, S.n2eStart, S.n2e, S.slotStart, S.slots, S.fprops, S.padj, S.nadj,
+    void *olds[] = {S.nodes, S.rec, S.rec32, S.internal, S.ps, S.normals0, S.nprops, S.corner, S.xuInd,
+                    S.facePtr, S.faceConn, S.n2eStart, S.n2e, S.slotStart, S.slots, S.fprops, S.adj,
                     S.ntmp, S.pb, S.fpropsd, c->xs, c->dxs, c->dxsIn, c->partial};
     for (void *p : olds) {
         void *q = p;
@@ -742,12 +745,12 @@ int idw_initialize_warping(idw_ctx *c, const double *pts, int64_t n_pts, const d
     S.nInternal = H.nInternal;
     S.nface = (int)H.nface;
     TRY(dalloc(c, &S.nodes, S.nnodes));
-    TRY(dalloc(c, &S.nd, S.nnodes));
-    TRY(dalloc(c, &S.nd32, S.nnodes));
+    TRY(dalloc(c, &S.rec, (size_t)S.nnodes + S.n));
+    TRY(dalloc(c, &S.rec32, (size_t)S.nnodes + S.n));
+    S.pd = S.rec + S.nnodes;
+    S.pd32 = S.rec32 + S.nnodes;
     TRY(dalloc(c, &S.internal, S.nInternal));
     TRY(dalloc(c, &S.ps, S.n));
-    TRY(dalloc(c, &S.pd, S.n));
-    TRY(dalloc(c, &S.pd32, S.n));
     TRY(dalloc(c, &S.normals0, 3 * (size_t)S.n));
     TRY(dalloc(c, &S.nprops, S.n));
     TRY(dalloc(c, &S.corner, S.n));
@@ -760,8 +763,9 @@ int idw_initialize_warping(idw_ctx *c, const double *pts, int64_t n_pts, const d
     TRY(dalloc(c, &S.slots, H.nconn));
     TRY(dalloc(c, &S.fprops, H.nface));
     TRY(dalloc(c, &S.fpropsd, H.nface));
-    TRY(dalloc(c, &S.padj, (size_t)S.n * idw::ADJ_STRIDE));
-    TRY(dalloc(c, &S.nadj, (size_t)S.nnodes * idw::ADJ_STRIDE));
+    TRY(dalloc(c, &S.adj, ((size_t)S.nnodes + S.n) * idw::ADJ_STRIDE));
+    S.nadj = S.adj;
+    S.padj = S.adj + (size_t)S.nnodes * idw::ADJ_STRIDE;
     TRY(dalloc(c, &S.ntmp, (size_t)S.n * 8));
     TRY(dalloc(c, &S.pb, 3 * (size_t)H.nconn));
     TRY(dalloc(c, &c->xs, 3 * (size_t)n_pts));

@@ -35,37 +35,30 @@ struct alignas(16) NodeHdr {
 };
 static_assert(sizeof(NodeHdr) == 64, "NodeHdr layout");
 
-// Per-call cluster data of an internal node (setData, kd_tree.F90:595-617),
-// stored as Kbar = mean(M_i) - I and e = Kbar X + mean(b_i), so that the
-// cluster displacement mean(M) r + mean(b) - r equals Kbar (r - X) + e.
-struct alignas(16) NodeDyn {
-    double K[9];  // row-major
-    double e[3];
-    double pad[4];
-};
-struct alignas(16) NodeDyn32 {
-    float K[9];
-    float e[3];
-};
-
-// Per-call surface point record in tree order: position Xu0, area weight A,
-// K = M_i - I and d = Xu_i - Xu0_i, so that M_i r + b_i - r == K (r - Xu0_i) + d
-// (b_i = Xu_i - M_i Xu0_i, kd_tree.F90:1648).  One record per point so a
-// bucket point is one contiguous fetch.
-struct alignas(16) PointDyn {
-    double x0[3];
+// Per-call record of one term of the IDW sum.  Surface points and tree
+// clusters share the layout, so the fast-mode kernels treat both alike: the
+// term's position x, weight factor A, and K, d such that the displacement it
+// contributes at r is K (r - x) + d.
+//  * surface point i (record nnodes + i, tree order): x = Xu0_i, A = A_i,
+//    K = M_i - I and d = Xu_i - Xu0_i, so that M_i r + b_i - r equals
+//    K (r - Xu0_i) + d (b_i = Xu_i - M_i Xu0_i, kd_tree.F90:1648);
+//  * internal tree node (record = node index): x = centroid X, A = Ai,
+//    K = mean(M_i) - I and d = K X + mean(b_i) (setData,
+//    kd_tree.F90:595-617), so that mean(M) r + mean(b) - r equals K (r - X) + d.
+struct alignas(16) Rec64 {
+    double x[3];
     double A;
-    double K[9];
+    double K[9];  // row-major
     double d[3];
 };
-static_assert(sizeof(PointDyn) == 128, "PointDyn layout");
-struct alignas(16) PointDyn32 {
-    double x0[3];  // positions stay FP64: r - Xu0 is formed before rounding
+static_assert(sizeof(Rec64) == 128, "Rec64 layout");
+struct alignas(16) Rec32 {
+    double x[3];  // positions stay FP64: r - x is formed before rounding
     float A;
     float K[9];
     float d[3];
 };
-static_assert(sizeof(PointDyn32) == 80, "PointDyn32 layout");
+static_assert(sizeof(Rec32) == 80, "Rec32 layout");
 
 // Host-side result of the surface definition (myCreateTree, kd_tree.F90:1365-1545).
 struct HostSurface {

@@ -43,12 +43,12 @@ struct WParams {
 struct DevSurface {
     int n = 0, nnodes = 0, nInternal = 0, nface = 0;
     NodeRec *nodes = nullptr;
-    NodeDyn *nd = nullptr;
-    NodeDyn32 *nd32 = nullptr;
     int32_t *internal = nullptr;
     double4 *ps = nullptr;  // Xu0, A (tree order)
-    PointDyn *pd = nullptr;
-    PointDyn32 *pd32 = nullptr;
+    // term records: nnodes cluster records (index = tree node), then n point
+    // records (index nnodes + tree-order point); pd / pd32 point at the latter
+    Rec64 *rec = nullptr, *pd = nullptr;
+    Rec32 *rec32 = nullptr, *pd32 = nullptr;
     double *normals0 = nullptr;
     double4 *nprops = nullptr;  // sumArea, sumNormal of the last evaluation
     int32_t *corner = nullptr, *xuInd = nullptr;
@@ -56,22 +56,27 @@ struct DevSurface {
     int32_t *slotStart = nullptr, *slots = nullptr;
     double4 *fprops = nullptr;  // dA = area/nPts, unit normal
     double4 *fpropsd = nullptr; // their tangents (warpDerivFwd)
-    double *padj = nullptr, *nadj = nullptr, *ntmp = nullptr, *pb = nullptr;
+    // adjoint records indexed like rec (ADJ_STRIDE doubles each): nadj = adj
+    // (clusters), padj = adj + nnodes * ADJ_STRIDE (points)
+    double *adj = nullptr, *padj = nullptr, *nadj = nullptr, *ntmp = nullptr, *pb = nullptr;
 };
 
-// One record of the forward walk: a tree node and the lanes of the wave it
-// served (accepted cluster or evaluated bucket).
+// One term of the forward walk: the record (accepted cluster, or one point
+// of an evaluated bucket) and the lanes of the wave it serves.
 struct alignas(16) PlanEntry {
     unsigned long long mask;
-    int32_t node;
-    int32_t flags;  // PLAN_FP64: bucket with a lane near the wall (mixed precision keeps FP64)
+    int32_t rec;  // index into the term records (DevSurface::rec)
+    int32_t pad;
 };
-constexpr int PLAN_FP64 = 1;
 
 // Per-wave plan of the fast mode (valid for one volume ordering, tree and errTol).
 struct DevPlan {
     PlanEntry *entries = nullptr;
     int64_t *waveOff = nullptr;  // nwave + 1 offsets
+    // first FP64 entry of each wave: the walk writes the terms evaluated in
+    // FP32 from the front of the wave's range and those kept in FP64 (a lane
+    // near the wall, mixed precision) from the back
+    int64_t *waveSplit = nullptr;
     int64_t *waveCount = nullptr;
     unsigned int *mismatch = nullptr;
     int64_t nwave = 0, total = 0, capacity = 0;

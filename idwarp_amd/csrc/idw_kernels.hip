@@ -252,7 +252,7 @@ __global__ void k_node_props(int n, int initial, int useRot, const int32_t *__re
                              const int32_t *__restrict__ n2e, const double4 *__restrict__ fprops,
                              const int32_t *__restrict__ xuInd, const int32_t *__restrict__ corner,
                              const double *__restrict__ xs, double4 *__restrict__ ps, double *__restrict__ normals0,
-                             double4 *__restrict__ nprops, PointDyn *__restrict__ pd, PointDyn32 *__restrict__ pd32) {
+                             double4 *__restrict__ nprops, Rec64 *__restrict__ pd, Rec32 *__restrict__ pd32) {
 #pragma clang fp contract(off)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -272,8 +272,8 @@ __global__ void k_node_props(int n, int initial, int useRot, const int32_t *__re
         return;
     }
     const double4 p = ps[i];
-    PointDyn out;
-    out.x0[0] = p.x; out.x0[1] = p.y; out.x0[2] = p.z;
+    Rec64 out;
+    out.x[0] = p.x; out.x[1] = p.y; out.x[2] = p.z;
     out.A = p.w;
     double M[9] = {1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0};
     double xu[3];
@@ -290,8 +290,8 @@ __global__ void k_node_props(int n, int initial, int useRot, const int32_t *__re
         out.d[0] = xu[0] - p.x; out.d[1] = xu[1] - p.y; out.d[2] = xu[2] - p.z;
     }
     pd[i] = out;
-    PointDyn32 o32;
-    o32.x0[0] = p.x; o32.x0[1] = p.y; o32.x0[2] = p.z;
+    Rec32 o32;
+    o32.x[0] = p.x; o32.x[1] = p.y; o32.x[2] = p.z;
     o32.A = (float)p.w;
     for (int k = 0; k < 9; ++k) o32.K[k] = (float)out.K[k];
     for (int k = 0; k < 3; ++k) o32.d[k] = (float)out.d[k];
@@ -420,7 +420,7 @@ __global__ void k_node_props_d(int n, int useRot, const int32_t *__restrict__ n2
                                const double4 *__restrict__ fpropsd, const int32_t *__restrict__ xuInd,
                                const int32_t *__restrict__ corner, const double *__restrict__ dxs,
                                const double4 *__restrict__ ps, const double *__restrict__ normals0,
-                               PointDyn *__restrict__ pd, PointDyn32 *__restrict__ pd32) {
+                               Rec64 *__restrict__ pd, Rec32 *__restrict__ pd32) {
 #pragma clang fp contract(off)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -436,8 +436,8 @@ __global__ void k_node_props_d(int n, int useRot, const int32_t *__restrict__ n2
         }
     }
     const double4 p = ps[i];
-    PointDyn out;
-    out.x0[0] = p.x; out.x0[1] = p.y; out.x0[2] = p.z;
+    Rec64 out;
+    out.x[0] = p.x; out.x[1] = p.y; out.x[2] = p.z;
     out.A = p.w;
     double Md[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     double xud[3];
@@ -456,8 +456,8 @@ __global__ void k_node_props_d(int n, int useRot, const int32_t *__restrict__ n2
     for (int k = 0; k < 9; ++k) out.K[k] = Md[k];
     for (int k = 0; k < 3; ++k) out.d[k] = xud[k];
     pd[i] = out;
-    PointDyn32 o32;
-    o32.x0[0] = p.x; o32.x0[1] = p.y; o32.x0[2] = p.z;
+    Rec32 o32;
+    o32.x[0] = p.x; o32.x[1] = p.y; o32.x[2] = p.z;
     o32.A = (float)p.w;
     for (int k = 0; k < 9; ++k) o32.K[k] = (float)out.K[k];
     for (int k = 0; k < 3; ++k) o32.d[k] = (float)out.d[k];
@@ -481,13 +481,14 @@ __global__ void k_corners(int n, double threshold, const int32_t *__restrict__ n
 }
 
 // setData (kd_tree.F90:595-617): mean K and mean b = d - K Xu0 over an
-// internal node's points, stored as Kbar and e = Kbar X + mean(b)
-// = mean(d) + mean(K (X - Xu0)).  One block per internal node.
+// internal node's points, stored in the node's term record as K = Kbar and
+// d = Kbar X + mean(b) = mean(d) + mean(K (X - Xu0)), with x = X and
+// A = Ai.  One block per internal node.
 __global__ void __launch_bounds__(256) k_cluster_data(const int32_t *__restrict__ internal,
                                                       const NodeRec *__restrict__ nodes,
                                                       const double4 *__restrict__ ps,
-                                                      const PointDyn *__restrict__ pd, NodeDyn *__restrict__ nd,
-                                                      NodeDyn32 *__restrict__ nd32) {
+                                                      const Rec64 *__restrict__ pd, Rec64 *__restrict__ rec,
+                                                      Rec32 *__restrict__ rec32) {
     __shared__ double red[15][256 / 64];
     const int node = internal[blockIdx.x];
     const NodeRec &r = nodes[node];
@@ -495,7 +496,7 @@ __global__ void __launch_bounds__(256) k_cluster_data(const int32_t *__restrict_
     double acc[15];
     for (int k = 0; k < 15; ++k) acc[k] = 0.0;
     for (int i = r.l + threadIdx.x; i <= r.u; i += blockDim.x) {
-        const PointDyn &q = pd[i];
+        const Rec64 &q = pd[i];
         const double4 p = ps[i];
         const double dx = X0 - p.x, dy = X1 - p.y, dz = X2 - p.z;
         for (int k = 0; k < 9; ++k) acc[k] += q.K[k];
@@ -518,13 +519,15 @@ __global__ void __launch_bounds__(256) k_cluster_data(const int32_t *__restrict_
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        NodeDyn o;
-        NodeDyn32 o32;
+        Rec64 o;
+        Rec32 o32;
+        for (int k = 0; k < 3; ++k) o.x[k] = o32.x[k] = r.X[k];
+        o.A = r.Ai;
+        o32.A = (float)r.Ai;
         for (int k = 0; k < 9; ++k) { o.K[k] = red[k][0]; o32.K[k] = (float)o.K[k]; }
-        for (int k = 0; k < 3; ++k) { o.e[k] = red[9 + k][0] + red[12 + k][0]; o32.e[k] = (float)o.e[k]; }
-        for (int k = 0; k < 4; ++k) o.pad[k] = 0.0;
-        nd[node] = o;
-        nd32[node] = o32;
+        for (int k = 0; k < 3; ++k) { o.d[k] = red[9 + k][0] + red[12 + k][0]; o32.d[k] = (float)o.d[k]; }
+        rec[node] = o;
+        rec32[node] = o32;
     }
 }
 
@@ -534,8 +537,8 @@ __global__ void __launch_bounds__(256) k_cluster_data(const int32_t *__restrict_
 template <int PREC, int EXPM>
 __global__ void __launch_bounds__(256) k_exact_fwd(int nvol, const double4 *__restrict__ vol,
                                                    const int32_t *__restrict__ perm, int n,
-                                                   const double4 *__restrict__ ps, const PointDyn *__restrict__ pd,
-                                                   const PointDyn32 *__restrict__ pd32, WParams P, Mirror M,
+                                                   const double4 *__restrict__ ps, const Rec64 *__restrict__ pd,
+                                                   const Rec32 *__restrict__ pd32, WParams P, Mirror M,
                                                    double *__restrict__ xv, double *__restrict__ den,
                                                    double4 *__restrict__ acc) {
     const int nb = (nvol + 255) / 256;
@@ -550,7 +553,7 @@ __global__ void __launch_bounds__(256) k_exact_fwd(int nvol, const double4 *__re
     if (PREC == IDW_PREC_FP64) {
         for (int i = 0; i < n; ++i) {
             const double4 s = ps[i];
-            const PointDyn &q = pd[i];
+            const Rec64 &q = pd[i];
             const double rx = v.x - s.x, ry = v.y - s.y, rz = v.z - s.z;
             const double lo = P.Ldef * rsqrt_nr(rx * rx + ry * ry + rz * rz + EPS_W);
             const double w = s.w * wfun<EXPM>(lo, P);
@@ -573,7 +576,7 @@ __global__ void __launch_bounds__(256) k_exact_fwd(int nvol, const double4 *__re
                 const double rx = v.x - s.x, ry = v.y - s.y, rz = v.z - s.z;
                 const double d2 = rx * rx + ry * ry + rz * rz;
                 if (__any(valid && d2 < T2)) {
-                    const PointDyn &q = pd[i];
+                    const Rec64 &q = pd[i];
                     const double lo = P.Ldef * rsqrt_nr(d2 + EPS_W);
                     const double w = s.w * wfun<EXPM>(lo, P);
                     n0 += w * (q.K[0] * rx + q.K[1] * ry + q.K[2] * rz + q.d[0]);
@@ -581,7 +584,7 @@ __global__ void __launch_bounds__(256) k_exact_fwd(int nvol, const double4 *__re
                     n2 += w * (q.K[6] * rx + q.K[7] * ry + q.K[8] * rz + q.d[2]);
                     dd += w;
                 } else {
-                    const PointDyn32 &q = pd32[i];
+                    const Rec32 &q = pd32[i];
                     const float fx = (float)rx, fy = (float)ry, fz = (float)rz;
                     const float lo = Lf * __frsqrt_rn(fx * fx + fy * fy + fz * fz);
                     const float w = (float)s.w * wfun<EXPM>(lo, P);
@@ -692,31 +695,38 @@ __device__ __forceinline__ double get_error(const NodeRec *nd, double dOvrR) {
 
 __device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
 
-// Wave-level record of the walk: every bucket the wave evaluated and every
-// cluster some lane accepted, with the mask of lanes concerned, in depth-first
-// order.  Slots [off, end) of the wave come from the counting pass; writes
-// beyond them are dropped and counted in *mismatch.
+// Wave-level record of the walk: one entry per term (accepted cluster, or
+// point of an evaluated bucket) with the mask of the lanes it serves, in
+// depth-first order.  The wave owns slots [off, end) sized by the counting
+// pass; FP32 terms fill them from the front, FP64 terms (mixed precision
+// near the wall) from the back, and the split is stored per wave.  Writes
+// outside the range are dropped and a count that disagrees with the
+// counting pass is reported in *mismatch.
 struct PlanWriter {
     PlanEntry *plan;
-    int64_t at, end;
+    int64_t off, end, nf = 0, nb = 0;
     unsigned int *mismatch;
-    __device__ __forceinline__ void put(int node, unsigned long long mask, int flags) {
-        if (!plan) return;
-        if (at < end) {
-            if ((threadIdx.x & 63) == 0) {
+    // terms base .. base+n-1 for the lanes of mask
+    __device__ __forceinline__ void put(int base, int n, unsigned long long mask, bool fp64) {
+        const int lane = threadIdx.x & 63;
+        for (int i = lane; i < n; i += 64) {
+            const int64_t at = fp64 ? end - 1 - (nb + i) : off + nf + i;
+            if (at >= off && at < end) {
                 PlanEntry e;
                 e.mask = mask;
-                e.node = node;
-                e.flags = flags;
+                e.rec = base + i;
+                e.pad = 0;
                 plan[at] = e;
             }
-        } else if ((threadIdx.x & 63) == 0) {
-            atomicAdd(mismatch, 1u);
         }
-        ++at;
+        if (fp64) nb += n;
+        else nf += n;
     }
-    __device__ __forceinline__ void finish() {
-        if (plan && at != end && (threadIdx.x & 63) == 0) atomicAdd(mismatch, 1u);
+    __device__ __forceinline__ void finish(int64_t *split, int wave) {
+        if ((threadIdx.x & 63) == 0) {
+            split[wave] = off + nf;
+            if (nf + nb != end - off) atomicAdd(mismatch, 1u);
+        }
     }
 };
 
@@ -724,56 +734,14 @@ __device__ __forceinline__ NodeHdr load_hdr(const NodeRec *nodes, int node) {
     return *reinterpret_cast<const NodeHdr *>(nodes + node);
 }
 
-// Bucket contribution (evalNodeExact over the bucket's points) to one lane.
-// Point records are fetched with vector loads of a wave-uniform address
-// (index + (lane & zero)): those return in order, so the next point's record
-// is in flight while the current one is evaluated without blocking the
-// wave's scalar loads.
-template <int PREC, int EXPM>
-__device__ __forceinline__ void bucket_fwd(const double4 &v, bool active, int l, int u, int lz,
-                                           const PointDyn *__restrict__ pd, const PointDyn32 *__restrict__ pd32,
-                                           const WParams &P, double &n0, double &n1, double &n2, double &dd,
-                                           bool fp64) {
-    if (PREC == IDW_PREC_FP64 || fp64) {
-        PointDyn q = pd[l];
-        for (int i = l; i <= u; ++i) {
-            const PointDyn qn = pd[min(i + 1, u)];
-            const double rx = v.x - q.x0[0], ry = v.y - q.x0[1], rz = v.z - q.x0[2];
-            const double lo = P.Ldef * rsqrt_nr(rx * rx + ry * ry + rz * rz + EPS_W);
-            const double w = active ? q.A * wfun<EXPM>(lo, P) : 0.0;
-            n0 += w * (q.K[0] * rx + q.K[1] * ry + q.K[2] * rz + q.d[0]);
-            n1 += w * (q.K[3] * rx + q.K[4] * ry + q.K[5] * rz + q.d[1]);
-            n2 += w * (q.K[6] * rx + q.K[7] * ry + q.K[8] * rz + q.d[2]);
-            dd += w;
-            q = qn;
-        }
-    } else {
-        float a0 = 0.f, a1 = 0.f, a2 = 0.f, ad = 0.f;
-        const float Lf = (float)P.Ldef;
-        PointDyn32 q = pd32[l];
-        for (int i = l; i <= u; ++i) {
-            const PointDyn32 qn = pd32[min(i + 1, u)];
-            const float fx = (float)(v.x - q.x0[0]), fy = (float)(v.y - q.x0[1]), fz = (float)(v.z - q.x0[2]);
-            const float lo = Lf * __frsqrt_rn(fx * fx + fy * fy + fz * fz);
-            const float w = active ? q.A * wfun<EXPM>(lo, P) : 0.f;
-            a0 += w * (q.K[0] * fx + q.K[1] * fy + q.K[2] * fz + q.d[0]);
-            a1 += w * (q.K[3] * fx + q.K[4] * fy + q.K[5] * fz + q.d[1]);
-            a2 += w * (q.K[6] * fx + q.K[7] * fy + q.K[8] * fz + q.d[2]);
-            ad += w;
-            q = qn;
-        }
-        n0 += a0; n1 += a1; n2 += a2; dd += ad;
-    }
-}
-
 // The walk of evalNode (kd_tree.F90:619-662) for 64 volume nodes per wave:
-// records every bucket the wave evaluates and every cluster some lane
-// accepts (evalNodeApprox, :692-713), with the lane masks, in depth-first
-// order.  The decisions use only init-time data (centroids, radii, error
-// tables, denominator0) and errTol.  COUNT: entries per wave only.
-// Bucket entries carry PLAN_FP64 when an active lane lies within
-// sqrt(nearT2) of the bucket's bounding sphere (mixed precision keeps those
-// pairs in FP64).  The headers of both possible successors (node+1 when a
+// records every point of every bucket the wave evaluates (evalNodeExact)
+// and every cluster some lane accepts (evalNodeApprox, :692-713), with the
+// lane masks, in depth-first order.  The decisions use only init-time data
+// (centroids, radii, error tables, denominator0) and errTol.  COUNT: terms
+// per wave only.  A bucket's points are FP64 terms when an active lane lies
+// within sqrt(nearT2) of the bucket's bounding sphere (mixed precision keeps
+// those pairs in FP64).  The headers of both possible successors (node+1 when a
 // lane descends, skip otherwise) are requested before the current node is
 // evaluated.
 template <bool COUNT>
@@ -781,6 +749,7 @@ __global__ void __launch_bounds__(256) k_plan_walk(int nvol, const double4 *__re
                                                    const NodeRec *__restrict__ nodes, double errTol, double nearT2,
                                                    int64_t *__restrict__ waveCount, PlanEntry *__restrict__ plan,
                                                    const int64_t *__restrict__ waveOff,
+                                                   int64_t *__restrict__ waveSplit,
                                                    unsigned int *__restrict__ mismatch, Mirror M) {
 #pragma clang fp contract(off)
     const int nb = (nvol + 255) / 256;
@@ -793,7 +762,7 @@ __global__ void __launch_bounds__(256) k_plan_walk(int nvol, const double4 *__re
     const double tol = errTol * v.w;  // errTol * denominator0
     int stop = valid ? INT32_MAX : -1;  // depth at which this lane stopped descending
     const int wave = uniform(blk * 4 + (threadIdx.x >> 6));
-    PlanWriter pw{COUNT ? nullptr : plan, COUNT ? 0 : waveOff[wave], COUNT ? 0 : waveOff[wave + 1], mismatch};
+    PlanWriter pw{plan, COUNT ? 0 : waveOff[wave], COUNT ? 0 : waveOff[wave + 1], 0, 0, mismatch};
     int64_t cnt = 0;
     int node = 0;
     NodeHdr h = load_hdr(nodes, 0);
@@ -805,14 +774,14 @@ __global__ void __launch_bounds__(256) k_plan_walk(int nvol, const double4 *__re
         int next;
         if (h.leaf) {
             if (COUNT) {
-                ++cnt;
+                cnt += h.u - h.l + 1;
             } else {
-                int flags = 0;
+                bool near = false;
                 if (nearT2 > 0.0) {
                     const double dc = sqrt(cx * cx + cy * cy + cz * cz) - 1.0 / h.orad;
-                    flags = __any(active && dc * fabs(dc) < nearT2) ? PLAN_FP64 : 0;
+                    near = __any(active && dc * fabs(dc) < nearT2);
                 }
-                pw.put(node, __ballot(active), flags);
+                pw.put(nnodes + h.l, h.u - h.l + 1, __ballot(active), near);
             }
             next = h.skip;
         } else {
@@ -827,7 +796,7 @@ __global__ void __launch_bounds__(256) k_plan_walk(int nvol, const double4 *__re
             const unsigned long long acc = __ballot(accept);
             if (acc) {
                 if (COUNT) ++cnt;
-                else pw.put(node, acc, 0);
+                else pw.put(node, 1, acc, false);
             }
             next = __any(descend) ? node + 1 : h.skip;
         }
@@ -839,7 +808,7 @@ __global__ void __launch_bounds__(256) k_plan_walk(int nvol, const double4 *__re
     if (COUNT) {
         if ((threadIdx.x & 63) == 0) waveCount[wave] = cnt;
     } else {
-        pw.finish();
+        pw.finish(waveSplit, wave);
     }
 }
 
@@ -910,89 +879,12 @@ __global__ void k_permute_vol(int64_t nvol, const int32_t *__restrict__ order, c
     pout[k] = pin[o];
 }
 
-// Forward sums replaying the plan: accepted clusters add
-// Ai w(dist) (Kbar (x - X) + e) (evalNodeApprox), buckets the exact terms
-// of their points (evalNodeExact); then Xv = Xv0 + num / den
-// (warpMesh.F90:65-70).  Entries are fetched two ahead, headers one ahead.
-template <int PREC, int EXPM>
-__global__ void __launch_bounds__(256) k_fast_fwd(int nvol, const double4 *__restrict__ vol,
-                                                  const int32_t *__restrict__ perm, const PlanEntry *__restrict__ plan,
-                                                  const int64_t *__restrict__ waveOff, const NodeRec *__restrict__ nodes,
-                                                  const NodeDyn *__restrict__ nd, const NodeDyn32 *__restrict__ nd32,
-                                                  const double4 *__restrict__ ps, const PointDyn *__restrict__ pd,
-                                                  const PointDyn32 *__restrict__ pd32, WParams P, Mirror M,
-                                                  double *__restrict__ xv, double *__restrict__ den,
-                                                  double4 *__restrict__ acc) {
-    const int nb = (nvol + 255) / 256;
-    const int blk = xcd_block(nb);
-    if (blk >= nb) return;
-    const int k = blk * 256 + threadIdx.x;
-    const bool valid = k < nvol;
-    const int wave = uniform(blk * 4 + (threadIdx.x >> 6));
-    const double4 v0 = valid ? vol[k] : make_double4(0.0, 0.0, 0.0, 1.0);
-    double4 v = v0;
-    if (M.mask) mirror_pt(M, v.x, v.y, v.z);
-    const int lane = threadIdx.x & 63, lz = lane & P.zero;
-    double n0 = 0.0, n1 = 0.0, n2 = 0.0, dd = 0.0;
-    const int64_t e0 = waveOff[wave], e1 = waveOff[wave + 1];
-    if (e0 < e1) {
-        PlanEntry pe = plan[e0];
-        PlanEntry pn = plan[min(e0 + 1, e1 - 1)];
-        NodeHdr h = load_hdr(nodes, pe.node);
-        NodeDyn32 c32 = nd32[pe.node];  // cluster record (FP32 path), one entry ahead
-        for (int64_t e = e0; e < e1; ++e) {
-            const PlanEntry pnn = plan[min(e + 2, e1 - 1)];
-            const NodeHdr hn = load_hdr(nodes, pn.node);
-            NodeDyn32 c32n;
-            if (PREC == IDW_PREC_MIXED) c32n = nd32[pn.node];
-            const bool active = (pe.mask >> lane) & 1ull;
-            if (h.leaf) {
-                bucket_fwd<PREC, EXPM>(v, active, h.l, h.u, lz, pd, pd32, P, n0, n1, n2, dd,
-                                       (pe.flags & PLAN_FP64) != 0);
-            } else {
-                const double cx = v.x - h.X[0], cy = v.y - h.X[1], cz = v.z - h.X[2];
-                const double d2 = cx * cx + cy * cy + cz * cz;
-                if (PREC == IDW_PREC_FP64) {
-                    const NodeDyn q = nd[pe.node];
-                    if (active) {
-                        const double w = h.Ai * wfun<EXPM>(P.Ldef * rsqrt_nr(d2), P);
-                        n0 += w * (q.K[0] * cx + q.K[1] * cy + q.K[2] * cz + q.e[0]);
-                        n1 += w * (q.K[3] * cx + q.K[4] * cy + q.K[5] * cz + q.e[1]);
-                        n2 += w * (q.K[6] * cx + q.K[7] * cy + q.K[8] * cz + q.e[2]);
-                        dd += w;
-                    }
-                } else {
-                    const NodeDyn32 &q = c32;
-                    if (active) {
-                        const float fx = (float)cx, fy = (float)cy, fz = (float)cz;
-                        const float w = (float)h.Ai * wfun<EXPM>((float)P.Ldef * __frsqrt_rn((float)d2), P);
-                        n0 += (double)(w * (q.K[0] * fx + q.K[1] * fy + q.K[2] * fz + q.e[0]));
-                        n1 += (double)(w * (q.K[3] * fx + q.K[4] * fy + q.K[5] * fz + q.e[1]));
-                        n2 += (double)(w * (q.K[6] * fx + q.K[7] * fy + q.K[8] * fz + q.e[2]));
-                        dd += (double)w;
-                    }
-                }
-            }
-            pe = pn;
-            pn = pnn;
-            h = hn;
-            if (PREC == IDW_PREC_MIXED) c32 = c32n;
-        }
-    }
-    if (valid) finish_fwd(P, M, k, perm[k], v0, n0, n1, n2, dd, xv, den, acc);
-}
-
 typedef double dvec4 __attribute__((ext_vector_type(4)));
 
 // wave-uniform copy of lane 0's value
 __device__ __forceinline__ double first_lane(double x) {
     return __hiloint2double(__builtin_amdgcn_readfirstlane(__double2hiint(x)),
                             __builtin_amdgcn_readfirstlane(__double2loint(x)));
-}
-__device__ __forceinline__ unsigned long long uniform64(unsigned long long x) {
-    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)(x & 0xffffffffu));
-    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(x >> 32));
-    return ((unsigned long long)hi << 32) | lo;
 }
 // LDS written by one lane and read by another lane of the same wave: the
 // wave's LDS operations execute in order, the barrier keeps the compiler
@@ -1003,61 +895,303 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-constexpr int WROW = 65;  // padded stride of the 16 x 64 weight tile (doubles)
-constexpr int GROW = 17;  // padded stride of the 64 x 16 seed tile
+// ---------------------------------------------------------------------------
+// Fast-mode sums over the plan on the matrix cores
+// ---------------------------------------------------------------------------
+// The wave's volume nodes are taken relative to the wave origin o (lane 0's
+// node, g = v - o).  A term at x contributes w (K (v - x) + d) = w (K g + c)
+// with c = d - K (x - o), so the wave's numerators and denominators are
+//   S(node, :) = sum over terms of w(node, term) [K (9), c (3), 1],
+// a (64 nodes x terms) by (terms x 16) product evaluated with
+// v_mfma_f64_16x16x4f64.  The weights are computed directly in the
+// A-operand layout: for node block t and term group q, lane l computes the
+// weight of node 16t + l%16 with term 4q + l/16; the 16-wide B rows of the
+// terms are staged in LDS.  The reverse reduces over the wave's nodes the
+// same way (terms x nodes by nodes x 16).
+//
+// MFMA lane layout (16x16x4 f64): lane l supplies A[l%16][l/16] and
+// B[l/16][l%16]; register j of D holds D[4j + l/16][l%16].
 
-// Reduce 16 rows of per-lane weights against the per-lane 16-vectors G:
-// P(16x16) = W(16x64) G(64x16) as 16 v_mfma_f64_16x16x4f64 over 4-lane
-// chunks (A lane l = W[l%16][4q + l/16], B_q lane l = G_{4q+l/16}[l%16],
-// D lane l reg t = P[4t + l/16][l%16]).  Row r's 12 outputs are
-// Mb'(a,c) = P[3a+c] - P[9+a] * d_r[c] and Bb(a) = P[9+a], added to dst_r.
-__device__ __forceinline__ void flush_rows(const double *W, const double *B, double *const *dst,
-                                           const double (*dd)[4], int nrow) {
-    wave_sync();
-    const int lane = threadIdx.x & 63, i = lane & 15, g = lane >> 4;
-    // four independent accumulation chains keep the matrix pipe busy
-    dvec4 a0 = {0.0, 0.0, 0.0, 0.0}, a1 = a0, a2 = a0, a3 = a0;
-#pragma unroll
-    for (int q = 0; q < 16; q += 4) {
-        a0 = __builtin_amdgcn_mfma_f64_16x16x4f64(W[i * WROW + 4 * q + g], B[q], a0, 0, 0, 0);
-        a1 = __builtin_amdgcn_mfma_f64_16x16x4f64(W[i * WROW + 4 * (q + 1) + g], B[q + 1], a1, 0, 0, 0);
-        a2 = __builtin_amdgcn_mfma_f64_16x16x4f64(W[i * WROW + 4 * (q + 2) + g], B[q + 2], a2, 0, 0, 0);
-        a3 = __builtin_amdgcn_mfma_f64_16x16x4f64(W[i * WROW + 4 * (q + 3) + g], B[q + 3], a3, 0, 0, 0);
+constexpr int FCH = 32;   // terms staged per chunk in the forward
+constexpr int BROW = 17;  // padded LDS row (doubles) of a staged B row
+
+struct FwdChunk {
+    double B[FCH][BROW];  // K (9), c (3), 1, 0, 0, 0
+    double X[FCH][4];     // x (3), A
+    double eps[FCH];      // added to |r|^2: EPS_W for points (evalNodeExact), 0 for clusters (evalNodeApprox)
+    unsigned long long mask[FCH];
+};
+union FwdLds {
+    FwdChunk c;
+    double acc[32][BROW];  // half of the wave's 64 x 16 sums
+};
+
+// Stage FCH terms of the plan (entries c0 .. c0+cnt-1, the rest zero) in
+// LDS, two lanes per term: lane r writes row 0 of K and c, x, A, eps and
+// the mask; lane r + FCH rows 1 and 2.  W32 reads the FP32 records (mixed
+// precision).
+template <bool W32>
+__device__ __forceinline__ void stage_terms(FwdChunk &C, const PlanEntry *__restrict__ plan, int64_t c0, int cnt,
+                                            int nnodes, int nrec, const Rec64 *__restrict__ rec,
+                                            const Rec32 *__restrict__ rec32, double ox, double oy, double oz) {
+    const int lane = threadIdx.x & 63, r = lane & (FCH - 1);
+    const bool hi = lane >= FCH;
+    if (r < cnt) {
+        PlanEntry pe = plan[c0 + r];
+        pe.rec = min((unsigned)pe.rec, (unsigned)(nrec - 1));  // a plan that disagreed with its count stays in bounds
+        double x[3];
+        if (W32) for (int i = 0; i < 3; ++i) x[i] = rec32[pe.rec].x[i];
+        else for (int i = 0; i < 3; ++i) x[i] = rec[pe.rec].x[i];
+        const double y0 = x[0] - ox, y1 = x[1] - oy, y2 = x[2] - oz;
+        const int a0 = hi ? 1 : 0, a1 = hi ? 3 : 1;  // K rows [a0, a1) of this lane
+        for (int a = a0; a < a1; ++a) {
+            double K0, K1, K2, d;
+            if (W32) {
+                const Rec32 &q = rec32[pe.rec];
+                K0 = q.K[3 * a]; K1 = q.K[3 * a + 1]; K2 = q.K[3 * a + 2]; d = q.d[a];
+            } else {
+                const Rec64 &q = rec[pe.rec];
+                K0 = q.K[3 * a]; K1 = q.K[3 * a + 1]; K2 = q.K[3 * a + 2]; d = q.d[a];
+            }
+            C.B[r][3 * a] = K0;
+            C.B[r][3 * a + 1] = K1;
+            C.B[r][3 * a + 2] = K2;
+            C.B[r][9 + a] = d - (K0 * y0 + K1 * y1 + K2 * y2);
+        }
+        if (hi) {
+            C.B[r][12] = 1.0;
+            C.B[r][13] = C.B[r][14] = C.B[r][15] = 0.0;
+        } else {
+            for (int i = 0; i < 3; ++i) C.X[r][i] = x[i];
+            C.X[r][3] = W32 ? (double)rec32[pe.rec].A : rec[pe.rec].A;
+            C.eps[r] = pe.rec < nnodes ? 0.0 : EPS_W;
+            C.mask[r] = pe.mask;
+        }
+    } else {
+        for (int j = hi ? 8 : 0; j < (hi ? 16 : 8); ++j) C.B[r][j] = 0.0;
+        if (!hi) {
+            for (int i = 0; i < 4; ++i) C.X[r][i] = 0.0;
+            C.eps[r] = 1.0;
+            C.mask[r] = 0ull;
+        }
     }
-    const dvec4 acc = (a0 + a1) + (a2 + a3);
-    const int j = i;
-    const int src = (lane & ~15) | (9 + (j < 9 ? j / 3 : 0));
+}
+
+// Forward sums of the terms in plan entries [a, b) into D (node block t in
+// D[t]); vx/vy/vz: positions of the lane's rows (nodes 16t + lane%16).
+template <bool W32, int EXPM>
+__device__ __forceinline__ void fwd_terms(int64_t a, int64_t b, const PlanEntry *__restrict__ plan, int nnodes,
+                                          int nrec, const Rec64 *__restrict__ rec, const Rec32 *__restrict__ rec32,
+                                          const WParams &P, FwdChunk &C, double ox, double oy, double oz,
+                                          const double (&vx)[4], const double (&vy)[4], const double (&vz)[4],
+                                          dvec4 (&D)[4]) {
+    const int lane = threadIdx.x & 63, n = lane & 15, R = lane >> 4;
+    const float Lf = (float)P.Ldef;
+    for (int64_t c0 = a; c0 < b; c0 += FCH) {
+        const int cnt = (int)min((int64_t)FCH, b - c0);
+        wave_sync();  // the previous chunk is consumed
+        stage_terms<W32>(C, plan, c0, cnt, nnodes, nrec, rec, rec32, ox, oy, oz);
+        wave_sync();
+        const int ng = (cnt + 3) >> 2;
+#pragma unroll 1
+        for (int q = 0; q < ng; ++q) {
+            const int tr = 4 * q + R;
+            const double Bv = C.B[tr][n];
+            const unsigned long long m = C.mask[tr] >> n;
+            const double x0 = C.X[tr][0], x1 = C.X[tr][1], x2 = C.X[tr][2], A = C.X[tr][3];
+            if (W32) {
+                const float Af = (float)A;
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const float fx = (float)(vx[t] - x0), fy = (float)(vy[t] - x1), fz = (float)(vz[t] - x2);
+                    const float w = Af * wfun<EXPM>(Lf * __frsqrt_rn(fx * fx + fy * fy + fz * fz), P);
+                    const bool on = (m >> (16 * t)) & 1ull;
+                    D[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(on ? (double)w : 0.0, Bv, D[t], 0, 0, 0);
+                }
+            } else {
+                const double eps = C.eps[tr];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const double rx = vx[t] - x0, ry = vy[t] - x1, rz = vz[t] - x2;
+                    const double w = A * wfun<EXPM>(P.Ldef * rsqrt_nr(rx * rx + ry * ry + rz * rz + eps), P);
+                    const bool on = (m >> (16 * t)) & 1ull;
+                    D[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(on ? w : 0.0, Bv, D[t], 0, 0, 0);
+                }
+            }
+        }
+    }
+}
+
+// Forward of the fast mode replaying the plan: accepted clusters add
+// Ai w(dist) (Kbar (x - X) + e) (evalNodeApprox), bucket points their exact
+// terms (evalNodeExact); then Xv = Xv0 + num / den (warpMesh.F90:65-70).
+// Mixed precision: FP32 weights for the wave's FP32 terms, FP64 for the
+// rest; the products are summed in FP64 either way.
+template <int PREC, int EXPM>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_fast_fwd(int nvol, const double4 *__restrict__ vol,
+                                                  const int32_t *__restrict__ perm, const PlanEntry *__restrict__ plan,
+                                                  const int64_t *__restrict__ waveOff,
+                                                  const int64_t *__restrict__ waveSplit, int nnodes, int nrec,
+                                                  const Rec64 *__restrict__ rec, const Rec32 *__restrict__ rec32,
+                                                  WParams P, Mirror M, double *__restrict__ xv,
+                                                  double *__restrict__ den, double4 *__restrict__ acc) {
+    __shared__ FwdLds lds[4];
+    const int nb = (nvol + 255) / 256;
+    const int blk = xcd_block(nb);
+    if (blk >= nb) return;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, n = lane & 15, R = lane >> 4;
+    const int k = blk * 256 + threadIdx.x;
+    const bool valid = k < nvol;
+    const int wave = uniform(blk * 4 + wv);
+    double4 v = valid ? vol[k] : make_double4(0.0, 0.0, 0.0, 1.0);
+    if (M.mask) mirror_pt(M, v.x, v.y, v.z);
+    // wave origin: lane 0 is valid whenever any lane of the wave is
+    const double ox = first_lane(v.x), oy = first_lane(v.y), oz = first_lane(v.z);
+    double vx[4], vy[4], vz[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-        const int row = 4 * t + g;
-        double val = acc[t];
-        const double pb = __shfl(val, src, 64);
-        if (j < 9) val -= pb * dd[row][j % 3];
-        if (row < nrow && j < 12 && val != 0.0) atomicAdd(dst[row] + j, val);
+        const int src = 16 * t + n;
+        vx[t] = __shfl(v.x, src, 64);
+        vy[t] = __shfl(v.y, src, 64);
+        vz[t] = __shfl(v.z, src, 64);
     }
-    wave_sync();
+    dvec4 D[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) D[t] = dvec4{0.0, 0.0, 0.0, 0.0};
+    FwdLds &L = lds[wv];
+    const int64_t e0 = waveOff[wave], e1 = waveOff[wave + 1];
+    if (PREC == IDW_PREC_MIXED) {
+        const int64_t es = min(max(waveSplit[wave], e0), e1);
+        fwd_terms<true, EXPM>(e0, es, plan, nnodes, nrec, rec, rec32, P, L.c, ox, oy, oz, vx, vy, vz, D);
+        fwd_terms<false, EXPM>(es, e1, plan, nnodes, nrec, rec, rec32, P, L.c, ox, oy, oz, vx, vy, vz, D);
+    } else {
+        fwd_terms<false, EXPM>(e0, e1, plan, nnodes, nrec, rec, rec32, P, L.c, ox, oy, oz, vx, vy, vz, D);
+    }
+    // D[t] register j: node 16t + 4j + R, column n.  Through LDS, 32 nodes
+    // at a time, each lane gathers its node's 13 sums.  The lane's own
+    // (mirrored) position is row R of its four rows.
+    const double gx = (R == 0 ? vx[0] : R == 1 ? vx[1] : R == 2 ? vx[2] : vx[3]) - ox;
+    const double gy = (R == 0 ? vy[0] : R == 1 ? vy[1] : R == 2 ? vy[2] : vy[3]) - oy;
+    const double gz = (R == 0 ? vz[0] : R == 1 ? vz[1] : R == 2 ? vz[2] : vz[3]) - oz;
+    double n0 = 0.0, n1 = 0.0, n2 = 0.0, dd = 0.0;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        wave_sync();
+#pragma unroll
+        for (int tt = 0; tt < 2; ++tt)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) L.acc[16 * tt + 4 * j + R][n] = D[2 * h + tt][j];
+        wave_sync();
+        if ((lane >> 5) == h) {
+            const double *s = L.acc[lane & 31];
+            n0 = s[0] * gx + s[1] * gy + s[2] * gz + s[9];
+            n1 = s[3] * gx + s[4] * gy + s[5] * gz + s[10];
+            n2 = s[6] * gx + s[7] * gy + s[8] * gz + s[11];
+            dd = s[12];
+        }
+    }
+    if (valid) {
+        // reloaded rather than kept live through the sums
+        const double4 *vk = vol + k;
+        const double4 vb = make_double4(__builtin_nontemporal_load(&vk->x), __builtin_nontemporal_load(&vk->y),
+                                        __builtin_nontemporal_load(&vk->z), 0.0);
+        finish_fwd(P, M, k, perm[k], vb, n0, n1, n2, dd, xv, den, acc);
+    }
+}
+
+constexpr int RT = 16;  // terms per reverse tile
+
+struct RevLds {
+    double4 node[64];    // mirrored positions of the wave's nodes
+    double sg[64][6];    // seeds s (3) and g = v - o (3): B-operand set-up
+    double Y[RT][4];     // x - o of the tile's terms
+    int32_t idx[RT];     // their records (-1: padding)
+};
+
+// Reverse sums of the terms in plan entries [a, b): per tile of 16 terms,
+// P(term, :) = sum over the wave's nodes of w(node, term) G(node, :) with
+// G = [s (x) g (9), s (3)] (B in registers), then
+// Mb'(a, c) += P(3a + c) - P(9 + a) (x - o)_c and Bb(a) += P(9 + a) into
+// the term's adjoint record.  Lane l computes the weight of term l%16 with
+// node 4q + l/16 for node group q.
+template <bool W32, int EXPM>
+__device__ __forceinline__ void rev_terms(int64_t a, int64_t b, const PlanEntry *__restrict__ plan, int nnodes,
+                                          int nrec, const Rec64 *__restrict__ rec, const WParams &P, RevLds &L,
+                                          const double (&B)[16], double ox, double oy, double oz,
+                                          double *__restrict__ adj) {
+    const int lane = threadIdx.x & 63, m = lane & 15, R = lane >> 4;
+    const float Lf = (float)P.Ldef;
+    const int src = (lane & ~15) | (9 + (m < 9 ? m / 3 : 0));
+    for (int64_t t0 = a; t0 < b; t0 += RT) {
+        const int64_t e = t0 + m;
+        unsigned long long mask = 0ull;
+        double4 xa = make_double4(0.0, 0.0, 0.0, 0.0);
+        int idx = -1;
+        if (e < b) {
+            const PlanEntry pe = plan[e];
+            mask = pe.mask;
+            idx = min((unsigned)pe.rec, (unsigned)(nrec - 1));  // a plan that disagreed with its count stays in bounds
+            xa = *reinterpret_cast<const double4 *>(rec + idx);  // x, A
+        }
+        const double eps = (idx >= 0 && idx < nnodes) ? 0.0 : EPS_W;
+        wave_sync();  // the previous tile's reads of Y / idx are done
+        if (R == 0) {
+            L.Y[m][0] = xa.x - ox;
+            L.Y[m][1] = xa.y - oy;
+            L.Y[m][2] = xa.z - oz;
+            L.idx[m] = idx;
+        }
+        const unsigned long long mR = mask >> R;
+        const float Af = (float)xa.w;
+        dvec4 D0 = {0.0, 0.0, 0.0, 0.0}, D1 = D0;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const double4 c = L.node[4 * q + R];
+            double w;
+            if (W32) {
+                const float fx = (float)(c.x - xa.x), fy = (float)(c.y - xa.y), fz = (float)(c.z - xa.z);
+                w = (double)(Af * wfun<EXPM>(Lf * __frsqrt_rn(fx * fx + fy * fy + fz * fz), P));
+            } else {
+                const double rx = c.x - xa.x, ry = c.y - xa.y, rz = c.z - xa.z;
+                w = xa.w * wfun<EXPM>(P.Ldef * rsqrt_nr(rx * rx + ry * ry + rz * rz + eps), P);
+            }
+            w = ((mR >> (4 * q)) & 1ull) ? w : 0.0;
+            if (q & 1) D1 = __builtin_amdgcn_mfma_f64_16x16x4f64(w, B[q], D1, 0, 0, 0);
+            else D0 = __builtin_amdgcn_mfma_f64_16x16x4f64(w, B[q], D0, 0, 0, 0);
+        }
+        const dvec4 Ds = D0 + D1;
+        wave_sync();  // Y / idx of this tile are written
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int tr = 4 * t + R;
+            double val = Ds[t];
+            const double pb = __shfl(val, src, 64);
+            if (m < 9) val -= pb * L.Y[tr][m % 3];
+            const int id = L.idx[tr];
+            if (id >= 0 && m < 12 && val != 0.0) atomicAdd(adj + (int64_t)id * ADJ_STRIDE + m, val);
+        }
+    }
 }
 
 // Reverse of the fast mode (evalNode_b / evalNodeApprox_b, kd_tree.F90:
 // 942-1031) replaying the forward's plan: for accepted cluster c,
 // C(a,b) += sum s_a (x_b - X_b), B(a) += sum s_a; for bucket point i,
 // Mb'(a,b) += sum s_a (x_b - Xu0_b), Bb(a) += sum s_a, with
-// s = w * dXv / den (warpDeriv.F90:76-78).  Plan entries are fetched two
-// ahead and node headers one ahead with scalar loads.
+// s = w * dXv / den (warpDeriv.F90:76-78).  The adjoint records are indexed
+// like the term records (clusters, then points).
 template <int PREC, int EXPM>
 __global__ void __launch_bounds__(256) k_fast_rev(int nvol, const double4 *__restrict__ vol,
                                                   const int32_t *__restrict__ perm, const double *__restrict__ dxv,
                                                   const double *__restrict__ den, const PlanEntry *__restrict__ plan,
-                                                  const int64_t *__restrict__ waveOff, const NodeRec *__restrict__ nodes,
-                                                  const double4 *__restrict__ ps, WParams P, Mirror M,
-                                                  double *__restrict__ nadj, double *__restrict__ padj) {
-    __shared__ double sW[4][64 * GROW];
-    __shared__ double sD[4][16][4];
-    __shared__ double *sDst[4][16];
+                                                  const int64_t *__restrict__ waveOff,
+                                                  const int64_t *__restrict__ waveSplit, int nnodes, int nrec,
+                                                  const Rec64 *__restrict__ rec, WParams P, Mirror M,
+                                                  double *__restrict__ adj) {
+    __shared__ RevLds lds[4];
     const int nb = (nvol + 255) / 256;
     const int blk = xcd_block(nb);
     if (blk >= nb) return;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, n = lane & 15, R = lane >> 4;
     const int k = blk * 256 + threadIdx.x;
     const bool valid = k < nvol;
     const int wave = uniform(blk * 4 + wv);
@@ -1076,84 +1210,31 @@ __global__ void __launch_bounds__(256) k_fast_rev(int nvol, const double4 *__res
     }
     // wave origin: lane 0 is valid whenever any lane of the wave is
     const double ox = first_lane(v.x), oy = first_lane(v.y), oz = first_lane(v.z);
-    const double gx = v.x - ox, gy = v.y - oy, gz = v.z - oz;
-    double *W = sW[wv];
-    {
-        const double G[12] = {s0 * gx, s0 * gy, s0 * gz, s1 * gx, s1 * gy, s1 * gz,
-                              s2 * gx, s2 * gy, s2 * gz, s0, s1, s2};
-#pragma unroll
-        for (int c = 0; c < 16; ++c) W[lane * GROW + c] = c < 12 ? G[c] : 0.0;
-    }
+    RevLds &L = lds[wv];
+    L.node[lane] = make_double4(v.x, v.y, v.z, 0.0);
+    L.sg[lane][0] = s0;
+    L.sg[lane][1] = s1;
+    L.sg[lane][2] = s2;
+    L.sg[lane][3] = v.x - ox;
+    L.sg[lane][4] = v.y - oy;
+    L.sg[lane][5] = v.z - oz;
     wave_sync();
+    // B operand of node group q: G(node 4q + R, column n)
+    const int sa = n < 9 ? n / 3 : (n < 12 ? n - 9 : 0), gb = 3 + n % 3;
     double B[16];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) B[q] = W[(4 * q + (lane >> 4)) * GROW + (lane & 15)];
-    wave_sync();
-    const float Lf = (float)P.Ldef;
-    const int lz = lane & P.zero;
-    int nrow = 0;
-    const int64_t e0 = waveOff[wave], e1 = waveOff[wave + 1];
-    if (e0 < e1) {
-        PlanEntry pe = plan[e0];
-        PlanEntry pn = plan[min(e0 + 1, e1 - 1)];
-        NodeHdr h = load_hdr(nodes, pe.node);
-        for (int64_t e = e0; e < e1; ++e) {
-            const PlanEntry pnn = plan[min(e + 2, e1 - 1)];
-            const NodeHdr hn = load_hdr(nodes, pn.node);
-            const int node = pe.node;
-            const bool active = (pe.mask >> lane) & 1ull;
-            if (h.leaf) {
-                double4 p = ps[h.l];
-                for (int i = h.l; i <= h.u; ++i) {
-                    const double4 pnext = ps[min(i + 1, h.u)];
-                    const double rx = v.x - p.x, ry = v.y - p.y, rz = v.z - p.z;
-                    double w;
-                    if (PREC == IDW_PREC_FP64 || (pe.flags & PLAN_FP64)) {
-                        w = active ? p.w * wfun<EXPM>(P.Ldef * rsqrt_nr(rx * rx + ry * ry + rz * rz + EPS_W), P) : 0.0;
-                    } else {
-                        const float fx = (float)rx, fy = (float)ry, fz = (float)rz;
-                        w = active ? (double)((float)p.w * wfun<EXPM>(Lf * __frsqrt_rn(fx * fx + fy * fy + fz * fz), P))
-                                   : 0.0;
-                    }
-                    W[nrow * WROW + lane] = w;
-                    if (lane == 0) {
-                        sDst[wv][nrow] = padj + (int64_t)i * ADJ_STRIDE;
-                        sD[wv][nrow][0] = p.x - ox;
-                        sD[wv][nrow][1] = p.y - oy;
-                        sD[wv][nrow][2] = p.z - oz;
-                    }
-                    if (++nrow == 16) {
-                        flush_rows(W, B, sDst[wv], sD[wv], 16);
-                        nrow = 0;
-                    }
-                    p = pnext;
-                }
-            } else {
-                const double cx = v.x - h.X[0], cy = v.y - h.X[1], cz = v.z - h.X[2];
-                const double d2 = cx * cx + cy * cy + cz * cz;
-                double w = 0.0;
-                if (active) {
-                    if (PREC == IDW_PREC_FP64) w = h.Ai * wfun<EXPM>(P.Ldef * rsqrt_nr(d2), P);
-                    else w = (double)((float)h.Ai * wfun<EXPM>(Lf * __frsqrt_rn((float)d2), P));
-                }
-                W[nrow * WROW + lane] = w;
-                if (lane == 0) {
-                    sDst[wv][nrow] = nadj + (int64_t)node * ADJ_STRIDE;
-                    sD[wv][nrow][0] = h.X[0] - ox;
-                    sD[wv][nrow][1] = h.X[1] - oy;
-                    sD[wv][nrow][2] = h.X[2] - oz;
-                }
-                if (++nrow == 16) {
-                    flush_rows(W, B, sDst[wv], sD[wv], 16);
-                    nrow = 0;
-                }
-            }
-            pe = pn;
-            pn = pnn;
-            h = hn;
-        }
+    for (int q = 0; q < 16; ++q) {
+        const double sv = L.sg[4 * q + R][sa], gv = L.sg[4 * q + R][gb];
+        B[q] = n < 9 ? sv * gv : (n < 12 ? sv : 0.0);
     }
-    if (nrow) flush_rows(W, B, sDst[wv], sD[wv], nrow);
+    const int64_t e0 = waveOff[wave], e1 = waveOff[wave + 1];
+    if (PREC == IDW_PREC_MIXED) {
+        const int64_t es = min(max(waveSplit[wave], e0), e1);
+        rev_terms<true, EXPM>(e0, es, plan, nnodes, nrec, rec, P, L, B, ox, oy, oz, adj);
+        rev_terms<false, EXPM>(es, e1, plan, nnodes, nrec, rec, P, L, B, ox, oy, oz, adj);
+    } else {
+        rev_terms<false, EXPM>(e0, e1, plan, nnodes, nrec, rec, P, L, B, ox, oy, oz, adj);
+    }
 }
 
 // Per-lane work of the fast walk, summed over all volume nodes: internal
@@ -1597,7 +1678,7 @@ void launch_corners(hipStream_t st, const DevSurface &S, double threshold) {
 }
 
 void launch_cluster_data(hipStream_t st, const DevSurface &S) {
-    if (S.nInternal) k_cluster_data<<<S.nInternal, 256, 0, st>>>(S.internal, S.nodes, S.ps, S.pd, S.nd, S.nd32);
+    if (S.nInternal) k_cluster_data<<<S.nInternal, 256, 0, st>>>(S.internal, S.nodes, S.ps, S.pd, S.rec, S.rec32);
 }
 
 void launch_errors(hipStream_t st, const DevSurface &S, const WParams &P, const double *sphere60) {
@@ -1663,11 +1744,11 @@ static void forward_impl(hipStream_t st, const DevSurface &S, const DevVolume &V
     } else {
         if (prec == IDW_PREC_FP64)
             k_fast_fwd<IDW_PREC_FP64, EXPM><<<g, 256, 0, st>>>((int)V.n, V.vol, V.perm, plan->entries, plan->waveOff,
-                                                              S.nodes, S.nd, S.nd32, S.ps, S.pd, S.pd32, P, M, xv,
+                                                              plan->waveSplit, S.nnodes, S.nnodes + S.n, S.rec, S.rec32, P, M, xv,
                                                               V.den, V.acc);
         else
             k_fast_fwd<IDW_PREC_MIXED, EXPM><<<g, 256, 0, st>>>((int)V.n, V.vol, V.perm, plan->entries, plan->waveOff,
-                                                               S.nodes, S.nd, S.nd32, S.ps, S.pd, S.pd32, P, M, xv,
+                                                               plan->waveSplit, S.nnodes, S.nnodes + S.n, S.rec, S.rec32, P, M, xv,
                                                                V.den, V.acc);
     }
 }
@@ -1679,13 +1760,13 @@ void launch_prep_numb(hipStream_t st, const DevVolume &V, const Mirror &M, const
 void launch_plan_count(hipStream_t st, const DevSurface &S, const DevVolume &V, const Mirror &M, double errTol,
                        int64_t *waveCount) {
     k_plan_walk<true><<<grid_for(V.n, 256), 256, 0, st>>>((int)V.n, V.vol, S.nnodes, S.nodes, errTol, 0.0, waveCount,
-                                                          nullptr, nullptr, nullptr, M);
+                                                          nullptr, nullptr, nullptr, nullptr, M);
 }
 
 void launch_plan_fill(hipStream_t st, const DevSurface &S, const DevVolume &V, const Mirror &M, double errTol,
                       double nearT2, const DevPlan &plan) {
     k_plan_walk<false><<<xcd_grid(V.n), 256, 0, st>>>((int)V.n, V.vol, S.nnodes, S.nodes, errTol, nearT2, nullptr,
-                                                      plan.entries, plan.waveOff, plan.mismatch, M);
+                                                      plan.entries, plan.waveOff, plan.waveSplit, plan.mismatch, M);
 }
 
 template <int EXPM>
@@ -1707,18 +1788,17 @@ static void reverse_impl(hipStream_t st, const DevSurface &S, const DevVolume &V
         const unsigned g = xcd_grid(V.n);
         if (prec == IDW_PREC_FP64)
             k_fast_rev<IDW_PREC_FP64, EXPM><<<g, 256, 0, st>>>((int)V.n, V.vol, V.perm, dxv, V.den, plan->entries,
-                                                        plan->waveOff, S.nodes, S.ps, P, M, S.nadj, S.padj);
+                                                              plan->waveOff, plan->waveSplit, S.nnodes, S.nnodes + S.n, S.rec, P, M,
+                                                              S.adj);
         else
             k_fast_rev<IDW_PREC_MIXED, EXPM><<<g, 256, 0, st>>>((int)V.n, V.vol, V.perm, dxv, V.den, plan->entries,
-                                                         plan->waveOff, S.nodes, S.ps, P, M, S.nadj, S.padj);
+                                                               plan->waveOff, plan->waveSplit, S.nnodes, S.nnodes + S.n, S.rec, P, M,
+                                                               S.adj);
     }
 }
 
 void launch_reverse_prep(hipStream_t st, const DevSurface &S, int fast) {
-    if (fast) {
-        launch_fill(st, S.padj, (int64_t)S.n * ADJ_STRIDE, 0.0);
-        launch_fill(st, S.nadj, (int64_t)S.nnodes * ADJ_STRIDE, 0.0);
-    }
+    if (fast) launch_fill(st, S.adj, ((int64_t)S.nnodes + S.n) * ADJ_STRIDE, 0.0);
 }
 
 void launch_fast_count(hipStream_t st, const DevSurface &S, const DevVolume &V, const Mirror &M, double errTol,
