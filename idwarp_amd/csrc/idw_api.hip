@@ -56,9 +56,15 @@ double plan_near_t2(const idw_ctx *c);
 
 }  // namespace
 
+constexpr int MAX_CHUNKS = 8;  // volume chunks of an overlapped fast-mode warp
+
 struct idw_ctx {
     int device = 0;
     hipStream_t st = nullptr;
+    // second stream: forwards of volume chunks whose walks are done, while
+    // the context stream walks the next chunk
+    hipStream_t st2 = nullptr;
+    hipEvent_t evChunk[MAX_CHUNKS] = {}, evFork = nullptr, evJoin = nullptr;
     idw_options opt{};
     idw_options initOpt{};
     idw_options denOpt{};
@@ -381,6 +387,42 @@ int surface_pass(idw_ctx *c, const double *xs, bool clusters) {
     return IDW_OK;
 }
 
+// Fast-mode warp that (re)writes plans: the volume blocks are split into up
+// to MAX_CHUNKS chunks; the walk of each chunk runs on the context stream
+// and the forward of that chunk on the second stream once its walk is done,
+// so the walk of one chunk overlaps the forward of the previous one.  Passes
+// stay in order on each stream (the mirror passes accumulate per node).
+// Timing: phase 2 ends after the last walk, phase 3 after the last forward.
+int overlapped_fast_warp(idw_ctx *c, idw::WParams P, double nt2, const std::vector<char> &fill, double *xv) {
+    const int64_t nb = (c->V.n + 255) / 256;
+    const int nch = (int)std::max<int64_t>(1, std::min<int64_t>(MAX_CHUNKS, nb / 512));
+    HIPCHK(hipEventRecord(c->evFork, c->st));
+    HIPCHK(hipStreamWaitEvent(c->st2, c->evFork, 0));
+    for (int kk = 0; kk < nloop(c); ++kk) {
+        idw::DevPlan &pl = c->plans[kk];
+        if (fill[kk]) idw::launch_zero_u32(c->st, pl.mismatch, 1);
+        P.pass = pass_of(c, kk);
+        for (int ch = 0; ch < nch; ++ch) {
+            const int64_t b0 = nb * ch / nch, b1 = nb * (ch + 1) / nch;
+            if (fill[kk]) {
+                idw::launch_plan_fill(c->st, c->S, c->V, mirror_of(c, kk), c->opt.errTol, nt2, pl, b0, b1 - b0);
+                HIPCHK(hipEventRecord(c->evChunk[ch], c->st));
+                HIPCHK(hipStreamWaitEvent(c->st2, c->evChunk[ch], 0));
+            }
+            idw::launch_forward(c->st2, c->S, c->V, P, mirror_of(c, kk), 1, c->opt.precision, xv, &pl, b0, b1 - b0);
+        }
+        if (fill[kk]) {
+            pl.filled = true;
+            pl.nearT2 = nt2;
+        }
+    }
+    HIPCHK(hipGetLastError());
+    REC(c, 2);
+    HIPCHK(hipEventRecord(c->evJoin, c->st2));
+    HIPCHK(hipStreamWaitEvent(c->st, c->evJoin, 0));
+    return IDW_OK;
+}
+
 int run_warp(idw_ctx *c, const double *xs, double *xv) {
     if (!c->surfInit) return fail(IDW_ESTATE, "warp before initialize_warping");
     if (!c->volSet) return fail(IDW_ESTATE, "no volume nodes set");
@@ -391,26 +433,38 @@ int run_warp(idw_ctx *c, const double *xs, double *xv) {
     REC(c, 0);
     TRY(surface_pass(c, xs, fast));
     REC(c, 1);
+    std::vector<char> fill(nloop(c), 0);
+    bool anyFill = false;
+    const double nt2 = fast ? plan_near_t2(c) : 0.0;
     if (fast) {
-        const double nt2 = plan_near_t2(c);
         for (int kk = 0; kk < nloop(c); ++kk) {
             idw::DevPlan &pl = c->plans[kk];
             if (!c->opt.reusePlan || !pl.filled || pl.errTol != c->opt.errTol || pl.nearT2 != nt2) {
                 if (c->capturing && !(pl.filled && pl.errTol == c->opt.errTol && pl.nearT2 == nt2))
                     return fail(IDW_ESTATE, "a captured warp needs the plan of a previous warp at the same options");
-                idw::launch_zero_u32(c->st, pl.mismatch, 1);
-                if (c->V.n) idw::launch_plan_fill(c->st, c->S, c->V, mirror_of(c, kk), c->opt.errTol, nt2, pl);
-                HIPCHK(hipGetLastError());
-                pl.filled = true;
-                pl.nearT2 = nt2;
+                fill[kk] = 1;
+                anyFill = true;
             }
         }
     }
-    REC(c, 2);
-    for (int kk = 0; kk < nloop(c) && c->V.n; ++kk) {
-        P.pass = pass_of(c, kk);
-        idw::launch_forward(c->st, c->S, c->V, P, mirror_of(c, kk), fast, c->opt.precision, xv,
-                            fast ? &c->plans[kk] : nullptr);
+    if (anyFill && !c->capturing && c->V.n) {
+        TRY(overlapped_fast_warp(c, P, nt2, fill, xv));
+    } else {
+        for (int kk = 0; kk < nloop(c); ++kk) {
+            if (!fill[kk]) continue;
+            idw::DevPlan &pl = c->plans[kk];
+            idw::launch_zero_u32(c->st, pl.mismatch, 1);
+            if (c->V.n) idw::launch_plan_fill(c->st, c->S, c->V, mirror_of(c, kk), c->opt.errTol, nt2, pl);
+            HIPCHK(hipGetLastError());
+            pl.filled = true;
+            pl.nearT2 = nt2;
+        }
+        REC(c, 2);
+        for (int kk = 0; kk < nloop(c) && c->V.n; ++kk) {
+            P.pass = pass_of(c, kk);
+            idw::launch_forward(c->st, c->S, c->V, P, mirror_of(c, kk), fast, c->opt.precision, xv,
+                                fast ? &c->plans[kk] : nullptr);
+        }
     }
     HIPCHK(hipGetLastError());
     REC(c, 3);
@@ -578,7 +632,12 @@ int idw_create(idw_ctx **out, int device, const idw_options *opt) {
     c->opt = *opt;
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking);
     for (int k = 0; k < 8 && e == hipSuccess; ++k) e = hipEventCreate(&c->ev[k]);
+    for (int k = 0; k < MAX_CHUNKS && e == hipSuccess; ++k)
+        e = hipEventCreateWithFlags(&c->evChunk[k], hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->evFork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->evJoin, hipEventDisableTiming);
     if (e != hipSuccess) {
         idw_destroy(c);
         return fail(IDW_EHIP, "stream/event creation: %s", hipGetErrorString(e));
@@ -591,9 +650,15 @@ int idw_destroy(idw_ctx *c) {
     if (!c) return IDW_OK;
     (void)hipSetDevice(c->device);
     if (c->st) (void)hipStreamSynchronize(c->st);
+    if (c->st2) (void)hipStreamSynchronize(c->st2);
     for (void *p : c->allocs) (void)hipFree(p);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto &e : c->evChunk)
+        if (e) (void)hipEventDestroy(e);
+    if (c->evFork) (void)hipEventDestroy(c->evFork);
+    if (c->evJoin) (void)hipEventDestroy(c->evJoin);
+    if (c->st2) (void)hipStreamDestroy(c->st2);
     if (c->st) (void)hipStreamDestroy(c->st);
     delete c;
     return IDW_OK;

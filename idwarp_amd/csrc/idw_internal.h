@@ -22,16 +22,17 @@ struct alignas(16) NodeRec {
     int32_t depth;    // root = 0
     int32_t leaf;     // 1 for a bucket
     int32_t n;        // u - l + 1
-    double Ai;        // summed area weights (setAi, kd_tree.F90:536-556)
+    double rad;       // 1 / orad
     double err[NERR]; // RMS denominator error at rstar radii (computeErrors, :832-880)
+    double Ai;        // summed area weights (setAi, kd_tree.F90:536-556)
 };
-static_assert(sizeof(NodeRec) == 160, "NodeRec layout");
+static_assert(sizeof(NodeRec) == 176, "NodeRec layout");
 
 struct alignas(16) NodeHdr {
     double X[3];
     double orad;
     int32_t l, u, skip, depth, leaf, n;
-    double Ai;
+    double rad;
 };
 static_assert(sizeof(NodeHdr) == 64, "NodeHdr layout");
 

@@ -117,13 +117,14 @@ void launch_permute_vol(hipStream_t st, int64_t n, const int32_t *order, const d
 void launch_fill(hipStream_t st, double *a, int64_t n, double v);
 // zero n counters with a kernel (safe inside graph capture, unlike a memset node)
 void launch_zero_u32(hipStream_t st, unsigned int *a, int64_t n);
+// fast mode: blocks [blk0, blk0 + nblk) of 256 volume nodes (nblk < 0: to the end)
 void launch_forward(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, const Mirror &M,
-                    int fast, int prec, double *xv, const DevPlan *plan);
+                    int fast, int prec, double *xv, const DevPlan *plan, int64_t blk0 = 0, int64_t nblk = -1);
 void launch_prep_numb(hipStream_t st, const DevVolume &V, const Mirror &M, const double *dxv);
 void launch_plan_count(hipStream_t st, const DevSurface &S, const DevVolume &V, const Mirror &M, double errTol,
                        int64_t *waveCount);
 void launch_plan_fill(hipStream_t st, const DevSurface &S, const DevVolume &V, const Mirror &M, double errTol,
-                      double nearT2, const DevPlan &plan);
+                      double nearT2, const DevPlan &plan, int64_t blk0 = 0, int64_t nblk = -1);
 void launch_reverse(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, const Mirror &M,
                     int fast, int prec, double *partial, int nchunk, const double *dxv, const DevPlan *plan);
 hipError_t exclusive_scan(hipStream_t st, const int64_t *in, int64_t *out, int64_t n);

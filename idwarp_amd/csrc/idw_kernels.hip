@@ -681,14 +681,27 @@ __global__ void k_exact_rev_reduce(int n, int nchunk, const double *__restrict__
 // ---------------------------------------------------------------------------
 // Fast mode: wave-uniform kd-tree walk
 // ---------------------------------------------------------------------------
+// Segment i of the error table, rstar(i) <= d < rstar(i+1), for d >= 2
+// (NERR-1 from the last radius on).  The radii 2, 3, 4, 5, 7.5 and 10 * 2^k
+// are exact thresholds on the binary exponent e and mantissa m of
+// d = m 2^e, m in [0.5, 1).
+__host__ __device__ __forceinline__ int err_segment(double d) {
+    int e;
+    const double m = frexp(d, &e);
+    int i;
+    if (e >= 4) i = e + 1 - (m < 0.625 ? 1 : 0);
+    else if (e == 3) i = 2 + (m >= 0.625 ? 1 : 0) + (m >= 0.9375 ? 1 : 0);
+    else i = m >= 0.75 ? 1 : 0;
+    return i < NERR - 1 ? i : NERR - 1;
+}
+
 // getError (kd_tree.F90:882-906): linear interpolation of the node's error
-// table at dist*oradius.
-__device__ __forceinline__ double get_error(const NodeRec *nd, double dOvrR) {
-    int i = 0;
-#pragma unroll
-    for (int k = 1; k < NERR; ++k) i += (dOvrR >= c_rstar[k]) ? 1 : 0;
+// table at dOvrR = dist * oradius >= 2; rseg holds {rstar(i), orstar(i)}.
+__device__ __forceinline__ double get_error(const NodeRec *nd, double dOvrR, const double2 *rseg) {
+#pragma clang fp contract(off)
+    const int i = err_segment(dOvrR);
     if (i == NERR - 1) return nd->err[NERR - 1];
-    const double2 seg = c_rseg[i];
+    const double2 seg = rseg[i];
     const double fact = (dOvrR - seg.x) * seg.y;
     return (1.0 - fact) * nd->err[i] + fact * nd->err[i + 1];
 }
@@ -745,16 +758,21 @@ __device__ __forceinline__ NodeHdr load_hdr(const NodeRec *nodes, int node) {
 // lane descends, skip otherwise) are requested before the current node is
 // evaluated.
 template <bool COUNT>
-__global__ void __launch_bounds__(256) k_plan_walk(int nvol, const double4 *__restrict__ vol, int nnodes,
+__global__ void __launch_bounds__(256) k_plan_walk(int nvol, int blk0, int nbc, const double4 *__restrict__ vol,
+                                                   int nnodes,
                                                    const NodeRec *__restrict__ nodes, double errTol, double nearT2,
                                                    int64_t *__restrict__ waveCount, PlanEntry *__restrict__ plan,
                                                    const int64_t *__restrict__ waveOff,
                                                    int64_t *__restrict__ waveSplit,
                                                    unsigned int *__restrict__ mismatch, Mirror M) {
 #pragma clang fp contract(off)
-    const int nb = (nvol + 255) / 256;
-    const int blk = COUNT ? (int)blockIdx.x : xcd_block(nb);
-    if (blk >= nb) return;
+    __shared__ double2 rseg[NERR - 1];
+    const int lb = COUNT ? (int)blockIdx.x : xcd_block(nbc);
+    if (lb >= nbc) return;  // block-uniform
+    const int blk = blk0 + lb;
+    if (threadIdx.x < NERR - 1) rseg[threadIdx.x] = c_rseg[threadIdx.x];
+    __syncthreads();
+    const double nearT = sqrt(nearT2);
     const int k = blk * 256 + threadIdx.x;
     const bool valid = k < nvol;
     double4 v = valid ? vol[k] : make_double4(0.0, 0.0, 0.0, 1.0);
@@ -778,19 +796,29 @@ __global__ void __launch_bounds__(256) k_plan_walk(int nvol, const double4 *__re
             } else {
                 bool near = false;
                 if (nearT2 > 0.0) {
-                    const double dc = sqrt(cx * cx + cy * cy + cz * cz) - 1.0 / h.orad;
-                    near = __any(active && dc * fabs(dc) < nearT2);
+                    // distance to the bounding sphere below nearT
+                    const double rn = h.rad + nearT;
+                    near = __any(active && cx * cx + cy * cy + cz * cz < rn * rn);
                 }
                 pw.put(nnodes + h.l, h.u - h.l + 1, __ballot(active), near);
             }
             next = h.skip;
         } else {
             bool descend = false, accept = false;
+            const double d2 = cx * cx + cy * cy + cz * cz;
+            // dOvrR = sqrt(d2) orad < 2 is certain without the square root
+            // when d2 orad^2 < 4 (1 - 2.5e-12): the roundings move either
+            // side by a few 1e-16
+            const bool sure = (d2 * h.orad) * h.orad < 3.99999999999;
             if (active) {
-                const double dOvrR = sqrt(cx * cx + cy * cy + cz * cz) * h.orad;
-                if (dOvrR < 2.0) descend = true;
-                else if (get_error(nodes + node, dOvrR) < tol) accept = true;
-                else descend = true;
+                if (sure) {
+                    descend = true;
+                } else {
+                    const double dOvrR = sqrt(d2) * h.orad;
+                    if (dOvrR < 2.0) descend = true;
+                    else if (get_error(nodes + node, dOvrR, rseg) < tol) accept = true;
+                    else descend = true;
+                }
                 stop = descend ? INT32_MAX : h.depth;
             }
             const unsigned long long acc = __ballot(accept);
@@ -825,6 +853,7 @@ __global__ void __launch_bounds__(256) k_near_keys(int nvol, const double4 *__re
     const bool valid = k < nvol;
     const double4 v = valid ? vol[k] : make_double4(0.0, 0.0, 0.0, 1.0);
     const double tol = errTol * v.w;
+    const double nearT = sqrt(nearT2);
     int stop = valid ? INT32_MAX : -1;
     bool near = false;
     int node = 0;
@@ -836,15 +865,20 @@ __global__ void __launch_bounds__(256) k_near_keys(int nvol, const double4 *__re
         const double cx = v.x - h.X[0], cy = v.y - h.X[1], cz = v.z - h.X[2];
         int next;
         if (h.leaf) {
-            const double dc = sqrt(cx * cx + cy * cy + cz * cz) - 1.0 / h.orad;
-            near = near || (active && dc * fabs(dc) < nearT2);
+            const double rn = h.rad + nearT;
+            near = near || (active && cx * cx + cy * cy + cz * cz < rn * rn);
             next = h.skip;
         } else {
             bool descend = false;
             if (active) {
-                const double dOvrR = sqrt(cx * cx + cy * cy + cz * cz) * h.orad;
-                if (dOvrR < 2.0) descend = true;
-                else if (!(get_error(nodes + node, dOvrR) < tol)) descend = true;
+                const double d2 = cx * cx + cy * cy + cz * cz;
+                if ((d2 * h.orad) * h.orad < 3.99999999999) {
+                    descend = true;
+                } else {
+                    const double dOvrR = sqrt(d2) * h.orad;
+                    if (dOvrR < 2.0) descend = true;
+                    else if (!(get_error(nodes + node, dOvrR, c_rseg) < tol)) descend = true;
+                }
                 stop = descend ? INT32_MAX : h.depth;
             }
             next = __any(descend) ? node + 1 : h.skip;
@@ -1028,7 +1062,8 @@ __device__ __forceinline__ void fwd_terms(int64_t a, int64_t b, const PlanEntry 
 // Mixed precision: FP32 weights for the wave's FP32 terms, FP64 for the
 // rest; the products are summed in FP64 either way.
 template <int PREC, int EXPM>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_fast_fwd(int nvol, const double4 *__restrict__ vol,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_fast_fwd(int nvol, int blk0, int nbc,
+                                                  const double4 *__restrict__ vol,
                                                   const int32_t *__restrict__ perm, const PlanEntry *__restrict__ plan,
                                                   const int64_t *__restrict__ waveOff,
                                                   const int64_t *__restrict__ waveSplit, int nnodes, int nrec,
@@ -1036,9 +1071,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
                                                   WParams P, Mirror M, double *__restrict__ xv,
                                                   double *__restrict__ den, double4 *__restrict__ acc) {
     __shared__ FwdLds lds[4];
-    const int nb = (nvol + 255) / 256;
-    const int blk = xcd_block(nb);
-    if (blk >= nb) return;
+    const int lb = xcd_block(nbc);
+    if (lb >= nbc) return;
+    const int blk = blk0 + lb;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, n = lane & 15, R = lane >> 4;
     const int k = blk * 256 + threadIdx.x;
     const bool valid = k < nvol;
@@ -1277,7 +1312,7 @@ __global__ void __launch_bounds__(256) k_fast_count(int nvol, const double4 *__r
             if (dOvrR < 2.0) descend = true;
             else {
                 ++cE;
-                if (get_error(r, dOvrR) < tol) ++cA;
+                if (get_error(r, dOvrR, c_rseg) < tol) ++cA;
                 else descend = true;
             }
             stop = descend ? INT32_MAX : depth;
@@ -1607,7 +1642,7 @@ __global__ void __launch_bounds__(256) k_dry_run(int nvol, const double4 *__rest
             const double dOvrR = sqrt(cx * cx + cy * cy + cz * cz + 1e-16) * r->orad;
             c += 0.5;
             if (dOvrR < 2.0) descend = true;
-            else if (get_error(r, dOvrR) < tol) c += 1.1;
+            else if (get_error(r, dOvrR, c_rseg) < tol) c += 1.1;
             else descend = true;
             stop = descend ? INT32_MAX : depth;
         }
@@ -1651,10 +1686,8 @@ __global__ void k_build_vol(int64_t nvol, const double *__restrict__ xv0, const 
 // launchers
 // ---------------------------------------------------------------------------
 static inline unsigned grid_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
-static inline unsigned xcd_grid(int64_t n) {
-    const int64_t nb = (n + 255) / 256;
-    return (unsigned)(((nb + 7) / 8) * 8);
-}
+static inline unsigned xcd_blocks(int64_t nb) { return (unsigned)(((nb + 7) / 8) * 8); }
+static inline unsigned xcd_grid(int64_t n) { return xcd_blocks((n + 255) / 256); }
 
 void launch_face_props(hipStream_t st, const DevSurface &S, const double *xs) {
     if (S.nface) k_face_props<<<grid_for(S.nface, 256), 256, 0, st>>>(S.nface, S.facePtr, S.faceConn, S.xuInd, xs, S.fprops);
@@ -1732,8 +1765,11 @@ void launch_fill(hipStream_t st, double *a, int64_t n, double v) {
 
 template <int EXPM>
 static void forward_impl(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, const Mirror &M,
-                         int fast, int prec, double *xv, const DevPlan *plan) {
+                         int fast, int prec, double *xv, const DevPlan *plan, int64_t blk0, int64_t nblk) {
     const unsigned g = xcd_grid(V.n);
+    if (nblk < 0) nblk = grid_for(V.n, 256) - blk0;
+    if (fast && nblk <= 0) return;
+    const unsigned gc = xcd_blocks(nblk);
     if (!fast) {
         if (prec == IDW_PREC_FP64)
             k_exact_fwd<IDW_PREC_FP64, EXPM><<<g, 256, 0, st>>>((int)V.n, V.vol, V.perm, S.n, S.ps, S.pd, S.pd32, P, M,
@@ -1743,13 +1779,13 @@ static void forward_impl(hipStream_t st, const DevSurface &S, const DevVolume &V
                                                                 xv, V.den, V.acc);
     } else {
         if (prec == IDW_PREC_FP64)
-            k_fast_fwd<IDW_PREC_FP64, EXPM><<<g, 256, 0, st>>>((int)V.n, V.vol, V.perm, plan->entries, plan->waveOff,
-                                                              plan->waveSplit, S.nnodes, S.nnodes + S.n, S.rec, S.rec32, P, M, xv,
-                                                              V.den, V.acc);
+            k_fast_fwd<IDW_PREC_FP64, EXPM><<<gc, 256, 0, st>>>(
+                (int)V.n, (int)blk0, (int)nblk, V.vol, V.perm, plan->entries, plan->waveOff, plan->waveSplit,
+                S.nnodes, S.nnodes + S.n, S.rec, S.rec32, P, M, xv, V.den, V.acc);
         else
-            k_fast_fwd<IDW_PREC_MIXED, EXPM><<<g, 256, 0, st>>>((int)V.n, V.vol, V.perm, plan->entries, plan->waveOff,
-                                                               plan->waveSplit, S.nnodes, S.nnodes + S.n, S.rec, S.rec32, P, M, xv,
-                                                               V.den, V.acc);
+            k_fast_fwd<IDW_PREC_MIXED, EXPM><<<gc, 256, 0, st>>>(
+                (int)V.n, (int)blk0, (int)nblk, V.vol, V.perm, plan->entries, plan->waveOff, plan->waveSplit,
+                S.nnodes, S.nnodes + S.n, S.rec, S.rec32, P, M, xv, V.den, V.acc);
     }
 }
 
@@ -1759,14 +1795,18 @@ void launch_prep_numb(hipStream_t st, const DevVolume &V, const Mirror &M, const
 
 void launch_plan_count(hipStream_t st, const DevSurface &S, const DevVolume &V, const Mirror &M, double errTol,
                        int64_t *waveCount) {
-    k_plan_walk<true><<<grid_for(V.n, 256), 256, 0, st>>>((int)V.n, V.vol, S.nnodes, S.nodes, errTol, 0.0, waveCount,
-                                                          nullptr, nullptr, nullptr, nullptr, M);
+    const int nb = (int)grid_for(V.n, 256);
+    k_plan_walk<true><<<nb, 256, 0, st>>>((int)V.n, 0, nb, V.vol, S.nnodes, S.nodes, errTol, 0.0, waveCount, nullptr,
+                                          nullptr, nullptr, nullptr, M);
 }
 
 void launch_plan_fill(hipStream_t st, const DevSurface &S, const DevVolume &V, const Mirror &M, double errTol,
-                      double nearT2, const DevPlan &plan) {
-    k_plan_walk<false><<<xcd_grid(V.n), 256, 0, st>>>((int)V.n, V.vol, S.nnodes, S.nodes, errTol, nearT2, nullptr,
-                                                      plan.entries, plan.waveOff, plan.waveSplit, plan.mismatch, M);
+                      double nearT2, const DevPlan &plan, int64_t blk0, int64_t nblk) {
+    if (nblk < 0) nblk = grid_for(V.n, 256) - blk0;
+    if (nblk <= 0) return;
+    k_plan_walk<false><<<xcd_blocks(nblk), 256, 0, st>>>((int)V.n, (int)blk0, (int)nblk, V.vol, S.nnodes, S.nodes,
+                                                         errTol, nearT2, nullptr, plan.entries, plan.waveOff,
+                                                         plan.waveSplit, plan.mismatch, M);
 }
 
 template <int EXPM>
@@ -1787,13 +1827,13 @@ static void reverse_impl(hipStream_t st, const DevSurface &S, const DevVolume &V
     } else {
         const unsigned g = xcd_grid(V.n);
         if (prec == IDW_PREC_FP64)
-            k_fast_rev<IDW_PREC_FP64, EXPM><<<g, 256, 0, st>>>((int)V.n, V.vol, V.perm, dxv, V.den, plan->entries,
-                                                              plan->waveOff, plan->waveSplit, S.nnodes, S.nnodes + S.n, S.rec, P, M,
-                                                              S.adj);
+            k_fast_rev<IDW_PREC_FP64, EXPM><<<g, 256, 0, st>>>(
+                (int)V.n, V.vol, V.perm, dxv, V.den, plan->entries, plan->waveOff, plan->waveSplit, S.nnodes,
+                S.nnodes + S.n, S.rec, P, M, S.adj);
         else
-            k_fast_rev<IDW_PREC_MIXED, EXPM><<<g, 256, 0, st>>>((int)V.n, V.vol, V.perm, dxv, V.den, plan->entries,
-                                                               plan->waveOff, plan->waveSplit, S.nnodes, S.nnodes + S.n, S.rec, P, M,
-                                                               S.adj);
+            k_fast_rev<IDW_PREC_MIXED, EXPM><<<g, 256, 0, st>>>(
+                (int)V.n, V.vol, V.perm, dxv, V.den, plan->entries, plan->waveOff, plan->waveSplit, S.nnodes,
+                S.nnodes + S.n, S.rec, P, M, S.adj);
     }
 }
 
@@ -1807,9 +1847,9 @@ void launch_fast_count(hipStream_t st, const DevSurface &S, const DevVolume &V, 
 }
 
 void launch_forward(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, const Mirror &M,
-                    int fast, int prec, double *xv, const DevPlan *plan) {
-    if (P.expMode == EXP_35) forward_impl<EXP_35>(st, S, V, P, M, fast, prec, xv, plan);
-    else forward_impl<EXP_INT>(st, S, V, P, M, fast, prec, xv, plan);
+                    int fast, int prec, double *xv, const DevPlan *plan, int64_t blk0, int64_t nblk) {
+    if (P.expMode == EXP_35) forward_impl<EXP_35>(st, S, V, P, M, fast, prec, xv, plan, blk0, nblk);
+    else forward_impl<EXP_INT>(st, S, V, P, M, fast, prec, xv, plan, blk0, nblk);
 }
 
 void launch_reverse(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, const Mirror &M,

@@ -193,6 +193,7 @@ bool build_surface(const double *uniquePts, int64_t nu, const int32_t *link, int
             radius = std::max(radius, std::sqrt(dx * dx + dy * dy + dz * dz));
         }
         r.orad = 1.0 / radius;
+        r.rad = 1.0 / r.orad;
     }
 
     // Xu0 in tree order and the inverse permutation
