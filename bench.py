@@ -215,22 +215,26 @@ def main():
         res["mixed_speedup_vs_fp64"] = round(b["ms"] / a["ms"], 3)
         res["mixed_max_err_over_Ldef"] = float((a["xv"] - b["xv"]).abs().max().item() / ldef)
         res["mixed_adjoint_rel_err"] = float(((a["dxs"] - b["dxs"]).norm() / b["dxs"].norm()).item())
-    # roofline of the dominant volume kernel of the headline leg
+    # roofline of the dominant kernel launch of the headline leg.  A warp that
+    # writes its plan launches the walk and the forward per volume chunk on
+    # two streams (they overlap), so the longest single launch is the
+    # reverse, one launch per step, bracketed by HIP events on the context
+    # stream (phase 4).
     ph = head["phases"]
     # kernel names as rocprofv3 reports them: <precision, exponent mode>,
     # exponent mode 0 = the default exponents (3, 5) compiled as products
     pi = 1 if head_prec == "mixed" else 0
-    cand = [("idw::k_plan_walk<false>", ph[1], kflops["walk"]), ("idw::k_fast_fwd<%d, 0>" % pi, ph[2], kflops["fwd"]),
-            ("idw::k_fast_rev<%d, 0>" % pi, ph[4], kflops["rev"])]
-    kname, kms, kfl = max(cand, key=lambda t: t[1])
+    kname, kms, kfl = "idw::k_fast_rev<%d, 0>" % pi, ph[4], kflops["rev"]
     achieved = kfl / (kms * 1e-3) / 1e12
     res["roofline"] = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                        "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
                        "traffic": _traffic_record(kname)["bytes"] if _traffic_record(kname) else None,
                        "kernel": kname, "kernel_ms": round(float(kms), 3),
                        "algorithmic_flops_per_launch": kfl,
-                       "compute": "VALU FP64/FP32 weights (the reverse reduces its lane sums with FP64 MFMA): the 78.6 TFLOP/s FP64 vector roof equals the FP64 "
-                                  "matrix roof and non-packed FP32 issues at the same rate"}
+                       "compute": "weights on VALU (FP32 far from the wall, FP64 near it), the sums over each wave's "
+                                  "64 volume nodes on FP64 MFMA (v_mfma_f64_16x16x4f64); the 78.6 TFLOP/s FP64 "
+                                  "vector roof equals the FP64 matrix roof and non-packed FP32 issues at the same "
+                                  "rate"}
     tr = _traffic_record(kname)
     if tr:
         res["roofline"]["traffic_detail"] = {"fetch_bytes": tr["fetch_bytes"], "write_bytes": tr["write_bytes"],

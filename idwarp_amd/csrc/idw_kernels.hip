@@ -708,34 +708,75 @@ __device__ __forceinline__ double get_error(const NodeRec *nd, double dOvrR, con
 
 __device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
 
+// LDS written by one lane and read by another lane of the same wave: the
+// wave's LDS operations execute in order, the barrier keeps the compiler
+// from moving them.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+constexpr int WBUF = 128;  // FP32 plan entries a wave gathers in LDS before writing them out
+
 // Wave-level record of the walk: one entry per term (accepted cluster, or
 // point of an evaluated bucket) with the mask of the lanes it serves, in
 // depth-first order.  The wave owns slots [off, end) sized by the counting
 // pass; FP32 terms fill them from the front, FP64 terms (mixed precision
-// near the wall) from the back, and the split is stored per wave.  Writes
-// outside the range are dropped and a count that disagrees with the
-// counting pass is reported in *mismatch.
+// near the wall) from the back, and the split is stored per wave.  FP32
+// entries are gathered in LDS and written out WBUF at a time, so the walk's
+// vector-memory queue (which retires in order) is rarely held up by stores
+// in front of its error-table loads.  Writes outside the range are dropped
+// and a count that disagrees with the counting pass is reported in
+// *mismatch.
 struct PlanWriter {
     PlanEntry *plan;
     int64_t off, end, nf = 0, nb = 0;
     unsigned int *mismatch;
+    PlanEntry *buf;  // this wave's LDS buffer (WBUF entries)
+    int cf = 0;      // entries in buf (wave-uniform)
+    __device__ __forceinline__ void flush() {
+        const int lane = threadIdx.x & 63;
+        wave_sync();
+        for (int j = lane; j < cf; j += 64) {
+            const int64_t at = off + nf + j;
+            if (at < end) plan[at] = buf[j];
+        }
+        nf += cf;
+        cf = 0;
+        wave_sync();
+    }
     // terms base .. base+n-1 for the lanes of mask
     __device__ __forceinline__ void put(int base, int n, unsigned long long mask, bool fp64) {
         const int lane = threadIdx.x & 63;
-        for (int i = lane; i < n; i += 64) {
-            const int64_t at = fp64 ? end - 1 - (nb + i) : off + nf + i;
-            if (at >= off && at < end) {
-                PlanEntry e;
-                e.mask = mask;
-                e.rec = base + i;
-                e.pad = 0;
-                plan[at] = e;
+        if (fp64 || n > WBUF / 2) {
+            if (!fp64) flush();
+            for (int i = lane; i < n; i += 64) {
+                const int64_t at = fp64 ? end - 1 - (nb + i) : off + nf + i;
+                if (at >= off && at < end) {
+                    PlanEntry e;
+                    e.mask = mask;
+                    e.rec = base + i;
+                    e.pad = 0;
+                    plan[at] = e;
+                }
             }
+            if (fp64) nb += n;
+            else nf += n;
+            return;
         }
-        if (fp64) nb += n;
-        else nf += n;
+        if (cf + n > WBUF) flush();
+        if (lane < n) {
+            PlanEntry e;
+            e.mask = mask;
+            e.rec = base + lane;
+            e.pad = 0;
+            buf[cf + lane] = e;
+        }
+        cf += n;
     }
     __device__ __forceinline__ void finish(int64_t *split, int wave) {
+        flush();
         if ((threadIdx.x & 63) == 0) {
             split[wave] = off + nf;
             if (nf + nb != end - off) atomicAdd(mismatch, 1u);
@@ -767,6 +808,7 @@ __global__ void __launch_bounds__(256) k_plan_walk(int nvol, int blk0, int nbc, 
                                                    unsigned int *__restrict__ mismatch, Mirror M) {
 #pragma clang fp contract(off)
     __shared__ double2 rseg[NERR - 1];
+    __shared__ PlanEntry wbuf[COUNT ? 1 : 4][COUNT ? 1 : WBUF];
     const int lb = COUNT ? (int)blockIdx.x : xcd_block(nbc);
     if (lb >= nbc) return;  // block-uniform
     const int blk = blk0 + lb;
@@ -780,7 +822,8 @@ __global__ void __launch_bounds__(256) k_plan_walk(int nvol, int blk0, int nbc, 
     const double tol = errTol * v.w;  // errTol * denominator0
     int stop = valid ? INT32_MAX : -1;  // depth at which this lane stopped descending
     const int wave = uniform(blk * 4 + (threadIdx.x >> 6));
-    PlanWriter pw{plan, COUNT ? 0 : waveOff[wave], COUNT ? 0 : waveOff[wave + 1], 0, 0, mismatch};
+    PlanWriter pw{plan, COUNT ? 0 : waveOff[wave], COUNT ? 0 : waveOff[wave + 1], 0, 0, mismatch,
+                  wbuf[COUNT ? 0 : threadIdx.x >> 6]};
     int64_t cnt = 0;
     int node = 0;
     NodeHdr h = load_hdr(nodes, 0);
@@ -920,14 +963,6 @@ __device__ __forceinline__ double first_lane(double x) {
     return __hiloint2double(__builtin_amdgcn_readfirstlane(__double2hiint(x)),
                             __builtin_amdgcn_readfirstlane(__double2loint(x)));
 }
-// LDS written by one lane and read by another lane of the same wave: the
-// wave's LDS operations execute in order, the barrier keeps the compiler
-// from moving them.
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // ---------------------------------------------------------------------------
 // Fast-mode sums over the plan on the matrix cores
@@ -960,19 +995,50 @@ union FwdLds {
     double acc[32][BROW];  // half of the wave's 64 x 16 sums
 };
 
-// Stage FCH terms of the plan (entries c0 .. c0+cnt-1, the rest zero) in
-// LDS, two lanes per term: lane r writes row 0 of K and c, x, A, eps and
-// the mask; lane r + FCH rows 1 and 2.  W32 reads the FP32 records (mixed
-// precision).
+// Entry of plan slot e for this lane, or an empty entry (no lanes, record
+// -1) beyond the end of the range.
+// The load is unconditional (a slot inside the range is read), so the
+// loops that prefetch entries carry no branches around their loads.
+// Requires end > 0.
+__device__ __forceinline__ PlanEntry plan_entry(const PlanEntry *__restrict__ plan, int64_t e, int64_t end) {
+    PlanEntry pe = plan[min(e, end - 1)];
+    if (e >= end) {
+        pe.mask = 0ull;
+        pe.rec = -1;
+    }
+    return pe;
+}
+
+// Wait for every outstanding vector-memory operation.  Placed after a
+// prefetch and before atomics: the counter retires in order, so waiting
+// for a load issued after atomics would also wait for the atomics.
+__device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }  // vmcnt(0)
+
+// FP32 weight of one pair from |r|^2 (mixed precision).  The default
+// exponents (3, 5) use w = A L^3 p^3 (1 + alpha^5 L^2 p^2) with p = 1/|r|,
+// which is A (q^3 + alpha^5 q^5) for q = L p; AL3 = A L^3 per term and
+// a5L2 = alpha^5 L^2 per launch.
+template <int EXPM>
+__device__ __forceinline__ float weight_f32(float d2, float A, float AL3, float Lf, float a5L2, const WParams &P) {
+    if (EXPM == EXP_35) {
+        const float p = __frsqrt_rn(d2), p2 = p * p;
+        return (AL3 * (p2 * p)) * fmaf(a5L2, p2, 1.0f);
+    }
+    return A * wfun<EXPM>(Lf * __frsqrt_rn(d2), P);
+}
+
+// Stage FCH terms of the plan in LDS, two lanes per term: lane r writes
+// row 0 of K and c, x, A, eps and the mask; lane r + FCH rows 1 and 2.
+// pe: this lane's entry (record -1: padding row).  W32 reads the FP32
+// records (mixed precision).
 template <bool W32>
-__device__ __forceinline__ void stage_terms(FwdChunk &C, const PlanEntry *__restrict__ plan, int64_t c0, int cnt,
-                                            int nnodes, int nrec, const Rec64 *__restrict__ rec,
-                                            const Rec32 *__restrict__ rec32, double ox, double oy, double oz) {
+__device__ __forceinline__ void stage_terms(FwdChunk &C, PlanEntry pe, int nnodes, int nrec,
+                                            const Rec64 *__restrict__ rec, const Rec32 *__restrict__ rec32,
+                                            double ox, double oy, double oz) {
     const int lane = threadIdx.x & 63, r = lane & (FCH - 1);
     const bool hi = lane >= FCH;
-    if (r < cnt) {
-        PlanEntry pe = plan[c0 + r];
-        pe.rec = min((unsigned)pe.rec, (unsigned)(nrec - 1));  // a plan that disagreed with its count stays in bounds
+    if (pe.rec >= 0) {
+        pe.rec = min(pe.rec, nrec - 1);  // a plan that disagreed with its count stays in bounds
         double x[3];
         if (W32) for (int i = 0; i < 3; ++i) x[i] = rec32[pe.rec].x[i];
         else for (int i = 0; i < 3; ++i) x[i] = rec[pe.rec].x[i];
@@ -1013,18 +1079,23 @@ __device__ __forceinline__ void stage_terms(FwdChunk &C, const PlanEntry *__rest
 
 // Forward sums of the terms in plan entries [a, b) into D (node block t in
 // D[t]); vx/vy/vz: positions of the lane's rows (nodes 16t + lane%16).
+// The entries of the next chunk are requested while this one is summed.
 template <bool W32, int EXPM>
 __device__ __forceinline__ void fwd_terms(int64_t a, int64_t b, const PlanEntry *__restrict__ plan, int nnodes,
                                           int nrec, const Rec64 *__restrict__ rec, const Rec32 *__restrict__ rec32,
                                           const WParams &P, FwdChunk &C, double ox, double oy, double oz,
                                           const double (&vx)[4], const double (&vy)[4], const double (&vz)[4],
                                           dvec4 (&D)[4]) {
-    const int lane = threadIdx.x & 63, n = lane & 15, R = lane >> 4;
-    const float Lf = (float)P.Ldef;
+    const int lane = threadIdx.x & 63, n = lane & 15, R = lane >> 4, r = lane & (FCH - 1);
+    const float Lf = (float)P.Ldef, a5L2 = (float)(P.a5 * P.Ldef * P.Ldef);
+    const double L3 = P.Ldef * P.Ldef * P.Ldef;
+    if (a >= b) return;
+    PlanEntry pe = plan_entry(plan, a + r, b);
     for (int64_t c0 = a; c0 < b; c0 += FCH) {
         const int cnt = (int)min((int64_t)FCH, b - c0);
         wave_sync();  // the previous chunk is consumed
-        stage_terms<W32>(C, plan, c0, cnt, nnodes, nrec, rec, rec32, ox, oy, oz);
+        stage_terms<W32>(C, pe, nnodes, nrec, rec, rec32, ox, oy, oz);
+        pe = plan_entry(plan, c0 + FCH + r, b);  // next chunk
         wave_sync();
         const int ng = (cnt + 3) >> 2;
 #pragma unroll 1
@@ -1034,13 +1105,13 @@ __device__ __forceinline__ void fwd_terms(int64_t a, int64_t b, const PlanEntry 
             const unsigned long long m = C.mask[tr] >> n;
             const double x0 = C.X[tr][0], x1 = C.X[tr][1], x2 = C.X[tr][2], A = C.X[tr][3];
             if (W32) {
-                const float Af = (float)A;
+                const float Af = (float)A, AL3 = (float)(A * L3);
 #pragma unroll
                 for (int t = 0; t < 4; ++t) {
                     const float fx = (float)(vx[t] - x0), fy = (float)(vy[t] - x1), fz = (float)(vz[t] - x2);
-                    const float w = Af * wfun<EXPM>(Lf * __frsqrt_rn(fx * fx + fy * fy + fz * fz), P);
+                    const float w = weight_f32<EXPM>(fx * fx + fy * fy + fz * fz, Af, AL3, Lf, a5L2, P);
                     const bool on = (m >> (16 * t)) & 1ull;
-                    D[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(on ? (double)w : 0.0, Bv, D[t], 0, 0, 0);
+                    D[t] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)(on ? w : 0.0f), Bv, D[t], 0, 0, 0);
                 }
             } else {
                 const double eps = C.eps[tr];
@@ -1148,26 +1219,30 @@ struct RevLds {
 // G = [s (x) g (9), s (3)] (B in registers), then
 // Mb'(a, c) += P(3a + c) - P(9 + a) (x - o)_c and Bb(a) += P(9 + a) into
 // the term's adjoint record.  Lane l computes the weight of term l%16 with
-// node 4q + l/16 for node group q.
+// node 4q + l/16 for node group q.  The next tile's record and the entries
+// of the tile after it are requested before this tile's sums and are in
+// registers before its atomics are issued.
 template <bool W32, int EXPM>
 __device__ __forceinline__ void rev_terms(int64_t a, int64_t b, const PlanEntry *__restrict__ plan, int nnodes,
                                           int nrec, const Rec64 *__restrict__ rec, const WParams &P, RevLds &L,
                                           const double (&B)[16], double ox, double oy, double oz,
                                           double *__restrict__ adj) {
     const int lane = threadIdx.x & 63, m = lane & 15, R = lane >> 4;
-    const float Lf = (float)P.Ldef;
+    const float Lf = (float)P.Ldef, a5L2 = (float)(P.a5 * P.Ldef * P.Ldef);
+    const double L3 = P.Ldef * P.Ldef * P.Ldef;
     const int src = (lane & ~15) | (9 + (m < 9 ? m / 3 : 0));
+    if (a >= b) return;
+    auto record = [&](const PlanEntry &pe) {  // x, A of an entry (zero for padding); unconditional load
+        const double4 v = *reinterpret_cast<const double4 *>(rec + min(max(pe.rec, 0), nrec - 1));
+        return pe.rec >= 0 ? v : make_double4(0.0, 0.0, 0.0, 0.0);
+    };
+    PlanEntry pe = plan_entry(plan, a + m, b);
+    double4 xa = record(pe);
+    PlanEntry pn = plan_entry(plan, a + RT + m, b);
     for (int64_t t0 = a; t0 < b; t0 += RT) {
-        const int64_t e = t0 + m;
-        unsigned long long mask = 0ull;
-        double4 xa = make_double4(0.0, 0.0, 0.0, 0.0);
-        int idx = -1;
-        if (e < b) {
-            const PlanEntry pe = plan[e];
-            mask = pe.mask;
-            idx = min((unsigned)pe.rec, (unsigned)(nrec - 1));  // a plan that disagreed with its count stays in bounds
-            xa = *reinterpret_cast<const double4 *>(rec + idx);  // x, A
-        }
+        const double4 xn = record(pn);
+        const PlanEntry pnn = plan_entry(plan, t0 + 2 * RT + m, b);
+        const int idx = pe.rec >= 0 ? min(pe.rec, nrec - 1) : -1;
         const double eps = (idx >= 0 && idx < nnodes) ? 0.0 : EPS_W;
         wave_sync();  // the previous tile's reads of Y / idx are done
         if (R == 0) {
@@ -1176,25 +1251,28 @@ __device__ __forceinline__ void rev_terms(int64_t a, int64_t b, const PlanEntry 
             L.Y[m][2] = xa.z - oz;
             L.idx[m] = idx;
         }
-        const unsigned long long mR = mask >> R;
-        const float Af = (float)xa.w;
+        const unsigned long long mR = pe.mask >> R;
+        const float Af = (float)xa.w, AL3 = (float)(xa.w * L3);
         dvec4 D0 = {0.0, 0.0, 0.0, 0.0}, D1 = D0;
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
             const double4 c = L.node[4 * q + R];
+            const bool on = (mR >> (4 * q)) & 1ull;
             double w;
             if (W32) {
                 const float fx = (float)(c.x - xa.x), fy = (float)(c.y - xa.y), fz = (float)(c.z - xa.z);
-                w = (double)(Af * wfun<EXPM>(Lf * __frsqrt_rn(fx * fx + fy * fy + fz * fz), P));
+                const float wf = weight_f32<EXPM>(fx * fx + fy * fy + fz * fz, Af, AL3, Lf, a5L2, P);
+                w = (double)(on ? wf : 0.0f);
             } else {
                 const double rx = c.x - xa.x, ry = c.y - xa.y, rz = c.z - xa.z;
                 w = xa.w * wfun<EXPM>(P.Ldef * rsqrt_nr(rx * rx + ry * ry + rz * rz + eps), P);
+                w = on ? w : 0.0;
             }
-            w = ((mR >> (4 * q)) & 1ull) ? w : 0.0;
             if (q & 1) D1 = __builtin_amdgcn_mfma_f64_16x16x4f64(w, B[q], D1, 0, 0, 0);
             else D0 = __builtin_amdgcn_mfma_f64_16x16x4f64(w, B[q], D0, 0, 0, 0);
         }
         const dvec4 Ds = D0 + D1;
+        wait_vmem();  // the prefetches are in before this tile's atomics go out
         wave_sync();  // Y / idx of this tile are written
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
@@ -1205,6 +1283,9 @@ __device__ __forceinline__ void rev_terms(int64_t a, int64_t b, const PlanEntry 
             const int id = L.idx[tr];
             if (id >= 0 && m < 12 && val != 0.0) atomicAdd(adj + (int64_t)id * ADJ_STRIDE + m, val);
         }
+        pe = pn;
+        xa = xn;
+        pn = pnn;
     }
 }
 
@@ -1215,7 +1296,7 @@ __device__ __forceinline__ void rev_terms(int64_t a, int64_t b, const PlanEntry 
 // s = w * dXv / den (warpDeriv.F90:76-78).  The adjoint records are indexed
 // like the term records (clusters, then points).
 template <int PREC, int EXPM>
-__global__ void __launch_bounds__(256) k_fast_rev(int nvol, const double4 *__restrict__ vol,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_fast_rev(int nvol, const double4 *__restrict__ vol,
                                                   const int32_t *__restrict__ perm, const double *__restrict__ dxv,
                                                   const double *__restrict__ den, const PlanEntry *__restrict__ plan,
                                                   const int64_t *__restrict__ waveOff,
