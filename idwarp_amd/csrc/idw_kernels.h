@@ -35,7 +35,6 @@ enum { PASS_ONLY = 0, PASS_FIRST = 1, PASS_MID = 2, PASS_LAST = 3 };
 struct WParams {
     double Ldef, a5, aExp, bExp, errTol, nearT2;
     int32_t expMode, ia, ib;
-    int32_t zero;  // always 0: lane & zero turns a uniform index into a per-lane one (vector load)
     int32_t tangent;  // 1: the point/cluster records hold tangents; write num/den only (warpDerivFwd)
     int32_t pass;     // PASS_* of this launch
 };

@@ -16,6 +16,7 @@
 
 #include <cstdint>
 
+#include "idw_err_segment.h"
 #include "idw_internal.h"
 #include "idw_kernels.h"
 
@@ -681,22 +682,9 @@ __global__ void k_exact_rev_reduce(int n, int nchunk, const double *__restrict__
 // ---------------------------------------------------------------------------
 // Fast mode: wave-uniform kd-tree walk
 // ---------------------------------------------------------------------------
-// Segment i of the error table, rstar(i) <= d < rstar(i+1), for d >= 2
-// (NERR-1 from the last radius on).  The radii 2, 3, 4, 5, 7.5 and 10 * 2^k
-// are exact thresholds on the binary exponent e and mantissa m of
-// d = m 2^e, m in [0.5, 1).
-__host__ __device__ __forceinline__ int err_segment(double d) {
-    int e;
-    const double m = frexp(d, &e);
-    int i;
-    if (e >= 4) i = e + 1 - (m < 0.625 ? 1 : 0);
-    else if (e == 3) i = 2 + (m >= 0.625 ? 1 : 0) + (m >= 0.9375 ? 1 : 0);
-    else i = m >= 0.75 ? 1 : 0;
-    return i < NERR - 1 ? i : NERR - 1;
-}
-
 // getError (kd_tree.F90:882-906): linear interpolation of the node's error
 // table at dOvrR = dist * oradius >= 2; rseg holds {rstar(i), orstar(i)}.
+static_assert(NERR == 12, "err_segment covers the 12 reference radii");
 __device__ __forceinline__ double get_error(const NodeRec *nd, double dOvrR, const double2 *rseg) {
 #pragma clang fp contract(off)
     const int i = err_segment(dOvrR);
