@@ -76,9 +76,7 @@ def read_boundary(path):
 def read_polymesh(case_dir):
     """Volume points, faces and boundary patches of an OpenFOAM case (the
     case directory or its constant/polyMesh directory)."""
-    d = case_dir
-    if os.path.isdir(os.path.join(d, "constant", "polyMesh")):
-        d = os.path.join(d, "constant", "polyMesh")
+    d = _polymesh_dir(case_dir)
     pts_file = os.path.join(d, "points")
     if not os.path.exists(pts_file):
         pts_file = os.path.join(d, "points_orig")
@@ -114,6 +112,25 @@ _HEADER = """FoamFile
     object      %s;
 }
 """
+
+
+def _polymesh_dir(case_dir):
+    d = case_dir
+    if os.path.isdir(os.path.join(d, "constant", "polyMesh")):
+        d = os.path.join(d, "constant", "polyMesh")
+    return d
+
+
+def write_points(case_dir, x):
+    """Overwrite the points file of a case (the case directory or its
+    constant/polyMesh) with x (N, 3), as pyOFM's writeVolumeMeshPoints does
+    for writeGrid (UnstructuredMesh.py:507-511)."""
+    x = np.asarray(x, dtype=np.float64).reshape(-1, 3)
+    with open(os.path.join(_polymesh_dir(case_dir), "points"), "w") as f:
+        f.write(_HEADER % ("vectorField", "points"))
+        f.write("\n%d\n(\n" % len(x))
+        f.write("".join("(%.17g %.17g %.17g)\n" % tuple(p) for p in x))
+        f.write(")\n")
 
 
 def write_polymesh(case_dir, x0, face_sizes, face_conn, boundaries):

@@ -2,10 +2,11 @@
 
 Method and option names, argument meaning and error behaviour follow
 idwarp/UnstructuredMesh.py (class USMesh, :43).  The volume grid comes from
-memory (``gridFile`` may be an (N, 3) array or a ``.npy`` path) instead of a
-CGNS/PLOT3D/OpenFOAM reader, and two options are added: ``precision``
-("fp64" or "mixed") and ``nearWallFactor``.  All numerics run in HIP kernels
-through ``libidwarp_hip.so``; nothing here computes a warp on the CPU.
+an OpenFOAM ASCII case, a multiblock PLOT3D file, or memory (``gridFile``
+may be an (N, 3) array or a ``.npy`` path); CGNS files are not read.  Two
+options are added: ``precision`` ("fp64" or "mixed") and ``nearWallFactor``.
+All numerics run in HIP kernels through ``libidwarp_hip.so``; nothing here
+computes a warp on the CPU.
 """
 import ctypes
 
@@ -72,6 +73,7 @@ class USMesh:
         self.nSurf = 0
         self.nVol = 0
         self.OFData = {}
+        self.plot3dSizes = None
         self.fileType = self.getOption("fileType")
         grid = self.getOption("gridFile")
         if grid is not None:
@@ -81,10 +83,15 @@ class USMesh:
 
                 self.OFData = openfoam.read_polymesh(grid)
                 grid = self.OFData["x0"]
+            elif self.fileType == "PLOT3D":
+                # UnstructuredMesh.py:121-122, readPlot3d.F90:1-162
+                from . import plot3d
+
+                self.plot3dSizes, grid = plot3d.read_grid(grid)
             elif isinstance(grid, str):
                 if self.fileType not in ("NPY", "memory") or not grid.endswith(".npy"):
-                    raise Error("Grid files other than OpenFOAM cases and .npy arrays are not supported "
-                                "(CGNS and PLOT3D readers are not part of this library)")
+                    raise Error("Grid files other than OpenFOAM cases, PLOT3D files and .npy arrays are not "
+                                "supported (the CGNS reader is not part of this library)")
                 grid = np.load(grid)
             self.setVolumeNodes(grid)
 
@@ -169,6 +176,23 @@ class USMesh:
         self.nSurf = len(pts)
         self.warpInitialized = True
 
+    def setSurfaceDefinitionFromFile(self, surfFile):
+        """Surface definition from a multiblock PLOT3D surface file: every
+        cell of every zone is a quadrilateral (UnstructuredMesh.py:263-294,
+        readPlot3dSurface, readPlot3d.F90:164-262)."""
+        from . import plot3d
+
+        pts, conn, _ = plot3d.read_surface(surfFile)
+        self.setSurfaceDefinition(pts, conn, 4 * np.ones(len(conn), dtype=np.int32))
+
+    def setSurfaceFromFile(self, surfFile):
+        """New surface coordinates from a PLOT3D surface file with the zones
+        and sizes of the one given to setSurfaceDefinitionFromFile
+        (UnstructuredMesh.py:296-323)."""
+        from . import plot3d
+
+        self.setSurfaceCoordinates(plot3d.read_surface(surfFile)[0])
+
     def _setSymmetryConditions(self):
         """UnstructuredMesh.py:752-962 for user-given planes: the
         'symmetryPlanes' option [[point, normal], ...].  Planes found from
@@ -240,6 +264,25 @@ class USMesh:
 
     def getCommonGrid(self):
         return self.getWarpGrid()
+
+    def writeGrid(self, fileName=None):
+        """Write the current grid in the input's format
+        (UnstructuredMesh.py:480-511): a PLOT3D grid to fileName, an
+        OpenFOAM case's constant/polyMesh/points in place."""
+        if self.fileType in ("CGNS", "PLOT3D"):
+            if fileName is None:
+                raise Error("fileName is not optional for writeGrid with gridType of CGNS or PLOT3D")
+            if self.fileType == "CGNS":
+                raise Error("CGNS output is not part of this library")
+            from . import plot3d
+
+            plot3d.write_grid(fileName, self.plot3dSizes, self.getCommonGrid().reshape(-1, 3))
+        elif self.fileType == "OpenFOAM":
+            from . import openfoam
+
+            openfoam.write_points(self.getOption("gridFile"), self.getCommonGrid().reshape(-1, 3))
+        else:
+            raise Error("writeGrid needs a CGNS, PLOT3D or OpenFOAM grid; this mesh was given in memory")
 
     def _solverDOF(self):
         n = ctypes.c_int64(0)

@@ -604,3 +604,61 @@ def test_chunked_walk_and_forward_overlap():
         err = np.abs(x1.reshape(-1, 3)[idx] - ref.reshape(-1, 3)).max() / ldef
         print(prec, "sampled max |dXv|/Ldef", err)
         assert err <= (FP64_TOL if prec == "fp64" else MIXED_TOL)
+
+
+@pytest.mark.parametrize("mode", ["exact", "fast"])
+def test_plot3d_grid_and_surface_files(tmp_path, mode):
+    """USMesh driven by PLOT3D files as the reference's PLOT3D users drive
+    it (fileType='PLOT3D', setSurfaceDefinitionFromFile, setSurfaceFromFile,
+    writeGrid): a 13x9x9 sphere O-mesh in two volume zones that share their
+    k=4 layer, its wall in two surface zones that share a row.  The warp must
+    equal an in-memory USMesh on the same nodes bit for bit and the oracle
+    within the FP64 bar; the written grid reads back as the warped grid."""
+    from idwarp_amd import plot3d
+
+    ni, nj, nk = 13, 9, 9
+    m = synthetic.sphere_omesh(ni, nj, nk)
+    lay = ni * nj
+    grid = str(tmp_path / "grid.xyz")
+    plot3d.write_grid(grid, [[ni, nj, 5], [ni, nj, 5]], np.concatenate([m["xv0"][:5 * lay], m["xv0"][4 * lay:]]))
+
+    def wall_file(path, p):
+        w = p.reshape(nj, ni, 3)
+        plot3d.write_surface(path, [[ni, 5], [ni, 5]], np.concatenate([w[:5].reshape(-1, 3), w[4:].reshape(-1, 3)]))
+
+    surf0, surf1 = str(tmp_path / "wall0.xyz"), str(tmp_path / "wall1.xyz")
+    wall_file(surf0, m["pts"])
+    wall_file(surf1, m["perturb"](m["pts"]))
+
+    gpu = USMesh(options=dict(gridFile=grid, fileType="PLOT3D", evalMode=mode))
+    gpu.setSurfaceDefinitionFromFile(surf0)
+    gpu.setSurfaceFromFile(surf1)
+    gpu.warpMesh()
+
+    _, xvol = plot3d.read_grid(grid)
+    pts, conn, _ = plot3d.read_surface(surf0)
+    fs = np.full(len(conn), 4, np.int32)
+    xs = plot3d.read_surface(surf1)[0]
+    mem = USMesh(options=dict(gridFile=xvol, fileType="memory", evalMode=mode))
+    mem.setSurfaceDefinition(pts, conn, fs)
+    mem.setSurfaceCoordinates(xs)
+    mem.warpMesh()
+    ora = O.OracleMesh(pts, conn.ravel(), fs, xvol, evalMode=mode)
+    ora.setSurfaceCoordinates(xs)
+    ref = ora.warpMesh()
+
+    got = gpu.getWarpGrid()
+    assert np.array_equal(got, mem.getWarpGrid())
+    err = np.abs(got - ref).max() / ora.Ldef0
+    print("PLOT3D", mode, "max |dXv|/Ldef", err, flush=True)
+    assert err <= FP64_TOL
+    assert np.abs(got - xvol.ravel()).max() > 1e-3
+    out = str(tmp_path / "warped.xyz")
+    gpu.writeGrid(out)
+    sizes, back = plot3d.read_grid(out)
+    assert sizes.tolist() == [[ni, nj, 5], [ni, nj, 5]]
+    assert np.array_equal(back, got.reshape(-1, 3))
+    # the shared layer moves as one (fast mode sums about a per-wave origin, so to rounding)
+    assert np.abs(back[4 * lay:5 * lay] - back[5 * lay:6 * lay]).max() <= FP64_TOL * ora.Ldef0
+    with pytest.raises(Exception):
+        gpu.writeGrid()
