@@ -1,0 +1,200 @@
+"""MultiUSMesh bookkeeping (idwarp_amd/multiusmesh.py) on the CPU: the
+splitting of the solver's volume and surface vectors among the instances
+(idwarp/MultiUnstructuredMesh.py:336-594) checked with a linear stand-in
+for USMesh, xv = xv0 + B (xs - xs0), so that the assembled products of the
+whole object must equal one global matrix built independently here."""
+import numpy as np
+import pytest
+
+from idwarp_amd import multiusmesh as MM
+from idwarp_amd import plot3d
+
+
+class LinearMesh:
+    """USMesh stand-in with a dense random Jacobian B per instance."""
+
+    made = []
+
+    def __init__(self, options, comm=None, debug=False):
+        self.options = options
+        self.xv0 = np.asarray(options["gridFile"], np.float64).ravel().copy()
+        self.xv = self.xv0.copy()
+        LinearMesh.made.append(self)
+
+    def setExternalMeshIndices(self, ind):
+        self.ind = np.asarray(ind)
+
+    def setSurfaceDefinition(self, pts, conn, faceSizes):
+        self.pts0, self.conn, self.fs = np.array(pts, np.float64), np.array(conn), np.array(faceSizes)
+        self.xs0 = self.pts0.ravel().copy()
+        self.xs = self.xs0.copy()
+        self.B = np.random.default_rng(len(self.xv0)).standard_normal((len(self.xv0), len(self.xs0)))
+
+    def getSurfaceCoordinates(self):
+        return self.xs.reshape(-1, 3).copy()
+
+    def setSurfaceCoordinates(self, p):
+        assert p.shape == (len(self.xs0) // 3, 3)
+        self.xs = np.array(p, np.float64).ravel()
+
+    def warpMesh(self):
+        self.xv = self.xv0 + self.B @ (self.xs - self.xs0)
+
+    def getCommonGrid(self):
+        return self.xv.copy()
+
+    def getSolverGrid(self):
+        return self.xv[self.ind]
+
+    def warpDeriv(self, dXv, solverVec=True):
+        full = np.zeros(len(self.xv0))
+        np.add.at(full, self.ind, dXv)
+        self.dxs = self.B.T @ full
+
+    def getdXs(self):
+        return self.dxs.reshape(-1, 3)
+
+    def warpDerivFwd(self, dXs):
+        self.warpMesh()
+        return (self.B @ np.ravel(dXs))[self.ind]
+
+
+def _grid():
+    """Blocks in CGNS order: body (2 blocks), background (1), fin (1)."""
+    rng = np.random.default_rng(0)
+    blocks = [("body.00000", rng.random((2, 3, 4, 3))), ("body.00001", rng.random((7, 3))),
+              ("background.00000", rng.random((3, 2, 2, 3)) + 5.0), ("fin.00000", rng.random((4, 1, 2, 3)) - 3.0)]
+    return blocks
+
+
+def _surface():
+    """Solver surface: two quads in body block 0, a triangle in
+    body block 1, a background triangle (block 2) and two fin triangles (block
+    3); point 3 is shared by body and fin elements."""
+    pts = np.random.default_rng(1).random((14, 3))
+    conn = [0, 1, 2, 3, 2, 3, 4, 5, 4, 5, 6, 7, 8, 9, 3, 10, 11, 12, 13, 11]
+    fs = [4, 4, 3, 3, 3, 3]
+    ids = [0, 0, 1, 2, 3, 3]
+    return pts, np.array(conn), np.array(fs), np.array(ids)
+
+
+@pytest.fixture
+def multi(monkeypatch):
+    monkeypatch.setattr(MM, "USMesh", LinearMesh)
+    LinearMesh.made = []
+    blocks = _grid()
+    m = MM.MultiUSMesh(blocks, {"body": {"aExp": 4.0}, "fin": {}})
+    nnodes = sum(np.asarray(b).size // 3 for _, b in blocks)
+    perm = np.random.default_rng(2).permutation(nnodes)
+    ind = (3 * perm[:, None] + np.arange(3)).ravel()
+    m.setExternalMeshIndices(ind)
+    pts, conn, fs, ids = _surface()
+    m.setSurfaceDefinition(pts, conn, fs, ids)
+    return m, blocks, ind, pts, conn, fs, ids
+
+
+def test_zones_and_intervals(multi):
+    m, blocks, *_ = multi
+    assert m.zoneNames == ["body", "fin"]
+    assert m.cgnsBlockIntervals == [[0, 2], [3, 4]]
+    assert m.cgnsVolNodeIntervals == [[0, 3 * 31], [3 * 43, 3 * 51]]
+    assert [lm.options["aExp"] for lm in LinearMesh.made[:1]] == [4.0]
+    assert m.blockSizes == [[(4, 3, 2), (7, 1, 1)], [(2, 1, 4)]]
+    assert MM.zone_name("near_wing_vol_L3.00012") == "near_wing_vol_L3" and MM.zone_name("plain") == "plain"
+
+
+def test_surface_split(multi):
+    m, blocks, ind, pts, conn, fs, ids = multi
+    body, fin = m.meshes
+    assert m.filtered2fullMaps[0].tolist() == [0, 1, 2, 3, 4, 5, 6]
+    assert m.filtered2fullMaps[1].tolist() == [3, 10, 11, 12, 13]
+    # every instance element names the same physical points as the solver's element
+    ptr = np.concatenate([[0], np.cumsum(fs)])
+    for mesh, want_ids in ((body, (0, 1)), (fin, (3,))):
+        sel = [e for e in range(len(fs)) if ids[e] in want_ids]
+        ref = np.concatenate([pts[conn[ptr[e]:ptr[e + 1]]] for e in sel])
+        assert np.array_equal(mesh.pts0[mesh.conn], ref)
+        assert mesh.fs.tolist() == [fs[e] for e in sel]
+    got = m.getSurfaceCoordinates()
+    assert np.array_equal(got[[0, 1, 2, 3, 4, 5, 6, 10, 11, 12, 13]], pts[[0, 1, 2, 3, 4, 5, 6, 10, 11, 12, 13]])
+    assert not got[[7, 8, 9]].any()  # points of the background element belong to no instance
+
+
+def _global_jacobian(m, nsurf):
+    """Solver-order Jacobian of the whole object from the instances' B."""
+    J = np.zeros((m.numVolNodes, 3 * nsurf))
+    for ii, mesh in enumerate(m.meshes):
+        cols = (3 * m.filtered2fullMaps[ii][:, None] + np.arange(3)).ravel()
+        rows = np.flatnonzero(m.cgnsVolNodeMasks[ii])
+        J[np.ix_(rows, cols)] = mesh.B[mesh.ind]
+    return J
+
+
+def test_products_match_one_global_matrix(multi):
+    m, blocks, ind, pts, conn, fs, ids = multi
+    xcgns = np.concatenate([np.asarray(b, np.float64).ravel() for _, b in blocks])
+    assert np.array_equal(m.getSolverGrid(), xcgns[ind])
+    J = _global_jacobian(m, len(pts))
+    rng = np.random.default_rng(4)
+    x0 = m.getSurfaceCoordinates()
+    dx = rng.standard_normal(x0.shape)
+    dx[[7, 8, 9]] = 0.0
+    m.setSurfaceCoordinates(x0 + dx)
+    m.warpMesh()
+    np.testing.assert_allclose(m.getSolverGrid(), xcgns[ind] + J @ dx.ravel(), rtol=0, atol=1e-12)
+    vols, ncoor = m.getWarpGrid()
+    assert ncoor == 3 * 31 + 3 * 8 and [len(v) for v in vols] == [93, 24]
+    dXv = np.linspace(0.0, 1.0, len(ind))
+    dXs = m.warpDeriv(dXv)
+    np.testing.assert_allclose(dXs.ravel(), J.T @ dXv, rtol=0, atol=1e-12)
+    assert np.array_equal(m.getdXs(), dXs)
+    s = rng.standard_normal(x0.shape)
+    fwd = m.warpDerivFwd(s)
+    np.testing.assert_allclose(fwd, J @ s.ravel(), rtol=0, atol=1e-12)
+    bg = np.ones(len(ind), bool)
+    for mask in m.cgnsVolNodeMasks:
+        bg &= ~mask
+    assert bg.sum() == 3 * 12 and not fwd[bg].any()
+    assert abs(dXv @ fwd - dXs.ravel() @ s.ravel()) <= 1e-12 * abs(dXv @ fwd)
+
+
+def test_write_grid_per_instance(multi, tmp_path):
+    m, blocks, *_ = multi
+    m.warpMesh()
+    names = m.writeGrid(str(tmp_path / "out"))
+    assert [n.rsplit("/", 1)[1] for n in names] == ["out_inst000.xyz", "out_inst001.xyz"]
+    sizes, nodes = plot3d.read_grid(names[0])
+    assert sizes.tolist() == [[4, 3, 2], [7, 1, 1]]
+    assert np.array_equal(nodes.ravel(), m.meshes[0].getCommonGrid())
+
+
+def test_plot3d_input_and_errors(monkeypatch, tmp_path):
+    monkeypatch.setattr(MM, "USMesh", LinearMesh)
+    blocks = _grid()
+    f = str(tmp_path / "g.xyz")
+    sizes = [MM._block_sizes(b) for _, b in blocks]
+    plot3d.write_grid(f, sizes, np.concatenate([np.asarray(b).reshape(-1, 3) for _, b in blocks]))
+    names = [n for n, _ in blocks]
+    m = MM.MultiUSMesh(f, {"fin": {}}, blockNames=names)
+    assert m.zoneNames == ["body", "background", "fin"]
+    assert np.array_equal(m.meshes[2].xv0, np.asarray(blocks[3][1]).ravel())
+    assert np.array_equal(m.meshes[0].getCommonGrid(),
+                          np.concatenate([np.asarray(b).ravel() for _, b in blocks[:2]]))
+    with pytest.raises(MM.Error):
+        MM.MultiUSMesh(f, {"fin": {}})
+    with pytest.raises(MM.Error):
+        MM.MultiUSMesh("grid.cgns", {"fin": {}})
+    with pytest.raises(MM.Error):
+        MM.MultiUSMesh(blocks, {"wing": {}})
+    with pytest.raises(MM.Error):
+        MM.MultiUSMesh(blocks[:1] + blocks[2:3] + blocks[1:2], {"body": {}})
+    with pytest.raises(MM.Error):
+        MM.MultiUSMesh(blocks, {"body": {}}, dtype="D")
+    pts, conn, fs, ids = _surface()
+    with pytest.raises(NameError):
+        m.setSurfaceDefinition(pts, conn, fs, ids)
+    m.setExternalMeshIndices(np.arange(3 * 51))
+    with pytest.raises(ValueError):
+        m.setSurfaceDefinition(pts, conn, fs, ids[:-1])
+    with pytest.raises(NameError):
+        m.getSurfaceCoordinates()
