@@ -1,4 +1,5 @@
-"""In-tree builds: the HIP library (gfx950) and, for tests, the CPU oracle."""
+"""In-tree builds: the HIP library (gfx950) and, for tests, a HIP test library
+and the CPU oracle."""
 import os
 import subprocess
 
@@ -35,6 +36,19 @@ def build_library():
     return LIB
 
 
+TEST_LIB = os.path.join(ROOT, "tests", "native", "libwalkmath.so")
+
+
+def build_test_library():
+    """Compile tests/native/libwalkmath.so, the walk's device arithmetic on
+    arrays for the tests (same flags as the library)."""
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    tmp = TEST_LIB + ".tmp"
+    _run([hipcc] + HIPCC_FLAGS + ["-I", CSRC, "-o", tmp, "walkmath.hip"], os.path.dirname(TEST_LIB))
+    os.replace(tmp, TEST_LIB)
+    return TEST_LIB
+
+
 def build_oracle(reference="/root/reference/src"):
     """Compile oracle/liboracle.so and, where the reference sources exist,
     oracle/_ref/libidwref.so (test-only checkers)."""
@@ -46,4 +60,5 @@ def build_oracle(reference="/root/reference/src"):
 
 if __name__ == "__main__":
     print(build_library())
+    print(build_test_library())
     build_oracle()

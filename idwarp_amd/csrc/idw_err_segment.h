@@ -26,4 +26,19 @@ IDW_HD int err_segment(double d) {
     return i < 11 ? i : 11;
 }
 
+// Cell of d >= 2 on a grid of eighths of each binade: the exponent and the
+// three leading mantissa bits of d, (bits >> 49) - (1024 << 3).  Every radius
+// lies on a cell boundary, so all of a cell's values share one segment; the
+// cells from ERR_CELLS - 1 on (d >= 1024) all lie in the last segment.
+constexpr int ERR_CELLS = 73;
+IDW_HD int err_cell(double d) {
+    unsigned long long bits;
+    __builtin_memcpy(&bits, &d, sizeof bits);
+    const int c = (int)(bits >> 49) - 8192;
+    return c < ERR_CELLS - 1 ? c : ERR_CELLS - 1;
+}
+
+// Lowest value of cell c.
+IDW_HD double err_cell_low(int c) { return ldexp(1.0 + (c & 7) * 0.125, 1 + (c >> 3)); }
+
 }  // namespace idw

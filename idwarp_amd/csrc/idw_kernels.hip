@@ -14,11 +14,13 @@
 //    are fetched with scalar loads.
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
 #include <cstdint>
 
 #include "idw_err_segment.h"
 #include "idw_internal.h"
 #include "idw_kernels.h"
+#include "idw_walk_math.h"
 
 namespace idw {
 
@@ -795,12 +797,12 @@ __global__ void __launch_bounds__(256) k_plan_walk(int nvol, int blk0, int nbc, 
                                                    int64_t *__restrict__ waveSplit,
                                                    unsigned int *__restrict__ mismatch, Mirror M) {
 #pragma clang fp contract(off)
-    __shared__ double2 rseg[NERR - 1];
+    __shared__ SegCell cells[ERR_CELLS];
     __shared__ PlanEntry wbuf[COUNT ? 1 : 4][COUNT ? 1 : WBUF];
     const int lb = COUNT ? (int)blockIdx.x : xcd_block(nbc);
     if (lb >= nbc) return;  // block-uniform
     const int blk = blk0 + lb;
-    if (threadIdx.x < NERR - 1) rseg[threadIdx.x] = c_rseg[threadIdx.x];
+    if (threadIdx.x < ERR_CELLS) cells[threadIdx.x] = seg_cell(threadIdx.x, c_rseg, c_rstar[NERR - 1]);
     __syncthreads();
     const double nearT = sqrt(nearT2);
     const int k = blk * 256 + threadIdx.x;
@@ -845,9 +847,11 @@ __global__ void __launch_bounds__(256) k_plan_walk(int nvol, int blk0, int nbc, 
                 if (sure) {
                     descend = true;
                 } else {
-                    const double dOvrR = sqrt(d2) * h.orad;
+                    // d2 >= 3.99 rad^2 here, and the tree build keeps every
+                    // internal radius above 1e-100
+                    const double dOvrR = sqrt_normal(d2) * h.orad;
                     if (dOvrR < 2.0) descend = true;
-                    else if (get_error(nodes + node, dOvrR, rseg) < tol) accept = true;
+                    else if (get_error_cells(nodes + node, dOvrR, cells) < tol) accept = true;
                     else descend = true;
                 }
                 stop = descend ? INT32_MAX : h.depth;

@@ -192,6 +192,13 @@ bool build_surface(const double *uniquePts, int64_t nu, const int32_t *link, int
             const double dx = p[0] - r.X[0], dy = p[1] - r.X[1], dz = p[2] - r.X[2];
             radius = std::max(radius, std::sqrt(dx * dx + dy * dy + dz * dz));
         }
+        // an internal node holds distinct points, so its radius is far above
+        // the range where the walk's square root would need rescaling
+        if (!r.leaf && !(radius > 1e-100)) {
+            msg = "degenerate kd-tree node: points " + std::to_string(r.l) + ".." + std::to_string(r.u) +
+                  " lie within 1e-100 of their centroid";
+            return false;
+        }
         r.orad = 1.0 / radius;
         r.rad = 1.0 / r.orad;
     }

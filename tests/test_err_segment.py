@@ -3,7 +3,8 @@ the reference's linear search over rstar (getError, kd_tree.F90:882-906,
 radii :34-35): first i with dOvrR < rstar(i+1), the last entry beyond 640.
 
 The header is compiled for the host with g++ and evaluated on every radius,
-its floating-point neighbours and a log-uniform sample of 200,000 values."""
+its floating-point neighbours and a log-uniform sample of 200,000 values,
+directly and through the walk's table of binade eighths (err_cell)."""
 import os
 import shutil
 import subprocess
@@ -20,7 +21,8 @@ PROGRAM = r"""
 #include "idw_err_segment.h"
 int main() {
     double d;
-    while (std::scanf("%la", &d) == 1) std::printf("%d\n", idw::err_segment(d));
+    while (std::scanf("%la", &d) == 1)
+        std::printf("%d %d\n", idw::err_segment(d), idw::err_segment(idw::err_cell_low(idw::err_cell(d))));
     return 0;
 }
 """
@@ -48,7 +50,10 @@ def test_err_segment_matches_linear_search(tmp_path):
     vals = [v for v in vals if v >= 2.0]
     out = subprocess.run([str(exe)], input="\n".join(float(v).hex() for v in vals), capture_output=True, text=True,
                          check=True).stdout.split()
-    got = np.array(out, dtype=int)
+    got = np.array(out, dtype=int).reshape(-1, 2)
     want = np.array([linear_search(v) for v in vals])
-    bad = np.nonzero(got != want)[0]
-    assert bad.size == 0, [(vals[i], got[i], want[i]) for i in bad[:5]]
+    bad = np.nonzero(got[:, 0] != want)[0]
+    assert bad.size == 0, [(vals[i], got[i, 0], want[i]) for i in bad[:5]]
+    # the walk's table: the segment of a value's cell, taken at the cell's lowest value
+    bad = np.nonzero(got[:, 1] != want)[0]
+    assert bad.size == 0, [(vals[i], got[i, 1], want[i]) for i in bad[:5]]
