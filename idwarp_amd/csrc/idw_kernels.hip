@@ -698,6 +698,11 @@ __device__ __forceinline__ double get_error(const NodeRec *nd, double dOvrR, con
 
 __device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
 
+// Wait for every outstanding vector-memory operation (vmcnt(0)).  The
+// counter retires in order, so a wait for a load issued after stores or
+// atomics also waits for those: the kernels place it where that costs least.
+__device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // LDS written by one lane and read by another lane of the same wave: the
 // wave's LDS operations execute in order, the barrier keeps the compiler
 // from moving them.
@@ -707,14 +712,14 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-constexpr int WBUF = 128;  // FP32 plan entries a wave gathers in LDS before writing them out
+constexpr int WBUF = 128;  // plan entries of each kind a wave gathers in LDS before writing them out
 
 // Wave-level record of the walk: one entry per term (accepted cluster, or
 // point of an evaluated bucket) with the mask of the lanes it serves, in
 // depth-first order.  The wave owns slots [off, end) sized by the counting
 // pass; FP32 terms fill them from the front, FP64 terms (mixed precision
-// near the wall) from the back, and the split is stored per wave.  FP32
-// entries are gathered in LDS and written out WBUF at a time, so the walk's
+// near the wall) from the back, and the split is stored per wave.  Entries of
+// both kinds are gathered in LDS and written out WBUF at a time, so the walk's
 // vector-memory queue (which retires in order) is rarely held up by stores
 // in front of its error-table loads.  Writes outside the range are dropped
 // and a count that disagrees with the counting pass is reported in
@@ -723,8 +728,8 @@ struct PlanWriter {
     PlanEntry *plan;
     int64_t off, end, nf = 0, nb = 0;
     unsigned int *mismatch;
-    PlanEntry *buf;  // this wave's LDS buffer (WBUF entries)
-    int cf = 0;      // entries in buf (wave-uniform)
+    PlanEntry *buf, *buf64;  // this wave's LDS buffers (WBUF entries each)
+    int cf = 0, cb = 0;      // entries in buf, buf64 (wave-uniform)
     __device__ __forceinline__ void flush() {
         const int lane = threadIdx.x & 63;
         wave_sync();
@@ -736,11 +741,23 @@ struct PlanWriter {
         cf = 0;
         wave_sync();
     }
+    __device__ __forceinline__ void flush64() {
+        const int lane = threadIdx.x & 63;
+        wave_sync();
+        for (int j = lane; j < cb; j += 64) {
+            const int64_t at = end - 1 - (nb + j);
+            if (at >= off) plan[at] = buf64[j];
+        }
+        nb += cb;
+        cb = 0;
+        wave_sync();
+    }
     // terms base .. base+n-1 for the lanes of mask
     __device__ __forceinline__ void put(int base, int n, unsigned long long mask, bool fp64) {
         const int lane = threadIdx.x & 63;
-        if (fp64 || n > WBUF / 2) {
-            if (!fp64) flush();
+        if (n > WBUF / 2) {
+            if (fp64) flush64();
+            else flush();
             for (int i = lane; i < n; i += 64) {
                 const int64_t at = fp64 ? end - 1 - (nb + i) : off + nf + i;
                 if (at >= off && at < end) {
@@ -755,18 +772,23 @@ struct PlanWriter {
             else nf += n;
             return;
         }
-        if (cf + n > WBUF) flush();
+        int &c = fp64 ? cb : cf;
+        if (c + n > WBUF) {
+            if (fp64) flush64();
+            else flush();
+        }
         if (lane < n) {
             PlanEntry e;
             e.mask = mask;
             e.rec = base + lane;
             e.pad = 0;
-            buf[cf + lane] = e;
+            (fp64 ? buf64 : buf)[c + lane] = e;
         }
-        cf += n;
+        c += n;
     }
     __device__ __forceinline__ void finish(int64_t *split, int wave) {
         flush();
+        flush64();
         if ((threadIdx.x & 63) == 0) {
             split[wave] = off + nf;
             if (nf + nb != end - off) atomicAdd(mismatch, 1u);
@@ -798,7 +820,7 @@ __global__ void __launch_bounds__(256) k_plan_walk(int nvol, int blk0, int nbc, 
                                                    unsigned int *__restrict__ mismatch, Mirror M) {
 #pragma clang fp contract(off)
     __shared__ SegCell cells[ERR_CELLS];
-    __shared__ PlanEntry wbuf[COUNT ? 1 : 4][COUNT ? 1 : WBUF];
+    __shared__ PlanEntry wbuf[COUNT ? 1 : 4][COUNT ? 1 : WBUF], wbuf64[COUNT ? 1 : 4][COUNT ? 1 : WBUF];
     const int lb = COUNT ? (int)blockIdx.x : xcd_block(nbc);
     if (lb >= nbc) return;  // block-uniform
     const int blk = blk0 + lb;
@@ -812,8 +834,11 @@ __global__ void __launch_bounds__(256) k_plan_walk(int nvol, int blk0, int nbc, 
     const double tol = errTol * v.w;  // errTol * denominator0
     int stop = valid ? INT32_MAX : -1;  // depth at which this lane stopped descending
     const int wave = uniform(blk * 4 + (threadIdx.x >> 6));
+    // the node's coordinates are in before the loop, so that the loop need
+    // not wait for the plan's stores in flight to learn it
+    wait_vmem();
     PlanWriter pw{plan, COUNT ? 0 : waveOff[wave], COUNT ? 0 : waveOff[wave + 1], 0, 0, mismatch,
-                  wbuf[COUNT ? 0 : threadIdx.x >> 6]};
+                  wbuf[COUNT ? 0 : threadIdx.x >> 6], wbuf64[COUNT ? 0 : threadIdx.x >> 6]};
     int64_t cnt = 0;
     int node = 0;
     NodeHdr h = load_hdr(nodes, 0);
@@ -1000,11 +1025,6 @@ __device__ __forceinline__ PlanEntry plan_entry(const PlanEntry *__restrict__ pl
     }
     return pe;
 }
-
-// Wait for every outstanding vector-memory operation.  Placed after a
-// prefetch and before atomics: the counter retires in order, so waiting
-// for a load issued after atomics would also wait for the atomics.
-__device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }  // vmcnt(0)
 
 // FP32 weight of one pair from |r|^2 (mixed precision).  The default
 // exponents (3, 5) use w = A L^3 p^3 (1 + alpha^5 L^2 p^2) with p = 1/|r|,
