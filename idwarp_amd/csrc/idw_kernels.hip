@@ -1001,8 +1001,13 @@ __device__ __forceinline__ double first_lane(double x) {
 constexpr int FCH = 32;   // terms staged per chunk in the forward
 constexpr int BROW = 17;  // padded LDS row (doubles) of a staged B row
 
+constexpr int BROWF = 17;  // padded LDS row (floats) of a staged FP32 B row
+
 struct FwdChunk {
-    double B[FCH][BROW];  // K (9), c (3), 1, 0, 0, 0
+    union {
+        double B[FCH][BROW];    // K (9), c (3), 1, 0, 0, 0
+        float Bf[FCH][BROWF];   // the same in FP32 (mixed precision, FP32 terms)
+    };
     double X[FCH][4];     // x (3), A
     double eps[FCH];      // added to |r|^2: EPS_W for points (evalNodeExact), 0 for clusters (evalNodeApprox)
     unsigned long long mask[FCH];
@@ -1065,14 +1070,27 @@ __device__ __forceinline__ void stage_terms(FwdChunk &C, PlanEntry pe, int nnode
                 const Rec64 &q = rec[pe.rec];
                 K0 = q.K[3 * a]; K1 = q.K[3 * a + 1]; K2 = q.K[3 * a + 2]; d = q.d[a];
             }
-            C.B[r][3 * a] = K0;
-            C.B[r][3 * a + 1] = K1;
-            C.B[r][3 * a + 2] = K2;
-            C.B[r][9 + a] = d - (K0 * y0 + K1 * y1 + K2 * y2);
+            const double c = d - (K0 * y0 + K1 * y1 + K2 * y2);
+            if (W32) {
+                C.Bf[r][3 * a] = (float)K0;
+                C.Bf[r][3 * a + 1] = (float)K1;
+                C.Bf[r][3 * a + 2] = (float)K2;
+                C.Bf[r][9 + a] = (float)c;
+            } else {
+                C.B[r][3 * a] = K0;
+                C.B[r][3 * a + 1] = K1;
+                C.B[r][3 * a + 2] = K2;
+                C.B[r][9 + a] = c;
+            }
         }
         if (hi) {
-            C.B[r][12] = 1.0;
-            C.B[r][13] = C.B[r][14] = C.B[r][15] = 0.0;
+            if (W32) {
+                C.Bf[r][12] = 1.0f;
+                C.Bf[r][13] = C.Bf[r][14] = C.Bf[r][15] = 0.0f;
+            } else {
+                C.B[r][12] = 1.0;
+                C.B[r][13] = C.B[r][14] = C.B[r][15] = 0.0;
+            }
         } else {
             for (int i = 0; i < 3; ++i) C.X[r][i] = x[i];
             C.X[r][3] = W32 ? (double)rec32[pe.rec].A : rec[pe.rec].A;
@@ -1080,7 +1098,10 @@ __device__ __forceinline__ void stage_terms(FwdChunk &C, PlanEntry pe, int nnode
             C.mask[r] = pe.mask;
         }
     } else {
-        for (int j = hi ? 8 : 0; j < (hi ? 16 : 8); ++j) C.B[r][j] = 0.0;
+        for (int j = hi ? 8 : 0; j < (hi ? 16 : 8); ++j) {
+            if (W32) C.Bf[r][j] = 0.0f;
+            else C.B[r][j] = 0.0;
+        }
         if (!hi) {
             for (int i = 0; i < 4; ++i) C.X[r][i] = 0.0;
             C.eps[r] = 1.0;
@@ -1089,9 +1110,42 @@ __device__ __forceinline__ void stage_terms(FwdChunk &C, PlanEntry pe, int nnode
     }
 }
 
+typedef float fvec4 __attribute__((ext_vector_type(4)));
+
+// Swap helpers over the lane rows (16 lanes each) of two registers:
+// rows 2-3 of x with rows 0-1 of y, and odd rows of x with even rows of y.
+__device__ __forceinline__ void swap_rows32(float &x, float &y) {
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(y), false, false);
+    x = __uint_as_float(r[0]);
+    y = __uint_as_float(r[1]);
+}
+__device__ __forceinline__ void swap_rows16(float &x, float &y) {
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(y), false, false);
+    x = __uint_as_float(r[0]);
+    y = __uint_as_float(r[1]);
+}
+
+// Add FP32 sums in the f32 16x16x4 layout (register j of lane l: row
+// 4(l/16) + j, column l%16) to FP64 sums in the f64 layout (register j:
+// row 4j + l/16): the 4 x 4 blocks over (lane row, register) are transposed
+// with two rounds of row swaps, then converted and added.
+__device__ __forceinline__ void add_f32_sums(dvec4 &D, fvec4 F) {
+    float f0 = F[0], f1 = F[1], f2 = F[2], f3 = F[3];
+    swap_rows32(f0, f2);
+    swap_rows32(f1, f3);
+    swap_rows16(f0, f1);
+    swap_rows16(f2, f3);
+    D[0] += (double)f0;
+    D[1] += (double)f1;
+    D[2] += (double)f2;
+    D[3] += (double)f3;
+}
+
 // Forward sums of the terms in plan entries [a, b) into D (node block t in
 // D[t]); vx/vy/vz: positions of the lane's rows (nodes 16t + lane%16).
 // The entries of the next chunk are requested while this one is summed.
+// W32 (mixed precision, FP32 terms): FP32 weights and B rows on
+// v_mfma_f32_16x16x4f32, each chunk's FP32 sums added to the FP64 sums.
 template <bool W32, int EXPM>
 __device__ __forceinline__ void fwd_terms(int64_t a, int64_t b, const PlanEntry *__restrict__ plan, int nnodes,
                                           int nrec, const Rec64 *__restrict__ rec, const Rec32 *__restrict__ rec32,
@@ -1110,22 +1164,34 @@ __device__ __forceinline__ void fwd_terms(int64_t a, int64_t b, const PlanEntry 
         pe = plan_entry(plan, c0 + FCH + r, b);  // next chunk
         wave_sync();
         const int ng = (cnt + 3) >> 2;
+        if (W32) {
+            fvec4 F[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) F[t] = fvec4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll 1
-        for (int q = 0; q < ng; ++q) {
-            const int tr = 4 * q + R;
-            const double Bv = C.B[tr][n];
-            const unsigned long long m = C.mask[tr] >> n;
-            const double x0 = C.X[tr][0], x1 = C.X[tr][1], x2 = C.X[tr][2], A = C.X[tr][3];
-            if (W32) {
+            for (int q = 0; q < ng; ++q) {
+                const int tr = 4 * q + R;
+                const float Bv = C.Bf[tr][n];
+                const unsigned long long m = C.mask[tr] >> n;
+                const double x0 = C.X[tr][0], x1 = C.X[tr][1], x2 = C.X[tr][2], A = C.X[tr][3];
                 const float Af = (float)A, AL3 = (float)(A * L3);
 #pragma unroll
                 for (int t = 0; t < 4; ++t) {
                     const float fx = (float)(vx[t] - x0), fy = (float)(vy[t] - x1), fz = (float)(vz[t] - x2);
                     const float w = weight_f32<EXPM>(fx * fx + fy * fy + fz * fz, Af, AL3, Lf, a5L2, P);
                     const bool on = (m >> (16 * t)) & 1ull;
-                    D[t] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)(on ? w : 0.0f), Bv, D[t], 0, 0, 0);
+                    F[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(on ? w : 0.0f, Bv, F[t], 0, 0, 0);
                 }
-            } else {
+            }
+#pragma unroll
+            for (int t = 0; t < 4; ++t) add_f32_sums(D[t], F[t]);
+        } else {
+#pragma unroll 1
+            for (int q = 0; q < ng; ++q) {
+                const int tr = 4 * q + R;
+                const double Bv = C.B[tr][n];
+                const unsigned long long m = C.mask[tr] >> n;
+                const double x0 = C.X[tr][0], x1 = C.X[tr][1], x2 = C.X[tr][2], A = C.X[tr][3];
                 const double eps = C.eps[tr];
 #pragma unroll
                 for (int t = 0; t < 4; ++t) {
