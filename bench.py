@@ -25,6 +25,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector (= FP64 matrix) peak
+FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32 matrix peak
 
 # algorithmic flops per unit of fast-mode work (FMA = 2, sqrt/div = 1), per
 # (V internal visit, A accepted cluster, L bucket point pair, E error check):
@@ -221,20 +222,26 @@ def main():
     # reverse, one launch per step, bracketed by HIP events on the context
     # stream (phase 4).
     ph = head["phases"]
-    # kernel names as rocprofv3 reports them: <precision, exponent mode>,
-    # exponent mode 0 = the default exponents (3, 5) compiled as products
-    pi = 1 if head_prec == "mixed" else 0
-    kname, kms, kfl = "idw::k_fast_rev<%d, 0>" % pi, ph[4], kflops["rev"]
+    # kernel names as rocprofv3 reports them; the template argument 0 is the
+    # exponent mode of the default exponents (3, 5), compiled as products
+    if head_prec == "mixed":
+        kname, peak = "idw::k_fast_rev_pair<0>", FP32_PEAK_TFLOPS
+        compute = ("FP32 terms (the bulk): FP32 weights on VALU and the sums over the 128 volume nodes of a pair of "
+                   "walk waves on v_mfma_f32_16x16x4f32, each term shared by the pair added once; FP64 terms near "
+                   "the wall: FP64 weights and v_mfma_f64_16x16x4f64 per wave.  Priced against the 157.3 TFLOP/s "
+                   "dense FP32 roof")
+    else:
+        kname, peak = "idw::k_fast_rev<0, 0>", FP64_PEAK_TFLOPS
+        compute = ("FP64 weights on VALU, the sums over each wave's 64 volume nodes on v_mfma_f64_16x16x4f64; the "
+                   "78.6 TFLOP/s FP64 vector roof equals the FP64 matrix roof")
+    kms, kfl = ph[4], kflops["rev"]
     achieved = kfl / (kms * 1e-3) / 1e12
-    res["roofline"] = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                       "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+    res["roofline"] = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
+                       "frac": round(achieved / peak, 4),
                        "traffic": _traffic_record(kname)["bytes"] if _traffic_record(kname) else None,
                        "kernel": kname, "kernel_ms": round(float(kms), 3),
                        "algorithmic_flops_per_launch": kfl,
-                       "compute": "weights on VALU (FP32 far from the wall, FP64 near it), the sums over each wave's "
-                                  "64 volume nodes on FP64 MFMA (v_mfma_f64_16x16x4f64); the 78.6 TFLOP/s FP64 "
-                                  "vector roof equals the FP64 matrix roof and non-packed FP32 issues at the same "
-                                  "rate"}
+                       "compute": compute}
     tr = _traffic_record(kname)
     if tr:
         res["roofline"]["traffic_detail"] = {"fetch_bytes": tr["fetch_bytes"], "write_bytes": tr["write_bytes"],

@@ -64,8 +64,8 @@ struct DevSurface {
 // of an evaluated bucket) and the lanes of the wave it serves.
 struct alignas(16) PlanEntry {
     unsigned long long mask;
-    int32_t rec;  // index into the term records (DevSurface::rec)
-    int32_t pad;
+    int32_t rec;   // index into the term records (DevSurface::rec)
+    int32_t node;  // tree node where the walk found the term (the cluster, or the point's bucket)
 };
 
 // Per-wave plan of the fast mode (valid for one volume ordering, tree and errTol).

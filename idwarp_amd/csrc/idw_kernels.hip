@@ -752,8 +752,8 @@ struct PlanWriter {
         cb = 0;
         wave_sync();
     }
-    // terms base .. base+n-1 for the lanes of mask
-    __device__ __forceinline__ void put(int base, int n, unsigned long long mask, bool fp64) {
+    // terms base .. base+n-1 for the lanes of mask, found at tree node node
+    __device__ __forceinline__ void put(int base, int n, unsigned long long mask, bool fp64, int node) {
         const int lane = threadIdx.x & 63;
         if (n > WBUF / 2) {
             if (fp64) flush64();
@@ -764,7 +764,7 @@ struct PlanWriter {
                     PlanEntry e;
                     e.mask = mask;
                     e.rec = base + i;
-                    e.pad = 0;
+                    e.node = node;
                     plan[at] = e;
                 }
             }
@@ -781,7 +781,7 @@ struct PlanWriter {
             PlanEntry e;
             e.mask = mask;
             e.rec = base + lane;
-            e.pad = 0;
+            e.node = node;
             (fp64 ? buf64 : buf)[c + lane] = e;
         }
         c += n;
@@ -858,7 +858,7 @@ __global__ void __launch_bounds__(256) k_plan_walk(int nvol, int blk0, int nbc, 
                     const double rn = h.rad + nearT;
                     near = __any(active && cx * cx + cy * cy + cz * cz < rn * rn);
                 }
-                pw.put(nnodes + h.l, h.u - h.l + 1, __ballot(active), near);
+                pw.put(nnodes + h.l, h.u - h.l + 1, __ballot(active), near, node);
             }
             next = h.skip;
         } else {
@@ -884,7 +884,7 @@ __global__ void __launch_bounds__(256) k_plan_walk(int nvol, int blk0, int nbc, 
             const unsigned long long acc = __ballot(accept);
             if (acc) {
                 if (COUNT) ++cnt;
-                else pw.put(node, 1, acc, false);
+                else pw.put(node, 1, acc, false, node);
             }
             next = __any(descend) ? node + 1 : h.skip;
         }
@@ -1027,6 +1027,7 @@ __device__ __forceinline__ PlanEntry plan_entry(const PlanEntry *__restrict__ pl
     if (e >= end) {
         pe.mask = 0ull;
         pe.rec = -1;
+        pe.node = -1;
     }
     return pe;
 }
@@ -1302,7 +1303,8 @@ struct RevLds {
 // registers before its atomics are issued.
 template <bool W32, int EXPM>
 __device__ __forceinline__ void rev_terms(int64_t a, int64_t b, const PlanEntry *__restrict__ plan, int nnodes,
-                                          int nrec, const Rec64 *__restrict__ rec, const WParams &P, RevLds &L,
+                                          int nrec, const Rec64 *__restrict__ rec, const WParams &P,
+                                          const double4 *nodes, double (*Y)[4], int32_t *idxs,
                                           const double (&B)[16], double ox, double oy, double oz,
                                           double *__restrict__ adj) {
     const int lane = threadIdx.x & 63, m = lane & 15, R = lane >> 4;
@@ -1324,17 +1326,17 @@ __device__ __forceinline__ void rev_terms(int64_t a, int64_t b, const PlanEntry 
         const double eps = (idx >= 0 && idx < nnodes) ? 0.0 : EPS_W;
         wave_sync();  // the previous tile's reads of Y / idx are done
         if (R == 0) {
-            L.Y[m][0] = xa.x - ox;
-            L.Y[m][1] = xa.y - oy;
-            L.Y[m][2] = xa.z - oz;
-            L.idx[m] = idx;
+            Y[m][0] = xa.x - ox;
+            Y[m][1] = xa.y - oy;
+            Y[m][2] = xa.z - oz;
+            idxs[m] = idx;
         }
         const unsigned long long mR = pe.mask >> R;
         const float Af = (float)xa.w, AL3 = (float)(xa.w * L3);
         dvec4 D0 = {0.0, 0.0, 0.0, 0.0}, D1 = D0;
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
-            const double4 c = L.node[4 * q + R];
+            const double4 c = nodes[4 * q + R];
             const bool on = (mR >> (4 * q)) & 1ull;
             double w;
             if (W32) {
@@ -1357,8 +1359,8 @@ __device__ __forceinline__ void rev_terms(int64_t a, int64_t b, const PlanEntry 
             const int tr = 4 * t + R;
             double val = Ds[t];
             const double pb = __shfl(val, src, 64);
-            if (m < 9) val -= pb * L.Y[tr][m % 3];
-            const int id = L.idx[tr];
+            if (m < 9) val -= pb * Y[tr][m % 3];
+            const int id = idxs[tr];
             if (id >= 0 && m < 12 && val != 0.0) atomicAdd(adj + (int64_t)id * ADJ_STRIDE + m, val);
         }
         pe = pn;
@@ -1424,10 +1426,319 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     const int64_t e0 = waveOff[wave], e1 = waveOff[wave + 1];
     if (PREC == IDW_PREC_MIXED) {
         const int64_t es = min(max(waveSplit[wave], e0), e1);
-        rev_terms<true, EXPM>(e0, es, plan, nnodes, nrec, rec, P, L, B, ox, oy, oz, adj);
-        rev_terms<false, EXPM>(es, e1, plan, nnodes, nrec, rec, P, L, B, ox, oy, oz, adj);
+        rev_terms<true, EXPM>(e0, es, plan, nnodes, nrec, rec, P, L.node, L.Y, L.idx, B, ox, oy, oz, adj);
+        rev_terms<false, EXPM>(es, e1, plan, nnodes, nrec, rec, P, L.node, L.Y, L.idx, B, ox, oy, oz, adj);
     } else {
-        rev_terms<false, EXPM>(e0, e1, plan, nnodes, nrec, rec, P, L, B, ox, oy, oz, adj);
+        rev_terms<false, EXPM>(e0, e1, plan, nnodes, nrec, rec, P, L.node, L.Y, L.idx, B, ox, oy, oz, adj);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Reverse of a pair of walk waves (mixed precision)
+// ---------------------------------------------------------------------------
+// Neighbouring waves share most of their terms (at C3 a record appears in
+// 1.71 of two neighbouring waves' plans on average), and the reverse is held
+// back by its atomics.  So one wave takes the 128 volume nodes of walk waves A
+// = 2p and B = 2p + 1: their FP32 plan ranges, both in depth-first order, are
+// merged on the fly, and a term in both plans is summed over all 128 nodes
+// and added to its adjoint record once.  Terms of one plan only are summed
+// over that wave's 64 nodes, so no node is weighted against a term its wave
+// does not use.  FP32 weights and G on v_mfma_f32_16x16x4f32; the FP64 terms
+// (near the wall) take the per-wave path with FP64 sums.
+constexpr int MQ = 32;              // plan entries of each wave in the merge buffers
+constexpr int QCAP = RT + MQ;      // a queue holds < RT items before a merge round adds at most MQ to it
+
+struct PairMergeLds {
+    unsigned long long keyA[MQ], keyB[MQ], mskA[MQ], mskB[MQ];
+    // union terms by kind: 0 in both plans, 1 in A's only, 2 in B's only
+    int32_t qrec[3][QCAP];
+    unsigned long long qmA[2][QCAP];  // A masks of kinds 0 and 1
+    unsigned long long qmB[2][QCAP];  // B masks of kinds 0 and 2
+};
+
+struct PairLds {
+    double4 node[128];  // mirrored positions: wave A's nodes, then wave B's
+    union {
+        PairMergeLds q;      // during the merge
+        double s[128][3];    // seeds, while the B operands are built
+    };
+    double Y[RT][4];  // x - o of the tile's terms
+    int32_t idx[RT];  // their records (-1: padding)
+};
+
+// Depth-first position of a plan entry: its tree node, then its record (a
+// bucket's points in order).  Strictly increasing along a wave's FP32 range.
+__device__ __forceinline__ unsigned long long plan_key(const PlanEntry &e) {
+    return ((unsigned long long)(unsigned)e.node << 32) | (unsigned)e.rec;
+}
+
+// Number of the first n keys of a sorted LDS array below k.
+__device__ __forceinline__ int count_below(const unsigned long long *a, int n, unsigned long long k) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a[mid] < k) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// A tile of up to RT union terms of one kind (per lane m = lane % 16: term m).
+struct PairTile {
+    int kind, rec;
+    unsigned long long mA, mB;
+    double4 xa;  // x, A of the record (zero for padding)
+};
+
+template <int EXPM>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+k_fast_rev_pair(int nvol, int nwave, const double4 *__restrict__ vol, const int32_t *__restrict__ perm,
+                const double *__restrict__ dxv, const double *__restrict__ den, const PlanEntry *__restrict__ plan,
+                const int64_t *__restrict__ waveOff, const int64_t *__restrict__ waveSplit, int nnodes, int nrec,
+                const Rec64 *__restrict__ rec, WParams P, Mirror M, double *__restrict__ adj) {
+    __shared__ PairLds lds[4];
+    const int npair = (nwave + 1) >> 1;
+    const int nblk = (npair + 3) >> 2;
+    const int blk = xcd_block(nblk);
+    if (blk >= nblk) return;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, m = lane & 15, R = lane >> 4;
+    const int pair = uniform(blk * 4 + wv);
+    if (pair >= npair) return;  // wave-uniform
+    PairLds &L = lds[wv];
+    // the pair's nodes: lane j holds node j of wave A and node j of wave B
+    auto node_seed = [&](int k, double4 &v, double (&sd)[3]) {  // mirrored position and seed s = dXv / den
+        const bool valid = k < nvol;
+        v = valid ? vol[k] : make_double4(0.0, 0.0, 0.0, 1.0);
+        sd[0] = sd[1] = sd[2] = 0.0;
+        if (valid) {
+            const int64_t j = perm[k];
+            const double oden = 1.0 / den[k];
+            sd[0] = dxv[3 * j] * oden;
+            sd[1] = dxv[3 * j + 1] * oden;
+            sd[2] = dxv[3 * j + 2] * oden;
+        }
+        if (M.mask) {
+            mirror_pt(M, v.x, v.y, v.z);
+            mirror_vec_b(M, sd[0], sd[1], sd[2]);
+        }
+        v.w = 0.0;
+    };
+    double vx0;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        double4 v;
+        double sd[3];
+        node_seed(pair * 128 + 64 * h + lane, v, sd);
+        if (h == 0) vx0 = v.x;
+        L.node[64 * h + lane] = v;
+        for (int c = 0; c < 3; ++c) L.s[64 * h + lane][c] = sd[c];
+    }
+    // pair origin: lane 0's node of wave A (valid whenever any node of the pair is)
+    const double ox = first_lane(vx0), oy = first_lane(L.node[lane].y), oz = first_lane(L.node[lane].z);
+    wave_sync();
+    // G(node nd, column m) = s_a g_b (9), s_a (3), 0 with g = v - o, from LDS
+    const int sa = m < 9 ? m / 3 : (m < 12 ? m - 9 : 0), gb = m % 3;
+    const double og = gb == 0 ? ox : (gb == 1 ? oy : oz);
+    auto g_entry = [&](int nd) {
+        const double sv = L.s[nd][sa];
+        const double gv = reinterpret_cast<const double *>(&L.node[nd])[gb] - og;
+        return m < 9 ? sv * gv : (m < 12 ? sv : 0.0);
+    };
+    float GA[16], GB[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        GA[q] = (float)g_entry(4 * q + R);
+        GB[q] = (float)g_entry(64 + 4 * q + R);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    const float Lf = (float)P.Ldef, a5L2 = (float)(P.a5 * P.Ldef * P.Ldef);
+    const double L3 = P.Ldef * P.Ldef * P.Ldef;
+    const int src9 = (lane & ~15) | (9 + (m < 9 ? m / 3 : 0));
+    const int wA = 2 * pair, wB = min(2 * pair + 1, nwave);  // wave B == nwave: an empty range
+    const int64_t a0 = waveOff[wA], a1 = waveOff[wA + 1];
+    const int64_t b0 = wB < nwave ? waveOff[wB] : 0, b1 = wB < nwave ? waveOff[wB + 1] : 0;
+    const int64_t asp = min(max(waveSplit[wA], a0), a1);
+    const int64_t bsp = wB < nwave ? min(max(waveSplit[wB], b0), b1) : 0;
+    // merge state (wave-uniform): next unread entry and end of A's and B's FP32
+    // ranges, entries in the merge buffers, items in the three queues
+    int64_t ia = a0, ib = b0;
+    const int64_t ea = asp, eb = bsp;
+    int na = 0, nb = 0, q0 = 0, q1 = 0, q2 = 0;
+    auto record = [&](int r) {  // x, A of a record (zero for padding); unconditional load
+        const double4 v = *reinterpret_cast<const double4 *>(rec + min(max(r, 0), nrec - 1));
+        return r >= 0 ? v : make_double4(0.0, 0.0, 0.0, 0.0);
+    };
+    // Each pass: find the next tile (a full queue, or at the end what is left,
+    // merging as needed) and request its records; then add the previous
+    // tile's sums to the adjoint records; then sum the new tile.
+    PairTile T;
+    T.kind = -1;
+    fvec4 F0 = {0.0f, 0.0f, 0.0f, 0.0f}, F1 = F0;
+    for (;;) {
+        PairTile Tn;
+        Tn.kind = -1;
+        for (;;) {
+            int kind = q0 >= RT ? 0 : (q1 >= RT ? 1 : (q2 >= RT ? 2 : -1));
+            const bool done = ia >= ea && ib >= eb && na == 0 && nb == 0;
+            if (kind < 0 && done) kind = q0 > 0 ? 0 : (q1 > 0 ? 1 : (q2 > 0 ? 2 : -1));
+            if (kind >= 0) {
+                const int qk = kind == 0 ? q0 : (kind == 1 ? q1 : q2);
+                const int cnt = min(qk, RT);
+                const int base = qk - cnt;
+                const bool ok = m < cnt;
+                Tn.kind = kind;
+                Tn.rec = ok ? L.q.qrec[kind][base + m] : -1;
+                Tn.mA = (ok && kind != 2) ? L.q.qmA[kind][base + m] : 0ull;
+                Tn.mB = (ok && kind != 1) ? L.q.qmB[kind == 0 ? 0 : 1][base + m] : 0ull;
+                if (kind == 0) q0 = uniform(base);
+                else if (kind == 1) q1 = uniform(base);
+                else q2 = uniform(base);
+                Tn.xa = record(Tn.rec);
+                break;
+            }
+            if (done) break;
+            // One merge round: top up both buffers to MQ entries, then move every entry
+            // whose key is at most K = min(last key loaded of each plan not yet read to its
+            // end) into the queues.  No entry past K can precede or equal a key still
+            // unread, and at least one buffer empties, so the merge ends.  Lanes 0..31
+            // hold A's buffer, lanes 32..63 B's.
+            const bool isA = lane < MQ;
+            const int i = isA ? lane : lane - MQ;
+            const int la = (int)min((int64_t)(MQ - na), ea - ia);
+            const int lb = (int)min((int64_t)(MQ - nb), eb - ib);
+            wave_sync();
+            if (isA ? i < la : i < lb) {
+                const PlanEntry e = plan[isA ? ia + i : ib + i];
+                const int at = (isA ? na : nb) + i;
+                if (isA) {
+                    L.q.keyA[at] = plan_key(e);
+                    L.q.mskA[at] = e.mask;
+                } else {
+                    L.q.keyB[at] = plan_key(e);
+                    L.q.mskB[at] = e.mask;
+                }
+            }
+            ia += la;
+            na = uniform(na + la);
+            ib += lb;
+            nb = uniform(nb + lb);
+            wave_sync();
+            const unsigned long long KA = ia < ea ? L.q.keyA[na - 1] : ~0ull;
+            const unsigned long long KB = ib < eb ? L.q.keyB[nb - 1] : ~0ull;
+            const unsigned long long K = KA < KB ? KA : KB;
+            const int nown = isA ? na : nb, noth = isA ? nb : na;
+            const unsigned long long *other = isA ? L.q.keyB : L.q.keyA;
+            const bool have = i < nown;
+            const unsigned long long k = have ? (isA ? L.q.keyA[i] : L.q.keyB[i]) : 0ull;
+            const unsigned long long msk = have ? (isA ? L.q.mskA[i] : L.q.mskB[i]) : 0ull;
+            const int c = have ? count_below(other, noth, k) : 0;
+            const bool twin = have && c < noth && other[c] == k;
+            const bool emit = have && k <= K;
+            const unsigned long long m0 = __ballot(emit && isA && twin);
+            const unsigned long long m1 = __ballot(emit && isA && !twin);
+            const unsigned long long m2 = __ballot(emit && !isA && !twin);
+            const unsigned long long below = (1ull << lane) - 1ull;
+            if (emit && (isA || !twin)) {
+                const int kind = isA ? (twin ? 0 : 1) : 2;
+                const unsigned long long mk = kind == 0 ? m0 : (kind == 1 ? m1 : m2);
+                const int slot = (kind == 0 ? q0 : (kind == 1 ? q1 : q2)) + __popcll(mk & below);
+                L.q.qrec[kind][slot] = (int32_t)(unsigned)k;
+                if (kind == 0) {
+                    L.q.qmA[0][slot] = msk;
+                    L.q.qmB[0][slot] = L.q.mskB[c];  // the twin in B's buffer
+                } else if (kind == 1) {
+                    L.q.qmA[1][slot] = msk;
+                } else {
+                    L.q.qmB[1][slot] = msk;
+                }
+            }
+            q0 = uniform(q0 + __popcll(m0));
+            q1 = uniform(q1 + __popcll(m1));
+            q2 = uniform(q2 + __popcll(m2));
+            // the emitted entries are a prefix of each buffer: move the rest to the front
+            const int nae = __popcll(__ballot(emit && isA)), nbe = __popcll(__ballot(emit && !isA));
+            const int e_own = isA ? nae : nbe;
+            const bool mv = i + e_own < nown;
+            const unsigned long long k2 = mv ? (isA ? L.q.keyA[i + e_own] : L.q.keyB[i + e_own]) : 0ull;
+            const unsigned long long m2v = mv ? (isA ? L.q.mskA[i + e_own] : L.q.mskB[i + e_own]) : 0ull;
+            wave_sync();
+            if (mv) {
+                if (isA) {
+                    L.q.keyA[i] = k2;
+                    L.q.mskA[i] = m2v;
+                } else {
+                    L.q.keyB[i] = k2;
+                    L.q.mskB[i] = m2v;
+                }
+            }
+            na = uniform(na - nae);
+            nb = uniform(nb - nbe);
+        }
+        if (T.kind >= 0) {
+            wait_vmem();  // the next tile's records are in before this tile's atomics go out
+            wave_sync();  // Y / idx of this tile are written
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int tr = 4 * R + t;  // f32 layout: register t holds term row 4 (l/16) + t
+                double val = (double)F0[t] + (double)F1[t];
+                const double pb = __shfl(val, src9, 64);
+                if (m < 9) val -= pb * L.Y[tr][m % 3];
+                const int id = L.idx[tr];
+                if (id >= 0 && m < 12 && val != 0.0) atomicAdd(adj + (int64_t)id * ADJ_STRIDE + m, val);
+            }
+        }
+        if (Tn.kind < 0) break;
+        T = Tn;
+        const int idx = T.rec >= 0 ? min(T.rec, nrec - 1) : -1;
+        wave_sync();  // the previous tile's reads of Y / idx are done
+        if (R == 0) {
+            L.Y[m][0] = T.xa.x - ox;
+            L.Y[m][1] = T.xa.y - oy;
+            L.Y[m][2] = T.xa.z - oz;
+            L.idx[m] = idx;
+        }
+        const float Af = (float)T.xa.w, AL3 = (float)(T.xa.w * L3);
+        F0 = fvec4{0.0f, 0.0f, 0.0f, 0.0f};
+        F1 = F0;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            if (h == 0 ? T.kind == 2 : T.kind == 1) continue;  // wave-uniform: the half has no lanes
+            const unsigned long long mR = (h == 0 ? T.mA : T.mB) >> R;
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const double4 c = L.node[64 * h + 4 * q + R];
+                const bool on = (mR >> (4 * q)) & 1ull;
+                const float fx = (float)(c.x - T.xa.x), fy = (float)(c.y - T.xa.y), fz = (float)(c.z - T.xa.z);
+                const float wf = weight_f32<EXPM>(fx * fx + fy * fy + fz * fz, Af, AL3, Lf, a5L2, P);
+                const float w = on ? wf : 0.0f;
+                const float g = h == 0 ? GA[q] : GB[q];
+                if (q & 1) F1 = __builtin_amdgcn_mfma_f32_16x16x4f32(w, g, F1, 0, 0, 0);
+                else F0 = __builtin_amdgcn_mfma_f32_16x16x4f32(w, g, F0, 0, 0, 0);
+            }
+        }
+    }
+    // the FP64 terms, per wave, with FP64 G (the merge buffers are free again)
+    if (asp >= a1 && bsp >= b1) return;
+    wave_sync();
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        double4 v;
+        double sd[3];
+        node_seed(pair * 128 + 64 * h + lane, v, sd);
+        for (int c = 0; c < 3; ++c) L.s[64 * h + lane][c] = sd[c];
+    }
+    wave_sync();
+#pragma unroll 1
+    for (int h = 0; h < 2; ++h) {
+        const int64_t e0 = h == 0 ? asp : bsp, e1 = h == 0 ? a1 : b1;
+        if (e0 >= e1) continue;
+        double B[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            B[q] = g_entry(64 * h + 4 * q + R);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        rev_terms<false, EXPM>(e0, e1, plan, nnodes, nrec, rec, P, L.node + 64 * h, L.Y, L.idx, B, ox, oy, oz, adj);
     }
 }
 
@@ -1990,9 +2301,9 @@ static void reverse_impl(hipStream_t st, const DevSurface &S, const DevVolume &V
                 (int)V.n, V.vol, V.perm, dxv, V.den, plan->entries, plan->waveOff, plan->waveSplit, S.nnodes,
                 S.nnodes + S.n, S.rec, P, M, S.adj);
         else
-            k_fast_rev<IDW_PREC_MIXED, EXPM><<<g, 256, 0, st>>>(
-                (int)V.n, V.vol, V.perm, dxv, V.den, plan->entries, plan->waveOff, plan->waveSplit, S.nnodes,
-                S.nnodes + S.n, S.rec, P, M, S.adj);
+            k_fast_rev_pair<EXPM><<<xcd_blocks(((plan->nwave + 1) / 2 + 3) / 4), 256, 0, st>>>(
+                (int)V.n, (int)plan->nwave, V.vol, V.perm, dxv, V.den, plan->entries, plan->waveOff, plan->waveSplit,
+                S.nnodes, S.nnodes + S.n, S.rec, P, M, S.adj);
     }
 }
 
