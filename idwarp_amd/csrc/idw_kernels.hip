@@ -1178,8 +1178,8 @@ __device__ __forceinline__ void fwd_terms(int64_t a, int64_t b, const PlanEntry 
                 const float Af = (float)A, AL3 = (float)(A * L3);
 #pragma unroll
                 for (int t = 0; t < 4; ++t) {
-                    const float fx = (float)(vx[t] - x0), fy = (float)(vy[t] - x1), fz = (float)(vz[t] - x2);
-                    const float w = weight_f32<EXPM>(fx * fx + fy * fy + fz * fz, Af, AL3, Lf, a5L2, P);
+                    const double rx = vx[t] - x0, ry = vy[t] - x1, rz = vz[t] - x2;
+                    const float w = weight_f32<EXPM>((float)(rx * rx + ry * ry + rz * rz), Af, AL3, Lf, a5L2, P);
                     const bool on = (m >> (16 * t)) & 1ull;
                     F[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(on ? w : 0.0f, Bv, F[t], 0, 0, 0);
                 }
@@ -1340,8 +1340,8 @@ __device__ __forceinline__ void rev_terms(int64_t a, int64_t b, const PlanEntry 
             const bool on = (mR >> (4 * q)) & 1ull;
             double w;
             if (W32) {
-                const float fx = (float)(c.x - xa.x), fy = (float)(c.y - xa.y), fz = (float)(c.z - xa.z);
-                const float wf = weight_f32<EXPM>(fx * fx + fy * fy + fz * fz, Af, AL3, Lf, a5L2, P);
+                const double rx = c.x - xa.x, ry = c.y - xa.y, rz = c.z - xa.z;
+                const float wf = weight_f32<EXPM>((float)(rx * rx + ry * ry + rz * rz), Af, AL3, Lf, a5L2, P);
                 w = (double)(on ? wf : 0.0f);
             } else {
                 const double rx = c.x - xa.x, ry = c.y - xa.y, rz = c.z - xa.z;
@@ -1708,8 +1708,8 @@ k_fast_rev_pair(int nvol, int nwave, const double4 *__restrict__ vol, const int3
             for (int q = 0; q < 16; ++q) {
                 const double4 c = L.node[64 * h + 4 * q + R];
                 const bool on = (mR >> (4 * q)) & 1ull;
-                const float fx = (float)(c.x - T.xa.x), fy = (float)(c.y - T.xa.y), fz = (float)(c.z - T.xa.z);
-                const float wf = weight_f32<EXPM>(fx * fx + fy * fy + fz * fz, Af, AL3, Lf, a5L2, P);
+                const double rx = c.x - T.xa.x, ry = c.y - T.xa.y, rz = c.z - T.xa.z;
+                const float wf = weight_f32<EXPM>((float)(rx * rx + ry * ry + rz * rz), Af, AL3, Lf, a5L2, P);
                 const float w = on ? wf : 0.0f;
                 const float g = h == 0 ? GA[q] : GB[q];
                 if (q & 1) F1 = __builtin_amdgcn_mfma_f32_16x16x4f32(w, g, F1, 0, 0, 0);

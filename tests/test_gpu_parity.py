@@ -684,6 +684,7 @@ def test_mixed_adjoint_of_wave_pairs(planes):
         xs = m["perturb"](m["pts"])
         gpu.setSurfaceCoordinates(xs)
         gpu.warpMesh()
+        res[prec + "_xv"] = gpu.getWarpGrid().copy()
         dxv = synthetic.random_dxv(len(m["xv0"]), seed=9, frac=0.7)
         gpu.warpDeriv(dxv, solverVec=False)
         res[prec] = (gpu, dxv, gpu.getdXs().ravel().copy())
@@ -694,6 +695,9 @@ def test_mixed_adjoint_of_wave_pairs(planes):
     rmix = _rel(res["mixed"][2], res["fp64"][2])
     rora = _rel(res["mixed"][2], dref)
     print("planes", planes, "fp64 vs oracle", r64, "mixed vs fp64", rmix, "mixed vs oracle", rora, flush=True)
+    fwd = np.abs(res["mixed_xv"] - res["fp64_xv"]).max() / gpu.getLdef0()
+    print("planes", planes, "forward mixed vs fp64 max |dXv|/Ldef", fwd, flush=True)
+    assert fwd <= 1e-6
     assert r64 <= 1e-10
     assert rmix <= 1e-6 and rora <= 1e-6
     assert rmix >= 1e-12  # the FP32 path (and so the pair merge) did run
