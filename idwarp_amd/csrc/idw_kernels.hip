@@ -698,6 +698,9 @@ __device__ __forceinline__ double get_error(const NodeRec *nd, double dOvrR, con
 
 __device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
 
+// This lane's bit of a wave-uniform lane mask, as a lane predicate.
+__device__ __forceinline__ bool lane_in(unsigned long long m) { return __builtin_amdgcn_inverse_ballot_w64(m); }
+
 // Wait for every outstanding vector-memory operation (vmcnt(0)).  The
 // counter retires in order, so a wait for a load issued after stores or
 // atomics also waits for those: the kernels place it where that costs least.
@@ -845,48 +848,48 @@ __global__ void __launch_bounds__(256) k_plan_walk(int nvol, int blk0, int nbc, 
     for (;;) {
         const NodeHdr hA = load_hdr(nodes, min(node + 1, nnodes - 1));
         const NodeHdr hB = load_hdr(nodes, min(h.skip, nnodes - 1));
-        const bool active = h.depth <= stop;
         const double cx = v.x - h.X[0], cy = v.y - h.X[1], cz = v.z - h.X[2];
         int next;
         if (h.leaf) {
             if (COUNT) {
                 cnt += h.u - h.l + 1;
             } else {
+                const unsigned long long mAct = __ballot(h.depth <= stop);
                 bool near = false;
                 if (nearT2 > 0.0) {
                     // distance to the bounding sphere below nearT
                     const double rn = h.rad + nearT;
-                    near = __any(active && cx * cx + cy * cy + cz * cz < rn * rn);
+                    near = (__ballot(cx * cx + cy * cy + cz * cz < rn * rn) & mAct) != 0;
                 }
-                pw.put(nnodes + h.l, h.u - h.l + 1, __ballot(active), near, node);
+                pw.put(nnodes + h.l, h.u - h.l + 1, mAct, near, node);
             }
             next = h.skip;
         } else {
-            bool descend = false, accept = false;
             const double d2 = cx * cx + cy * cy + cz * cz;
-            // dOvrR = sqrt(d2) orad < 2 is certain without the square root
-            // when d2 orad^2 < 4 (1 - 2.5e-12): the roundings move either
-            // side by a few 1e-16
-            const bool sure = (d2 * h.orad) * h.orad < 3.99999999999;
-            if (active) {
-                if (sure) {
-                    descend = true;
-                } else {
-                    // d2 >= 3.99 rad^2 here, and the tree build keeps every
-                    // internal radius above 1e-100
-                    const double dOvrR = sqrt_normal(d2) * h.orad;
-                    if (dOvrR < 2.0) descend = true;
-                    else if (get_error_cells(nodes + node, dOvrR, cells) < tol) accept = true;
-                    else descend = true;
-                }
-                stop = descend ? INT32_MAX : h.depth;
+            // The decisions are kept as lane masks (compare results) so that
+            // no lane predicate goes through a vector register.  dOvrR =
+            // sqrt(d2) orad < 2 is certain without the square root when
+            // d2 orad^2 < 4 (1 - 2.5e-12): the roundings move either side by a
+            // few 1e-16.
+            const unsigned long long mAct = __ballot(h.depth <= stop);
+            const unsigned long long mSure = __ballot((d2 * h.orad) * h.orad < 3.99999999999);
+            const unsigned long long mNeed = mAct & ~mSure;
+            unsigned long long mAcc = 0;
+            if (mNeed) {
+                // every lane evaluates the test, the lanes of mNeed use it
+                // (d2 >= 3.99 rad^2 for them, and the tree build keeps every
+                // internal radius above 1e-100)
+                const double dOvrR = sqrt_normal(d2) * h.orad;
+                const double err = get_error_cells(nodes + node, dOvrR, cells);
+                mAcc = mNeed & __ballot(dOvrR >= 2.0) & __ballot(err < tol);
             }
-            const unsigned long long acc = __ballot(accept);
-            if (acc) {
+            const unsigned long long mDesc = mAct & ~mAcc;
+            stop = lane_in(mAct) ? (lane_in(mDesc) ? INT32_MAX : h.depth) : stop;
+            if (mAcc) {
                 if (COUNT) ++cnt;
-                else pw.put(node, 1, acc, false, node);
+                else pw.put(node, 1, mAcc, false, node);
             }
-            next = __any(descend) ? node + 1 : h.skip;
+            next = mDesc ? node + 1 : h.skip;
         }
         next = uniform(next);
         if (next >= nnodes) break;

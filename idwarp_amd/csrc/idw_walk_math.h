@@ -34,10 +34,11 @@ __device__ __forceinline__ SegCell seg_cell(int c, const double2 *rseg, double r
 }
 
 // getError at dOvrR >= 2 (finite): (1 - fact) err(i) + fact err(i+1) with
-// fact = (dOvrR - rstar(i)) orstar(i), unfused.
+// fact = (dOvrR - rstar(i)) orstar(i), unfused.  Any other argument (below 2,
+// NaN) reads the first or the last cell and gives a value callers discard.
 __device__ __forceinline__ double get_error_cells(const NodeRec *nd, double dOvrR, const SegCell *cells) {
 #pragma clang fp contract(off)
-    const SegCell t = cells[err_cell(dOvrR)];
+    const SegCell t = cells[max(err_cell(dOvrR), 0)];
     // err(i), err(i+1) as words of the record; err(12) is followed by Ai
     const double *w = reinterpret_cast<const double *>(reinterpret_cast<const char *>(nd) + offsetof(NodeRec, err));
     const double e0 = w[t.i], e1 = w[t.i + 1];
