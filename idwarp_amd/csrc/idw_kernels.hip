@@ -849,12 +849,13 @@ __global__ void __launch_bounds__(256) k_plan_walk(int nvol, int blk0, int nbc, 
         const NodeHdr hA = load_hdr(nodes, min(node + 1, nnodes - 1));
         const NodeHdr hB = load_hdr(nodes, min(h.skip, nnodes - 1));
         const double cx = v.x - h.X[0], cy = v.y - h.X[1], cz = v.z - h.X[2];
+        // active lanes: not below the depth where they accepted a cluster
+        const unsigned long long mAct = __ballot(h.depth <= stop);
         int next;
         if (h.leaf) {
             if (COUNT) {
                 cnt += h.u - h.l + 1;
             } else {
-                const unsigned long long mAct = __ballot(h.depth <= stop);
                 bool near = false;
                 if (nearT2 > 0.0) {
                     // distance to the bounding sphere below nearT
@@ -871,7 +872,6 @@ __global__ void __launch_bounds__(256) k_plan_walk(int nvol, int blk0, int nbc, 
             // sqrt(d2) orad < 2 is certain without the square root when
             // d2 orad^2 < 4 (1 - 2.5e-12): the roundings move either side by a
             // few 1e-16.
-            const unsigned long long mAct = __ballot(h.depth <= stop);
             const unsigned long long mSure = __ballot((d2 * h.orad) * h.orad < 3.99999999999);
             const unsigned long long mNeed = mAct & ~mSure;
             unsigned long long mAcc = 0;
@@ -884,7 +884,9 @@ __global__ void __launch_bounds__(256) k_plan_walk(int nvol, int blk0, int nbc, 
                 mAcc = mNeed & __ballot(dOvrR >= 2.0) & __ballot(err < tol);
             }
             const unsigned long long mDesc = mAct & ~mAcc;
-            stop = lane_in(mAct) ? (lane_in(mDesc) ? INT32_MAX : h.depth) : stop;
+            // descending lanes stop nowhere, accepting lanes stop here
+            stop = lane_in(mDesc) ? INT32_MAX : stop;
+            stop = lane_in(mAcc) ? h.depth : stop;
             if (mAcc) {
                 if (COUNT) ++cnt;
                 else pw.put(node, 1, mAcc, false, node);

@@ -12,12 +12,15 @@
 
 namespace idw {
 
-// One cell of err_cell: {rstar(i), orstar(i), i} of its segment i.  The last
-// segment is {640, 0, 11}, whose interpolation (1 - 0) err(11) + 0 * Ai returns
-// err(11) exactly (Ai, the word after the table, is finite).
+// One cell of err_cell: {rstar(i), orstar(i), i} of its segment i, and the
+// byte offset of err(i) in a NodeRec.  The last segment is {640, 0, 11}, whose
+// interpolation (1 - 0) err(11) + 0 * Ai returns err(11) exactly (Ai, the word
+// after the table, is finite).
 struct SegCell {
     double r, o;
-    int i, pad[3];
+    int i;
+    unsigned int off;
+    int pad[2];
 };
 static_assert(sizeof(SegCell) == 32, "SegCell layout");
 static_assert(offsetof(NodeRec, Ai) == offsetof(NodeRec, err) + NERR * sizeof(double), "Ai follows the error table");
@@ -29,7 +32,8 @@ __device__ __forceinline__ SegCell seg_cell(int c, const double2 *rseg, double r
     t.r = i < NERR - 1 ? rseg[i].x : rlast;
     t.o = i < NERR - 1 ? rseg[i].y : 0.0;
     t.i = i;
-    t.pad[0] = t.pad[1] = t.pad[2] = 0;
+    t.off = (unsigned int)(offsetof(NodeRec, err) + i * sizeof(double));
+    t.pad[0] = t.pad[1] = 0;
     return t;
 }
 
@@ -38,10 +42,12 @@ __device__ __forceinline__ SegCell seg_cell(int c, const double2 *rseg, double r
 // NaN) reads the first or the last cell and gives a value callers discard.
 __device__ __forceinline__ double get_error_cells(const NodeRec *nd, double dOvrR, const SegCell *cells) {
 #pragma clang fp contract(off)
-    const SegCell t = cells[max(err_cell(dOvrR), 0)];
+    // err_cell(dOvrR) from the high word, clamped to the table at both ends
+    const int c = min(max((int)((unsigned)__double2hiint(dOvrR) >> 17) - 8192, 0), ERR_CELLS - 1);
+    const SegCell t = cells[c];
     // err(i), err(i+1) as words of the record; err(12) is followed by Ai
-    const double *w = reinterpret_cast<const double *>(reinterpret_cast<const char *>(nd) + offsetof(NodeRec, err));
-    const double e0 = w[t.i], e1 = w[t.i + 1];
+    const double *w = reinterpret_cast<const double *>(reinterpret_cast<const char *>(nd) + t.off);
+    const double e0 = w[0], e1 = w[1];
     const double fact = (dOvrR - t.r) * t.o;
     return (1.0 - fact) * e0 + fact * e1;
 }
