@@ -1183,10 +1183,13 @@ __device__ __forceinline__ void fwd_terms(int64_t a, int64_t b, const PlanEntry 
                 const float Af = (float)A, AL3 = (float)(A * L3);
 #pragma unroll
                 for (int t = 0; t < 4; ++t) {
+                    const unsigned long long on = __ballot((m >> (16 * t)) & 1ull);
+                    // node block t idle for all four terms: nothing to add
+                    // (about one block in seven)
+                    if (!on) continue;
                     const double rx = vx[t] - x0, ry = vy[t] - x1, rz = vz[t] - x2;
                     const float w = weight_f32<EXPM>((float)(rx * rx + ry * ry + rz * rz), Af, AL3, Lf, a5L2, P);
-                    const bool on = (m >> (16 * t)) & 1ull;
-                    F[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(on ? w : 0.0f, Bv, F[t], 0, 0, 0);
+                    F[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(lane_in(on) ? w : 0.0f, Bv, F[t], 0, 0, 0);
                 }
             }
 #pragma unroll
@@ -1201,10 +1204,11 @@ __device__ __forceinline__ void fwd_terms(int64_t a, int64_t b, const PlanEntry 
                 const double eps = C.eps[tr];
 #pragma unroll
                 for (int t = 0; t < 4; ++t) {
+                    const unsigned long long on = __ballot((m >> (16 * t)) & 1ull);
+                    if (!on) continue;  // node block t idle for all four terms
                     const double rx = vx[t] - x0, ry = vy[t] - x1, rz = vz[t] - x2;
                     const double w = A * wfun<EXPM>(P.Ldef * rsqrt_nr(rx * rx + ry * ry + rz * rz + eps), P);
-                    const bool on = (m >> (16 * t)) & 1ull;
-                    D[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(on ? w : 0.0, Bv, D[t], 0, 0, 0);
+                    D[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(lane_in(on) ? w : 0.0, Bv, D[t], 0, 0, 0);
                 }
             }
         }
