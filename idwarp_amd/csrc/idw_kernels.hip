@@ -1345,17 +1345,18 @@ __device__ __forceinline__ void rev_terms(int64_t a, int64_t b, const PlanEntry 
         dvec4 D0 = {0.0, 0.0, 0.0, 0.0}, D1 = D0;
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
+            const unsigned long long on = __ballot((mR >> (4 * q)) & 1ull);
+            if (!on) continue;  // nodes 4q..4q+3 idle for all the tile's terms
             const double4 c = nodes[4 * q + R];
-            const bool on = (mR >> (4 * q)) & 1ull;
             double w;
             if (W32) {
                 const double rx = c.x - xa.x, ry = c.y - xa.y, rz = c.z - xa.z;
                 const float wf = weight_f32<EXPM>((float)(rx * rx + ry * ry + rz * rz), Af, AL3, Lf, a5L2, P);
-                w = (double)(on ? wf : 0.0f);
+                w = (double)(lane_in(on) ? wf : 0.0f);
             } else {
                 const double rx = c.x - xa.x, ry = c.y - xa.y, rz = c.z - xa.z;
                 w = xa.w * wfun<EXPM>(P.Ldef * rsqrt_nr(rx * rx + ry * ry + rz * rz + eps), P);
-                w = on ? w : 0.0;
+                w = lane_in(on) ? w : 0.0;
             }
             if (q & 1) D1 = __builtin_amdgcn_mfma_f64_16x16x4f64(w, B[q], D1, 0, 0, 0);
             else D0 = __builtin_amdgcn_mfma_f64_16x16x4f64(w, B[q], D0, 0, 0, 0);
