@@ -809,7 +809,10 @@ int idw_initialize_warping(idw_ctx *c, const double *pts, int64_t n_pts, const d
     S.nnodes = (int)H.nodes.size();
     S.nInternal = H.nInternal;
     S.nface = (int)H.nface;
-    TRY(dalloc(c, &S.nodes, S.nnodes));
+    // one zeroed record past the last: the walk prefetches the successors of
+    // every node it visits without clamping their indices
+    TRY(dalloc(c, &S.nodes, (size_t)S.nnodes + 1));
+    HIPCHK(hipMemsetAsync(S.nodes + S.nnodes, 0, sizeof(idw::NodeRec), c->st));
     TRY(dalloc(c, &S.rec, (size_t)S.nnodes + S.n));
     TRY(dalloc(c, &S.rec32, (size_t)S.nnodes + S.n));
     S.pd = S.rec + S.nnodes;

@@ -803,6 +803,33 @@ __device__ __forceinline__ NodeHdr load_hdr(const NodeRec *nodes, int node) {
     return *reinterpret_cast<const NodeHdr *>(nodes + node);
 }
 
+// A node's header as eight words, so that choosing between two prefetched
+// headers takes eight 64-bit scalar selects.
+struct HdrW {
+    unsigned long long w[8];
+    __device__ double X(int k) const { return __longlong_as_double((long long)w[k]); }
+    __device__ double orad() const { return __longlong_as_double((long long)w[3]); }
+    __device__ int l() const { return (int)(unsigned)w[4]; }
+    __device__ int u() const { return (int)(unsigned)(w[4] >> 32); }
+    __device__ int skip() const { return (int)(unsigned)w[5]; }
+    __device__ int depth() const { return (int)(unsigned)(w[5] >> 32); }
+    __device__ int leaf() const { return (int)(unsigned)w[6]; }
+    __device__ double rad() const { return __longlong_as_double((long long)w[7]); }
+};
+static_assert(offsetof(NodeRec, l) == 32 && offsetof(NodeRec, u) == 36 && offsetof(NodeRec, skip) == 40 &&
+                  offsetof(NodeRec, depth) == 44 && offsetof(NodeRec, leaf) == 48 && offsetof(NodeRec, rad) == 56,
+              "HdrW word layout");
+
+// Header of the record at byte offset off (a multiple of sizeof(NodeRec)).
+__device__ __forceinline__ HdrW load_hdrw(const NodeRec *nodes, unsigned off) {
+    const unsigned long long *p =
+        reinterpret_cast<const unsigned long long *>(reinterpret_cast<const char *>(nodes) + off);
+    HdrW h;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) h.w[i] = p[i];
+    return h;
+}
+
 // The walk of evalNode (kd_tree.F90:619-662) for 64 volume nodes per wave:
 // records every point of every bucket the wave evaluates (evalNodeExact)
 // and every cluster some lane accepts (evalNodeApprox, :692-713), with the
@@ -843,28 +870,31 @@ __global__ void __launch_bounds__(256) k_plan_walk(int nvol, int blk0, int nbc, 
     PlanWriter pw{plan, COUNT ? 0 : waveOff[wave], COUNT ? 0 : waveOff[wave + 1], 0, 0, mismatch,
                   wbuf[COUNT ? 0 : threadIdx.x >> 6], wbuf64[COUNT ? 0 : threadIdx.x >> 6]};
     int64_t cnt = 0;
+    constexpr unsigned REC = sizeof(NodeRec);
     int node = 0;
-    NodeHdr h = load_hdr(nodes, 0);
+    unsigned off = 0;  // byte offset of node's record
+    HdrW h = load_hdrw(nodes, 0);
     for (;;) {
-        const NodeHdr hA = load_hdr(nodes, min(node + 1, nnodes - 1));
-        const NodeHdr hB = load_hdr(nodes, min(h.skip, nnodes - 1));
-        const double cx = v.x - h.X[0], cy = v.y - h.X[1], cz = v.z - h.X[2];
+        // record nnodes is a zeroed pad, so neither successor needs a clamp
+        const HdrW hA = load_hdrw(nodes, off + REC);
+        const HdrW hB = load_hdrw(nodes, (unsigned)h.skip() * REC);
+        const double cx = v.x - h.X(0), cy = v.y - h.X(1), cz = v.z - h.X(2);
         // active lanes: not below the depth where they accepted a cluster
-        const unsigned long long mAct = __ballot(h.depth <= stop);
+        const unsigned long long mAct = __ballot(h.depth() <= stop);
         int next;
-        if (h.leaf) {
+        if (h.leaf()) {
             if (COUNT) {
-                cnt += h.u - h.l + 1;
+                cnt += h.u() - h.l() + 1;
             } else {
                 bool near = false;
                 if (nearT2 > 0.0) {
                     // distance to the bounding sphere below nearT
-                    const double rn = h.rad + nearT;
+                    const double rn = h.rad() + nearT;
                     near = (__ballot(cx * cx + cy * cy + cz * cz < rn * rn) & mAct) != 0;
                 }
-                pw.put(nnodes + h.l, h.u - h.l + 1, mAct, near, node);
+                pw.put(nnodes + h.l(), h.u() - h.l() + 1, mAct, near, node);
             }
-            next = h.skip;
+            next = h.skip();
         } else {
             const double d2 = cx * cx + cy * cy + cz * cz;
             // The decisions are kept as lane masks (compare results) so that
@@ -872,30 +902,34 @@ __global__ void __launch_bounds__(256) k_plan_walk(int nvol, int blk0, int nbc, 
             // sqrt(d2) orad < 2 is certain without the square root when
             // d2 orad^2 < 4 (1 - 2.5e-12): the roundings move either side by a
             // few 1e-16.
-            const unsigned long long mSure = __ballot((d2 * h.orad) * h.orad < 3.99999999999);
+            const unsigned long long mSure = __ballot((d2 * h.orad()) * h.orad() < 3.99999999999);
             const unsigned long long mNeed = mAct & ~mSure;
             unsigned long long mAcc = 0;
             if (mNeed) {
                 // every lane evaluates the test, the lanes of mNeed use it
                 // (d2 >= 3.99 rad^2 for them, and the tree build keeps every
                 // internal radius above 1e-100)
-                const double dOvrR = sqrt_normal(d2) * h.orad;
-                const double err = get_error_cells(nodes + node, dOvrR, cells);
+                const double dOvrR = sqrt_normal(d2) * h.orad();
+                const double err = get_error_cells(
+                    reinterpret_cast<const NodeRec *>(reinterpret_cast<const char *>(nodes) + off), dOvrR, cells);
                 mAcc = mNeed & __ballot(dOvrR >= 2.0) & __ballot(err < tol);
             }
             const unsigned long long mDesc = mAct & ~mAcc;
             // descending lanes stop nowhere, accepting lanes stop here
             stop = lane_in(mDesc) ? INT32_MAX : stop;
-            stop = lane_in(mAcc) ? h.depth : stop;
+            stop = lane_in(mAcc) ? h.depth() : stop;
             if (mAcc) {
                 if (COUNT) ++cnt;
                 else pw.put(node, 1, mAcc, false, node);
             }
-            next = mDesc ? node + 1 : h.skip;
+            next = mDesc ? node + 1 : h.skip();
         }
         next = uniform(next);
         if (next >= nnodes) break;
-        h = (next == node + 1) ? hA : hB;
+        const bool adv = next == node + 1;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) h.w[i] = adv ? hA.w[i] : hB.w[i];
+        off = adv ? off + REC : (unsigned)next * REC;
         node = next;
     }
     if (COUNT) {
