@@ -56,15 +56,9 @@ double plan_near_t2(const idw_ctx *c);
 
 }  // namespace
 
-constexpr int MAX_CHUNKS = 8;  // volume chunks of an overlapped fast-mode warp
-
 struct idw_ctx {
     int device = 0;
     hipStream_t st = nullptr;
-    // second stream: forwards of volume chunks whose walks are done, while
-    // the context stream walks the next chunk
-    hipStream_t st2 = nullptr;
-    hipEvent_t evChunk[MAX_CHUNKS] = {}, evFork = nullptr, evJoin = nullptr;
     idw_options opt{};
     idw_options initOpt{};
     idw_options denOpt{};
@@ -387,42 +381,6 @@ int surface_pass(idw_ctx *c, const double *xs, bool clusters) {
     return IDW_OK;
 }
 
-// Fast-mode warp that (re)writes plans: the volume blocks are split into up
-// to MAX_CHUNKS chunks; the walk of each chunk runs on the context stream
-// and the forward of that chunk on the second stream once its walk is done,
-// so the walk of one chunk overlaps the forward of the previous one.  Passes
-// stay in order on each stream (the mirror passes accumulate per node).
-// Timing: phase 2 ends after the last walk, phase 3 after the last forward.
-int overlapped_fast_warp(idw_ctx *c, idw::WParams P, double nt2, const std::vector<char> &fill, double *xv) {
-    const int64_t nb = (c->V.n + 255) / 256;
-    const int nch = (int)std::max<int64_t>(1, std::min<int64_t>(MAX_CHUNKS, nb / 512));
-    HIPCHK(hipEventRecord(c->evFork, c->st));
-    HIPCHK(hipStreamWaitEvent(c->st2, c->evFork, 0));
-    for (int kk = 0; kk < nloop(c); ++kk) {
-        idw::DevPlan &pl = c->plans[kk];
-        if (fill[kk]) idw::launch_zero_u32(c->st, pl.mismatch, 1);
-        P.pass = pass_of(c, kk);
-        for (int ch = 0; ch < nch; ++ch) {
-            const int64_t b0 = nb * ch / nch, b1 = nb * (ch + 1) / nch;
-            if (fill[kk]) {
-                idw::launch_plan_fill(c->st, c->S, c->V, mirror_of(c, kk), c->opt.errTol, nt2, pl, b0, b1 - b0);
-                HIPCHK(hipEventRecord(c->evChunk[ch], c->st));
-                HIPCHK(hipStreamWaitEvent(c->st2, c->evChunk[ch], 0));
-            }
-            idw::launch_forward(c->st2, c->S, c->V, P, mirror_of(c, kk), 1, c->opt.precision, xv, &pl, b0, b1 - b0);
-        }
-        if (fill[kk]) {
-            pl.filled = true;
-            pl.nearT2 = nt2;
-        }
-    }
-    HIPCHK(hipGetLastError());
-    REC(c, 2);
-    HIPCHK(hipEventRecord(c->evJoin, c->st2));
-    HIPCHK(hipStreamWaitEvent(c->st, c->evJoin, 0));
-    return IDW_OK;
-}
-
 int run_warp(idw_ctx *c, const double *xs, double *xv) {
     if (!c->surfInit) return fail(IDW_ESTATE, "warp before initialize_warping");
     if (!c->volSet) return fail(IDW_ESTATE, "no volume nodes set");
@@ -434,7 +392,6 @@ int run_warp(idw_ctx *c, const double *xs, double *xv) {
     TRY(surface_pass(c, xs, fast));
     REC(c, 1);
     std::vector<char> fill(nloop(c), 0);
-    bool anyFill = false;
     const double nt2 = fast ? plan_near_t2(c) : 0.0;
     if (fast) {
         for (int kk = 0; kk < nloop(c); ++kk) {
@@ -443,28 +400,23 @@ int run_warp(idw_ctx *c, const double *xs, double *xv) {
                 if (c->capturing && !(pl.filled && pl.errTol == c->opt.errTol && pl.nearT2 == nt2))
                     return fail(IDW_ESTATE, "a captured warp needs the plan of a previous warp at the same options");
                 fill[kk] = 1;
-                anyFill = true;
             }
         }
     }
-    if (anyFill && !c->capturing && c->V.n) {
-        TRY(overlapped_fast_warp(c, P, nt2, fill, xv));
-    } else {
-        for (int kk = 0; kk < nloop(c); ++kk) {
-            if (!fill[kk]) continue;
-            idw::DevPlan &pl = c->plans[kk];
-            idw::launch_zero_u32(c->st, pl.mismatch, 1);
-            if (c->V.n) idw::launch_plan_fill(c->st, c->S, c->V, mirror_of(c, kk), c->opt.errTol, nt2, pl);
-            HIPCHK(hipGetLastError());
-            pl.filled = true;
-            pl.nearT2 = nt2;
-        }
-        REC(c, 2);
-        for (int kk = 0; kk < nloop(c) && c->V.n; ++kk) {
-            P.pass = pass_of(c, kk);
-            idw::launch_forward(c->st, c->S, c->V, P, mirror_of(c, kk), fast, c->opt.precision, xv,
-                                fast ? &c->plans[kk] : nullptr);
-        }
+    for (int kk = 0; kk < nloop(c); ++kk) {
+        if (!fill[kk]) continue;
+        idw::DevPlan &pl = c->plans[kk];
+        idw::launch_zero_u32(c->st, pl.mismatch, 1);
+        if (c->V.n) idw::launch_plan_fill(c->st, c->S, c->V, mirror_of(c, kk), c->opt.errTol, nt2, pl);
+        HIPCHK(hipGetLastError());
+        pl.filled = true;
+        pl.nearT2 = nt2;
+    }
+    REC(c, 2);
+    for (int kk = 0; kk < nloop(c) && c->V.n; ++kk) {
+        P.pass = pass_of(c, kk);
+        idw::launch_forward(c->st, c->S, c->V, P, mirror_of(c, kk), fast, c->opt.precision, xv,
+                            fast ? &c->plans[kk] : nullptr);
     }
     HIPCHK(hipGetLastError());
     REC(c, 3);
@@ -632,12 +584,7 @@ int idw_create(idw_ctx **out, int device, const idw_options *opt) {
     c->opt = *opt;
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking);
     for (int k = 0; k < 8 && e == hipSuccess; ++k) e = hipEventCreate(&c->ev[k]);
-    for (int k = 0; k < MAX_CHUNKS && e == hipSuccess; ++k)
-        e = hipEventCreateWithFlags(&c->evChunk[k], hipEventDisableTiming);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->evFork, hipEventDisableTiming);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->evJoin, hipEventDisableTiming);
     if (e != hipSuccess) {
         idw_destroy(c);
         return fail(IDW_EHIP, "stream/event creation: %s", hipGetErrorString(e));
@@ -650,15 +597,9 @@ int idw_destroy(idw_ctx *c) {
     if (!c) return IDW_OK;
     (void)hipSetDevice(c->device);
     if (c->st) (void)hipStreamSynchronize(c->st);
-    if (c->st2) (void)hipStreamSynchronize(c->st2);
     for (void *p : c->allocs) (void)hipFree(p);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
-    for (auto &e : c->evChunk)
-        if (e) (void)hipEventDestroy(e);
-    if (c->evFork) (void)hipEventDestroy(c->evFork);
-    if (c->evJoin) (void)hipEventDestroy(c->evJoin);
-    if (c->st2) (void)hipStreamDestroy(c->st2);
     if (c->st) (void)hipStreamDestroy(c->st);
     delete c;
     return IDW_OK;
@@ -789,7 +730,7 @@ int idw_initialize_warping(idw_ctx *c, const double *pts, int64_t n_pts, const d
         return fail(IDW_EINVAL, "%s", msg.c_str());
     // free a previous surface
     idw::DevSurface &S = c->S;
-    void *olds[] = {S.nodes, S.rec, S.rec32, S.internal, S.ps, S.normals0, S.nprops, S.corner, S.xuInd,
+    void *olds[] = {S.nodes, S.whdr, S.rec, S.rec32, S.internal, S.ps, S.normals0, S.nprops, S.corner, S.xuInd,
                     S.facePtr, S.faceConn, S.n2eStart, S.n2e, S.slotStart, S.slots, S.fprops, S.adj,
                     S.ntmp, S.pb, S.fpropsd, c->xs, c->dxs, c->dxsIn, c->partial};
     for (void *p : olds) {
@@ -809,10 +750,8 @@ int idw_initialize_warping(idw_ctx *c, const double *pts, int64_t n_pts, const d
     S.nnodes = (int)H.nodes.size();
     S.nInternal = H.nInternal;
     S.nface = (int)H.nface;
-    // one zeroed record past the last: the walk prefetches the successors of
-    // every node it visits without clamping their indices
-    TRY(dalloc(c, &S.nodes, (size_t)S.nnodes + 1));
-    HIPCHK(hipMemsetAsync(S.nodes + S.nnodes, 0, sizeof(idw::NodeRec), c->st));
+    TRY(dalloc(c, &S.nodes, S.nnodes));
+    TRY(dalloc(c, &S.whdr, (size_t)S.nnodes + 1));
     TRY(dalloc(c, &S.rec, (size_t)S.nnodes + S.n));
     TRY(dalloc(c, &S.rec32, (size_t)S.nnodes + S.n));
     S.pd = S.rec + S.nnodes;
@@ -871,6 +810,18 @@ int idw_initialize_warping(idw_ctx *c, const double *pts, int64_t n_pts, const d
     for (int i = 0; i < S.n; ++i) Ai[i] = ps2[i].w;
     idw::set_node_areas(c->hs, Ai.data());
     HIPCHK(hipMemcpyAsync(S.nodes, c->hs.nodes.data(), sizeof(idw::NodeRec) * S.nnodes, hipMemcpyHostToDevice, st));
+    std::vector<idw::WalkHdr> whdr(S.nnodes + 1);  // value-initialised: the pad record is zero
+    for (int k = 0; k < S.nnodes; ++k) {
+        const idw::NodeRec &r = c->hs.nodes[k];
+        idw::WalkHdr &w = whdr[k];
+        for (int i = 0; i < 3; ++i) w.X[i] = r.X[i];
+        w.r = r.leaf ? r.rad : r.orad;
+        w.skip = r.skip;
+        w.depth = r.depth;
+        w.l = r.l;
+        w.n = r.u - r.l + 1;
+    }
+    HIPCHK(hipMemcpyAsync(S.whdr, whdr.data(), sizeof(idw::WalkHdr) * whdr.size(), hipMemcpyHostToDevice, st));
     static double sphere[60];
     sphere_nominal(sphere);
     idw::launch_errors(st, S, make_params(c, true), sphere);

@@ -42,6 +42,7 @@ struct WParams {
 struct DevSurface {
     int n = 0, nnodes = 0, nInternal = 0, nface = 0;
     NodeRec *nodes = nullptr;
+    WalkHdr *whdr = nullptr;  // nnodes + 1 walk headers
     int32_t *internal = nullptr;
     double4 *ps = nullptr;  // Xu0, A (tree order)
     // term records: nnodes cluster records (index = tree node), then n point

@@ -803,46 +803,69 @@ __device__ __forceinline__ NodeHdr load_hdr(const NodeRec *nodes, int node) {
     return *reinterpret_cast<const NodeHdr *>(nodes + node);
 }
 
-// A node's header as eight words, so that choosing between two prefetched
-// headers takes eight 64-bit scalar selects.
-struct HdrW {
-    unsigned long long w[8];
-    __device__ double X(int k) const { return __longlong_as_double((long long)w[k]); }
-    __device__ double orad() const { return __longlong_as_double((long long)w[3]); }
-    __device__ int l() const { return (int)(unsigned)w[4]; }
-    __device__ int u() const { return (int)(unsigned)(w[4] >> 32); }
-    __device__ int skip() const { return (int)(unsigned)w[5]; }
-    __device__ int depth() const { return (int)(unsigned)(w[5] >> 32); }
-    __device__ int leaf() const { return (int)(unsigned)w[6]; }
-    __device__ double rad() const { return __longlong_as_double((long long)w[7]); }
-};
-static_assert(offsetof(NodeRec, l) == 32 && offsetof(NodeRec, u) == 36 && offsetof(NodeRec, skip) == 40 &&
-                  offsetof(NodeRec, depth) == 44 && offsetof(NodeRec, leaf) == 48 && offsetof(NodeRec, rad) == 56,
-              "HdrW word layout");
-
-// Header of the record at byte offset off (a multiple of sizeof(NodeRec)).
-__device__ __forceinline__ HdrW load_hdrw(const NodeRec *nodes, unsigned off) {
-    const unsigned long long *p =
-        reinterpret_cast<const unsigned long long *>(reinterpret_cast<const char *>(nodes) + off);
-    HdrW h;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) h.w[i] = p[i];
-    return h;
+// Walk header of the node at byte offset off (a multiple of sizeof(WalkHdr)).
+__device__ __forceinline__ WalkHdr load_whdr(const WalkHdr *w, unsigned off) {
+    return *reinterpret_cast<const WalkHdr *>(reinterpret_cast<const char *>(w) + off);
 }
 
-// The walk of evalNode (kd_tree.F90:619-662) for 64 volume nodes per wave:
-// records every point of every bucket the wave evaluates (evalNodeExact)
-// and every cluster some lane accepts (evalNodeApprox, :692-713), with the
-// lane masks, in depth-first order.  The decisions use only init-time data
-// (centroids, radii, error tables, denominator0) and errTol.  COUNT: terms
-// per wave only.  A bucket's points are FP64 terms when an active lane lies
-// within sqrt(nearT2) of the bucket's bounding sphere (mixed precision keeps
-// those pairs in FP64).  The headers of both possible successors (node+1 when a
-// lane descends, skip otherwise) are requested before the current node is
-// evaluated.
+// One visit of the walk below at node (walk header h):
+// the lane decisions, the plan entries, and the wave's next node.
+template <bool COUNT>
+__device__ __forceinline__ int walk_visit(const WalkHdr &h, int node, const double4 &v, double tol,
+                                          int &stop, PlanWriter &pw, int64_t &cnt, const NodeRec *nodes,
+                                          int nnodes, const SegCell *cells, double nearT2, double nearT) {
+#pragma clang fp contract(off)
+    const double cx = v.x - h.X[0], cy = v.y - h.X[1], cz = v.z - h.X[2];
+    // active lanes: not below the depth where they accepted a cluster
+    const unsigned long long mAct = __ballot(h.depth <= stop);
+    int next;
+    if (h.skip == node + 1) {  // a bucket
+        if (COUNT) {
+            cnt += h.n;
+        } else {
+            bool near = false;
+            if (nearT2 > 0.0) {
+                // distance to the bounding sphere below nearT
+                const double rn = h.r + nearT;
+                near = (__ballot(cx * cx + cy * cy + cz * cz < rn * rn) & mAct) != 0;
+            }
+            pw.put(nnodes + h.l, h.n, mAct, near, node);
+        }
+        next = h.skip;
+    } else {
+        const double d2 = cx * cx + cy * cy + cz * cz;
+        // The decisions are kept as lane masks (compare results) so that
+        // no lane predicate goes through a vector register.  dOvrR =
+        // sqrt(d2) orad < 2 is certain without the square root when
+        // d2 orad^2 < 4 (1 - 2.5e-12): the roundings move either side by a
+        // few 1e-16.
+        const unsigned long long mSure = __ballot((d2 * h.r) * h.r < 3.99999999999);
+        const unsigned long long mNeed = mAct & ~mSure;
+        unsigned long long mAcc = 0;
+        if (mNeed) {
+            // every lane evaluates the test, the lanes of mNeed use it
+            // (d2 >= 3.99 rad^2 for them, and the tree build keeps every
+            // internal radius above 1e-100)
+            const double dOvrR = sqrt_normal(d2) * h.r;
+            const double err = get_error_cells(nodes + node, dOvrR, cells);
+            mAcc = mNeed & __ballot(dOvrR >= 2.0) & __ballot(err < tol);
+        }
+        const unsigned long long mDesc = mAct & ~mAcc;
+        // descending lanes stop nowhere, accepting lanes stop here
+        stop = lane_in(mDesc) ? INT32_MAX : stop;
+        stop = lane_in(mAcc) ? h.depth : stop;
+        if (mAcc) {
+            if (COUNT) ++cnt;
+            else pw.put(node, 1, mAcc, false, node);
+        }
+        next = mDesc ? node + 1 : h.skip;
+    }
+    return uniform(next);
+}
+
 template <bool COUNT>
 __global__ void __launch_bounds__(256) k_plan_walk(int nvol, int blk0, int nbc, const double4 *__restrict__ vol,
-                                                   int nnodes,
+                                                   int nnodes, const WalkHdr *__restrict__ whdr,
                                                    const NodeRec *__restrict__ nodes, double errTol, double nearT2,
                                                    int64_t *__restrict__ waveCount, PlanEntry *__restrict__ plan,
                                                    const int64_t *__restrict__ waveOff,
@@ -870,66 +893,18 @@ __global__ void __launch_bounds__(256) k_plan_walk(int nvol, int blk0, int nbc, 
     PlanWriter pw{plan, COUNT ? 0 : waveOff[wave], COUNT ? 0 : waveOff[wave + 1], 0, 0, mismatch,
                   wbuf[COUNT ? 0 : threadIdx.x >> 6], wbuf64[COUNT ? 0 : threadIdx.x >> 6]};
     int64_t cnt = 0;
-    constexpr unsigned REC = sizeof(NodeRec);
+    // The walk headers of both possible successors of the node being visited
+    // (node + 1 when a lane descends, skip otherwise) are requested before
+    // the visit.  Header nnodes is a zeroed pad: neither needs a clamp.
+    constexpr unsigned HB = sizeof(WalkHdr);
     int node = 0;
-    unsigned off = 0;  // byte offset of node's record
-    HdrW h = load_hdrw(nodes, 0);
+    WalkHdr h = load_whdr(whdr, 0);
     for (;;) {
-        // record nnodes is a zeroed pad, so neither successor needs a clamp
-        const HdrW hA = load_hdrw(nodes, off + REC);
-        const HdrW hB = load_hdrw(nodes, (unsigned)h.skip() * REC);
-        const double cx = v.x - h.X(0), cy = v.y - h.X(1), cz = v.z - h.X(2);
-        // active lanes: not below the depth where they accepted a cluster
-        const unsigned long long mAct = __ballot(h.depth() <= stop);
-        int next;
-        if (h.leaf()) {
-            if (COUNT) {
-                cnt += h.u() - h.l() + 1;
-            } else {
-                bool near = false;
-                if (nearT2 > 0.0) {
-                    // distance to the bounding sphere below nearT
-                    const double rn = h.rad() + nearT;
-                    near = (__ballot(cx * cx + cy * cy + cz * cz < rn * rn) & mAct) != 0;
-                }
-                pw.put(nnodes + h.l(), h.u() - h.l() + 1, mAct, near, node);
-            }
-            next = h.skip();
-        } else {
-            const double d2 = cx * cx + cy * cy + cz * cz;
-            // The decisions are kept as lane masks (compare results) so that
-            // no lane predicate goes through a vector register.  dOvrR =
-            // sqrt(d2) orad < 2 is certain without the square root when
-            // d2 orad^2 < 4 (1 - 2.5e-12): the roundings move either side by a
-            // few 1e-16.
-            const unsigned long long mSure = __ballot((d2 * h.orad()) * h.orad() < 3.99999999999);
-            const unsigned long long mNeed = mAct & ~mSure;
-            unsigned long long mAcc = 0;
-            if (mNeed) {
-                // every lane evaluates the test, the lanes of mNeed use it
-                // (d2 >= 3.99 rad^2 for them, and the tree build keeps every
-                // internal radius above 1e-100)
-                const double dOvrR = sqrt_normal(d2) * h.orad();
-                const double err = get_error_cells(
-                    reinterpret_cast<const NodeRec *>(reinterpret_cast<const char *>(nodes) + off), dOvrR, cells);
-                mAcc = mNeed & __ballot(dOvrR >= 2.0) & __ballot(err < tol);
-            }
-            const unsigned long long mDesc = mAct & ~mAcc;
-            // descending lanes stop nowhere, accepting lanes stop here
-            stop = lane_in(mDesc) ? INT32_MAX : stop;
-            stop = lane_in(mAcc) ? h.depth() : stop;
-            if (mAcc) {
-                if (COUNT) ++cnt;
-                else pw.put(node, 1, mAcc, false, node);
-            }
-            next = mDesc ? node + 1 : h.skip();
-        }
-        next = uniform(next);
+        const WalkHdr hA = load_whdr(whdr, (unsigned)(node + 1) * HB);
+        const WalkHdr hB = load_whdr(whdr, (unsigned)h.skip * HB);
+        const int next = walk_visit<COUNT>(h, node, v, tol, stop, pw, cnt, nodes, nnodes, cells, nearT2, nearT);
         if (next >= nnodes) break;
-        const bool adv = next == node + 1;
-#pragma unroll
-        for (int i = 0; i < 8; ++i) h.w[i] = adv ? hA.w[i] : hB.w[i];
-        off = adv ? off + REC : (unsigned)next * REC;
+        h = next == node + 1 ? hA : hB;
         node = next;
     }
     if (COUNT) {
@@ -2310,7 +2285,7 @@ void launch_prep_numb(hipStream_t st, const DevVolume &V, const Mirror &M, const
 void launch_plan_count(hipStream_t st, const DevSurface &S, const DevVolume &V, const Mirror &M, double errTol,
                        int64_t *waveCount) {
     const int nb = (int)grid_for(V.n, 256);
-    k_plan_walk<true><<<nb, 256, 0, st>>>((int)V.n, 0, nb, V.vol, S.nnodes, S.nodes, errTol, 0.0, waveCount, nullptr,
+    k_plan_walk<true><<<nb, 256, 0, st>>>((int)V.n, 0, nb, V.vol, S.nnodes, S.whdr, S.nodes, errTol, 0.0, waveCount, nullptr,
                                           nullptr, nullptr, nullptr, M);
 }
 
@@ -2318,7 +2293,7 @@ void launch_plan_fill(hipStream_t st, const DevSurface &S, const DevVolume &V, c
                       double nearT2, const DevPlan &plan, int64_t blk0, int64_t nblk) {
     if (nblk < 0) nblk = grid_for(V.n, 256) - blk0;
     if (nblk <= 0) return;
-    k_plan_walk<false><<<xcd_blocks(nblk), 256, 0, st>>>((int)V.n, (int)blk0, (int)nblk, V.vol, S.nnodes, S.nodes,
+    k_plan_walk<false><<<xcd_blocks(nblk), 256, 0, st>>>((int)V.n, (int)blk0, (int)nblk, V.vol, S.nnodes, S.whdr, S.nodes,
                                                          errTol, nearT2, nullptr, plan.entries, plan.waveOff,
                                                          plan.waveSplit, plan.mismatch, M);
 }
