@@ -1316,18 +1316,16 @@ struct RevLds {
 // G = [s (x) g (9), s (3)] (B in registers), then
 // Mb'(a, c) += P(3a + c) - P(9 + a) (x - o)_c and Bb(a) += P(9 + a) into
 // the term's adjoint record.  Lane l computes the weight of term l%16 with
-// node 4q + l/16 for node group q.  The next tile's record and the entries
-// of the tile after it are requested before this tile's sums and are in
-// registers before its atomics are issued.
-template <bool W32, int EXPM>
+// node 4q + l/16 for node group q, in FP64.  The next tile's record and the
+// entries of the tile after it are requested before this tile's sums and are
+// in registers before its atomics are issued.
+template <int EXPM>
 __device__ __forceinline__ void rev_terms(int64_t a, int64_t b, const PlanEntry *__restrict__ plan, int nnodes,
                                           int nrec, const Rec64 *__restrict__ rec, const WParams &P,
                                           const double4 *nodes, double (*Y)[4], int32_t *idxs,
                                           const double (&B)[16], double ox, double oy, double oz,
                                           double *__restrict__ adj) {
     const int lane = threadIdx.x & 63, m = lane & 15, R = lane >> 4;
-    const float Lf = (float)P.Ldef, a5L2 = (float)(P.a5 * P.Ldef * P.Ldef);
-    const double L3 = P.Ldef * P.Ldef * P.Ldef;
     const int src = (lane & ~15) | (9 + (m < 9 ? m / 3 : 0));
     if (a >= b) return;
     auto record = [&](const PlanEntry &pe) {  // x, A of an entry (zero for padding); unconditional load
@@ -1350,23 +1348,15 @@ __device__ __forceinline__ void rev_terms(int64_t a, int64_t b, const PlanEntry 
             idxs[m] = idx;
         }
         const unsigned long long mR = pe.mask >> R;
-        const float Af = (float)xa.w, AL3 = (float)(xa.w * L3);
         dvec4 D0 = {0.0, 0.0, 0.0, 0.0}, D1 = D0;
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
             const unsigned long long on = __ballot((mR >> (4 * q)) & 1ull);
             if (!on) continue;  // nodes 4q..4q+3 idle for all the tile's terms
             const double4 c = nodes[4 * q + R];
-            double w;
-            if (W32) {
-                const double rx = c.x - xa.x, ry = c.y - xa.y, rz = c.z - xa.z;
-                const float wf = weight_f32<EXPM>((float)(rx * rx + ry * ry + rz * rz), Af, AL3, Lf, a5L2, P);
-                w = (double)(lane_in(on) ? wf : 0.0f);
-            } else {
-                const double rx = c.x - xa.x, ry = c.y - xa.y, rz = c.z - xa.z;
-                w = xa.w * wfun<EXPM>(P.Ldef * rsqrt_nr(rx * rx + ry * ry + rz * rz + eps), P);
-                w = lane_in(on) ? w : 0.0;
-            }
+            const double rx = c.x - xa.x, ry = c.y - xa.y, rz = c.z - xa.z;
+            double w = xa.w * wfun<EXPM>(P.Ldef * rsqrt_nr(rx * rx + ry * ry + rz * rz + eps), P);
+            w = lane_in(on) ? w : 0.0;
             if (q & 1) D1 = __builtin_amdgcn_mfma_f64_16x16x4f64(w, B[q], D1, 0, 0, 0);
             else D0 = __builtin_amdgcn_mfma_f64_16x16x4f64(w, B[q], D0, 0, 0, 0);
         }
@@ -1393,7 +1383,9 @@ __device__ __forceinline__ void rev_terms(int64_t a, int64_t b, const PlanEntry 
 // C(a,b) += sum s_a (x_b - X_b), B(a) += sum s_a; for bucket point i,
 // Mb'(a,b) += sum s_a (x_b - Xu0_b), Bb(a) += sum s_a, with
 // s = w * dXv / den (warpDeriv.F90:76-78).  The adjoint records are indexed
-// like the term records (clusters, then points).
+// like the term records (clusters, then points).  FP64 weights throughout:
+// mixed precision takes k_fast_rev_pair, whose FP64 terms come back here
+// through rev_terms.
 template <int PREC, int EXPM>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_fast_rev(int nvol, const double4 *__restrict__ vol,
                                                   const int32_t *__restrict__ perm, const double *__restrict__ dxv,
@@ -1402,6 +1394,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
                                                   const int64_t *__restrict__ waveSplit, int nnodes, int nrec,
                                                   const Rec64 *__restrict__ rec, WParams P, Mirror M,
                                                   double *__restrict__ adj) {
+    static_assert(PREC == IDW_PREC_FP64, "mixed precision reverses through k_fast_rev_pair");
     __shared__ RevLds lds[4];
     const int nb = (nvol + 255) / 256;
     const int blk = xcd_block(nb);
@@ -1443,13 +1436,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         B[q] = n < 9 ? sv * gv : (n < 12 ? sv : 0.0);
     }
     const int64_t e0 = waveOff[wave], e1 = waveOff[wave + 1];
-    if (PREC == IDW_PREC_MIXED) {
-        const int64_t es = min(max(waveSplit[wave], e0), e1);
-        rev_terms<true, EXPM>(e0, es, plan, nnodes, nrec, rec, P, L.node, L.Y, L.idx, B, ox, oy, oz, adj);
-        rev_terms<false, EXPM>(es, e1, plan, nnodes, nrec, rec, P, L.node, L.Y, L.idx, B, ox, oy, oz, adj);
-    } else {
-        rev_terms<false, EXPM>(e0, e1, plan, nnodes, nrec, rec, P, L.node, L.Y, L.idx, B, ox, oy, oz, adj);
-    }
+    rev_terms<EXPM>(e0, e1, plan, nnodes, nrec, rec, P, L.node, L.Y, L.idx, B, ox, oy, oz, adj);
 }
 
 // ---------------------------------------------------------------------------
@@ -1757,7 +1744,7 @@ k_fast_rev_pair(int nvol, int nwave, const double4 *__restrict__ vol, const int3
             B[q] = g_entry(64 * h + 4 * q + R);
             __builtin_amdgcn_sched_barrier(0);
         }
-        rev_terms<false, EXPM>(e0, e1, plan, nnodes, nrec, rec, P, L.node + 64 * h, L.Y, L.idx, B, ox, oy, oz, adj);
+        rev_terms<EXPM>(e0, e1, plan, nnodes, nrec, rec, P, L.node + 64 * h, L.Y, L.idx, B, ox, oy, oz, adj);
     }
 }
 
