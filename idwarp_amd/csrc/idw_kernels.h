@@ -39,11 +39,21 @@ struct WParams {
     int32_t pass;     // PASS_* of this launch
 };
 
+constexpr int MAX_LEVELS = 64;  // tree depths the cluster-data pass handles
+constexpr int CSUM = 16;        // per-node sums of the cluster data (15 used)
+
 struct DevSurface {
     int n = 0, nnodes = 0, nInternal = 0, nface = 0;
     NodeRec *nodes = nullptr;
     WalkHdr *whdr = nullptr;  // nnodes + 1 walk headers
     int32_t *internal = nullptr;
+    // the cluster data's bottom-up order: the buckets, then the internal nodes
+    // one depth at a time from the deepest (level i: sumOrder[lvlStart[i] ..
+    // lvlStart[i] + lvlCount[i])); csum holds CSUM sums per node
+    int32_t *sumOrder = nullptr;
+    double *csum = nullptr;
+    int nLeaves = 0, nLevels = 0;
+    int lvlStart[MAX_LEVELS] = {}, lvlCount[MAX_LEVELS] = {};
     double4 *ps = nullptr;  // Xu0, A (tree order)
     // term records: nnodes cluster records (index = tree node), then n point
     // records (index nnodes + tree-order point); pd / pd32 point at the latter

@@ -483,55 +483,72 @@ __global__ void k_corners(int n, double threshold, const int32_t *__restrict__ n
     corner[i] = c;
 }
 
-// setData (kd_tree.F90:595-617): mean K and mean b = d - K Xu0 over an
-// internal node's points, stored in the node's term record as K = Kbar and
-// d = Kbar X + mean(b) = mean(d) + mean(K (X - Xu0)), with x = X and
-// A = Ai.  One block per internal node.
-__global__ void __launch_bounds__(256) k_cluster_data(const int32_t *__restrict__ internal,
-                                                      const NodeRec *__restrict__ nodes,
-                                                      const double4 *__restrict__ ps,
-                                                      const Rec64 *__restrict__ pd, Rec64 *__restrict__ rec,
-                                                      Rec32 *__restrict__ rec32) {
-    __shared__ double red[15][256 / 64];
-    const int node = internal[blockIdx.x];
+// setData (kd_tree.F90:595-617) bottom up: per node the sums over its
+// points of K (9), d (3) and K (X - p) (3, X the node's centroid), first for
+// every bucket from its points, then for each internal node from its two
+// children (node + 1 and that child's skip), shifting the children's
+// K (X_c - p) sums to X with K_c (X - X_c).  An internal node's cluster
+// record is K = mean(K), d = mean(d) + mean(K (X - p)), the form of
+// d = K X + mean(b) with b = Xu - M Xu0 (the same for point and tangent
+// records, which enter linearly).
+__global__ void __launch_bounds__(256) k_leaf_sums(int n, const int32_t *__restrict__ list,
+                                                   const NodeRec *__restrict__ nodes,
+                                                   const double4 *__restrict__ ps, const Rec64 *__restrict__ pd,
+                                                   double *__restrict__ csum) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int node = list[t];
     const NodeRec &r = nodes[node];
-    const double X0 = r.X[0], X1 = r.X[1], X2 = r.X[2];
     double acc[15];
     for (int k = 0; k < 15; ++k) acc[k] = 0.0;
-    for (int i = r.l + threadIdx.x; i <= r.u; i += blockDim.x) {
+    for (int i = r.l; i <= r.u; ++i) {
         const Rec64 &q = pd[i];
         const double4 p = ps[i];
-        const double dx = X0 - p.x, dy = X1 - p.y, dz = X2 - p.z;
+        const double dx = r.X[0] - p.x, dy = r.X[1] - p.y, dz = r.X[2] - p.z;
         for (int k = 0; k < 9; ++k) acc[k] += q.K[k];
         for (int k = 0; k < 3; ++k) {
             acc[9 + k] += q.d[k];
             acc[12 + k] += q.K[3 * k] * dx + q.K[3 * k + 1] * dy + q.K[3 * k + 2] * dz;
         }
     }
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (int k = 0; k < 15; ++k) {
-        double v = acc[k];
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-        if (lane == 0) red[k][w] = v;
+    double *o = csum + (int64_t)node * CSUM;
+    for (int k = 0; k < 15; ++k) o[k] = acc[k];
+}
+
+__global__ void __launch_bounds__(256) k_node_sums(int n, const int32_t *__restrict__ list,
+                                                   const NodeRec *__restrict__ nodes, double *__restrict__ csum,
+                                                   Rec64 *__restrict__ rec, Rec32 *__restrict__ rec32) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int node = list[t];
+    const NodeRec &r = nodes[node];
+    const int ch[2] = {node + 1, nodes[node + 1].skip};
+    double acc[15];
+    for (int k = 0; k < 15; ++k) acc[k] = 0.0;
+    for (int c = 0; c < 2; ++c) {
+        const double *cs = csum + (int64_t)ch[c] * CSUM;
+        const NodeRec &rc = nodes[ch[c]];
+        const double sx = r.X[0] - rc.X[0], sy = r.X[1] - rc.X[1], sz = r.X[2] - rc.X[2];
+        for (int k = 0; k < 12; ++k) acc[k] += cs[k];
+        for (int k = 0; k < 3; ++k) acc[12 + k] += cs[12 + k] + (cs[3 * k] * sx + cs[3 * k + 1] * sy + cs[3 * k + 2] * sz);
     }
-    __syncthreads();
-    if (threadIdx.x < 15) {
-        double v = 0.0;
-        for (int q = 0; q < (int)(blockDim.x >> 6); ++q) v += red[threadIdx.x][q];
-        red[threadIdx.x][0] = v / r.n;
+    double *o = csum + (int64_t)node * CSUM;
+    for (int k = 0; k < 15; ++k) o[k] = acc[k];
+    Rec64 e;
+    Rec32 e32;
+    for (int k = 0; k < 3; ++k) e.x[k] = e32.x[k] = r.X[k];
+    e.A = r.Ai;
+    e32.A = (float)r.Ai;
+    for (int k = 0; k < 9; ++k) {
+        e.K[k] = acc[k] / r.n;
+        e32.K[k] = (float)e.K[k];
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        Rec64 o;
-        Rec32 o32;
-        for (int k = 0; k < 3; ++k) o.x[k] = o32.x[k] = r.X[k];
-        o.A = r.Ai;
-        o32.A = (float)r.Ai;
-        for (int k = 0; k < 9; ++k) { o.K[k] = red[k][0]; o32.K[k] = (float)o.K[k]; }
-        for (int k = 0; k < 3; ++k) { o.d[k] = red[9 + k][0] + red[12 + k][0]; o32.d[k] = (float)o.d[k]; }
-        rec[node] = o;
-        rec32[node] = o32;
+    for (int k = 0; k < 3; ++k) {
+        e.d[k] = acc[9 + k] / r.n + acc[12 + k] / r.n;
+        e32.d[k] = (float)e.d[k];
     }
+    rec[node] = e;
+    rec32[node] = e32;
 }
 
 // ---------------------------------------------------------------------------
@@ -2187,7 +2204,11 @@ void launch_corners(hipStream_t st, const DevSurface &S, double threshold) {
 }
 
 void launch_cluster_data(hipStream_t st, const DevSurface &S) {
-    if (S.nInternal) k_cluster_data<<<S.nInternal, 256, 0, st>>>(S.internal, S.nodes, S.ps, S.pd, S.rec, S.rec32);
+    if (!S.nInternal) return;  // a single bucket: no cluster records
+    k_leaf_sums<<<grid_for(S.nLeaves, 256), 256, 0, st>>>(S.nLeaves, S.sumOrder, S.nodes, S.ps, S.pd, S.csum);
+    for (int i = 0; i < S.nLevels; ++i)
+        k_node_sums<<<grid_for(S.lvlCount[i], 256), 256, 0, st>>>(S.lvlCount[i], S.sumOrder + S.lvlStart[i], S.nodes,
+                                                                  S.csum, S.rec, S.rec32);
 }
 
 void launch_errors(hipStream_t st, const DevSurface &S, const WParams &P, const double *sphere60) {
