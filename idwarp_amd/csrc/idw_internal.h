@@ -28,17 +28,18 @@ struct alignas(16) NodeRec {
 };
 static_assert(sizeof(NodeRec) == 176, "NodeRec layout");
 
-// What the fast walk reads at every visit, 48 bytes per node, in a copy of
-// its own (with one zeroed record past the last node, so that the walk can
-// prefetch both successors of any node without clamping their indices).
-// A node is a bucket exactly when skip == its index + 1.
-struct alignas(16) WalkHdr {
+// What the fast walk reads at every visit, one 64-byte cache line per node,
+// in a copy of its own (with one zeroed record past the last node, so that
+// the walk can prefetch both successors of any node without clamping their
+// indices).  A node is a bucket exactly when skip == its index + 1.
+struct alignas(64) WalkHdr {
     double X[3];
     double r;              // orad for an internal node, rad for a bucket
     int32_t skip, depth;
     int32_t l, n;          // the bucket's points
+    int32_t pad[4];
 };
-static_assert(sizeof(WalkHdr) == 48, "WalkHdr layout");
+static_assert(sizeof(WalkHdr) == 64, "WalkHdr layout");
 
 struct alignas(16) NodeHdr {
     double X[3];

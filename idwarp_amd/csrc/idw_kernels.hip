@@ -910,10 +910,10 @@ __global__ void __launch_bounds__(256) k_plan_walk(int nvol, int blk0, int nbc, 
     PlanWriter pw{plan, COUNT ? 0 : waveOff[wave], COUNT ? 0 : waveOff[wave + 1], 0, 0, mismatch,
                   wbuf[COUNT ? 0 : threadIdx.x >> 6], wbuf64[COUNT ? 0 : threadIdx.x >> 6]};
     int64_t cnt = 0;
-    // Before each visit the walk touches the first and last word of both
-    // possible successors' headers (node + 1 when a lane descends, skip
-    // otherwise), which brings their cache lines into the scalar cache; after
-    // the visit it loads the chosen one, a cache hit.  Loading both headers
+    // Before each visit the walk touches a word of both possible successors'
+    // headers (node + 1 when a lane descends, skip otherwise), which brings
+    // their cache lines into the scalar cache; after the visit it loads the
+    // chosen one, a cache hit.  Loading both headers
     // whole and selecting between them took twelve scalar selects per visit
     // on a scalar-issue-bound loop.  Header nnodes is a zeroed pad: neither
     // successor needs a clamp.
@@ -925,10 +925,10 @@ __global__ void __launch_bounds__(256) k_plan_walk(int nvol, int blk0, int nbc, 
         const unsigned *pa =
             reinterpret_cast<const unsigned *>(reinterpret_cast<const char *>(whdr) + (unsigned)(node + 1) * HB);
         const unsigned *pb = reinterpret_cast<const unsigned *>(reinterpret_cast<const char *>(whdr) + (unsigned)h.skip * HB);
-        const unsigned ta0 = pa[0], ta1 = pa[HB / 4 - 1], tb0 = pb[0], tb1 = pb[HB / 4 - 1];
+        const unsigned ta = pa[0], tb = pb[0];
         const int next = walk_visit<COUNT>(h, node, v, tol, stop, pw, cnt, nodes, nnodes, cells, nearT2, nearT);
         if (next >= nnodes) break;
-        touched += (ta0 ^ ta1) + (tb0 ^ tb1);
+        touched += ta ^ tb;
         h = load_whdr(whdr, (unsigned)next * HB);
         node = next;
     }
