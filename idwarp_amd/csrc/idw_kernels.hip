@@ -913,9 +913,9 @@ __global__ void __launch_bounds__(256) k_plan_walk(int nvol, int blk0, int nbc, 
     // Before each visit the walk touches a word of both possible successors'
     // headers (node + 1 when a lane descends, skip otherwise), which brings
     // their cache lines into the scalar cache; after the visit it loads the
-    // chosen one, a cache hit.  Loading both headers
-    // whole and selecting between them took twelve scalar selects per visit
-    // on a scalar-issue-bound loop.  Header nnodes is a zeroed pad: neither
+    // chosen one, a cache hit.  Loading both headers whole and selecting
+    // between them took twelve scalar selects per visit on a
+    // scalar-issue-bound loop.  Header nnodes is a zeroed pad: neither
     // successor needs a clamp.
     constexpr unsigned HB = sizeof(WalkHdr);
     int node = 0;
@@ -932,9 +932,12 @@ __global__ void __launch_bounds__(256) k_plan_walk(int nvol, int blk0, int nbc, 
         h = load_whdr(whdr, (unsigned)next * HB);
         node = next;
     }
-    // The touches have no other use.  This adds zero to the mismatch counter
-    // under a condition the compiler cannot rule out, so they are not dropped.
-    if (!COUNT && touched == 0x9e3779b9u && (threadIdx.x & 63) == 0) atomicAdd(mismatch, 0u);
+    // The touches have no other use.  This adds zero to an output under a
+    // condition the compiler cannot rule out, so they are not dropped.
+    if (touched == 0x9e3779b9u && (threadIdx.x & 63) == 0) {
+        if (COUNT) atomicAdd(reinterpret_cast<unsigned long long *>(waveCount + wave), 0ull);
+        else atomicAdd(mismatch, 0u);
+    }
     if (COUNT) {
         if ((threadIdx.x & 63) == 0) waveCount[wave] = cnt;
     } else {
