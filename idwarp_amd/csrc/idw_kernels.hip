@@ -880,6 +880,14 @@ __device__ __forceinline__ int walk_visit(const WalkHdr &h, int node, const doub
     return uniform(next);
 }
 
+// The walk of evalNode (kd_tree.F90:619-662) for 64 volume nodes per wave:
+// records every point of every bucket the wave evaluates (evalNodeExact)
+// and every cluster some lane accepts (evalNodeApprox, :692-713), with the
+// lane masks, in depth-first order.  The decisions use only init-time data
+// (centroids, radii, error tables, denominator0) and errTol.  COUNT: terms
+// per wave only.  A bucket's points are FP64 terms when an active lane lies
+// within sqrt(nearT2) of the bucket's bounding sphere (mixed precision keeps
+// those pairs in FP64).
 template <bool COUNT>
 __global__ void __launch_bounds__(256) k_plan_walk(int nvol, int blk0, int nbc, const double4 *__restrict__ vol,
                                                    int nnodes, const WalkHdr *__restrict__ whdr,
