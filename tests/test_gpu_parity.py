@@ -704,3 +704,35 @@ def test_mixed_adjoint_of_wave_pairs(planes):
     p = np.random.default_rng(10).standard_normal(jtv.size)
     jp = gpu.warpDerivFwd(p.reshape(-1, 3), solverVec=False)
     assert abs(dxv @ jp - jtv @ p) <= 1e-6 * abs(dxv @ jp)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_single_bucket_tree(prec):
+    """A surface of 8 points (a cube, 6 quads) is one bucket: the tree has no
+    internal node, the walk visits the root bucket only and the cluster data
+    has nothing to compute.  Fast mode then equals the exact sum, so the
+    device's fast warp and reverse must match the oracle's exact ones."""
+    c = np.array([[x, y, z] for x in (0.0, 1.0) for y in (0.0, 1.0) for z in (0.0, 1.0)]) - 0.5
+    quads = [[0, 1, 3, 2], [4, 6, 7, 5], [0, 4, 5, 1], [2, 3, 7, 6], [0, 2, 6, 4], [1, 5, 7, 3]]
+    conn = np.array(quads, np.int32).ravel()
+    fs = np.full(6, 4, np.int32)
+    rng = np.random.default_rng(21)
+    d = rng.standard_normal((3000, 3))
+    xv0 = (d / np.linalg.norm(d, axis=1, keepdims=True)) * rng.uniform(0.9, 3.0, (3000, 1))
+    xs = c + 0.02 * rng.standard_normal(c.shape)
+    ora = O.OracleMesh(c, conn, fs, xv0, evalMode="exact")
+    ora.setSurfaceCoordinates(xs)
+    ref = ora.warpMesh()
+    v = synthetic.random_dxv(len(xv0), seed=3, frac=0.5)
+    refb = ora.warpDeriv(v)
+    gpu = USMesh(options=dict(gridFile=xv0, fileType="memory", evalMode="fast", errTol=1e-4, precision=prec))
+    gpu.setSurfaceDefinition(c, conn, fs)
+    gpu.setSurfaceCoordinates(xs)
+    gpu.warpMesh()
+    print("case single bucket", prec, flush=True)
+    err = np.abs(gpu.getWarpGrid() - ref.ravel()).max() / ora.Ldef0
+    gpu.warpDeriv(v, solverVec=False)
+    rb = _rel(gpu.getdXs().ravel(), refb)
+    print(prec, "max |dXv|/Ldef", err, "adjoint rel", rb)
+    assert err <= (FP64_TOL if prec == "fp64" else MIXED_TOL)
+    assert rb <= (1e-10 if prec == "fp64" else 1e-6)
