@@ -1609,6 +1609,18 @@ k_fast_rev_pair(int nvol, int nwave, const double4 *__restrict__ vol, const int3
     int64_t ia = a0, ib = b0;
     const int64_t ea = asp, eb = bsp;
     int na = 0, nb = 0, q0 = 0, q1 = 0, q2 = 0;
+    // Lane i < MQ tops up A's buffer from entry ia + i, lane MQ + i B's from
+    // ib + i.  Those entries are requested one merge round ahead (when ia and
+    // ib last moved), so a round does not wait for its loads.
+    const bool mA = lane < MQ;
+    const int mi = mA ? lane : lane - MQ;
+    auto next_entry = [&](int64_t ia_, int64_t ib_) {
+        const int64_t idx = (mA ? ia_ : ib_) + mi;
+        PlanEntry e{};
+        if (idx < (mA ? ea : eb)) e = plan[idx];
+        return e;
+    };
+    PlanEntry pre = next_entry(ia, ib);
     auto record = [&](int r) {  // x, A of a record (zero for padding); unconditional load
         const double4 v = *reinterpret_cast<const double4 *>(rec + min(max(r, 0), nrec - 1));
         return r >= 0 ? v : make_double4(0.0, 0.0, 0.0, 0.0);
@@ -1653,7 +1665,7 @@ k_fast_rev_pair(int nvol, int nwave, const double4 *__restrict__ vol, const int3
             const int lb = (int)min((int64_t)(MQ - nb), eb - ib);
             wave_sync();
             if (isA ? i < la : i < lb) {
-                const PlanEntry e = plan[isA ? ia + i : ib + i];
+                const PlanEntry e = pre;
                 const int at = (isA ? na : nb) + i;
                 if (isA) {
                     L.q.keyA[at] = plan_key(e);
@@ -1667,6 +1679,7 @@ k_fast_rev_pair(int nvol, int nwave, const double4 *__restrict__ vol, const int3
             na = uniform(na + la);
             ib += lb;
             nb = uniform(nb + lb);
+            pre = next_entry(ia, ib);
             wave_sync();
             const unsigned long long KA = ia < ea ? L.q.keyA[na - 1] : ~0ull;
             const unsigned long long KB = ib < eb ? L.q.keyB[nb - 1] : ~0ull;
