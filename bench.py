@@ -216,11 +216,10 @@ def main():
         res["mixed_speedup_vs_fp64"] = round(b["ms"] / a["ms"], 3)
         res["mixed_max_err_over_Ldef"] = float((a["xv"] - b["xv"]).abs().max().item() / ldef)
         res["mixed_adjoint_rel_err"] = float(((a["dxs"] - b["dxs"]).norm() / b["dxs"].norm()).item())
-    # roofline of the dominant kernel launch of the headline leg.  A warp that
-    # writes its plan launches the walk and the forward per volume chunk on
-    # two streams (they overlap), so the longest single launch is the
-    # reverse, one launch per step, bracketed by HIP events on the context
-    # stream (phase 4).
+    # roofline of the dominant kernel launch of the headline leg: a warp that
+    # writes its plan runs the walk, then the forward, back to back on the
+    # context stream; the reverse is the longest of the three launches of a
+    # step, bracketed by HIP events on that stream (phase 4).
     ph = head["phases"]
     # kernel names as rocprofv3 reports them; the template argument 0 is the
     # exponent mode of the default exponents (3, 5), compiled as products

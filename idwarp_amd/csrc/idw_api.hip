@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -47,6 +48,13 @@ void sphere_nominal(double *v) {
     std::memcpy(v, t, sizeof(t));
 }
 
+// Index of the first non-finite value of a host array, or -1.
+int64_t first_non_finite(const double *a, int64_t n) {
+    for (int64_t k = 0; k < n; ++k)
+        if (!std::isfinite(a[k])) return k;
+    return -1;
+}
+
 bool same_numeric_options(const idw_options &a, const idw_options &b) {
     return a.LdefFact == b.LdefFact && a.alpha == b.alpha && a.bExp == b.bExp && a.errTol == b.errTol &&
            a.evalMode == b.evalMode && a.precision == b.precision && a.nearWallFactor == b.nearWallFactor;
@@ -83,6 +91,15 @@ struct idw_ctx {
     double ms[6] = {0, 0, 0, 0, 0, 0};
     bool fwdTimed = false, revTimed = false;
     bool capturing = false;  // work is being recorded into a HIP graph: no events, syncs or allocations
+    bool batchPlan = false;  // a batch is running: keep the filled plans (the options keep reusePlan as set)
+    // fill walks of the *_dev entry points report a plan that disagreed with
+    // its counting pass through these pinned flags (one per mirror pass),
+    // read at a later call once the event is reached
+    unsigned int *mmHost = nullptr;
+    hipEvent_t mmEvent = nullptr;
+    bool mmPending = false;
+    int64_t failAllocAfter = -1;  // test hook IDW_TEST_FAIL_ALLOC_AFTER: device allocations allowed before failing
+    int64_t nAllocs = 0;
     uint64_t gen = 0;        // bumped whenever device buffers, plans or options change (recorded graphs check it)
     std::string restartFile;        // denominator estimate + costs file (initializeWarping.F90:98-312)
     std::string restartStatus;      // "", "loaded", "saved"
@@ -95,6 +112,10 @@ template <class T>
 int dalloc(idw_ctx *c, T **p, size_t count) {
     *p = nullptr;
     if (count == 0) count = 1;
+    if (c->failAllocAfter >= 0 && c->nAllocs >= c->failAllocAfter)
+        return fail(IDW_ENOMEM, "hipMalloc(%zu bytes): failure injected by IDW_TEST_FAIL_ALLOC_AFTER=%lld",
+                    count * sizeof(T), (long long)c->failAllocAfter);
+    ++c->nAllocs;
     void *q = nullptr;
     hipError_t e = hipMalloc(&q, count * sizeof(T));
     if (e != hipSuccess) return fail(IDW_ENOMEM, "hipMalloc(%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
@@ -358,6 +379,7 @@ int size_one_plan(idw_ctx *c, int kk) {
 }
 
 int check_plan(idw_ctx *c) {
+    c->mmPending = false;  // superseded: the synchronous check below reads the device flags
     for (idw::DevPlan &P : c->plans) {
         if (!P.mismatch) continue;
         unsigned int mm = 0;
@@ -369,6 +391,39 @@ int check_plan(idw_ctx *c) {
             return fail(IDW_EHIP, "fast-mode plan of %u waves disagreed with its counting pass", mm);
         }
     }
+    return IDW_OK;
+}
+
+// The outcome of the fill walks of an earlier *_dev call, if they have
+// completed: a plan that disagreed with its counting pass is dropped (the
+// next warp sizes and fills it again) and reported by this call.
+int poll_plan(idw_ctx *c) {
+    if (!c->mmPending || c->capturing) return IDW_OK;
+    const hipError_t q = hipEventQuery(c->mmEvent);
+    if (q == hipErrorNotReady) return IDW_OK;
+    c->mmPending = false;
+    if (q != hipSuccess) return fail(IDW_EHIP, "hipEventQuery: %s", hipGetErrorString(q));
+    for (size_t kk = 0; kk < c->plans.size() && kk < (size_t)(1 << idw::MAX_PLANES); ++kk)
+        if (c->mmHost[kk]) {
+            c->planValid = false;
+            c->plans[kk].sized = false;
+            return fail(IDW_EHIP, "the fast-mode plan written by an earlier warp disagreed with its counting pass "
+                                  "in %u waves; that warp's results are invalid", c->mmHost[kk]);
+        }
+    return IDW_OK;
+}
+
+// Mixed precision: the near-wall distance below which pairs stay FP64 must
+// keep the FP32 weights of every farther pair finite.
+int check_mixed_range(const idw_ctx *c) {
+    if (c->opt.precision != IDW_PREC_MIXED) return IDW_OK;
+    const double f = c->opt.nearWallFactor;
+    if (!(f >= idw::mixed_near_wall_floor(c->initOpt.aExp, c->initOpt.bExp, c->opt.alpha)))
+        return fail(IDW_EINVAL, "nearWallFactor %g is below the mixed-precision floor %g for the tree's exponents", f,
+                    idw::mixed_near_wall_floor(c->initOpt.aExp, c->initOpt.bExp, c->opt.alpha));
+    if (!(f * c->hs.ldef0 * c->opt.LdefFact >= idw::MIN_NEAR_WALL_DISTANCE))
+        return fail(IDW_EINVAL, "mixed precision needs nearWallFactor * Ldef >= %g (got %g)", idw::MIN_NEAR_WALL_DISTANCE,
+                    f * c->hs.ldef0 * c->opt.LdefFact);
     return IDW_OK;
 }
 
@@ -385,6 +440,8 @@ int run_warp(idw_ctx *c, const double *xs, double *xv) {
     if (!c->surfInit) return fail(IDW_ESTATE, "warp before initialize_warping");
     if (!c->volSet) return fail(IDW_ESTATE, "no volume nodes set");
     const int fast = c->opt.evalMode == IDW_EVAL_FAST;
+    TRY(poll_plan(c));
+    TRY(check_mixed_range(c));
     idw::WParams P = make_params(c, false);
     if (fast) TRY(size_plan(c));
     TRY(ensure_mirror_buffers(c));
@@ -396,7 +453,7 @@ int run_warp(idw_ctx *c, const double *xs, double *xv) {
     if (fast) {
         for (int kk = 0; kk < nloop(c); ++kk) {
             idw::DevPlan &pl = c->plans[kk];
-            if (!c->opt.reusePlan || !pl.filled || pl.errTol != c->opt.errTol || pl.nearT2 != nt2) {
+            if (!(c->opt.reusePlan || c->batchPlan) || !pl.filled || pl.errTol != c->opt.errTol || pl.nearT2 != nt2) {
                 if (c->capturing && !(pl.filled && pl.errTol == c->opt.errTol && pl.nearT2 == nt2))
                     return fail(IDW_ESTATE, "a captured warp needs the plan of a previous warp at the same options");
                 fill[kk] = 1;
@@ -409,9 +466,14 @@ int run_warp(idw_ctx *c, const double *xs, double *xv) {
         idw::launch_zero_u32(c->st, pl.mismatch, 1);
         if (c->V.n) idw::launch_plan_fill(c->st, c->S, c->V, mirror_of(c, kk), c->opt.errTol, nt2, pl);
         HIPCHK(hipGetLastError());
+        if (!c->capturing && kk < (1 << idw::MAX_PLANES)) {
+            HIPCHK(hipMemcpyAsync(c->mmHost + kk, pl.mismatch, sizeof(unsigned int), hipMemcpyDeviceToHost, c->st));
+            c->mmPending = true;
+        }
         pl.filled = true;
         pl.nearT2 = nt2;
     }
+    if (c->mmPending && !c->capturing) HIPCHK(hipEventRecord(c->mmEvent, c->st));
     REC(c, 2);
     for (int kk = 0; kk < nloop(c) && c->V.n; ++kk) {
         P.pass = pass_of(c, kk);
@@ -442,6 +504,8 @@ int run_deriv(idw_ctx *c, const double *xs, const double *dxv, double *dxs) {
     if (!c->surfInit) return fail(IDW_ESTATE, "warp_deriv before initialize_warping");
     if (!c->volSet) return fail(IDW_ESTATE, "no volume nodes set");
     const int fast = c->opt.evalMode == IDW_EVAL_FAST;
+    TRY(poll_plan(c));
+    TRY(check_mixed_range(c));
     // The denominator (and the plan) depend on the options, not on the surface coordinates.
     if (!c->denValid || !same_numeric_options(c->denOpt, c->opt) || (fast && !c->planValid))
         TRY(run_warp(c, xs, c->xv));
@@ -473,6 +537,8 @@ int run_deriv_fwd(idw_ctx *c, const double *xs, const double *dxs, double *dxv) 
     if (!c->surfInit) return fail(IDW_ESTATE, "warp_deriv_fwd before initialize_warping");
     if (!c->volSet) return fail(IDW_ESTATE, "no volume nodes set");
     const int fast = c->opt.evalMode == IDW_EVAL_FAST;
+    TRY(poll_plan(c));
+    TRY(check_mixed_range(c));
     if (fast && (!c->planValid || !same_numeric_options(c->denOpt, c->opt))) TRY(run_warp(c, xs, c->xv));
     TRY(ensure_mirror_buffers(c));
     idw::WParams P = make_params(c, false);
@@ -506,11 +572,21 @@ void collect_timings(idw_ctx *c) {
     }
 }
 
-bool valid_opts(const idw_options *o) {
-    return o && o->bucketSize >= 1 && (o->evalMode == 0 || o->evalMode == 1) && (o->precision == 0 || o->precision == 1) &&
-           (o->reusePlan == 0 || o->reusePlan == 1) &&
-           std::isfinite(o->aExp) && std::isfinite(o->bExp) && o->LdefFact > 0 && o->errTol >= 0 &&
-           o->nearWallFactor >= 0;
+// Why the options are unusable, or nullptr.
+const char *invalid_opts(const idw_options *o) {
+    if (!o) return "null options";
+    if (o->bucketSize < 1) return "bucketSize must be at least 1";
+    if (!(o->evalMode == 0 || o->evalMode == 1)) return "evalMode must be exact (0) or fast (1)";
+    if (!(o->precision == 0 || o->precision == 1)) return "precision must be fp64 (0) or mixed (1)";
+    if (!(o->reusePlan == 0 || o->reusePlan == 1)) return "reusePlan must be 0 or 1";
+    if (!std::isfinite(o->aExp) || !std::isfinite(o->bExp)) return "aExp and bExp must be finite";
+    if (!(o->LdefFact > 0)) return "LdefFact must be positive";
+    if (!(o->errTol >= 0)) return "errTol must be non-negative";
+    if (!(o->nearWallFactor >= 0)) return "nearWallFactor must be non-negative";
+    if (o->precision == IDW_PREC_MIXED &&
+        !(o->nearWallFactor >= idw::mixed_near_wall_floor(o->aExp, o->bExp, o->alpha)))
+        return "nearWallFactor is below the mixed-precision floor (FP32 weights of nearer pairs would overflow)";
+    return nullptr;
 }
 
 }  // namespace
@@ -570,7 +646,7 @@ const char *idw_last_error(void) { return t_err.c_str(); }
 int idw_create(idw_ctx **out, int device, const idw_options *opt) {
     if (!out) return fail(IDW_EINVAL, "null output pointer");
     *out = nullptr;
-    if (!valid_opts(opt)) return fail(IDW_EINVAL, "invalid options");
+    if (const char *why = invalid_opts(opt)) return fail(IDW_EINVAL, "invalid options: %s", why);
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(IDW_ENODEV, "no HIP device available");
     if (device < 0 || device >= ndev) return fail(IDW_ENODEV, "device %d out of range (%d devices)", device, ndev);
@@ -585,6 +661,10 @@ int idw_create(idw_ctx **out, int device, const idw_options *opt) {
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
     for (int k = 0; k < 8 && e == hipSuccess; ++k) e = hipEventCreate(&c->ev[k]);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->mmEvent, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&c->mmHost, sizeof(unsigned int) << idw::MAX_PLANES);
+    if (e == hipSuccess) std::memset(c->mmHost, 0, sizeof(unsigned int) << idw::MAX_PLANES);
+    if (const char *fa = std::getenv("IDW_TEST_FAIL_ALLOC_AFTER")) c->failAllocAfter = std::atoll(fa);
     if (e != hipSuccess) {
         idw_destroy(c);
         return fail(IDW_EHIP, "stream/event creation: %s", hipGetErrorString(e));
@@ -600,6 +680,8 @@ int idw_destroy(idw_ctx *c) {
     for (void *p : c->allocs) (void)hipFree(p);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
+    if (c->mmEvent) (void)hipEventDestroy(c->mmEvent);
+    if (c->mmHost) (void)hipHostFree(c->mmHost);
     if (c->st) (void)hipStreamDestroy(c->st);
     delete c;
     return IDW_OK;
@@ -607,7 +689,7 @@ int idw_destroy(idw_ctx *c) {
 
 int idw_set_options(idw_ctx *c, const idw_options *opt) {
     if (!c) return fail(IDW_EINVAL, "null context");
-    if (!valid_opts(opt)) return fail(IDW_EINVAL, "invalid options");
+    if (const char *why = invalid_opts(opt)) return fail(IDW_EINVAL, "invalid options: %s", why);
     if (std::memcmp(&c->opt, opt, sizeof(idw_options)) != 0) ++c->gen;
     c->opt = *opt;
     return IDW_OK;
@@ -618,6 +700,8 @@ int idw_set_volume_nodes(idw_ctx *c, const double *xv0, int64_t ndof) {
     if (ndof < 0 || ndof % 3) return fail(IDW_EINVAL, "volume DOF count %lld is not a multiple of 3", (long long)ndof);
     const int64_t n = ndof / 3;
     if (n >= INT32_MAX / 2) return fail(IDW_EINVAL, "too many volume nodes for 32-bit indexing");
+    if (const int64_t bad = first_non_finite(xv0, ndof); bad >= 0)
+        return fail(IDW_EINVAL, "volume coordinate %lld is not finite", (long long)bad);
     TRY(ensure_device(c));
     dfree(c, c->xv0w); dfree(c, c->xv); dfree(c, c->dxv);
     dfree(c, c->V.vol); dfree(c, c->V.perm); dfree(c, c->V.den); dfree(c, c->V.numb);
@@ -709,6 +793,10 @@ int idw_get_restart_info(idw_ctx *c, int32_t *status, double *costs, int64_t n) 
     return IDW_OK;
 }
 
+double idw_mixed_near_wall_floor(double aExp, double bExp, double alpha) {
+    return idw::mixed_near_wall_floor(aExp, bExp, alpha);
+}
+
 int idw_point_reduce(const double *pts, int64_t n, double tol, double *unique_pts, int32_t *link, int64_t *n_unique) {
     if (n < 0 || (n && (!pts || !unique_pts || !link)) || !n_unique) return fail(IDW_EINVAL, "bad argument");
     if (n >= INT32_MAX) return fail(IDW_EINVAL, "too many points");
@@ -722,12 +810,18 @@ int idw_initialize_warping(idw_ctx *c, const double *pts, int64_t n_pts, const d
     if (!c || !pts || !unique_pts || !link || (n_faces && (!face_sizes || !face_conn)))
         return fail(IDW_EINVAL, "null argument");
     if (n_pts <= 0 || n_unique <= 0 || n_unique > n_pts) return fail(IDW_EINVAL, "bad point counts");
+    if (const int64_t bad = first_non_finite(pts, 3 * n_pts); bad >= 0)
+        return fail(IDW_EINVAL, "surface coordinate %lld is not finite", (long long)bad);
+    if (first_non_finite(unique_pts, 3 * n_unique) >= 0) return fail(IDW_EINVAL, "unique surface points are not finite");
     TRY(ensure_device(c));
     std::string msg;
     idw::HostSurface hs;
     if (!idw::build_surface(unique_pts, n_unique, link, n_pts, face_sizes, n_faces, face_conn, n_conn,
                             c->opt.bucketSize, hs, msg))
         return fail(IDW_EINVAL, "%s", msg.c_str());
+    if (!idw::walk_offsets_fit((int64_t)hs.nodes.size()))
+        return fail(IDW_EINVAL, "kd-tree of %lld nodes exceeds the walk's %lld-node limit (32-bit header offsets)",
+                    (long long)hs.nodes.size(), (long long)idw::MAX_WALK_NODES);
     // free a previous surface
     idw::DevSurface &S = c->S;
     void *olds[] = {S.nodes, S.whdr, S.rec, S.rec32, S.internal, S.sumOrder, S.csum, S.ps, S.normals0, S.nprops, S.corner, S.xuInd,
@@ -857,6 +951,8 @@ int idw_set_surface_coordinates(idw_ctx *c, const double *xs, int64_t ndof) {
     if (ndof != 3 * c->npts)
         return fail(IDW_EINVAL, "Incorrect length of coordinates: expected %lld, received %lld", (long long)(3 * c->npts),
                     (long long)ndof);
+    if (const int64_t bad = first_non_finite(xs, ndof); bad >= 0)
+        return fail(IDW_EINVAL, "surface coordinate %lld is not finite", (long long)bad);
     TRY(ensure_device(c));
     HIPCHK(hipMemcpyAsync(c->xs, xs, sizeof(double) * ndof, hipMemcpyHostToDevice, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
@@ -1261,22 +1357,29 @@ int idw_batch_run(idw_batch *b, void *hip_stream) {
         HIPCHK(hipEventRecord(b->in, user));
         for (idw_ctx *c : cs) {
             HIPCHK(hipStreamWaitEvent(c->st, b->in, 0));
-            if (!c->opt.reusePlan) ++c->gen;
-            c->opt.reusePlan = 1;
+            c->batchPlan = true;
         }
-        for (auto &o : b->ops) TRY(batch_op(o));
+        int r = IDW_OK;
+        for (auto &o : b->ops)
+            if (r == IDW_OK) r = batch_op(o);
         for (idw_ctx *c : cs) {
-            HIPCHK(hipStreamSynchronize(c->st));
-            TRY(check_plan(c));
+            if (r == IDW_OK && hipStreamSynchronize(c->st) != hipSuccess) r = fail(IDW_EHIP, "batch warm-up failed");
+            if (r == IDW_OK) r = check_plan(c);
+        }
+        if (r != IDW_OK) {
+            for (idw_ctx *c : cs) c->batchPlan = false;
+            return r;
         }
         // choose the graph layout by timing both (two replays each)
         float t = 0.f;
-        for (int layout = 0; layout < 2; ++layout) {
-            TRY(batch_capture(b, layout));
-            TRY(batch_time(b, &t));
+        for (int layout = 0; layout < 2 && r == IDW_OK; ++layout) {
+            r = batch_capture(b, layout);
+            if (r == IDW_OK) r = batch_time(b, &t);
             b->ms[layout] = t / 2.0;
         }
-        TRY(batch_capture(b, b->ms[0] <= b->ms[1] ? 0 : 1));
+        if (r == IDW_OK) r = batch_capture(b, b->ms[0] <= b->ms[1] ? 0 : 1);
+        for (idw_ctx *c : cs) c->batchPlan = false;
+        TRY(r);
     }
     HIPCHK(hipEventRecord(b->in, user));
     HIPCHK(hipStreamWaitEvent(b->origin, b->in, 0));

@@ -41,6 +41,21 @@ struct alignas(64) WalkHdr {
 };
 static_assert(sizeof(WalkHdr) == 64, "WalkHdr layout");
 
+// The walk addresses headers by 32-bit byte offsets: the tree's nodes plus
+// the pad record must fit below 4 GiB of headers (67,108,863 nodes).
+constexpr int64_t MAX_WALK_NODES = (int64_t)(0x100000000ll / (int64_t)sizeof(WalkHdr)) - 1;
+inline bool walk_offsets_fit(int64_t nnodes) { return nnodes >= 1 && nnodes <= MAX_WALK_NODES; }
+
+// Mixed precision evaluates weights A (q^aExp + alpha^bExp q^bExp), q = Ldef/d,
+// in FP32 for every pair farther than nearWallFactor * Ldef, so q <=
+// 1 / nearWallFactor there.  The smallest nearWallFactor that keeps both
+// powers below 1e32 (six decades under the FP32 maximum, for the area factor
+// and the sums of up to 128 weighted terms).
+double mixed_near_wall_floor(double aExp, double bExp, double alpha);
+// The FP32 fast-mode weights form 1/d^3 before scaling by Ldef^3: the
+// near-wall distance itself must keep 1/d^3 below 1e36.
+constexpr double MIN_NEAR_WALL_DISTANCE = 1e-12;
+
 struct alignas(16) NodeHdr {
     double X[3];
     double orad;

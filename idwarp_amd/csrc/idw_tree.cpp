@@ -48,6 +48,16 @@ int64_t point_reduce(const double *pts, int64_t n, double tol, double *unique, i
     return nu;
 }
 
+double mixed_near_wall_floor(double aExp, double bExp, double alpha) {
+    // q^e <= c  <=>  q <= c^(1/e); the floor is 1 / the smallest such q
+    const double cap = 1e32;
+    double qmax = INFINITY;
+    if (aExp > 0.0) qmax = std::min(qmax, std::pow(cap, 1.0 / aExp));
+    const double ab = std::pow(std::fabs(alpha), bExp);
+    if (bExp > 0.0 && ab > 0.0) qmax = std::min(qmax, std::pow(cap / ab, 1.0 / bExp));
+    return std::isfinite(qmax) ? 1.0 / qmax : 0.0;
+}
+
 namespace {
 
 struct Builder {

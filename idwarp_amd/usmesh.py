@@ -107,7 +107,17 @@ class USMesh:
             raise Error("Option '%s' must be one of %s" % (name, spec[1]))
         if spec[0] is float and isinstance(value, (int, np.integer)) and not isinstance(value, bool):
             value = float(value)
-        self._opts[key] = (self._opts[key][0], value)
+        old = self._opts[key]
+        self._opts[key] = (old[0], value)
+        if key in ("precision", "nearwallfactor", "aexp", "bexp", "alpha") and self.getOption("precision") == "mixed":
+            floor = _lib.load().idw_mixed_near_wall_floor(self.getOption("aExp"), self.getOption("bExp"),
+                                                          self.getOption("alpha"))
+            if not self.getOption("nearWallFactor") >= floor:
+                self._opts[key] = old
+                raise Error("nearWallFactor %g is below the mixed-precision floor %.3g for aExp=%g, bExp=%g, "
+                            "alpha=%g: FP32 weights of nearer pairs would overflow" %
+                            (self.getOption("nearWallFactor"), floor, self.getOption("aExp"),
+                             self.getOption("bExp"), self.getOption("alpha")))
 
     def getOption(self, name):
         key = name.lower()

@@ -59,7 +59,11 @@ typedef struct idw_options {
                                1 = keep the walk's plan while errTol, the grid and the surface
                                definition are unchanged (its decisions depend on nothing else) */
     double nearWallFactor;  /* mixed mode: buckets with a volume node closer than
-                               nearWallFactor*Ldef stay FP64 (1e-6: keeps q^5 < 1e30 in FP32) */
+                               nearWallFactor*Ldef stay FP64 (1e-6: keeps q^5 < 1e30 in FP32).
+                               With precision mixed it must be at least the floor that keeps
+                               q^aExp and alpha^bExp q^bExp below 1e32 for q = 1/nearWallFactor
+                               (9.95e-8 for the defaults), and nearWallFactor*Ldef >= 1e-12;
+                               smaller values are rejected with IDW_EINVAL */
 } idw_options;
 
 typedef struct idw_ctx idw_ctx;
@@ -108,6 +112,11 @@ int idw_set_restart_file(idw_ctx *ctx, const char *path);
  * the costs it holds (n = n_vol; costs may be NULL).  No counterpart in the
  * reference. */
 int idw_get_restart_info(idw_ctx *ctx, int32_t *status, double *costs, int64_t n);
+
+/* Smallest nearWallFactor that precision mixed accepts for these exponents
+ * and alpha (see idw_options).  Context-free.  No counterpart in the
+ * reference, which has no mixed precision. */
+double idw_mixed_near_wall_floor(double aExp, double bExp, double alpha);
 
 /* pointreduce (libidwarp.pyf:195-202, src/utils/pointReduce.F90:1-83):
  * merge points within tol.  pts and unique_pts are 3*n; link (n) is 1-based
@@ -163,7 +172,13 @@ int idw_warp_deriv_fwd(idw_ctx *ctx, const double *dxs, int64_t ndof_s, double *
 
 /* Device-resident forms of warpMesh and warpDeriv for frameworks that keep
  * coordinates on the GPU (torch autograd wrapper).  d_xs: 3*n_pts surface,
- * d_xv: 3*n_vol output, d_dxv: 3*n_vol seed, d_dxs: 3*n_pts output. */
+ * d_xv: 3*n_vol output, d_dxv: 3*n_vol seed, d_dxs: 3*n_pts output.
+ * They do not wait for the device: a fast-mode plan whose fill walk
+ * disagreed with its counting pass (the host forms report it at once) is
+ * reported with IDW_EHIP by the first later call on the context that finds
+ * the warp completed, or by idw_get_timings, which waits for it.  The plan
+ * is then dropped and rebuilt by the next warp.  Coordinates are not checked
+ * for NaN here (the host forms reject non-finite input with IDW_EINVAL). */
 int idw_warp_mesh_dev(idw_ctx *ctx, const double *d_xs, double *d_xv, void *hip_stream);
 int idw_warp_deriv_dev(idw_ctx *ctx, const double *d_xs, const double *d_dxv, double *d_dxs, void *hip_stream);
 /* Device-resident warpDerivFwd: d_dxs 3*n_pts tangent, d_dxv 3*n_vol output. */
