@@ -25,7 +25,7 @@ EXPORTS = (
     "idw_get_denominator0", "idw_get_timings", "idw_fast_counts", "idw_warp_deriv_fwd", "idw_warp_deriv_fwd_dev",
     "idw_set_symmetry_planes", "idw_batch_create", "idw_batch_destroy", "idw_batch_add_warp",
     "idw_batch_add_warp_deriv", "idw_batch_run", "idw_batch_info", "idw_set_restart_file",
-    "idw_get_restart_info", "idw_mixed_near_wall_floor",
+    "idw_get_restart_info", "idw_mixed_near_wall_floor", "idw_batch_stats",
 )
 
 
@@ -114,6 +114,7 @@ def load():
         "idw_set_restart_file": (i32, [vp, ctypes.c_char_p]),
         "idw_get_restart_info": (i32, [vp, P(i32), vp, i64]),
         "idw_mixed_near_wall_floor": (d, [d, d, d]),
+        "idw_batch_stats": (i32, [vp, P(i64), P(i64)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

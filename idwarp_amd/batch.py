@@ -67,6 +67,14 @@ class Batch:
         _lib.check(self._L.idw_batch_info(self._h, ctypes.byref(layout), ms))
         return layout.value, [ms[0], ms[1]]
 
+    def stats(self):
+        """(recordings, graph launches) so far: a recording is the eager
+        warm-up run, the timed layouts and the graph kept; later runs that
+        find every mesh unchanged only launch the graph."""
+        rec, lau = ctypes.c_int64(0), ctypes.c_int64(0)
+        _lib.check(self._L.idw_batch_stats(self._h, ctypes.byref(rec), ctypes.byref(lau)))
+        return rec.value, lau.value
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:

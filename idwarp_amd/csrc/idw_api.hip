@@ -489,15 +489,49 @@ int run_warp(idw_ctx *c, const double *xs, double *xv) {
     return IDW_OK;
 }
 
-int ensure_partial(idw_ctx *c) {
+// Volume chunks of the exact reverse: enough block rows (chunks x 256-node
+// surface blocks) to fill the device, each chunk at least 64 volume nodes.
+int exact_rev_chunks(const idw_ctx *c) {
     const int64_t nbs = (c->S.n + 255) / 256;
     int nchunk = (int)std::min<int64_t>(128, std::max<int64_t>(1, (4096 + nbs - 1) / nbs));
     while (nchunk > 1 && c->V.n / nchunk < 64) nchunk /= 2;
+    return nchunk;
+}
+
+int ensure_partial(idw_ctx *c) {
+    const int nchunk = exact_rev_chunks(c);
     if (c->partial && nchunk == c->nchunk) return IDW_OK;
     dfree(c, c->partial);
     TRY(dalloc(c, &c->partial, (size_t)nchunk * c->S.n * 12));
     c->nchunk = nchunk;
     return IDW_OK;
+}
+
+// The kernels of a reverse product, given the exact mode's chunk partials
+// (nchunk * n_unique * 12 doubles); enqueues work only.
+int deriv_kernels(idw_ctx *c, const double *xs, const double *dxv, double *dxs, double *partial, int nchunk) {
+    const int fast = c->opt.evalMode == IDW_EVAL_FAST;
+    REC(c, 4);
+    TRY(surface_pass(c, xs, false));
+    idw::launch_reverse_prep(c->st, c->S, fast || c->V.n == 0);
+    HIPCHK(hipGetLastError());
+    REC(c, 5);
+    for (int kk = 0; kk < nloop(c) && c->V.n; ++kk)
+        idw::launch_reverse(c->st, c->S, c->V, make_params(c, false), mirror_of(c, kk), fast, c->opt.precision,
+                            partial, nchunk, dxv, fast ? &c->plans[kk] : nullptr);
+    HIPCHK(hipGetLastError());
+    REC(c, 6);
+    idw::launch_nodal_reverse(c->st, c->S, xs, fast, c->opt.useRotations, dxs, c->npts);
+    HIPCHK(hipGetLastError());
+    REC(c, 7);
+    c->revTimed = !c->capturing;
+    return IDW_OK;
+}
+
+bool deriv_needs_warp(const idw_ctx *c) {
+    // the denominator (and the plan) depend on the options, not on the surface coordinates
+    return !c->denValid || !same_numeric_options(c->denOpt, c->opt) ||
+           (c->opt.evalMode == IDW_EVAL_FAST && !c->planValid);
 }
 
 int run_deriv(idw_ctx *c, const double *xs, const double *dxv, double *dxs) {
@@ -506,26 +540,10 @@ int run_deriv(idw_ctx *c, const double *xs, const double *dxv, double *dxs) {
     const int fast = c->opt.evalMode == IDW_EVAL_FAST;
     TRY(poll_plan(c));
     TRY(check_mixed_range(c));
-    // The denominator (and the plan) depend on the options, not on the surface coordinates.
-    if (!c->denValid || !same_numeric_options(c->denOpt, c->opt) || (fast && !c->planValid))
-        TRY(run_warp(c, xs, c->xv));
+    if (deriv_needs_warp(c)) TRY(run_warp(c, xs, c->xv));
     if (!fast) TRY(ensure_partial(c));
     TRY(ensure_mirror_buffers(c));
-    REC(c, 4);
-    TRY(surface_pass(c, xs, false));
-    idw::launch_reverse_prep(c->st, c->S, fast || c->V.n == 0);
-    HIPCHK(hipGetLastError());
-    REC(c, 5);
-    for (int kk = 0; kk < nloop(c) && c->V.n; ++kk)
-        idw::launch_reverse(c->st, c->S, c->V, make_params(c, false), mirror_of(c, kk), fast, c->opt.precision,
-                            c->partial, c->nchunk, dxv, fast ? &c->plans[kk] : nullptr);
-    HIPCHK(hipGetLastError());
-    REC(c, 6);
-    idw::launch_nodal_reverse(c->st, c->S, xs, fast, c->opt.useRotations, dxs, c->npts);
-    HIPCHK(hipGetLastError());
-    REC(c, 7);
-    c->revTimed = !c->capturing;
-    return IDW_OK;
+    return deriv_kernels(c, xs, dxv, dxs, c->partial, c->nchunk);
 }
 
 // warpDerivFwd (warpDerivFwd.F90:62-99): dxv = (dXv/dXs) dxs.  The tangent
@@ -1194,6 +1212,7 @@ struct idw_batch {
     std::vector<hipEvent_t> events;                      // events recorded into the graph
     hipEvent_t in = nullptr, out = nullptr;              // ordering with the caller's stream
     std::vector<std::pair<idw_ctx *, uint64_t>> sig;     // context generations at recording time
+    int64_t recordings = 0, launches = 0;                // warm-up + recording passes, graph launches
 };
 
 namespace {
@@ -1353,6 +1372,7 @@ int idw_batch_run(idw_batch *b, void *hip_stream) {
         // uses the same accept/recurse decisions (they depend on the mesh and
         // errTol only).
         batch_drop_graph(b);
+        ++b->recordings;
         std::vector<idw_ctx *> cs = batch_contexts(b);
         HIPCHK(hipEventRecord(b->in, user));
         for (idw_ctx *c : cs) {
@@ -1384,8 +1404,16 @@ int idw_batch_run(idw_batch *b, void *hip_stream) {
     HIPCHK(hipEventRecord(b->in, user));
     HIPCHK(hipStreamWaitEvent(b->origin, b->in, 0));
     HIPCHK(hipGraphLaunch(b->exec, b->origin));
+    ++b->launches;
     HIPCHK(hipEventRecord(b->out, b->origin));
     HIPCHK(hipStreamWaitEvent(user, b->out, 0));
+    return IDW_OK;
+}
+
+int idw_batch_stats(idw_batch *b, int64_t *recordings, int64_t *launches) {
+    if (!b) return fail(IDW_EINVAL, "null batch");
+    if (recordings) *recordings = b->recordings;
+    if (launches) *launches = b->launches;
     return IDW_OK;
 }
 

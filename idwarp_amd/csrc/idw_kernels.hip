@@ -606,7 +606,8 @@ __global__ void __launch_bounds__(256) k_exact_fwd(int nvol, const double4 *__re
                 } else {
                     const Rec32 &q = pd32[i];
                     const float fx = (float)rx, fy = (float)ry, fz = (float)rz;
-                    const float lo = Lf * __frsqrt_rn(fx * fx + fy * fy + fz * fz);
+                    // |r|^2 + eps (evalNodeExact's eps) in FP64, rounded once
+                    const float lo = Lf * __frsqrt_rn((float)(d2 + EPS_W));
                     const float w = (float)s.w * wfun<EXPM>(lo, P);
                     a0 += w * (q.K[0] * fx + q.K[1] * fy + q.K[2] * fz + q.d[0]);
                     a1 += w * (q.K[3] * fx + q.K[4] * fy + q.K[5] * fz + q.d[1]);
@@ -670,8 +671,7 @@ __global__ void __launch_bounds__(256) k_exact_rev(int nvol, int chunk, const do
                 const double lo = P.Ldef * rsqrt_nr(d2 + EPS_W);
                 w = s.w * wfun<EXPM>(lo, P);
             } else {
-                const float fx = (float)rx, fy = (float)ry, fz = (float)rz;
-                const float lo = (float)P.Ldef * __frsqrt_rn(fx * fx + fy * fy + fz * fz);
+                const float lo = (float)P.Ldef * __frsqrt_rn((float)(d2 + EPS_W));
                 w = (double)((float)s.w * wfun<EXPM>(lo, P));
             }
         }
@@ -1228,6 +1228,7 @@ __device__ __forceinline__ void fwd_terms(int64_t a, int64_t b, const PlanEntry 
                 const float Bv = C.Bf[tr][n];
                 const unsigned long long m = C.mask[tr] >> n;
                 const double x0 = C.X[tr][0], x1 = C.X[tr][1], x2 = C.X[tr][2], A = C.X[tr][3];
+                const double eps = C.eps[tr];
                 const float Af = (float)A, AL3 = (float)(A * L3);
 #pragma unroll
                 for (int t = 0; t < 4; ++t) {
@@ -1236,7 +1237,8 @@ __device__ __forceinline__ void fwd_terms(int64_t a, int64_t b, const PlanEntry 
                     // (about one block in seven)
                     if (!on) continue;
                     const double rx = vx[t] - x0, ry = vy[t] - x1, rz = vz[t] - x2;
-                    const float w = weight_f32<EXPM>((float)(rx * rx + ry * ry + rz * rz), Af, AL3, Lf, a5L2, P);
+                    // |r|^2 + eps in FP64 (eps: points only), rounded once
+                    const float w = weight_f32<EXPM>((float)(rx * rx + ry * ry + rz * rz + eps), Af, AL3, Lf, a5L2, P);
                     F[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(lane_in(on) ? w : 0.0f, Bv, F[t], 0, 0, 0);
                 }
             }
@@ -1756,6 +1758,7 @@ k_fast_rev_pair(int nvol, int nwave, const double4 *__restrict__ vol, const int3
             L.idx[m] = idx;
         }
         const float Af = (float)T.xa.w, AL3 = (float)(T.xa.w * L3);
+        const double epsT = T.rec >= nnodes ? EPS_W : 0.0;  // points carry evalNodeExact's eps, clusters none
         F0 = fvec4{0.0f, 0.0f, 0.0f, 0.0f};
         F1 = F0;
 #pragma unroll
@@ -1767,7 +1770,7 @@ k_fast_rev_pair(int nvol, int nwave, const double4 *__restrict__ vol, const int3
                 const double4 c = L.node[64 * h + 4 * q + R];
                 const bool on = (mR >> (4 * q)) & 1ull;
                 const double rx = c.x - T.xa.x, ry = c.y - T.xa.y, rz = c.z - T.xa.z;
-                const float wf = weight_f32<EXPM>((float)(rx * rx + ry * ry + rz * rz), Af, AL3, Lf, a5L2, P);
+                const float wf = weight_f32<EXPM>((float)(rx * rx + ry * ry + rz * rz + epsT), Af, AL3, Lf, a5L2, P);
                 const float w = on ? wf : 0.0f;
                 const float g = h == 0 ? GA[q] : GB[q];
                 if (q & 1) F1 = __builtin_amdgcn_mfma_f32_16x16x4f32(w, g, F1, 0, 0, 0);

@@ -236,6 +236,9 @@ int idw_batch_run(idw_batch *batch, void *hip_stream);
 /* The chosen layout (0: a branch per context, 1: a chain) and the measured
  * ms of one replay of each layout. */
 int idw_batch_info(idw_batch *batch, int *layout, double *ms2);
+/* How often the batch was (re)recorded (warm-up run, layouts timed, graph
+ * recorded) and how many graph launches its runs made. */
+int idw_batch_stats(idw_batch *batch, int64_t *recordings, int64_t *launches);
 
 #ifdef __cplusplus
 }
