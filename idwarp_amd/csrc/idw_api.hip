@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -100,6 +101,8 @@ struct idw_ctx {
     bool mmPending = false;
     int64_t failAllocAfter = -1;  // test hook IDW_TEST_FAIL_ALLOC_AFTER: device allocations allowed before failing
     int64_t nAllocs = 0;
+    hipEvent_t evOrder = nullptr;  // orders the context stream with a caller's stream (allocation-free entry point)
+    double *scratch = nullptr;     // this device's copy of the allocation-free reverse's scratch array
     uint64_t gen = 0;        // bumped whenever device buffers, plans or options change (recorded graphs check it)
     std::string restartFile;        // denominator estimate + costs file (initializeWarping.F90:98-312)
     std::string restartStatus;      // "", "loaded", "saved"
@@ -639,6 +642,41 @@ hipError_t exclusive_scan(hipStream_t st, const int64_t *in, int64_t *out, int64
 }
 }  // namespace idw
 
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free reverse product
+// ---------------------------------------------------------------------------
+// The exact mode's per-chunk partial sums of idw_warp_deriv_dev_noalloc live
+// in this array, one copy per device, held until the process ends.  Host
+// code never passes the array's name to a kernel (on the host it is only a
+// stand-in for the symbol): each device's address comes from
+// hipGetSymbolAddress, taken once per device when its first context is
+// created, and that address is what the kernels receive.
+namespace {
+constexpr int64_t NOALLOC_SCRATCH_DOUBLES = int64_t(4) << 20;  // 32 MiB
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratchMu;
+double *g_scratchPtr[MAX_DEVICES] = {};
+}  // namespace
+
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_DOUBLES];
+
+namespace {
+// With `device` current: this device's address of the scratch array.
+int scratch_address(int device, double **out) {
+    *out = nullptr;
+    if (device < 0 || device >= MAX_DEVICES) return fail(IDW_ENODEV, "device %d beyond the scratch table", device);
+    std::lock_guard<std::mutex> lock(g_scratchMu);
+    if (!g_scratchPtr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return fail(IDW_EHIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        g_scratchPtr[device] = static_cast<double *>(p);
+    }
+    *out = g_scratchPtr[device];
+    return IDW_OK;
+}
+}  // namespace
+
 extern "C" {
 
 void idw_default_options(idw_options *o) {
@@ -680,12 +718,18 @@ int idw_create(idw_ctx **out, int device, const idw_options *opt) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
     for (int k = 0; k < 8 && e == hipSuccess; ++k) e = hipEventCreate(&c->ev[k]);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->mmEvent, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->evOrder, hipEventDisableTiming);
     if (e == hipSuccess) e = hipHostMalloc((void **)&c->mmHost, sizeof(unsigned int) << idw::MAX_PLANES);
     if (e == hipSuccess) std::memset(c->mmHost, 0, sizeof(unsigned int) << idw::MAX_PLANES);
     if (const char *fa = std::getenv("IDW_TEST_FAIL_ALLOC_AFTER")) c->failAllocAfter = std::atoll(fa);
     if (e != hipSuccess) {
         idw_destroy(c);
         return fail(IDW_EHIP, "stream/event creation: %s", hipGetErrorString(e));
+    }
+    if (const int r = scratch_address(device, &c->scratch); r != IDW_OK) {
+        const std::string msg = t_err;
+        idw_destroy(c);
+        return fail(r, "%s", msg.c_str());
     }
     *out = c;
     return IDW_OK;
@@ -699,6 +743,7 @@ int idw_destroy(idw_ctx *c) {
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->mmEvent) (void)hipEventDestroy(c->mmEvent);
+    if (c->evOrder) (void)hipEventDestroy(c->evOrder);
     if (c->mmHost) (void)hipHostFree(c->mmHost);
     if (c->st) (void)hipStreamDestroy(c->st);
     delete c;
@@ -1094,6 +1139,40 @@ int idw_warp_deriv_dev(idw_ctx *c, const double *d_xs, const double *d_dxv, doub
         HIPCHK(hipStreamWaitEvent(user, e, 0));
     }
     (void)hipEventDestroy(e);
+    return r;
+}
+
+int idw_warp_deriv_dev_noalloc(idw_ctx *c, const double *d_xs, const double *d_dxv, double *d_dxs, void *stream) {
+    if (!c || !d_xs || !d_dxv || !d_dxs) return fail(IDW_EINVAL, "null argument");
+    if (!c->surfInit) return fail(IDW_ESTATE, "warp_deriv before initialize_warping");
+    if (!c->volSet) return fail(IDW_ESTATE, "no volume nodes set");
+    const int fast = c->opt.evalMode == IDW_EVAL_FAST;
+    int nchunk = 0;
+    if (!fast) {
+        const int64_t per = 12 * (int64_t)c->S.n;  // doubles of one volume chunk's partial sums
+        if (per > NOALLOC_SCRATCH_DOUBLES)
+            return fail(IDW_EINVAL,
+                        "the allocation-free reverse holds %lld bytes of scratch: exact mode over %d unique surface "
+                        "nodes needs %lld (at most %lld nodes)",
+                        (long long)(8 * NOALLOC_SCRATCH_DOUBLES), c->S.n, (long long)(8 * per),
+                        (long long)(NOALLOC_SCRATCH_DOUBLES / 12));
+        nchunk = (int)std::min<int64_t>(exact_rev_chunks(c), NOALLOC_SCRATCH_DOUBLES / per);
+    }
+    if (c->capturing) return fail(IDW_ESTATE, "the allocation-free reverse is not recorded into batches");
+    TRY(poll_plan(c));
+    TRY(check_mixed_range(c));
+    if (deriv_needs_warp(c))
+        return fail(IDW_ESTATE, "the allocation-free reverse needs a warp of this context at the current options first");
+    if (nloop(c) > 1 && !c->V.acc) return fail(IDW_ESTATE, "the allocation-free reverse needs a warp with the planes first");
+    TRY(ensure_device(c));
+    hipStream_t user = static_cast<hipStream_t>(stream);
+    HIPCHK(hipEventRecord(c->evOrder, user));
+    HIPCHK(hipStreamWaitEvent(c->st, c->evOrder, 0));
+    const int r = deriv_kernels(c, d_xs, d_dxv, d_dxs, fast ? nullptr : c->scratch, nchunk);
+    if (r == IDW_OK) {
+        HIPCHK(hipEventRecord(c->evOrder, c->st));
+        HIPCHK(hipStreamWaitEvent(user, c->evOrder, 0));
+    }
     return r;
 }
 

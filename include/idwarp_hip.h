@@ -181,6 +181,22 @@ int idw_warp_deriv_fwd(idw_ctx *ctx, const double *dxs, int64_t ndof_s, double *
  * for NaN here (the host forms reject non-finite input with IDW_EINVAL). */
 int idw_warp_mesh_dev(idw_ctx *ctx, const double *d_xs, double *d_xv, void *hip_stream);
 int idw_warp_deriv_dev(idw_ctx *ctx, const double *d_xs, const double *d_dxv, double *d_dxs, void *hip_stream);
+/* Allocation-free form of idw_warp_deriv_dev.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory during the
+ * call (code being captured into HIP graphs by the caller, serving loops,
+ * harnesses that forbid allocation).  It calls no hipMalloc, hipFree, event
+ * or stream creation, nor any allocating library routine.  Requires a warp
+ * of this context at the current options first (IDW_ESTATE otherwise); in
+ * fast mode it replays that warp's plan.  Exact mode keeps its per-chunk
+ * partial sums in a 32 MiB array of the library's own (one per device, held
+ * until the process ends), which bounds the surface to 349,525 unique nodes
+ * (IDW_EINVAL beyond).  Every call on a device shares that device's array:
+ * calls that overlap on one device, from any context or thread, race and
+ * must be ordered by the caller.  Same arguments and result as
+ * idw_warp_deriv_dev (exact mode may split the volume into fewer chunks, so
+ * sums can differ in the last bits). */
+int idw_warp_deriv_dev_noalloc(idw_ctx *ctx, const double *d_xs, const double *d_dxv, double *d_dxs, void *hip_stream);
+
 /* Device-resident warpDerivFwd: d_dxs 3*n_pts tangent, d_dxv 3*n_vol output. */
 int idw_warp_deriv_fwd_dev(idw_ctx *ctx, const double *d_xs, const double *d_dxs, double *d_dxv, void *hip_stream);
 
