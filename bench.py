@@ -76,7 +76,9 @@ def log(*a):
 def cpu_baseline(m, xs, dxv, errtol, target_s):
     """The oracle (C restatement of the reference loops) on a random sample
     of the volume nodes, on all host cores; set-up time is added once and the
-    volume loops are scaled by N / sample."""
+    volume loops are scaled by N / sample.  Also returns the last sample's
+    results (node indices, warped nodes, and the reverse product of the seed
+    restricted to the sample), the checker for the GPU legs."""
     from oracle import oracle as O
 
     cores = len(os.sched_getaffinity(0))
@@ -88,26 +90,26 @@ def cpu_baseline(m, xs, dxv, errtol, target_s):
         idx = np.sort(rng.choice(nvol, size=k, replace=False))
         om = O.OracleMesh(m["pts"], m["conn"], m["faceSizes"], m["xv0"][idx], evalMode="fast", errTol=errtol)
         om.setSurfaceCoordinates(xs)
-        om.warpMesh()
+        xv = om.warpMesh()
         sw, lw = O.last_times()
-        om.warpDeriv(dxv.reshape(-1, 3)[idx].ravel())
+        dxs = om.warpDeriv(dxv.reshape(-1, 3)[idx].ravel())
         sd, ld = O.last_times()
-        return sw + sd, lw + ld
+        return sw + sd, lw + ld, dict(idx=idx, xv=xv.reshape(-1, 3), dxs=dxs, ldef=om.Ldef0)
 
     # grow the sample until its volume loops take about target_s (small
     # samples overstate the per-node cost: thread start-up, cold caches)
     k = min(nvol, 2000)
-    setup, loop = run(k)
+    setup, loop, chk = run(k)
     for _ in range(4):
         if loop >= 0.6 * target_s or k >= nvol:
             break
         k = int(min(nvol, max(2 * k, k * target_s / max(loop, 1e-3))))
-        setup, loop = run(k)
+        setup, loop, chk = run(k)
     ms = 1e3 * (setup + loop * nvol / k)
     return dict(value=round(ms, 1), unit="ms per warpMesh+warpDeriv (full mesh, extrapolated)", cores=cores,
                 kind="port",
                 sample="%d of %d volume nodes (seed 7), fast mode errTol=%g; volume loops %.2fs scaled x%.0f, "
-                       "surface set-up %.2fs added once" % (k, nvol, errtol, loop, nvol / k, setup))
+                       "surface set-up %.2fs added once" % (k, nvol, errtol, loop, nvol / k, setup)), chk
 
 
 def main():
@@ -251,9 +253,31 @@ def main():
     res["walk_lane_utilisation"] = {k: round(float(v), 3) for k, v in util.items()}
     if not args.no_cpu:
         t0 = time.time()
-        res["cpu_baseline"] = cpu_baseline(m, xs, dxv, args.errtol, args.cpu_seconds)
+        res["cpu_baseline"], chk = cpu_baseline(m, xs, dxv, args.errtol, args.cpu_seconds)
         res["cpu_baseline"]["speedup_gpu_vs_cpu"] = round(res["cpu_baseline"]["value"] / head["ms"], 1)
         log("cpu baseline: %s (%.1fs)" % (res["cpu_baseline"], time.time() - t0))
+        # the baseline's sample as the checker of the GPU legs: the warped
+        # nodes of the sample, and the reverse product of the seed restricted
+        # to it (one more reverse per precision, outside the timed steps)
+        idx = torch.as_tensor(chk["idx"], device=dev)
+        seed = torch.zeros(nvol, 3, dtype=torch.float64, device=dev)
+        seed[idx] = d_dxv.view(-1, 3)[idx]
+        vs = {}
+        for leg in out:
+            prec = leg.split("_")[0]
+            mesh.setOption("precision", prec)
+            mesh.setOption("reusePlan", leg.endswith("_reuse"))
+            mesh._setMeshOptions()
+            got = out[leg]["xv"].view(-1, 3)[idx].cpu().numpy()
+            d_dxs = torch.empty(3 * npts, dtype=torch.float64, device=dev)
+            _lib.check(L.idw_warp_mesh_dev(ctx, vp(d_xs), vp(out[leg]["xv"]), ctypes.c_void_p(stream.cuda_stream)))
+            _lib.check(L.idw_warp_deriv_dev(ctx, vp(d_xs), vp(seed), vp(d_dxs), ctypes.c_void_p(stream.cuda_stream)))
+            torch.cuda.synchronize()
+            vs[leg] = {"max_err_over_Ldef": float(np.abs(got - chk["xv"]).max() / chk["ldef"]),
+                       "adjoint_rel_err": _rel(d_dxs.cpu().numpy(), chk["dxs"])}
+        res["vs_oracle"] = dict(vs, sample="the CPU baseline's %d sampled volume nodes; reverse product of the "
+                                           "step's seed restricted to them" % len(chk["idx"]))
+        log("vs oracle: %s" % res["vs_oracle"])
     print(json.dumps(res), flush=True)
 
 
@@ -348,28 +372,10 @@ def ellipsoid_exact(args):
                                        "rev": round(41.0 * pairs / (kms["fp64"][1] * 1e-3) / 1e12, 3)},
                        "mixed_TFLOPs": {"fwd": round(fwd_tf, 3), "rev": round(rev_tf, 3)},
                        "compute": "VALU (no MFMA): all-pairs weights and displacement sums"}
-    # FP64 against the oracle on a sample of volume nodes (forward) and for
-    # a seed carried by a sample (reverse)
-    t0 = time.time()
-    O.set_threads(len(os.sched_getaffinity(0)))
-    rng = np.random.default_rng(7)
-    idx = np.sort(rng.choice(nvol, size=2000, replace=False))
-    om = O.OracleMesh(m["pts"], m["conn"], m["faceSizes"], m["xv0"][idx], evalMode="exact")
-    om.setSurfaceCoordinates(xs)
-    ref = om.warpMesh().reshape(-1, 3)
-    got = b["xv"].view(-1, 3)[torch.as_tensor(idx, device=dev)].cpu().numpy()
-    res["fp64_vs_oracle"] = {"sample": "2000 volume nodes (seed 7)",
-                             "max_err_over_Ldef": float(np.abs(got - ref).max() / ldef)}
-    seed = np.zeros(3 * nvol)
-    g = np.random.default_rng(5).standard_normal((2000, 3))
-    seed.reshape(-1, 3)[idx] = g
+    # central differences of the FP64 warp on the device along unit surface
+    # directions (the quotient formed from the difference of the two grids)
     mesh.setOption("precision", "fp64")
     mesh.setSurfaceCoordinates(xs)
-    mesh.warpMesh()
-    mesh.warpDeriv(seed, solverVec=False)
-    res["fp64_vs_oracle"]["adjoint_rel_err"] = float(_rel(mesh.getdXs().ravel(), om.warpDeriv(g.ravel())))
-    log("oracle checks %.1fs: %s" % (time.time() - t0, res["fp64_vs_oracle"]))
-    # central differences on the device along unit surface directions
     mesh.warpMesh()
     mesh.warpDeriv(dxv, solverVec=False)
     dxs = mesh.getdXs().ravel()
@@ -379,17 +385,39 @@ def ellipsoid_exact(args):
         p = rng.standard_normal(xs.size)
         p /= np.linalg.norm(p)
         h = 1e-6
-        vals = []
+        grids = []
         for sgn in (1.0, -1.0):
             mesh.setSurfaceCoordinates((xs.ravel() + sgn * h * p).reshape(-1, 3))
             mesh.warpMesh()
-            vals.append(dxv @ mesh.getWarpGrid())
-        fd = (vals[0] - vals[1]) / (2 * h)
+            grids.append(mesh.getWarpGrid())
+        fd = dxv @ (grids[0] - grids[1]) / (2 * h)
         worst = max(worst, abs(fd - dxs @ p) / max(abs(fd), 1e-300))
     res["adjoint_fd"] = {"directions": args.fd_directions, "h": 1e-6, "max_rel_diff": worst,
                          "seed": "dXv ~ N(0,1) on a random 10% of the volume nodes (seed 5)"}
     log("FD check: %s" % res["adjoint_fd"])
     if not args.no_cpu:
+        # the CPU baseline leg: the oracle on a sample of volume nodes, which
+        # also checks the FP64 forward on it and the reverse for a seed carried
+        # by it
+        t0 = time.time()
+        O.set_threads(len(os.sched_getaffinity(0)))
+        rng = np.random.default_rng(7)
+        idx = np.sort(rng.choice(nvol, size=2000, replace=False))
+        om = O.OracleMesh(m["pts"], m["conn"], m["faceSizes"], m["xv0"][idx], evalMode="exact")
+        om.setSurfaceCoordinates(xs)
+        ref = om.warpMesh().reshape(-1, 3)
+        got = b["xv"].view(-1, 3)[torch.as_tensor(idx, device=dev)].cpu().numpy()
+        res["fp64_vs_oracle"] = {"sample": "2000 volume nodes (seed 7)",
+                                 "max_err_over_Ldef": float(np.abs(got - ref).max() / ldef)}
+        seed = np.zeros(3 * nvol)
+        g = np.random.default_rng(5).standard_normal((2000, 3))
+        seed.reshape(-1, 3)[idx] = g
+        mesh.setOption("precision", "fp64")
+        mesh.setSurfaceCoordinates(xs)
+        mesh.warpMesh()
+        mesh.warpDeriv(seed, solverVec=False)
+        res["fp64_vs_oracle"]["adjoint_rel_err"] = float(_rel(mesh.getdXs().ravel(), om.warpDeriv(g.ravel())))
+        log("oracle checks %.1fs: %s" % (time.time() - t0, res["fp64_vs_oracle"]))
         t0 = time.time()
         om.setSurfaceCoordinates(xs)
         om.warpMesh()

@@ -26,7 +26,7 @@ EXPORTS = (
     "idw_set_symmetry_planes", "idw_batch_create", "idw_batch_destroy", "idw_batch_add_warp",
     "idw_batch_add_warp_deriv", "idw_batch_run", "idw_batch_info", "idw_set_restart_file",
     "idw_get_restart_info", "idw_mixed_near_wall_floor", "idw_batch_stats",
-    "idw_warp_deriv_dev_noalloc",
+    "idw_warp_deriv_dev_noalloc", "idw_warp_mesh_async", "idw_warp_deriv_async", "idw_wait",
 )
 
 
@@ -117,6 +117,9 @@ def load():
         "idw_mixed_near_wall_floor": (d, [d, d, d]),
         "idw_batch_stats": (i32, [vp, P(i64), P(i64)]),
         "idw_warp_deriv_dev_noalloc": (i32, [vp, vp, vp, vp, vp]),
+        "idw_warp_mesh_async": (i32, [vp]),
+        "idw_warp_deriv_async": (i32, [vp, vp, i64]),
+        "idw_wait": (i32, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -1041,6 +1041,20 @@ int idw_warp_mesh(idw_ctx *c) {
     return check_plan(c);
 }
 
+int idw_warp_mesh_async(idw_ctx *c) {
+    if (!c) return fail(IDW_EINVAL, "null context");
+    TRY(ensure_device(c));
+    return run_warp(c, c->xs, c->xv);
+}
+
+int idw_wait(idw_ctx *c) {
+    if (!c) return fail(IDW_EINVAL, "null context");
+    TRY(ensure_device(c));
+    HIPCHK(hipStreamSynchronize(c->st));
+    collect_timings(c);
+    return check_plan(c);
+}
+
 int idw_get_volume_coordinates(idw_ctx *c, double *xv, int64_t ndof) {
     if (!c || (!xv && ndof)) return fail(IDW_EINVAL, "null argument");
     if (!c->volSet) return fail(IDW_ESTATE, "no volume nodes set");
@@ -1093,6 +1107,18 @@ int idw_warp_deriv(idw_ctx *c, const double *dxv, int64_t ndof) {
     HIPCHK(hipStreamSynchronize(c->st));
     collect_timings(c);
     TRY(check_plan(c));
+    c->haveDxs = true;
+    return IDW_OK;
+}
+
+int idw_warp_deriv_async(idw_ctx *c, const double *dxv, int64_t ndof) {
+    if (!c || (!dxv && ndof)) return fail(IDW_EINVAL, "null argument");
+    if (!c->volSet) return fail(IDW_ESTATE, "no volume nodes set");
+    if (ndof != 3 * c->V.n) return fail(IDW_EINVAL, "expected %lld values", (long long)(3 * c->V.n));
+    TRY(ensure_device(c));
+    // from pageable memory the copy has read dxv when it returns
+    if (ndof) HIPCHK(hipMemcpyAsync(c->dxv, dxv, sizeof(double) * ndof, hipMemcpyHostToDevice, c->st));
+    TRY(run_deriv(c, c->xs, c->dxv, c->dxs));
     c->haveDxs = true;
     return IDW_OK;
 }

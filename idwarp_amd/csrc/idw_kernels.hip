@@ -1237,8 +1237,10 @@ __device__ __forceinline__ void fwd_terms(int64_t a, int64_t b, const PlanEntry 
                     // (about one block in seven)
                     if (!on) continue;
                     const double rx = vx[t] - x0, ry = vy[t] - x1, rz = vz[t] - x2;
-                    // |r|^2 + eps in FP64 (eps: points only), rounded once
-                    const float w = weight_f32<EXPM>((float)(rx * rx + ry * ry + rz * rz + eps), Af, AL3, Lf, a5L2, P);
+                    // |r|^2 + eps in FP64 (eps: points only; folded into the
+                    // first product, so it costs no instruction), rounded once
+                    const float w = weight_f32<EXPM>((float)fma(rz, rz, fma(ry, ry, fma(rx, rx, eps))), Af, AL3, Lf,
+                                                     a5L2, P);
                     F[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(lane_in(on) ? w : 0.0f, Bv, F[t], 0, 0, 0);
                 }
             }
@@ -1770,7 +1772,8 @@ k_fast_rev_pair(int nvol, int nwave, const double4 *__restrict__ vol, const int3
                 const double4 c = L.node[64 * h + 4 * q + R];
                 const bool on = (mR >> (4 * q)) & 1ull;
                 const double rx = c.x - T.xa.x, ry = c.y - T.xa.y, rz = c.z - T.xa.z;
-                const float wf = weight_f32<EXPM>((float)(rx * rx + ry * ry + rz * rz + epsT), Af, AL3, Lf, a5L2, P);
+                const float wf =
+                    weight_f32<EXPM>((float)fma(rz, rz, fma(ry, ry, fma(rx, rx, epsT))), Af, AL3, Lf, a5L2, P);
                 const float w = on ? wf : 0.0f;
                 const float g = h == 0 ? GA[q] : GB[q];
                 if (q & 1) F1 = __builtin_amdgcn_mfma_f32_16x16x4f32(w, g, F1, 0, 0, 0);

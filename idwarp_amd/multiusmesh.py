@@ -141,7 +141,7 @@ class MultiUSMesh:
     # ------------------------------------------------------------------
     def _check_surface(self):
         if self.numSurfNodes is None:
-            raise NameError("IDWarpMulti surface is not initialized. Run self.setSurfaceDefinition first.")
+            raise NameError("MultiUSMesh has no surface yet: call setSurfaceDefinition() first")
 
     def getSurfaceCoordinates(self):
         """All surface coordinates in solver ordering (:287-312); points of
@@ -205,14 +205,14 @@ class MultiUSMesh:
         elements of background blocks go to no instance.  distTol is unused,
         as in the reference."""
         if self.backgroundInstanceIDs is not None:
-            raise NameError("The user should run self.setExternalMeshIndices before self.setSurfaceDefinition.")
+            raise NameError("MultiUSMesh.setSurfaceDefinition() needs setExternalMeshIndices() to have run first")
         pts = np.asarray(pts, dtype=self.dtype).reshape(-1, 3)
         faceSizes = np.asarray(faceSizes, dtype=np.int64).ravel()
         conn = np.asarray(conn, dtype=np.int64).ravel()
         cgnsBlockIDs = np.asarray(cgnsBlockIDs).ravel()
         numElems = len(faceSizes)
         if len(cgnsBlockIDs) != numElems:
-            raise ValueError("The user should provide block IDs for every element.")
+            raise ValueError("cgnsBlockIDs holds %d entries for %d elements" % (len(cgnsBlockIDs), numElems))
         if faceSizes.sum() != len(conn):
             raise ValueError("conn holds %d entries, faceSizes sum to %d" % (len(conn), int(faceSizes.sum())))
         self.numSurfNodes = pts.shape[0]
@@ -248,16 +248,24 @@ class MultiUSMesh:
         return dXs
 
     def warpMesh(self):
-        """Warp every instance in turn (:667-699)."""
+        """Warp every instance (:667-699).  The reference warps them one after
+        another; here every instance's warp is enqueued on its own stream
+        first and then all are waited for, so they run side by side on the
+        device."""
         for mesh in self.meshes:
-            mesh.warpMesh()
+            mesh._warpMeshAsync()
+        for mesh in self.meshes:
+            mesh._wait()
 
     def warpDeriv(self, dXv, solverVec=True):
         """dXs = (dXv/dXs)^T dXv over all instances (:701-755): each instance
-        takes the entries of dXv in its zone."""
+        takes the entries of dXv in its zone; the instances' reverse products
+        run side by side, as in warpMesh."""
         dXv = np.asarray(dXv, dtype=self.dtype).ravel()
         for instanceID, mesh in enumerate(self.meshes):
-            mesh.warpDeriv(dXv[self.cgnsVolNodeMasks[instanceID]], solverVec=solverVec)
+            mesh._warpDerivAsync(dXv[self.cgnsVolNodeMasks[instanceID]], solverVec=solverVec)
+        for mesh in self.meshes:
+            mesh._wait()
         return self.getdXs()
 
     def warpDerivFwd(self, dXs):

@@ -54,8 +54,7 @@ class USMesh:
 
     def __init__(self, options=None, comm=None, debug=False):
         if options is None:
-            raise Error("The 'options' keyword argument is *NOT* optional. An options dictionary must be passed upon "
-                        "creation of this object")
+            raise Error("USMesh needs an options dictionary (options=...) when it is created")
         self.comm = comm
         self.name = "IDWarp"
         self.category = "Volume mesh warping"
@@ -238,10 +237,9 @@ class USMesh:
 
         from . import openfoam
 
-        warnings.warn("Using internally generated IDWarp surfaces. If this mesh object is to be used with an "
-                      "external solver, ensure the mesh object is passed to the solver immediatedly after it is "
-                      "created. The external solver must then call 'setExternalMeshIndices()' and "
-                      "'setSurfaceDefinition()' routines.", stacklevel=2)
+        warnings.warn("No surface was defined: warping about the grid's own patch, wall and slip faces. A solver "
+                      "that drives this mesh should call setExternalMeshIndices() and setSurfaceDefinition() "
+                      "before the first warp.", stacklevel=2)
         pts, conn, faceSizes = openfoam.warp_surface(self.OFData)
         self.setSurfaceDefinition(pts, conn, faceSizes)
 
@@ -253,8 +251,8 @@ class USMesh:
 
     def setSurfaceCoordinates(self, coordinates):
         if len(coordinates) != self.nSurf:
-            raise Error("Incorrect length of coordinates supplied to setSurfaceCoordinates on proc %d. Expected "
-                        "aray of length %d, received an array of length %d." % (0, self.nSurf, len(coordinates)))
+            raise Error("setSurfaceCoordinates got %d surface points; the surface definition has %d" %
+                        (len(coordinates), self.nSurf))
         c = _lib.f64(coordinates).ravel()
         _lib.check(self._L.idw_set_surface_coordinates(self._ctx, _lib.ptr(c), c.size))
 
@@ -266,6 +264,21 @@ class USMesh:
         self._setInternalSurface()
         self._setMeshOptions()
         _lib.check(self._L.idw_warp_mesh(self._ctx))
+
+    def _warpMeshAsync(self):
+        """warpMesh enqueued on this mesh's stream; complete after _wait()."""
+        self._setInternalSurface()
+        self._setMeshOptions()
+        _lib.check(self._L.idw_warp_mesh_async(self._ctx))
+
+    def _warpDerivAsync(self, dXv, solverVec=True):
+        """warpDeriv enqueued on this mesh's stream; complete after _wait()."""
+        self._setMeshOptions()
+        dXvWarp = self._warpOrder(dXv, solverVec)
+        _lib.check(self._L.idw_warp_deriv_async(self._ctx, _lib.ptr(dXvWarp), dXvWarp.size))
+
+    def _wait(self):
+        _lib.check(self._L.idw_wait(self._ctx))
 
     def getWarpGrid(self):
         xv = np.zeros(3 * self.nVol, self.dtype)
@@ -281,7 +294,7 @@ class USMesh:
         OpenFOAM case's constant/polyMesh/points in place."""
         if self.fileType in ("CGNS", "PLOT3D"):
             if fileName is None:
-                raise Error("fileName is not optional for writeGrid with gridType of CGNS or PLOT3D")
+                raise Error("writeGrid needs a fileName for a CGNS or PLOT3D grid")
             if self.fileType == "CGNS":
                 raise Error("CGNS output is not part of this library")
             from . import plot3d
@@ -307,16 +320,19 @@ class USMesh:
                                                    solverGrid.size))
         return solverGrid
 
+    def _warpOrder(self, dXv, solverVec):
+        dXv = _lib.f64(dXv).ravel()
+        if not solverVec:
+            return dXv
+        dXvWarp = np.zeros(3 * self.nVol, self.dtype)
+        _lib.check(self._L.idw_solver_to_warp_grid(self._ctx, _lib.ptr(dXv), dXv.size, _lib.ptr(dXvWarp),
+                                                   dXvWarp.size))
+        return dXvWarp
+
     def warpDeriv(self, dXv, solverVec=True):
         """dXs = (dXv/dXs)^T dXv; the result is read with getdXs()."""
         self._setMeshOptions()
-        dXv = _lib.f64(dXv).ravel()
-        if solverVec:
-            dXvWarp = np.zeros(3 * self.nVol, self.dtype)
-            _lib.check(self._L.idw_solver_to_warp_grid(self._ctx, _lib.ptr(dXv), dXv.size, _lib.ptr(dXvWarp),
-                                                       dXvWarp.size))
-        else:
-            dXvWarp = dXv
+        dXvWarp = self._warpOrder(dXv, solverVec)
         _lib.check(self._L.idw_warp_deriv(self._ctx, _lib.ptr(dXvWarp), dXvWarp.size))
 
     def warpDerivFwd(self, dXs, solverVec=True):

@@ -141,6 +141,18 @@ int idw_get_surface_coordinates(idw_ctx *ctx, double *xs, int64_t ndof);
  * for the current surface coordinates; result stays on the device. */
 int idw_warp_mesh(idw_ctx *ctx);
 
+/* Non-blocking forms of idw_warp_mesh and idw_warp_deriv, for warping
+ * several contexts at once (each context has its own stream): they enqueue
+ * the work and return; idw_wait waits for the context's stream and reports
+ * what the blocking forms report (a plan that disagreed with its counting
+ * pass).  Later calls on the context are ordered after the work, so
+ * idw_get_volume_coordinates and idw_get_dxs read its results.  No
+ * counterpart in the reference, whose MultiUSMesh warps its instances one
+ * after another (MultiUnstructuredMesh.py:677-755). */
+int idw_warp_mesh_async(idw_ctx *ctx);
+int idw_warp_deriv_async(idw_ctx *ctx, const double *dxv, int64_t ndof);
+int idw_wait(idw_ctx *ctx);
+
 /* getvolumecoordinates (libidwarp.pyf:45-48): copy the warped grid out. */
 int idw_get_volume_coordinates(idw_ctx *ctx, double *xv, int64_t ndof);
 

@@ -163,6 +163,11 @@ int api() {
     expect(idw_set_surface_coordinates(nullptr, d, 3) == IDW_EINVAL, "set_surface_coordinates");
     expect(idw_get_surface_coordinates(nullptr, d, 3) == IDW_EINVAL, "get_surface_coordinates");
     expect(idw_warp_mesh(nullptr) == IDW_EINVAL, "warp_mesh");
+    expect(idw_warp_mesh_async(nullptr) == IDW_EINVAL, "warp_mesh_async");
+    expect(idw_warp_deriv_async(nullptr, d, 3) == IDW_EINVAL, "warp_deriv_async");
+    expect(idw_wait(nullptr) == IDW_EINVAL, "wait");
+    expect(idw_warp_deriv_dev_noalloc(nullptr, d, d, d, nullptr) == IDW_EINVAL, "warp_deriv_dev_noalloc");
+    expect(idw_batch_stats(nullptr, &n64, &n64) == IDW_EINVAL, "batch_stats");
     expect(idw_get_volume_coordinates(nullptr, d, 3) == IDW_EINVAL, "get_volume_coordinates");
     expect(idw_set_external_mesh_indices(nullptr, i, 1) == IDW_EINVAL, "set_external_mesh_indices");
     expect(idw_get_solver_dof(nullptr, &n64) == IDW_EINVAL, "get_solver_dof");

@@ -58,6 +58,20 @@ class LinearMesh:
         self.warpMesh()
         return (self.B @ np.ravel(dXs))[self.ind]
 
+    # the non-blocking forms: the work happens at once, the calls are logged
+    log = []
+
+    def _warpMeshAsync(self):
+        LinearMesh.log.append(("warp", id(self)))
+        self.warpMesh()
+
+    def _warpDerivAsync(self, dXv, solverVec=True):
+        LinearMesh.log.append(("deriv", id(self)))
+        self.warpDeriv(dXv, solverVec)
+
+    def _wait(self):
+        LinearMesh.log.append(("wait", id(self)))
+
 
 def _grid():
     """Blocks in CGNS order: body (2 blocks), background (1), fin (1)."""
@@ -91,6 +105,21 @@ def multi(monkeypatch):
     pts, conn, fs, ids = _surface()
     m.setSurfaceDefinition(pts, conn, fs, ids)
     return m, blocks, ind, pts, conn, fs, ids
+
+
+def test_instances_are_all_enqueued_before_any_wait(multi):
+    """warpMesh and warpDeriv start every instance's work (each context has
+    its own stream) before they wait for any, so the instances run side by
+    side on the device; the reference runs them one after another
+    (MultiUnstructuredMesh.py:677-755)."""
+    m, blocks, ind, *_ = multi
+    for call, args in ((m.warpMesh, ()), (m.warpDeriv, (np.ones(len(ind)),))):
+        LinearMesh.log = []
+        call(*args)
+        kinds = [k for k, _ in LinearMesh.log]
+        n = len(m.meshes)
+        assert n == 2 and kinds[:n] in (["warp"] * n, ["deriv"] * n) and kinds[n:] == ["wait"] * n
+        assert [i for _, i in LinearMesh.log[:n]] == [i for _, i in LinearMesh.log[n:]] == [id(x) for x in m.meshes]
 
 
 def test_zones_and_intervals(multi):
