@@ -44,6 +44,7 @@ def parse():
     ap.add_argument("--errtol", type=float, default=1e-4)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU work of the baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default tile parameters (no autotuner search)")
     ap.add_argument("--only", choices=["both", "fp64", "mixed"], default="both")
     ap.add_argument("--workload", choices=["star-fast", "ellipsoid-exact", "batch"], default="star-fast",
                     help="star-fast: 20M-node star body, fast mode (the headline); ellipsoid-exact: 3.2M-node "
@@ -159,13 +160,21 @@ def main():
     def vp(t):
         return ctypes.c_void_p(t.data_ptr())
 
-    out = {}
+    out, tuning = {}, {}
     legs = {"both": ["fp64", "mixed", "mixed_reuse"], "fp64": ["fp64"], "mixed": ["mixed"]}[args.only]
     for leg in legs:
         prec = leg.split("_")[0]
         mesh.setOption("precision", prec)
         mesh.setOption("reusePlan", leg.endswith("_reuse"))
         mesh._setMeshOptions()
+        if not args.no_tune:
+            # the tile parameters for this mesh and precision, searched once
+            # before the timed steps (kept for the same options)
+            t0 = time.time()
+            mesh.tune()
+            chosen, table, _ = mesh.getTuning()
+            tuning[prec] = {"chosen": chosen, "ms_per_launch": {k: v for k, v in table.items() if v}}
+            log("%s tuning (%.1fs): %s %s" % (leg, time.time() - t0, chosen, tuning[prec]["ms_per_launch"]))
         d_xv = torch.empty(3 * nvol, dtype=torch.float64, device=dev)
         d_dxs = torch.empty(3 * npts, dtype=torch.float64, device=dev)
         sp = ctypes.c_void_p(stream.cuda_stream)
@@ -248,6 +257,8 @@ def main():
         res["roofline"]["traffic_detail"] = {"fetch_bytes": tr["fetch_bytes"], "write_bytes": tr["write_bytes"],
                                              "hbm_GBps": round(tr["bytes"] / (kms * 1e-3) / 1e9, 1),
                                              "source": "profiles/traffic.json"}
+    if tuning:
+        res["tuning"] = tuning
     res["phases_ms"] = {p: dict(zip(("nodal", "walk", "fwd", "rev_setup", "rev", "nodal_rev"),
                                     [round(float(x), 3) for x in out[p]["phases"]])) for p in out}
     res["walk_lane_utilisation"] = {k: round(float(v), 3) for k, v in util.items()}
@@ -517,7 +528,10 @@ def batched(args):
         t_seq = timed(seq, 1)
         g.setOption("precision", "mixed")
         g._setMeshOptions()
-        per_mesh[m["name"]] = {"n_vol": m["nvol"], "graph_ms": round(t_graph * 1e3, 2),
+        chosen, table, _ = g.getTuning()
+        per_mesh[m["name"]] = {"n_vol": m["nvol"], "tuning": {"chosen": chosen,
+                                                               "ms_per_launch": {k: v for k, v in table.items() if v}},
+                               "graph_ms": round(t_graph * 1e3, 2),
                                "sequential_fp64_ms": round(t_seq * 1e3, 2),
                                "speedup": round(t_seq / t_graph, 2)}
         log("%s: graph %.1f ms, sequential FP64 %.1f ms" % (m["name"], t_graph * 1e3, t_seq * 1e3))

@@ -27,6 +27,7 @@ EXPORTS = (
     "idw_batch_add_warp_deriv", "idw_batch_run", "idw_batch_info", "idw_set_restart_file",
     "idw_get_restart_info", "idw_mixed_near_wall_floor", "idw_batch_stats",
     "idw_warp_deriv_dev_noalloc", "idw_warp_mesh_async", "idw_warp_deriv_async", "idw_wait",
+    "idw_tune", "idw_get_tuning", "idw_set_tuning",
 )
 
 
@@ -120,6 +121,9 @@ def load():
         "idw_warp_mesh_async": (i32, [vp]),
         "idw_warp_deriv_async": (i32, [vp, vp, i64]),
         "idw_wait": (i32, [vp]),
+        "idw_tune": (i32, [vp, i32]),
+        "idw_get_tuning": (i32, [vp, vp, vp, i64]),
+        "idw_set_tuning": (i32, [vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -101,6 +101,13 @@ struct idw_ctx {
     bool mmPending = false;
     int64_t failAllocAfter = -1;  // test hook IDW_TEST_FAIL_ALLOC_AFTER: device allocations allowed before failing
     int64_t nAllocs = 0;
+    // tile parameters of the volume kernels (idw_tune) and what they were
+    // chosen for: evalMode, precision, errTol, near-wall distance, sizes, planes
+    idw::Tuning tune;
+    bool tuned = false;
+    bool tuneFixed = false;  // set by idw_set_tuning: batches keep these parameters
+    double tuneKey[7] = {-1, -1, -1, -1, -1, -1, -1};
+    double tuneMs[idw::TUNE_SLOTS];
     hipEvent_t evOrder = nullptr;  // orders the context stream with a caller's stream (allocation-free entry point)
     double *scratch = nullptr;     // this device's copy of the allocation-free reverse's scratch array
     uint64_t gen = 0;        // bumped whenever device buffers, plans or options change (recorded graphs check it)
@@ -467,7 +474,7 @@ int run_warp(idw_ctx *c, const double *xs, double *xv) {
         if (!fill[kk]) continue;
         idw::DevPlan &pl = c->plans[kk];
         idw::launch_zero_u32(c->st, pl.mismatch, 1);
-        if (c->V.n) idw::launch_plan_fill(c->st, c->S, c->V, mirror_of(c, kk), c->opt.errTol, nt2, pl);
+        if (c->V.n) idw::launch_plan_fill(c->st, c->S, c->V, mirror_of(c, kk), c->opt.errTol, nt2, pl, c->tune);
         HIPCHK(hipGetLastError());
         if (!c->capturing && kk < (1 << idw::MAX_PLANES)) {
             HIPCHK(hipMemcpyAsync(c->mmHost + kk, pl.mismatch, sizeof(unsigned int), hipMemcpyDeviceToHost, c->st));
@@ -481,7 +488,7 @@ int run_warp(idw_ctx *c, const double *xs, double *xv) {
     for (int kk = 0; kk < nloop(c) && c->V.n; ++kk) {
         P.pass = pass_of(c, kk);
         idw::launch_forward(c->st, c->S, c->V, P, mirror_of(c, kk), fast, c->opt.precision, xv,
-                            fast ? &c->plans[kk] : nullptr);
+                            fast ? &c->plans[kk] : nullptr, c->tune);
     }
     HIPCHK(hipGetLastError());
     REC(c, 3);
@@ -494,12 +501,15 @@ int run_warp(idw_ctx *c, const double *xs, double *xv) {
 
 // Volume chunks of the exact reverse: enough block rows (chunks x 256-node
 // surface blocks) to fill the device, each chunk at least 64 volume nodes.
-int exact_rev_chunks(const idw_ctx *c) {
+int exact_rev_chunks(const idw_ctx *c, int shift) {
     const int64_t nbs = (c->S.n + 255) / 256;
     int nchunk = (int)std::min<int64_t>(128, std::max<int64_t>(1, (4096 + nbs - 1) / nbs));
+    nchunk = shift > 0 ? std::min(256, nchunk << shift) : std::max(1, nchunk >> (-shift));
     while (nchunk > 1 && c->V.n / nchunk < 64) nchunk /= 2;
     return nchunk;
 }
+
+int exact_rev_chunks(const idw_ctx *c) { return exact_rev_chunks(c, c->tune.exactChunkShift); }
 
 int ensure_partial(idw_ctx *c) {
     const int nchunk = exact_rev_chunks(c);
@@ -521,7 +531,7 @@ int deriv_kernels(idw_ctx *c, const double *xs, const double *dxv, double *dxs, 
     REC(c, 5);
     for (int kk = 0; kk < nloop(c) && c->V.n; ++kk)
         idw::launch_reverse(c->st, c->S, c->V, make_params(c, false), mirror_of(c, kk), fast, c->opt.precision,
-                            partial, nchunk, dxv, fast ? &c->plans[kk] : nullptr);
+                            partial, nchunk, dxv, fast ? &c->plans[kk] : nullptr, c->tune);
     HIPCHK(hipGetLastError());
     REC(c, 6);
     idw::launch_nodal_reverse(c->st, c->S, xs, fast, c->opt.useRotations, dxs, c->npts);
@@ -573,12 +583,144 @@ int run_deriv_fwd(idw_ctx *c, const double *xs, const double *dxs, double *dxv) 
     for (int kk = 0; kk < nloop(c) && c->V.n; ++kk) {
         P.pass = pass_of(c, kk);
         idw::launch_forward(c->st, c->S, c->V, P, mirror_of(c, kk), fast, c->opt.precision, dxv,
-                            fast ? &c->plans[kk] : nullptr);
+                            fast ? &c->plans[kk] : nullptr, c->tune);
     }
     HIPCHK(hipGetLastError());
     REC(c, 3);
     c->fwdTimed = !c->capturing;
     return IDW_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner of the volume kernels' tile parameters
+// ---------------------------------------------------------------------------
+// Each candidate of each parameter is timed on the context's own data at the
+// current options (one launch to warm up, then the mean of two, HIP events on
+// the context stream), with the other parameters at their current choice;
+// the fastest is kept.  Parameters: the walk's gather buffer, the forward's
+// chunk of staged terms, the mixed reverse's merge window (fast mode); the
+// FP32 group and the reverse's volume chunks (exact mode).
+void tune_key(const idw_ctx *c, double *k) {
+    k[0] = c->opt.evalMode;
+    k[1] = c->opt.precision;
+    k[2] = c->opt.evalMode == IDW_EVAL_FAST ? c->opt.errTol : 0.0;
+    k[3] = plan_near_t2(c);
+    k[4] = (double)c->V.n;
+    k[5] = (double)c->S.n;
+    k[6] = c->nplanes;
+}
+
+template <class F>
+int time_launches(idw_ctx *c, F &&launch, double *ms) {
+    launch();
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c->ev[0], c->st));
+    for (int r = 0; r < 2; ++r) launch();
+    HIPCHK(hipEventRecord(c->ev[1], c->st));
+    HIPCHK(hipEventSynchronize(c->ev[1]));
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, c->ev[0], c->ev[1]));
+    *ms = t / 2.0;
+    return IDW_OK;
+}
+
+int run_tune(idw_ctx *c) {
+    if (!c->surfInit) return fail(IDW_ESTATE, "tuning before initialize_warping");
+    if (!c->volSet) return fail(IDW_ESTATE, "no volume nodes set");
+    if (c->capturing) return fail(IDW_ESTATE, "tuning while recording a batch");
+    for (double &t : c->tuneMs) t = -1.0;
+    const int fast = c->opt.evalMode == IDW_EVAL_FAST, mixed = c->opt.precision == IDW_PREC_MIXED;
+    // a warp at the current options sizes and fills the plans and gives the denominator
+    TRY(run_warp(c, c->xs, c->xv));
+    HIPCHK(hipStreamSynchronize(c->st));
+    TRY(check_plan(c));
+    if (c->V.n == 0) {
+        c->tuned = true;
+        tune_key(c, c->tuneKey);
+        return IDW_OK;
+    }
+    idw::WParams P = make_params(c, false);
+    auto pick = [&](int slot0, const int *cand, int ncand, int32_t &param, auto &&launch) -> int {
+        int best = param;
+        double bestMs = INFINITY;
+        for (int k = 0; k < ncand; ++k) {
+            param = cand[k];
+            double ms = 0.0;
+            TRY(time_launches(c, launch, &ms));
+            c->tuneMs[slot0 + k] = ms;
+            if (ms < bestMs) {
+                bestMs = ms;
+                best = cand[k];
+            }
+        }
+        param = best;
+        return IDW_OK;
+    };
+    auto forward = [&]() {
+        for (int kk = 0; kk < nloop(c); ++kk) {
+            P.pass = pass_of(c, kk);
+            idw::launch_forward(c->st, c->S, c->V, P, mirror_of(c, kk), fast, c->opt.precision, c->xv,
+                                fast ? &c->plans[kk] : nullptr, c->tune);
+        }
+    };
+    if (fast) {
+        const double nt2 = plan_near_t2(c);
+        static const int wbufs[3] = {64, 128, 256}, fchs[3] = {16, 32, 64}, mqs[3] = {8, 16, 32};
+        TRY(pick(idw::TUNE_WBUF, wbufs, 3, c->tune.wbuf, [&]() {
+            for (int kk = 0; kk < nloop(c); ++kk) {
+                idw::launch_zero_u32(c->st, c->plans[kk].mismatch, 1);
+                idw::launch_plan_fill(c->st, c->S, c->V, mirror_of(c, kk), c->opt.errTol, nt2, c->plans[kk], c->tune);
+            }
+        }));
+        HIPCHK(hipStreamSynchronize(c->st));
+        TRY(check_plan(c));
+        TRY(pick(idw::TUNE_FCH, fchs, 3, c->tune.fch, forward));
+        if (mixed) {
+            // a seed of ones on every node (every term has work, as a real seed's)
+            idw::launch_fill(c->st, c->dxv, 3 * c->V.n, 1.0);
+            TRY(pick(idw::TUNE_MQ, mqs, 3, c->tune.mq, [&]() {
+                idw::launch_reverse_prep(c->st, c->S, 1);
+                for (int kk = 0; kk < nloop(c); ++kk)
+                    idw::launch_reverse(c->st, c->S, c->V, P, mirror_of(c, kk), 1, c->opt.precision, nullptr, 0, c->dxv,
+                                        &c->plans[kk], c->tune);
+            }));
+        }
+    } else {
+        static const int gs[3] = {4, 8, 16}, shifts[3] = {-1, 0, 1};
+        if (mixed) TRY(pick(idw::TUNE_EXACT_G, gs, 3, c->tune.exactG, forward));
+        // partial sums for the largest chunk count, freed afterwards
+        double *partial = nullptr;
+        TRY(dalloc(c, &partial, (size_t)exact_rev_chunks(c, 1) * c->S.n * 12));
+        idw::launch_fill(c->st, c->dxv, 3 * c->V.n, 1.0);
+        TRY(ensure_mirror_buffers(c));
+        const int r = pick(idw::TUNE_EXACT_CHUNKS, shifts, 3, c->tune.exactChunkShift, [&]() {
+            const int nchunk = exact_rev_chunks(c);
+            for (int kk = 0; kk < nloop(c); ++kk)
+                idw::launch_reverse(c->st, c->S, c->V, P, mirror_of(c, kk), 0, c->opt.precision, partial, nchunk,
+                                    c->dxv, nullptr, c->tune);
+        });
+        HIPCHK(hipStreamSynchronize(c->st));
+        dfree(c, partial);
+        TRY(r);
+    }
+    HIPCHK(hipStreamSynchronize(c->st));
+    // the last forward candidate wrote the warp of the current surface with
+    // another parameter: warp again with the chosen ones
+    TRY(run_warp(c, c->xs, c->xv));
+    HIPCHK(hipStreamSynchronize(c->st));
+    c->tuned = true;
+    tune_key(c, c->tuneKey);
+    ++c->gen;
+    return check_plan(c);
+}
+
+bool tune_current(const idw_ctx *c) {
+    if (!c->tuned) return false;
+    double k[7];
+    tune_key(c, k);
+    for (int i = 0; i < 7; ++i)
+        if (k[i] != c->tuneKey[i]) return false;
+    return true;
 }
 
 void collect_timings(idw_ctx *c) {
@@ -714,6 +856,7 @@ int idw_create(idw_ctx **out, int device, const idw_options *opt) {
     if (!c) return fail(IDW_ENOMEM, "out of host memory");
     c->device = device;
     c->opt = *opt;
+    for (double &t : c->tuneMs) t = -1.0;
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
     for (int k = 0; k < 8 && e == hipSuccess; ++k) e = hipEventCreate(&c->ev[k]);
@@ -1276,6 +1419,45 @@ int idw_get_timings(idw_ctx *c, double *ms6) {
     return check_plan(c);
 }
 
+int idw_tune(idw_ctx *c, int force) {
+    if (!c) return fail(IDW_EINVAL, "null context");
+    TRY(ensure_device(c));
+    if (!force && tune_current(c)) return IDW_OK;
+    c->tuneFixed = false;
+    return run_tune(c);
+}
+
+int idw_set_tuning(idw_ctx *c, const int32_t *params5) {
+    if (!c || !params5) return fail(IDW_EINVAL, "null argument");
+    auto in = [](int32_t v, std::initializer_list<int32_t> l) { return std::find(l.begin(), l.end(), v) != l.end(); };
+    if (!in(params5[0], {16, 32, 64}) || !in(params5[1], {8, 16, 32}) || !in(params5[2], {64, 128, 256}) ||
+        !in(params5[3], {4, 8, 16}) || !in(params5[4], {-1, 0, 1}))
+        return fail(IDW_EINVAL, "tuning parameters outside their candidates (16/32/64, 8/16/32, 64/128/256, 4/8/16, "
+                                "-1/0/1)");
+    c->tune.fch = params5[0];
+    c->tune.mq = params5[1];
+    c->tune.wbuf = params5[2];
+    c->tune.exactG = params5[3];
+    c->tune.exactChunkShift = params5[4];
+    c->tuned = false;
+    c->tuneFixed = true;
+    ++c->gen;
+    return IDW_OK;
+}
+
+int idw_get_tuning(idw_ctx *c, int32_t *params5, double *ms, int64_t nms) {
+    if (!c || !params5) return fail(IDW_EINVAL, "null argument");
+    if (ms && nms != idw::TUNE_SLOTS) return fail(IDW_EINVAL, "expected %d timing slots", idw::TUNE_SLOTS);
+    params5[0] = c->tune.fch;
+    params5[1] = c->tune.mq;
+    params5[2] = c->tune.wbuf;
+    params5[3] = c->tune.exactG;
+    params5[4] = c->tune.exactChunkShift;
+    if (ms)
+        for (int k = 0; k < idw::TUNE_SLOTS; ++k) ms[k] = c->tuned ? c->tuneMs[k] : -1.0;
+    return c->tuned ? (tune_current(c) ? 1 : 2) : 0;
+}
+
 int idw_fast_counts(idw_ctx *c, double *counts8) {
     if (!c || !counts8) return fail(IDW_EINVAL, "null argument");
     if (!c->volSet || !c->surfInit) return fail(IDW_ESTATE, "not initialized");
@@ -1479,6 +1661,15 @@ int idw_batch_run(idw_batch *b, void *hip_stream) {
         batch_drop_graph(b);
         ++b->recordings;
         std::vector<idw_ctx *> cs = batch_contexts(b);
+        HIPCHK(hipEventRecord(b->in, user));
+        // the autotuner picks each mesh's tile parameters for its own size and
+        // options before anything is recorded (it keeps an earlier choice made
+        // for the same options and sizes)
+        for (idw_ctx *c : cs)
+            if (!c->tuneFixed && !tune_current(c)) {
+                HIPCHK(hipStreamWaitEvent(c->st, b->in, 0));
+                TRY(run_tune(c));
+            }
         HIPCHK(hipEventRecord(b->in, user));
         for (idw_ctx *c : cs) {
             HIPCHK(hipStreamWaitEvent(c->st, b->in, 0));

@@ -95,6 +95,21 @@ struct DevPlan {
     bool sized = false, filled = false;
 };
 
+// Tile parameters of the volume kernels, chosen per context by the
+// autotuner (idw_tune); the defaults are those of an untuned context.
+struct Tuning {
+    int32_t fch = 32;         // fast forward: terms staged per chunk (16, 32, 64)
+    int32_t mq = 32;          // mixed fast reverse: merge window per walk wave (8, 16, 32)
+    int32_t wbuf = 128;       // walk: plan entries gathered in LDS per kind (64, 128, 256)
+    int32_t exactG = 8;       // exact mixed forward: surface points per FP32 partial sum (4, 8, 16)
+    int32_t exactChunkShift = 0;  // exact reverse: volume chunks = the default count * 2^shift (-1, 0, 1)
+};
+
+// Slots of the autotuner's timing table (ms per launch of each candidate):
+// walk buffer 64/128/256, forward chunk 16/32/64, merge window 8/16/32, exact
+// FP32 group 4/8/16, exact reverse chunks x1/2, x1, x2.
+enum { TUNE_WBUF = 0, TUNE_FCH = 3, TUNE_MQ = 6, TUNE_EXACT_G = 9, TUNE_EXACT_CHUNKS = 12, TUNE_SLOTS = 15 };
+
 struct DevVolume {
     int64_t n = 0;
     double4 *vol = nullptr;   // x, y, z, denominator0 in traversal (Morton) order
@@ -129,14 +144,16 @@ void launch_fill(hipStream_t st, double *a, int64_t n, double v);
 void launch_zero_u32(hipStream_t st, unsigned int *a, int64_t n);
 // fast mode: blocks [blk0, blk0 + nblk) of 256 volume nodes (nblk < 0: to the end)
 void launch_forward(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, const Mirror &M,
-                    int fast, int prec, double *xv, const DevPlan *plan, int64_t blk0 = 0, int64_t nblk = -1);
+                    int fast, int prec, double *xv, const DevPlan *plan, const Tuning &T, int64_t blk0 = 0,
+                    int64_t nblk = -1);
 void launch_prep_numb(hipStream_t st, const DevVolume &V, const Mirror &M, const double *dxv);
 void launch_plan_count(hipStream_t st, const DevSurface &S, const DevVolume &V, const Mirror &M, double errTol,
                        int64_t *waveCount);
 void launch_plan_fill(hipStream_t st, const DevSurface &S, const DevVolume &V, const Mirror &M, double errTol,
-                      double nearT2, const DevPlan &plan, int64_t blk0 = 0, int64_t nblk = -1);
+                      double nearT2, const DevPlan &plan, const Tuning &T, int64_t blk0 = 0, int64_t nblk = -1);
 void launch_reverse(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, const Mirror &M,
-                    int fast, int prec, double *partial, int nchunk, const double *dxv, const DevPlan *plan);
+                    int fast, int prec, double *partial, int nchunk, const double *dxv, const DevPlan *plan,
+                    const Tuning &T);
 hipError_t exclusive_scan(hipStream_t st, const int64_t *in, int64_t *out, int64_t n);
 void launch_reverse_prep(hipStream_t st, const DevSurface &S, int fast);
 void launch_fast_count(hipStream_t st, const DevSurface &S, const DevVolume &V, const Mirror &M, double errTol,

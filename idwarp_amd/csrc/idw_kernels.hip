@@ -554,7 +554,9 @@ __global__ void __launch_bounds__(256) k_node_sums(int n, const int32_t *__restr
 // ---------------------------------------------------------------------------
 // Exact mode (evalNodeExact on the root, kd_tree.F90:664-690)
 // ---------------------------------------------------------------------------
-template <int PREC, int EXPM>
+// G: surface points per FP32 partial sum in mixed precision (4, 8 or 16, a
+// tuning parameter).
+template <int PREC, int EXPM, int G>
 __global__ void __launch_bounds__(256) k_exact_fwd(int nvol, const double4 *__restrict__ vol,
                                                    const int32_t *__restrict__ perm, int n,
                                                    const double4 *__restrict__ ps, const Rec64 *__restrict__ pd,
@@ -587,7 +589,6 @@ __global__ void __launch_bounds__(256) k_exact_fwd(int nvol, const double4 *__re
         // nearer than nearWallFactor*Ldef for any lane of the wave stay FP64.
         const float Lf = (float)P.Ldef;
         const double T2 = P.nearT2;
-        constexpr int G = 8;
         for (int i0 = 0; i0 < n; i0 += G) {
             float a0 = 0.f, a1 = 0.f, a2 = 0.f, ad = 0.f;
             const int ie = min(i0 + G, n);
@@ -732,7 +733,8 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-constexpr int WBUF = 128;  // plan entries of each kind a wave gathers in LDS before writing them out
+// WBUF: plan entries of each kind a wave gathers in LDS before writing them
+// out (64, 128 or 256, a tuning parameter).
 
 // Wave-level record of the walk: one entry per term (accepted cluster, or
 // point of an evaluated bucket) with the mask of the lanes it serves, in
@@ -744,6 +746,7 @@ constexpr int WBUF = 128;  // plan entries of each kind a wave gathers in LDS be
 // in front of its error-table loads.  Writes outside the range are dropped
 // and a count that disagrees with the counting pass is reported in
 // *mismatch.
+template <int WBUF>
 struct PlanWriter {
     PlanEntry *plan;
     int64_t off, end, nf = 0, nb = 0;
@@ -827,9 +830,9 @@ __device__ __forceinline__ WalkHdr load_whdr(const WalkHdr *w, unsigned off) {
 
 // One visit of the walk below at node (walk header h):
 // the lane decisions, the plan entries, and the wave's next node.
-template <bool COUNT>
+template <bool COUNT, int WBUF>
 __device__ __forceinline__ int walk_visit(const WalkHdr &h, int node, const double4 &v, double tol,
-                                          int &stop, PlanWriter &pw, int64_t &cnt, const NodeRec *nodes,
+                                          int &stop, PlanWriter<WBUF> &pw, int64_t &cnt, const NodeRec *nodes,
                                           int nnodes, const SegCell *cells, double nearT2, double nearT) {
 #pragma clang fp contract(off)
     const double cx = v.x - h.X[0], cy = v.y - h.X[1], cz = v.z - h.X[2];
@@ -888,7 +891,7 @@ __device__ __forceinline__ int walk_visit(const WalkHdr &h, int node, const doub
 // per wave only.  A bucket's points are FP64 terms when an active lane lies
 // within sqrt(nearT2) of the bucket's bounding sphere (mixed precision keeps
 // those pairs in FP64).
-template <bool COUNT>
+template <bool COUNT, int WBUF>
 __global__ void __launch_bounds__(256) k_plan_walk(int nvol, int blk0, int nbc, const double4 *__restrict__ vol,
                                                    int nnodes, const WalkHdr *__restrict__ whdr,
                                                    const NodeRec *__restrict__ nodes, double errTol, double nearT2,
@@ -915,7 +918,7 @@ __global__ void __launch_bounds__(256) k_plan_walk(int nvol, int blk0, int nbc, 
     // the node's coordinates are in before the loop, so that the loop need
     // not wait for the plan's stores in flight to learn it
     wait_vmem();
-    PlanWriter pw{plan, COUNT ? 0 : waveOff[wave], COUNT ? 0 : waveOff[wave + 1], 0, 0, mismatch,
+    PlanWriter<WBUF> pw{plan, COUNT ? 0 : waveOff[wave], COUNT ? 0 : waveOff[wave + 1], 0, 0, mismatch,
                   wbuf[COUNT ? 0 : threadIdx.x >> 6], wbuf64[COUNT ? 0 : threadIdx.x >> 6]};
     int64_t cnt = 0;
     // Before each visit the walk touches a word of both possible successors'
@@ -934,7 +937,7 @@ __global__ void __launch_bounds__(256) k_plan_walk(int nvol, int blk0, int nbc, 
             reinterpret_cast<const unsigned *>(reinterpret_cast<const char *>(whdr) + (unsigned)(node + 1) * HB);
         const unsigned *pb = reinterpret_cast<const unsigned *>(reinterpret_cast<const char *>(whdr) + (unsigned)h.skip * HB);
         const unsigned ta = pa[0], tb = pb[0];
-        const int next = walk_visit<COUNT>(h, node, v, tol, stop, pw, cnt, nodes, nnodes, cells, nearT2, nearT);
+        const int next = walk_visit<COUNT, WBUF>(h, node, v, tol, stop, pw, cnt, nodes, nnodes, cells, nearT2, nearT);
         if (next >= nnodes) break;
         touched += ta ^ tb;
         h = load_whdr(whdr, (unsigned)next * HB);
@@ -1051,11 +1054,13 @@ __device__ __forceinline__ double first_lane(double x) {
 // MFMA lane layout (16x16x4 f64): lane l supplies A[l%16][l/16] and
 // B[l/16][l%16]; register j of D holds D[4j + l/16][l%16].
 
-constexpr int FCH = 32;   // terms staged per chunk in the forward
+// Terms staged per chunk in the forward (FCH: 16, 32 or 64, a tuning
+// parameter; 64 / FCH lanes stage each term).
 constexpr int BROW = 17;  // padded LDS row (doubles) of a staged B row
 
 constexpr int BROWF = 17;  // padded LDS row (floats) of a staged FP32 B row
 
+template <int FCH>
 struct FwdChunk {
     union {
         double B[FCH][BROW];    // K (9), c (3), 1, 0, 0, 0
@@ -1065,8 +1070,9 @@ struct FwdChunk {
     double eps[FCH];      // added to |r|^2: EPS_W for points (evalNodeExact), 0 for clusters (evalNodeApprox)
     unsigned long long mask[FCH];
 };
+template <int FCH>
 union FwdLds {
-    FwdChunk c;
+    FwdChunk<FCH> c;
     double acc[32][BROW];  // half of the wave's 64 x 16 sums
 };
 
@@ -1098,23 +1104,28 @@ __device__ __forceinline__ float weight_f32(float d2, float A, float AL3, float 
     return A * wfun<EXPM>(Lf * __frsqrt_rn(d2), P);
 }
 
-// Stage FCH terms of the plan in LDS, two lanes per term: lane r writes
-// row 0 of K and c, x, A, eps and the mask; lane r + FCH rows 1 and 2.
-// pe: this lane's entry (record -1: padding row).  W32 reads the FP32
-// records (mixed precision).
-template <bool W32>
-__device__ __forceinline__ void stage_terms(FwdChunk &C, PlanEntry pe, int nnodes, int nrec,
+// Stage FCH terms of the plan in LDS, 64 / FCH lanes per term: term r is
+// staged by lanes r + FCH p.  Part p = 0 writes row 0 of K and c (all rows
+// when one lane stages a term), x, A, eps and the mask; the other parts the
+// remaining rows; the last part the constant column.  pe: this lane's entry
+// (record -1: padding row).  W32 reads the FP32 records (mixed precision).
+template <bool W32, int FCH>
+__device__ __forceinline__ void stage_terms(FwdChunk<FCH> &C, PlanEntry pe, int nnodes, int nrec,
                                             const Rec64 *__restrict__ rec, const Rec32 *__restrict__ rec32,
                                             double ox, double oy, double oz) {
-    const int lane = threadIdx.x & 63, r = lane & (FCH - 1);
-    const bool hi = lane >= FCH;
+    static_assert(FCH == 16 || FCH == 32 || FCH == 64, "16, 32 or 64 terms per chunk");
+    constexpr int LPT = 64 / FCH;  // lanes per term
+    const int lane = threadIdx.x & 63, r = lane & (FCH - 1), part = lane / FCH;
+    // K rows [a0, a1) of this lane: 1 lane {0,1,2}; 2 lanes {0}, {1,2}; 4 lanes {0}, {1}, {2}, {}
+    const int a0 = LPT == 1 ? 0 : (LPT == 2 ? (part == 0 ? 0 : 1) : min(part, 3));
+    const int a1 = LPT == 1 ? 3 : (LPT == 2 ? (part == 0 ? 1 : 3) : min(part + 1, 3));
+    const bool lo = part == 0, hi = part == LPT - 1;
     if (pe.rec >= 0) {
         pe.rec = min(pe.rec, nrec - 1);  // a plan that disagreed with its count stays in bounds
         double x[3];
         if (W32) for (int i = 0; i < 3; ++i) x[i] = rec32[pe.rec].x[i];
         else for (int i = 0; i < 3; ++i) x[i] = rec[pe.rec].x[i];
         const double y0 = x[0] - ox, y1 = x[1] - oy, y2 = x[2] - oz;
-        const int a0 = hi ? 1 : 0, a1 = hi ? 3 : 1;  // K rows [a0, a1) of this lane
         for (int a = a0; a < a1; ++a) {
             double K0, K1, K2, d;
             if (W32) {
@@ -1145,18 +1156,20 @@ __device__ __forceinline__ void stage_terms(FwdChunk &C, PlanEntry pe, int nnode
                 C.B[r][12] = 1.0;
                 C.B[r][13] = C.B[r][14] = C.B[r][15] = 0.0;
             }
-        } else {
+        }
+        if (lo) {
             for (int i = 0; i < 3; ++i) C.X[r][i] = x[i];
             C.X[r][3] = W32 ? (double)rec32[pe.rec].A : rec[pe.rec].A;
             C.eps[r] = pe.rec < nnodes ? 0.0 : EPS_W;
             C.mask[r] = pe.mask;
         }
     } else {
-        for (int j = hi ? 8 : 0; j < (hi ? 16 : 8); ++j) {
+        constexpr int W = 16 / LPT;  // B columns each part clears
+        for (int j = part * W; j < (part + 1) * W; ++j) {
             if (W32) C.Bf[r][j] = 0.0f;
             else C.B[r][j] = 0.0;
         }
-        if (!hi) {
+        if (lo) {
             for (int i = 0; i < 4; ++i) C.X[r][i] = 0.0;
             C.eps[r] = 1.0;
             C.mask[r] = 0ull;
@@ -1200,10 +1213,10 @@ __device__ __forceinline__ void add_f32_sums(dvec4 &D, fvec4 F) {
 // The entries of the next chunk are requested while this one is summed.
 // W32 (mixed precision, FP32 terms): FP32 weights and B rows on
 // v_mfma_f32_16x16x4f32, each chunk's FP32 sums added to the FP64 sums.
-template <bool W32, int EXPM>
+template <bool W32, int EXPM, int FCH>
 __device__ __forceinline__ void fwd_terms(int64_t a, int64_t b, const PlanEntry *__restrict__ plan, int nnodes,
                                           int nrec, const Rec64 *__restrict__ rec, const Rec32 *__restrict__ rec32,
-                                          const WParams &P, FwdChunk &C, double ox, double oy, double oz,
+                                          const WParams &P, FwdChunk<FCH> &C, double ox, double oy, double oz,
                                           const double (&vx)[4], const double (&vy)[4], const double (&vz)[4],
                                           dvec4 (&D)[4]) {
     const int lane = threadIdx.x & 63, n = lane & 15, R = lane >> 4, r = lane & (FCH - 1);
@@ -1214,7 +1227,7 @@ __device__ __forceinline__ void fwd_terms(int64_t a, int64_t b, const PlanEntry 
     for (int64_t c0 = a; c0 < b; c0 += FCH) {
         const int cnt = (int)min((int64_t)FCH, b - c0);
         wave_sync();  // the previous chunk is consumed
-        stage_terms<W32>(C, pe, nnodes, nrec, rec, rec32, ox, oy, oz);
+        stage_terms<W32, FCH>(C, pe, nnodes, nrec, rec, rec32, ox, oy, oz);
         pe = plan_entry(plan, c0 + FCH + r, b);  // next chunk
         wave_sync();
         const int ng = (cnt + 3) >> 2;
@@ -1272,7 +1285,7 @@ __device__ __forceinline__ void fwd_terms(int64_t a, int64_t b, const PlanEntry 
 // terms (evalNodeExact); then Xv = Xv0 + num / den (warpMesh.F90:65-70).
 // Mixed precision: FP32 weights for the wave's FP32 terms, FP64 for the
 // rest; the products are summed in FP64 either way.
-template <int PREC, int EXPM>
+template <int PREC, int EXPM, int FCH>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_fast_fwd(int nvol, int blk0, int nbc,
                                                   const double4 *__restrict__ vol,
                                                   const int32_t *__restrict__ perm, const PlanEntry *__restrict__ plan,
@@ -1281,7 +1294,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
                                                   const Rec64 *__restrict__ rec, const Rec32 *__restrict__ rec32,
                                                   WParams P, Mirror M, double *__restrict__ xv,
                                                   double *__restrict__ den, double4 *__restrict__ acc) {
-    __shared__ FwdLds lds[4];
+    __shared__ FwdLds<FCH> lds[4];
     const int lb = xcd_block(nbc);
     if (lb >= nbc) return;
     const int blk = blk0 + lb;
@@ -1304,14 +1317,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     dvec4 D[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) D[t] = dvec4{0.0, 0.0, 0.0, 0.0};
-    FwdLds &L = lds[wv];
+    FwdLds<FCH> &L = lds[wv];
     const int64_t e0 = waveOff[wave], e1 = waveOff[wave + 1];
     if (PREC == IDW_PREC_MIXED) {
         const int64_t es = min(max(waveSplit[wave], e0), e1);
-        fwd_terms<true, EXPM>(e0, es, plan, nnodes, nrec, rec, rec32, P, L.c, ox, oy, oz, vx, vy, vz, D);
-        fwd_terms<false, EXPM>(es, e1, plan, nnodes, nrec, rec, rec32, P, L.c, ox, oy, oz, vx, vy, vz, D);
+        fwd_terms<true, EXPM, FCH>(e0, es, plan, nnodes, nrec, rec, rec32, P, L.c, ox, oy, oz, vx, vy, vz, D);
+        fwd_terms<false, EXPM, FCH>(es, e1, plan, nnodes, nrec, rec, rec32, P, L.c, ox, oy, oz, vx, vy, vz, D);
     } else {
-        fwd_terms<false, EXPM>(e0, e1, plan, nnodes, nrec, rec, rec32, P, L.c, ox, oy, oz, vx, vy, vz, D);
+        fwd_terms<false, EXPM, FCH>(e0, e1, plan, nnodes, nrec, rec, rec32, P, L.c, ox, oy, oz, vx, vy, vz, D);
     }
     // D[t] register j: node 16t + 4j + R, column n.  Through LDS, 32 nodes
     // at a time, each lane gathers its node's 13 sums.  The lane's own
@@ -1494,10 +1507,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
 // over that wave's 64 nodes, so no node is weighted against a term its wave
 // does not use.  FP32 weights and G on v_mfma_f32_16x16x4f32; the FP64 terms
 // (near the wall) take the per-wave path with FP64 sums.
-constexpr int MQ = 32;              // plan entries of each wave in the merge buffers
-constexpr int QCAP = RT + MQ;      // a queue holds < RT items before a merge round adds at most MQ to it
-
+// MQ: plan entries of each wave in the merge buffers (8, 16 or 32, a tuning
+// parameter); a queue holds < RT items before a merge round adds at most MQ
+// to it, so QCAP = RT + MQ.
+template <int MQ>
 struct PairMergeLds {
+    static constexpr int QCAP = RT + MQ;
     unsigned long long keyA[MQ], keyB[MQ], mskA[MQ], mskB[MQ];
     // union terms by kind: 0 in both plans, 1 in A's only, 2 in B's only
     int32_t qrec[3][QCAP];
@@ -1505,10 +1520,11 @@ struct PairMergeLds {
     unsigned long long qmB[2][QCAP];  // B masks of kinds 0 and 2
 };
 
+template <int MQ>
 struct PairLds {
     double4 node[128];  // mirrored positions: wave A's nodes, then wave B's
     union {
-        PairMergeLds q;      // during the merge
+        PairMergeLds<MQ> q;  // during the merge
         double s[128][3];    // seeds, while the B operands are built
     };
     double Y[RT][4];  // x - o of the tile's terms
@@ -1539,13 +1555,14 @@ struct PairTile {
     double4 xa;  // x, A of the record (zero for padding)
 };
 
-template <int EXPM>
+template <int EXPM, int MQ>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 k_fast_rev_pair(int nvol, int nwave, const double4 *__restrict__ vol, const int32_t *__restrict__ perm,
                 const double *__restrict__ dxv, const double *__restrict__ den, const PlanEntry *__restrict__ plan,
                 const int64_t *__restrict__ waveOff, const int64_t *__restrict__ waveSplit, int nnodes, int nrec,
                 const Rec64 *__restrict__ rec, WParams P, Mirror M, double *__restrict__ adj) {
-    __shared__ PairLds lds[4];
+    static_assert(MQ == 8 || MQ == 16 || MQ == 32, "both merge buffers are filled by one load per lane");
+    __shared__ PairLds<MQ> lds[4];
     const int npair = (nwave + 1) >> 1;
     const int nblk = (npair + 3) >> 2;
     const int blk = xcd_block(nblk);
@@ -1553,7 +1570,7 @@ k_fast_rev_pair(int nvol, int nwave, const double4 *__restrict__ vol, const int3
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, m = lane & 15, R = lane >> 4;
     const int pair = uniform(blk * 4 + wv);
     if (pair >= npair) return;  // wave-uniform
-    PairLds &L = lds[wv];
+    PairLds<MQ> &L = lds[wv];
     // the pair's nodes: lane j holds node j of wave A and node j of wave B
     auto node_seed = [&](int k, double4 &v, double (&sd)[3]) {  // mirrored position and seed s = dXv / den
         const bool valid = k < nvol;
@@ -2301,29 +2318,52 @@ void launch_fill(hipStream_t st, double *a, int64_t n, double v) {
     if (n > 0) k_fill<<<grid_for(n, 256), 256, 0, st>>>(n, a, v);
 }
 
+template <int PREC, int EXPM, int FCH>
+static void fast_fwd_launch(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, const Mirror &M,
+                            double *xv, const DevPlan *plan, int64_t blk0, int64_t nblk) {
+    k_fast_fwd<PREC, EXPM, FCH><<<xcd_blocks(nblk), 256, 0, st>>>(
+        (int)V.n, (int)blk0, (int)nblk, V.vol, V.perm, plan->entries, plan->waveOff, plan->waveSplit, S.nnodes,
+        S.nnodes + S.n, S.rec, S.rec32, P, M, xv, V.den, V.acc);
+}
+
+template <int PREC, int EXPM>
+static void fast_fwd_fch(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, const Mirror &M,
+                         double *xv, const DevPlan *plan, int64_t blk0, int64_t nblk, int fch) {
+    if (fch == 16) fast_fwd_launch<PREC, EXPM, 16>(st, S, V, P, M, xv, plan, blk0, nblk);
+    else if (fch == 64) fast_fwd_launch<PREC, EXPM, 64>(st, S, V, P, M, xv, plan, blk0, nblk);
+    else fast_fwd_launch<PREC, EXPM, 32>(st, S, V, P, M, xv, plan, blk0, nblk);
+}
+
+template <int PREC, int EXPM>
+static void exact_fwd_g(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, const Mirror &M,
+                        double *xv, int g) {
+    const unsigned grid = xcd_grid(V.n);
+    if (g == 4)
+        k_exact_fwd<PREC, EXPM, 4><<<grid, 256, 0, st>>>((int)V.n, V.vol, V.perm, S.n, S.ps, S.pd, S.pd32, P, M, xv,
+                                                        V.den, V.acc);
+    else if (g == 16)
+        k_exact_fwd<PREC, EXPM, 16><<<grid, 256, 0, st>>>((int)V.n, V.vol, V.perm, S.n, S.ps, S.pd, S.pd32, P, M, xv,
+                                                         V.den, V.acc);
+    else
+        k_exact_fwd<PREC, EXPM, 8><<<grid, 256, 0, st>>>((int)V.n, V.vol, V.perm, S.n, S.ps, S.pd, S.pd32, P, M, xv,
+                                                        V.den, V.acc);
+}
+
 template <int EXPM>
 static void forward_impl(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, const Mirror &M,
-                         int fast, int prec, double *xv, const DevPlan *plan, int64_t blk0, int64_t nblk) {
-    const unsigned g = xcd_grid(V.n);
+                         int fast, int prec, double *xv, const DevPlan *plan, int64_t blk0, int64_t nblk,
+                         const Tuning &T) {
     if (nblk < 0) nblk = grid_for(V.n, 256) - blk0;
     if (fast && nblk <= 0) return;
-    const unsigned gc = xcd_blocks(nblk);
     if (!fast) {
+        // G only shapes the mixed-precision kernel
         if (prec == IDW_PREC_FP64)
-            k_exact_fwd<IDW_PREC_FP64, EXPM><<<g, 256, 0, st>>>((int)V.n, V.vol, V.perm, S.n, S.ps, S.pd, S.pd32, P, M,
-                                                               xv, V.den, V.acc);
-        else
-            k_exact_fwd<IDW_PREC_MIXED, EXPM><<<g, 256, 0, st>>>((int)V.n, V.vol, V.perm, S.n, S.ps, S.pd, S.pd32, P, M,
-                                                                xv, V.den, V.acc);
+            k_exact_fwd<IDW_PREC_FP64, EXPM, 8><<<xcd_grid(V.n), 256, 0, st>>>((int)V.n, V.vol, V.perm, S.n, S.ps, S.pd,
+                                                                              S.pd32, P, M, xv, V.den, V.acc);
+        else exact_fwd_g<IDW_PREC_MIXED, EXPM>(st, S, V, P, M, xv, T.exactG);
     } else {
-        if (prec == IDW_PREC_FP64)
-            k_fast_fwd<IDW_PREC_FP64, EXPM><<<gc, 256, 0, st>>>(
-                (int)V.n, (int)blk0, (int)nblk, V.vol, V.perm, plan->entries, plan->waveOff, plan->waveSplit,
-                S.nnodes, S.nnodes + S.n, S.rec, S.rec32, P, M, xv, V.den, V.acc);
-        else
-            k_fast_fwd<IDW_PREC_MIXED, EXPM><<<gc, 256, 0, st>>>(
-                (int)V.n, (int)blk0, (int)nblk, V.vol, V.perm, plan->entries, plan->waveOff, plan->waveSplit,
-                S.nnodes, S.nnodes + S.n, S.rec, S.rec32, P, M, xv, V.den, V.acc);
+        if (prec == IDW_PREC_FP64) fast_fwd_fch<IDW_PREC_FP64, EXPM>(st, S, V, P, M, xv, plan, blk0, nblk, T.fch);
+        else fast_fwd_fch<IDW_PREC_MIXED, EXPM>(st, S, V, P, M, xv, plan, blk0, nblk, T.fch);
     }
 }
 
@@ -2334,22 +2374,39 @@ void launch_prep_numb(hipStream_t st, const DevVolume &V, const Mirror &M, const
 void launch_plan_count(hipStream_t st, const DevSurface &S, const DevVolume &V, const Mirror &M, double errTol,
                        int64_t *waveCount) {
     const int nb = (int)grid_for(V.n, 256);
-    k_plan_walk<true><<<nb, 256, 0, st>>>((int)V.n, 0, nb, V.vol, S.nnodes, S.whdr, S.nodes, errTol, 0.0, waveCount, nullptr,
-                                          nullptr, nullptr, nullptr, M);
+    k_plan_walk<true, 128><<<nb, 256, 0, st>>>((int)V.n, 0, nb, V.vol, S.nnodes, S.whdr, S.nodes, errTol, 0.0, waveCount,
+                                               nullptr, nullptr, nullptr, nullptr, M);
+}
+
+template <int WBUF>
+static void plan_fill_launch(hipStream_t st, const DevSurface &S, const DevVolume &V, const Mirror &M, double errTol,
+                             double nearT2, const DevPlan &plan, int64_t blk0, int64_t nblk) {
+    k_plan_walk<false, WBUF><<<xcd_blocks(nblk), 256, 0, st>>>((int)V.n, (int)blk0, (int)nblk, V.vol, S.nnodes, S.whdr,
+                                                               S.nodes, errTol, nearT2, nullptr, plan.entries,
+                                                               plan.waveOff, plan.waveSplit, plan.mismatch, M);
 }
 
 void launch_plan_fill(hipStream_t st, const DevSurface &S, const DevVolume &V, const Mirror &M, double errTol,
-                      double nearT2, const DevPlan &plan, int64_t blk0, int64_t nblk) {
+                      double nearT2, const DevPlan &plan, const Tuning &T, int64_t blk0, int64_t nblk) {
     if (nblk < 0) nblk = grid_for(V.n, 256) - blk0;
     if (nblk <= 0) return;
-    k_plan_walk<false><<<xcd_blocks(nblk), 256, 0, st>>>((int)V.n, (int)blk0, (int)nblk, V.vol, S.nnodes, S.whdr, S.nodes,
-                                                         errTol, nearT2, nullptr, plan.entries, plan.waveOff,
-                                                         plan.waveSplit, plan.mismatch, M);
+    if (T.wbuf == 64) plan_fill_launch<64>(st, S, V, M, errTol, nearT2, plan, blk0, nblk);
+    else if (T.wbuf == 256) plan_fill_launch<256>(st, S, V, M, errTol, nearT2, plan, blk0, nblk);
+    else plan_fill_launch<128>(st, S, V, M, errTol, nearT2, plan, blk0, nblk);
+}
+
+template <int EXPM, int MQ>
+static void rev_pair_launch(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, const Mirror &M,
+                            const double *dxv, const DevPlan *plan) {
+    k_fast_rev_pair<EXPM, MQ><<<xcd_blocks(((plan->nwave + 1) / 2 + 3) / 4), 256, 0, st>>>(
+        (int)V.n, (int)plan->nwave, V.vol, V.perm, dxv, V.den, plan->entries, plan->waveOff, plan->waveSplit, S.nnodes,
+        S.nnodes + S.n, S.rec, P, M, S.adj);
 }
 
 template <int EXPM>
 static void reverse_impl(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, const Mirror &M,
-                         int fast, int prec, double *partial, int nchunk, const double *dxv, const DevPlan *plan) {
+                         int fast, int prec, double *partial, int nchunk, const double *dxv, const DevPlan *plan,
+                         const Tuning &T) {
     if (!fast) {
         launch_prep_numb(st, V, M, dxv);
         const double4 *pos = M.mask ? V.volm : V.vol;
@@ -2368,10 +2425,12 @@ static void reverse_impl(hipStream_t st, const DevSurface &S, const DevVolume &V
             k_fast_rev<IDW_PREC_FP64, EXPM><<<g, 256, 0, st>>>(
                 (int)V.n, V.vol, V.perm, dxv, V.den, plan->entries, plan->waveOff, plan->waveSplit, S.nnodes,
                 S.nnodes + S.n, S.rec, P, M, S.adj);
+        else if (T.mq == 8)
+            rev_pair_launch<EXPM, 8>(st, S, V, P, M, dxv, plan);
+        else if (T.mq == 16)
+            rev_pair_launch<EXPM, 16>(st, S, V, P, M, dxv, plan);
         else
-            k_fast_rev_pair<EXPM><<<xcd_blocks(((plan->nwave + 1) / 2 + 3) / 4), 256, 0, st>>>(
-                (int)V.n, (int)plan->nwave, V.vol, V.perm, dxv, V.den, plan->entries, plan->waveOff, plan->waveSplit,
-                S.nnodes, S.nnodes + S.n, S.rec, P, M, S.adj);
+            rev_pair_launch<EXPM, 32>(st, S, V, P, M, dxv, plan);
     }
 }
 
@@ -2385,15 +2444,16 @@ void launch_fast_count(hipStream_t st, const DevSurface &S, const DevVolume &V, 
 }
 
 void launch_forward(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, const Mirror &M,
-                    int fast, int prec, double *xv, const DevPlan *plan, int64_t blk0, int64_t nblk) {
-    if (P.expMode == EXP_35) forward_impl<EXP_35>(st, S, V, P, M, fast, prec, xv, plan, blk0, nblk);
-    else forward_impl<EXP_INT>(st, S, V, P, M, fast, prec, xv, plan, blk0, nblk);
+                    int fast, int prec, double *xv, const DevPlan *plan, const Tuning &T, int64_t blk0, int64_t nblk) {
+    if (P.expMode == EXP_35) forward_impl<EXP_35>(st, S, V, P, M, fast, prec, xv, plan, blk0, nblk, T);
+    else forward_impl<EXP_INT>(st, S, V, P, M, fast, prec, xv, plan, blk0, nblk, T);
 }
 
 void launch_reverse(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, const Mirror &M,
-                    int fast, int prec, double *partial, int nchunk, const double *dxv, const DevPlan *plan) {
-    if (P.expMode == EXP_35) reverse_impl<EXP_35>(st, S, V, P, M, fast, prec, partial, nchunk, dxv, plan);
-    else reverse_impl<EXP_INT>(st, S, V, P, M, fast, prec, partial, nchunk, dxv, plan);
+                    int fast, int prec, double *partial, int nchunk, const double *dxv, const DevPlan *plan,
+                    const Tuning &T) {
+    if (P.expMode == EXP_35) reverse_impl<EXP_35>(st, S, V, P, M, fast, prec, partial, nchunk, dxv, plan, T);
+    else reverse_impl<EXP_INT>(st, S, V, P, M, fast, prec, partial, nchunk, dxv, plan, T);
 }
 
 void launch_nodal_reverse(hipStream_t st, const DevSurface &S, const double *xs, int fast, int useRot, double *dxs,

@@ -45,6 +45,7 @@ _DEFAULTS = {
     "precision": [str, ["fp64", "mixed"]],
     "nearWallFactor": [float, 1e-6],
     "reusePlan": [bool, False],
+    "autotune": [bool, False],
     "device": [int, 0],
 }
 
@@ -263,7 +264,47 @@ class USMesh:
         """Warp the volume to the surface set by setSurfaceCoordinates()."""
         self._setInternalSurface()
         self._setMeshOptions()
+        if self.getOption("autotune"):
+            _lib.check(self._L.idw_tune(self._ctx, 0))
         _lib.check(self._L.idw_warp_mesh(self._ctx))
+
+    def tune(self, force=False):
+        """Pick the volume kernels' tile parameters for this mesh at the
+        current options by timing the candidates (idw_tune); the option
+        autotune does this before the first warp at new options."""
+        self._setInternalSurface()
+        self._setMeshOptions()
+        _lib.check(self._L.idw_tune(self._ctx, int(bool(force))))
+
+    TUNING_NAMES = ("forwardChunk", "mergeWindow", "walkBuffer", "exactGroup", "exactChunkShift")
+    TUNING_SLOTS = (("walkBuffer", (64, 128, 256)), ("forwardChunk", (16, 32, 64)), ("mergeWindow", (8, 16, 32)),
+                    ("exactGroup", (4, 8, 16)), ("exactChunkShift", (-1, 0, 1)))
+
+    def setTuning(self, **params):
+        """Set tile parameters by name (forwardChunk, mergeWindow, walkBuffer,
+        exactGroup, exactChunkShift) without a search."""
+        chosen = self.getTuning()[0]
+        unknown = set(params) - set(self.TUNING_NAMES)
+        if unknown:
+            raise Error("unknown tuning parameters: %s" % ", ".join(sorted(unknown)))
+        chosen.update(params)
+        p = (ctypes.c_int32 * 5)(*[int(chosen[k]) for k in self.TUNING_NAMES])
+        _lib.check(self._L.idw_set_tuning(self._ctx, p))
+
+    def getTuning(self):
+        """(chosen parameters, {parameter: {candidate: ms per launch}}, state)
+        with state "untuned", "current" or "stale" (tuned at other options)."""
+        p = (ctypes.c_int32 * 5)()
+        ms = (ctypes.c_double * 15)()
+        rc = self._L.idw_get_tuning(self._ctx, p, ms, 15)
+        if rc < 0:
+            _lib.check(rc)
+        chosen = dict(zip(self.TUNING_NAMES, list(p)))
+        table, k = {}, 0
+        for name, cands in self.TUNING_SLOTS:
+            table[name] = {c: ms[k + i] for i, c in enumerate(cands) if ms[k + i] >= 0}
+            k += len(cands)
+        return chosen, table, ("untuned", "current", "stale")[rc]
 
     def _warpMeshAsync(self):
         """warpMesh enqueued on this mesh's stream; complete after _wait()."""

@@ -240,6 +240,31 @@ int idw_get_timings(idw_ctx *ctx, double *ms6);
  * No counterpart in the reference. */
 int idw_fast_counts(idw_ctx *ctx, double *counts8);
 
+/* Autotuner (no counterpart in the reference): times the candidates of the
+ * volume kernels' tile parameters on this context's mesh at its current
+ * options and keeps the fastest of each.  Fast mode: the walk's LDS gather
+ * buffer (64, 128, 256 plan entries), the forward's chunk of staged terms
+ * (16, 32, 64) and, in mixed precision, the reverse's merge window (8, 16,
+ * 32 entries per walk wave); exact mode: the mixed forward's FP32 group (4,
+ * 8, 16 surface points) and the reverse's volume chunks (half, the default,
+ * double).  Results do not depend on the choice beyond rounding (exact
+ * mode's chunking; the order of FP32 partial sums).  Warps the current
+ * surface.  Keeps an earlier choice made for the same evaluation mode,
+ * precision, errTol, sizes and planes unless force is set.  idw_batch_run
+ * calls it for every context it records. */
+int idw_tune(idw_ctx *ctx, int force);
+/* The chosen parameters {forward chunk, merge window, walk buffer, exact
+ * FP32 group, exact chunk shift (-1, 0, 1)} and, if ms is not NULL, the 15
+ * timings of the last search in ms per launch (walk buffer 64/128/256,
+ * forward chunk 16/32/64, merge window 8/16/32, exact group 4/8/16, exact
+ * chunks x1/2, x1, x2; -1 where not applicable).  Returns 0 if never tuned,
+ * 1 if tuned for the current options and sizes, 2 if tuned for others. */
+int idw_get_tuning(idw_ctx *ctx, int32_t *params5, double *ms, int64_t nms);
+/* Set the parameters (same order and candidates as idw_get_tuning) without
+ * a search; the context counts as untuned afterwards, and batches keep these
+ * parameters instead of searching (until idw_tune is called). */
+int idw_set_tuning(idw_ctx *ctx, const int32_t *params5);
+
 /* Batches (no counterpart in the reference, which warps one surface per
  * call): many warps and reverse products over one or more contexts on one
  * device, recorded at the first idw_batch_run into a HIP graph and replayed

@@ -442,3 +442,56 @@ def test_device_allocation_failure_is_reported(after, monkeypatch):
     g.setSurfaceCoordinates(m["pts"])
     g.warpMesh()
     assert np.array_equal(g.getWarpGrid(), m["xv0"].ravel())
+
+
+@pytest.mark.parametrize("case", ["fast-mixed", "fast-fp64", "exact-mixed", "exact-fp64"])
+def test_autotuner_candidates_and_choice(case):
+    """Every candidate of every tile parameter gives the oracle's warp and
+    reverse within the precision's bar (star body with 80 shells, 9-point
+    cube-sphere edges: 29,120 volume nodes; ellipsoid 33x17x13 in exact
+    mode); the search times each applicable candidate, keeps the fastest,
+    leaves its results within the bar and is kept for the same options."""
+    mode, prec = case.split("-")
+    m = synthetic.star_shells(n=9, nshell=80) if mode == "fast" else synthetic.ellipsoid_omesh(33, 17, 13)
+    opts = dict(evalMode=mode, errTol=1e-4, precision=prec)
+    ora = _oracle(m, np.arange(len(m["xv0"])), **opts)
+    xs = m["perturb"](m["pts"])
+    ora.setSurfaceCoordinates(xs)
+    ref = ora.warpMesh()
+    v = synthetic.random_dxv(len(m["xv0"]), seed=4, frac=0.5)
+    dref = ora.warpDeriv(v)
+    tol, atol = (MIXED_TOL, MIXED_TOL) if prec == "mixed" else (FP64_TOL, 1e-10)
+    g = _usmesh(m, **opts)
+    g.setSurfaceCoordinates(xs)
+    cands = dict(USMesh.TUNING_SLOTS)
+    for name, values in cands.items():
+        for val in values:
+            g.setTuning(**{name: val})
+            g.warpMesh()
+            err = np.abs(g.getWarpGrid() - ref).max() / ora.Ldef0
+            g.warpDeriv(v, solverVec=False)
+            r = _rel(g.getdXs().ravel(), dref)
+            assert err <= tol and r <= atol, (name, val, err, r)
+        g.setTuning(**{name: dict(forwardChunk=32, mergeWindow=32, walkBuffer=128, exactGroup=8,
+                                  exactChunkShift=0)[name]})
+    g.tune()
+    chosen, table, state = g.getTuning()
+    print(case, "tuning", chosen, table, flush=True)
+    assert state == "current"
+    timed = {"fast": ["walkBuffer", "forwardChunk"] + (["mergeWindow"] if prec == "mixed" else []),
+             "exact": (["exactGroup"] if prec == "mixed" else []) + ["exactChunkShift"]}[mode]
+    for name in timed:
+        assert sorted(table[name]) == sorted(cands[name]) and min(table[name].values()) > 0
+        assert chosen[name] == min(table[name], key=table[name].get)
+    for name in set(cands) - set(timed):
+        assert not table[name]
+    g.warpMesh()
+    assert np.abs(g.getWarpGrid() - ref).max() <= tol * ora.Ldef0
+    g.warpDeriv(v, solverVec=False)
+    assert _rel(g.getdXs().ravel(), dref) <= atol
+    g.setOption("autotune", True)
+    g.warpMesh()  # the choice is current: no new search
+    assert g.getTuning()[1] == table
+    g.setOption("errTol", 1e-3)
+    g.warpMesh()  # other options: searched again
+    assert g.getTuning()[2] == "current" and (mode == "exact" or g.getTuning()[1] != table)
