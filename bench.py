@@ -161,13 +161,19 @@ def main():
         return ctypes.c_void_p(t.data_ptr())
 
     out, tuning = {}, {}
-    legs = {"both": ["fp64", "mixed", "mixed_reuse"], "fp64": ["fp64"], "mixed": ["mixed"]}[args.only]
+    legs = {"both": ["fp64", "mixed", "mixed_reuse", "mixed_det"], "fp64": ["fp64"], "mixed": ["mixed"]}[args.only]
+
+    def set_leg(leg):
+        mesh.setOption("precision", leg.split("_")[0])
+        mesh.setOption("reusePlan", leg.endswith("_reuse"))
+        # bit-reproducible reverse: per-entry sums reduced per record in plan order
+        mesh.setOption("deterministic", leg.endswith("_det"))
+        mesh._setMeshOptions()
+
     for leg in legs:
         prec = leg.split("_")[0]
-        mesh.setOption("precision", prec)
-        mesh.setOption("reusePlan", leg.endswith("_reuse"))
-        mesh._setMeshOptions()
-        if not args.no_tune:
+        set_leg(leg)
+        if not args.no_tune and not leg.endswith("_det"):  # the deterministic leg keeps the mixed leg's choice
             # the tile parameters for this mesh and precision, searched once
             # before the timed steps (kept for the same options)
             t0 = time.time()
@@ -222,6 +228,8 @@ def main():
         res["fp64_ms_per_step"] = round(out["fp64"]["ms"], 3)
     if "mixed_reuse" in out:
         res["mixed_plan_reused_ms_per_step"] = round(out["mixed_reuse"]["ms"], 3)
+    if "mixed_det" in out:
+        res["mixed_deterministic_ms_per_step"] = round(out["mixed_det"]["ms"], 3)
     if "fp64" in out and "mixed" in out:
         a, b = out["mixed"], out["fp64"]
         res["mixed_speedup_vs_fp64"] = round(b["ms"] / a["ms"], 3)
@@ -275,10 +283,7 @@ def main():
         seed[idx] = d_dxv.view(-1, 3)[idx]
         vs = {}
         for leg in out:
-            prec = leg.split("_")[0]
-            mesh.setOption("precision", prec)
-            mesh.setOption("reusePlan", leg.endswith("_reuse"))
-            mesh._setMeshOptions()
+            set_leg(leg)
             got = out[leg]["xv"].view(-1, 3)[idx].cpu().numpy()
             d_dxs = torch.empty(3 * npts, dtype=torch.float64, device=dev)
             _lib.check(L.idw_warp_mesh_dev(ctx, vp(d_xs), vp(out[leg]["xv"]), ctypes.c_void_p(stream.cuda_stream)))

@@ -46,6 +46,7 @@ class IdwOptions(ctypes.Structure):
         ("precision", ctypes.c_int32),
         ("reusePlan", ctypes.c_int32),
         ("nearWallFactor", ctypes.c_double),
+        ("deterministic", ctypes.c_int32),
     ]
 
 

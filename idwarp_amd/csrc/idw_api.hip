@@ -108,6 +108,13 @@ struct idw_ctx {
     bool tuneFixed = false;  // set by idw_set_tuning: batches keep these parameters
     double tuneKey[7] = {-1, -1, -1, -1, -1, -1, -1};
     double tuneMs[idw::TUNE_SLOTS];
+    double *detPart = nullptr;     // deterministic fast reverse: 12 sums per plan slot
+    int64_t detPartCap = 0;        // slots it holds
+    unsigned *detKeys = nullptr, *detKeys2 = nullptr;  // the plan transpose's sort: keys in and out,
+    int32_t *detVals = nullptr;                        // slots in,
+    unsigned char *detTmp = nullptr;                   // and the sort's temporary storage
+    int64_t detSortCap = 0;
+    size_t detTmpBytes = 0;
     hipEvent_t evOrder = nullptr;  // orders the context stream with a caller's stream (allocation-free entry point)
     double *scratch = nullptr;     // this device's copy of the allocation-free reverse's scratch array
     uint64_t gen = 0;        // bumped whenever device buffers, plans or options change (recorded graphs check it)
@@ -287,6 +294,8 @@ int restart_file(idw_ctx *c) {
 void drop_plan(idw_ctx *c) {
     for (idw::DevPlan &P : c->plans) {
         dfree(c, P.entries);
+        dfree(c, P.detPos);
+        dfree(c, P.recStart);
         dfree(c, P.waveOff);
         dfree(c, P.waveSplit);
         dfree(c, P.waveCount);
@@ -379,9 +388,11 @@ int size_one_plan(idw_ctx *c, int kk) {
     HIPCHK(hipStreamSynchronize(c->st));
     if (total > P.capacity) {
         dfree(c, P.entries);
+        dfree(c, P.detPos);
         TRY(dalloc(c, &P.entries, (size_t)total));
         P.capacity = total;
     }
+    P.transposed = false;
     P.total = total;
     P.errTol = c->opt.errTol;
     P.sized = true;
@@ -481,6 +492,7 @@ int run_warp(idw_ctx *c, const double *xs, double *xv) {
             c->mmPending = true;
         }
         pl.filled = true;
+        pl.transposed = false;
         pl.nearT2 = nt2;
     }
     if (c->mmPending && !c->capturing) HIPCHK(hipEventRecord(c->mmEvent, c->st));
@@ -520,6 +532,72 @@ int ensure_partial(idw_ctx *c) {
     return IDW_OK;
 }
 
+// Deterministic fast reverse: the plan's slots listed by term record (a
+// stable sort keeps each record's slots in plan order) and the per-slot
+// buffer, made when a filled plan is first reversed.
+int ensure_transposed(idw_ctx *c, idw::DevPlan &P) {
+    if (P.transposed) return IDW_OK;
+    if (c->capturing) return fail(IDW_ESTATE, "a recorded deterministic reverse needs an earlier one of the same plan");
+    if (P.total >= INT32_MAX) return fail(IDW_EINVAL, "deterministic reverse: %lld plan entries (fewer than 2^31)",
+                                          (long long)P.total);
+    const int nrec = c->S.nnodes + c->S.n;
+    if (!P.detPos) TRY(dalloc(c, &P.detPos, (size_t)std::max<int64_t>(P.capacity, 1)));
+    if (!P.recStart) TRY(dalloc(c, &P.recStart, (size_t)nrec + 1));
+    // the sort's buffers are kept: a plan refilled at every warp is sorted
+    // again at every reverse, without allocations after the first
+    if (c->detSortCap < P.total) {
+        dfree(c, c->detKeys);
+        dfree(c, c->detKeys2);
+        dfree(c, c->detVals);
+        c->detSortCap = 0;
+        TRY(dalloc(c, &c->detKeys, (size_t)P.total));
+        TRY(dalloc(c, &c->detKeys2, (size_t)P.total));
+        TRY(dalloc(c, &c->detVals, (size_t)P.total));
+        c->detSortCap = P.total;
+    }
+    int bits = 1;  // keys run to nrec (slots without a record)
+    while (bits < 32 && (1ll << bits) <= nrec) ++bits;
+    if (P.total > 0) {
+        size_t bytes = 0;
+        HIPCHK(idw::sort_pairs32(c->st, nullptr, bytes, P.total, c->detKeys, c->detVals, c->detKeys2, P.detPos, bits));
+        if (bytes > c->detTmpBytes) {
+            dfree(c, c->detTmp);
+            c->detTmpBytes = 0;
+            TRY(dalloc(c, &c->detTmp, bytes));
+            c->detTmpBytes = bytes;
+        }
+        idw::launch_plan_recs(c->st, P, nrec, c->detKeys, c->detVals);
+        HIPCHK(hipGetLastError());
+        bytes = c->detTmpBytes;
+        HIPCHK(idw::sort_pairs32(c->st, c->detTmp, bytes, P.total, c->detKeys, c->detVals, c->detKeys2, P.detPos,
+                                 bits));
+    }
+    idw::launch_rec_bounds(c->st, nrec, P.total, c->detKeys2, P.recStart);
+    HIPCHK(hipGetLastError());
+    P.transposed = true;
+    return IDW_OK;
+}
+
+int ensure_det_part(idw_ctx *c) {
+    int64_t need = 1;
+    for (const idw::DevPlan &P : c->plans) need = std::max(need, P.total);
+    if (c->detPart && c->detPartCap >= need) return IDW_OK;
+    if (c->capturing) return fail(IDW_ESTATE, "a recorded deterministic reverse needs an earlier one of the same plan");
+    dfree(c, c->detPart);
+    TRY(dalloc(c, &c->detPart, (size_t)need * 12));
+    c->detPartCap = need;
+    return IDW_OK;
+}
+
+bool det_ready(const idw_ctx *c) {
+    int64_t need = 1;
+    for (const idw::DevPlan &P : c->plans) {
+        if (!P.transposed) return false;
+        need = std::max(need, P.total);
+    }
+    return c->detPart && c->detPartCap >= need;
+}
+
 // The kernels of a reverse product, given the exact mode's chunk partials
 // (nchunk * n_unique * 12 doubles); enqueues work only.
 int deriv_kernels(idw_ctx *c, const double *xs, const double *dxv, double *dxs, double *partial, int nchunk) {
@@ -529,9 +607,15 @@ int deriv_kernels(idw_ctx *c, const double *xs, const double *dxv, double *dxs, 
     idw::launch_reverse_prep(c->st, c->S, fast || c->V.n == 0);
     HIPCHK(hipGetLastError());
     REC(c, 5);
-    for (int kk = 0; kk < nloop(c) && c->V.n; ++kk)
+    const bool det = fast && c->opt.deterministic && c->V.n;
+    if (det && !det_ready(c)) return fail(IDW_ESTATE, "the deterministic reverse's plan transpose is not built");
+    for (int kk = 0; kk < nloop(c) && c->V.n; ++kk) {
         idw::launch_reverse(c->st, c->S, c->V, make_params(c, false), mirror_of(c, kk), fast, c->opt.precision,
-                            partial, nchunk, dxv, fast ? &c->plans[kk] : nullptr, c->tune);
+                            partial, nchunk, dxv, fast ? &c->plans[kk] : nullptr, c->tune,
+                            det ? c->detPart : nullptr);
+        // per term, the slots of this pass in plan order (stored by the first pass, added by later ones)
+        if (det) idw::launch_det_reduce(c->st, c->S, c->plans[kk], c->detPart, kk > 0);
+    }
     HIPCHK(hipGetLastError());
     REC(c, 6);
     idw::launch_nodal_reverse(c->st, c->S, xs, fast, c->opt.useRotations, dxs, c->npts);
@@ -556,6 +640,10 @@ int run_deriv(idw_ctx *c, const double *xs, const double *dxv, double *dxs) {
     if (deriv_needs_warp(c)) TRY(run_warp(c, xs, c->xv));
     if (!fast) TRY(ensure_partial(c));
     TRY(ensure_mirror_buffers(c));
+    if (fast && c->opt.deterministic && c->V.n) {
+        for (idw::DevPlan &P : c->plans) TRY(ensure_transposed(c, P));
+        TRY(ensure_det_part(c));
+    }
     return deriv_kernels(c, xs, dxv, dxs, c->partial, c->nchunk);
 }
 
@@ -742,6 +830,7 @@ const char *invalid_opts(const idw_options *o) {
     if (!(o->evalMode == 0 || o->evalMode == 1)) return "evalMode must be exact (0) or fast (1)";
     if (!(o->precision == 0 || o->precision == 1)) return "precision must be fp64 (0) or mixed (1)";
     if (!(o->reusePlan == 0 || o->reusePlan == 1)) return "reusePlan must be 0 or 1";
+    if (!(o->deterministic == 0 || o->deterministic == 1)) return "deterministic must be 0 or 1";
     if (!std::isfinite(o->aExp) || !std::isfinite(o->bExp)) return "aExp and bExp must be finite";
     if (!(o->LdefFact > 0)) return "LdefFact must be positive";
     if (!(o->errTol >= 0)) return "errTol must be non-negative";
@@ -770,6 +859,11 @@ hipError_t sort_pairs(hipStream_t st, int64_t n, uint64_t *keys, int32_t *ids, u
 }  // namespace idw
 
 namespace idw {
+hipError_t sort_pairs32(hipStream_t st, void *tmp, size_t &bytes, int64_t n, unsigned *keys, int32_t *vals,
+                        unsigned *keys_out, int32_t *vals_out, int end_bit) {
+    return hipcub::DeviceRadixSort::SortPairs(tmp, bytes, keys, keys_out, vals, vals_out, (int)n, 0, end_bit, st);
+}
+
 hipError_t exclusive_scan(hipStream_t st, const int64_t *in, int64_t *out, int64_t n) {
     size_t bytes = 0;
     hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, (int)n, st);
@@ -1333,6 +1427,8 @@ int idw_warp_deriv_dev_noalloc(idw_ctx *c, const double *d_xs, const double *d_d
     if (deriv_needs_warp(c))
         return fail(IDW_ESTATE, "the allocation-free reverse needs a warp of this context at the current options first");
     if (nloop(c) > 1 && !c->V.acc) return fail(IDW_ESTATE, "the allocation-free reverse needs a warp with the planes first");
+    if (fast && c->opt.deterministic && c->V.n && !det_ready(c))
+        return fail(IDW_ESTATE, "the allocation-free deterministic reverse needs an allocating one of this plan first");
     TRY(ensure_device(c));
     hipStream_t user = static_cast<hipStream_t>(stream);
     HIPCHK(hipEventRecord(c->evOrder, user));
@@ -1664,9 +1760,10 @@ int idw_batch_run(idw_batch *b, void *hip_stream) {
         HIPCHK(hipEventRecord(b->in, user));
         // the autotuner picks each mesh's tile parameters for its own size and
         // options before anything is recorded (it keeps an earlier choice made
-        // for the same options and sizes)
+        // for the same options and sizes); not for a deterministic context,
+        // whose bits would then depend on a timing
         for (idw_ctx *c : cs)
-            if (!c->tuneFixed && !tune_current(c)) {
+            if (!c->tuneFixed && !c->opt.deterministic && !tune_current(c)) {
                 HIPCHK(hipStreamWaitEvent(c->st, b->in, 0));
                 TRY(run_tune(c));
             }

@@ -89,6 +89,11 @@ struct DevPlan {
     int64_t *waveSplit = nullptr;
     int64_t *waveCount = nullptr;
     unsigned int *mismatch = nullptr;
+    // deterministic reverse: the plan's slots sorted by record (detPos) and
+    // each record's first sorted position (recStart, nrec + 1)
+    int32_t *detPos = nullptr;
+    int64_t *recStart = nullptr;
+    bool transposed = false;  // detPos / recStart describe the current entries
     int64_t nwave = 0, total = 0, capacity = 0;
     double errTol = -1.0, nearT2 = -1.0;  // key of the filled plan
     double orderTol = -1.0, orderT2 = -1.0;  // key of the near-wall ordering of the volume nodes
@@ -151,15 +156,24 @@ void launch_plan_count(hipStream_t st, const DevSurface &S, const DevVolume &V, 
                        int64_t *waveCount);
 void launch_plan_fill(hipStream_t st, const DevSurface &S, const DevVolume &V, const Mirror &M, double errTol,
                       double nearT2, const DevPlan &plan, const Tuning &T, int64_t blk0 = 0, int64_t nblk = -1);
+// detPart (fast mode, deterministic): per-slot sums instead of atomics
 void launch_reverse(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, const Mirror &M,
                     int fast, int prec, double *partial, int nchunk, const double *dxv, const DevPlan *plan,
-                    const Tuning &T);
+                    const Tuning &T, double *detPart = nullptr);
+// the deterministic reverse's transpose of a plan and its reduction into the adjoint records
+void launch_plan_recs(hipStream_t st, const DevPlan &plan, int nrec, unsigned *keys, int32_t *vals);
+void launch_rec_bounds(hipStream_t st, int nrec, int64_t total, const unsigned *keys, int64_t *recStart);
+void launch_det_reduce(hipStream_t st, const DevSurface &S, const DevPlan &plan, const double *part, int add);
 hipError_t exclusive_scan(hipStream_t st, const int64_t *in, int64_t *out, int64_t n);
 void launch_reverse_prep(hipStream_t st, const DevSurface &S, int fast);
 void launch_fast_count(hipStream_t st, const DevSurface &S, const DevVolume &V, const Mirror &M, double errTol,
                        unsigned long long *out);
 void launch_nodal_reverse(hipStream_t st, const DevSurface &S, const double *xs, int fast, int useRot, double *dxs,
                           int64_t npts);
+// Stable sort of 32-bit keys (bits [0, end_bit)) with int32 values into keys_out / vals_out; with
+// tmp null, sets bytes to the temporary storage it needs and sorts nothing.
+hipError_t sort_pairs32(hipStream_t st, void *tmp, size_t &bytes, int64_t n, unsigned *keys, int32_t *vals,
+                        unsigned *keys_out, int32_t *vals_out, int end_bit);
 // Sort keys/ids pairs (Morton order); temp storage allocated inside.
 hipError_t sort_pairs(hipStream_t st, int64_t n, uint64_t *keys, int32_t *ids, uint64_t *keys_out, int32_t *ids_out);
 

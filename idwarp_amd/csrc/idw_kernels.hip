@@ -1375,12 +1375,14 @@ struct RevLds {
 // node 4q + l/16 for node group q, in FP64.  The next tile's record and the
 // entries of the tile after it are requested before this tile's sums and are
 // in registers before its atomics are issued.
+// With part (deterministic reverse) the 12 sums of the term at plan slot e
+// are written to part[12 e ..] instead of being added to its record.
 template <int EXPM>
 __device__ __forceinline__ void rev_terms(int64_t a, int64_t b, const PlanEntry *__restrict__ plan, int nnodes,
                                           int nrec, const Rec64 *__restrict__ rec, const WParams &P,
                                           const double4 *nodes, double (*Y)[4], int32_t *idxs,
                                           const double (&B)[16], double ox, double oy, double oz,
-                                          double *__restrict__ adj) {
+                                          double *__restrict__ adj, double *__restrict__ part = nullptr) {
     const int lane = threadIdx.x & 63, m = lane & 15, R = lane >> 4;
     const int src = (lane & ~15) | (9 + (m < 9 ? m / 3 : 0));
     if (a >= b) return;
@@ -1426,7 +1428,11 @@ __device__ __forceinline__ void rev_terms(int64_t a, int64_t b, const PlanEntry 
             const double pb = __shfl(val, src, 64);
             if (m < 9) val -= pb * Y[tr][m % 3];
             const int id = idxs[tr];
-            if (id >= 0 && m < 12 && val != 0.0) atomicAdd(adj + (int64_t)id * ADJ_STRIDE + m, val);
+            if (part) {
+                if (id >= 0 && m < 12) part[(t0 + tr) * 12 + m] = val;
+            } else if (id >= 0 && m < 12 && val != 0.0) {
+                atomicAdd(adj + (int64_t)id * ADJ_STRIDE + m, val);
+            }
         }
         pe = pn;
         xa = xn;
@@ -1449,8 +1455,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
                                                   const int64_t *__restrict__ waveOff,
                                                   const int64_t *__restrict__ waveSplit, int nnodes, int nrec,
                                                   const Rec64 *__restrict__ rec, WParams P, Mirror M,
-                                                  double *__restrict__ adj) {
-    static_assert(PREC == IDW_PREC_FP64, "mixed precision reverses through k_fast_rev_pair");
+                                                  double *__restrict__ adj, double *__restrict__ part) {
+    static_assert(PREC == IDW_PREC_FP64, "mixed precision reverses through k_fast_rev_pair (or here, deterministic)");
     __shared__ RevLds lds[4];
     const int nb = (nvol + 255) / 256;
     const int blk = xcd_block(nb);
@@ -1492,7 +1498,66 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         B[q] = n < 9 ? sv * gv : (n < 12 ? sv : 0.0);
     }
     const int64_t e0 = waveOff[wave], e1 = waveOff[wave + 1];
-    rev_terms<EXPM>(e0, e1, plan, nnodes, nrec, rec, P, L.node, L.Y, L.idx, B, ox, oy, oz, adj);
+    rev_terms<EXPM>(e0, e1, plan, nnodes, nrec, rec, P, L.node, L.Y, L.idx, B, ox, oy, oz, adj, part);
+}
+
+// ---------------------------------------------------------------------------
+// Deterministic fast reverse: per-slot sums reduced per record in plan order
+// ---------------------------------------------------------------------------
+// The record of every plan slot and the slot itself, the keys and values of
+// the stable sort that lists each record's slots in increasing order.  A slot
+// without a record (never written by the reverse) takes key nrec, past every
+// record's range.
+__global__ void k_plan_recs(int64_t total, const PlanEntry *__restrict__ plan, unsigned *__restrict__ keys,
+                            int32_t *__restrict__ vals, int nrec) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= total) return;
+    const int r = plan[e].rec;
+    keys[e] = (unsigned)(r < 0 ? nrec : min(r, nrec - 1));
+    vals[e] = (int32_t)e;
+}
+
+// recStart[r] = first position of record r in the sorted keys (r <= nrec).
+__global__ void k_rec_bounds(int nrec, int64_t total, const unsigned *__restrict__ keys, int64_t *__restrict__ recStart) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r > nrec) return;
+    int64_t lo = 0, hi = total;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (keys[mid] < (unsigned)r) lo = mid + 1;
+        else hi = mid;
+    }
+    recStart[r] = lo;
+}
+
+// One wave per record: lane l sums the record's slots l, l + 64, ... in
+// order, then a fixed shuffle tree adds the lanes; the record's 12 sums are
+// stored (the first mirror pass) or added (later passes).  Every step has a
+// fixed order, so the result is the same on every run.
+__global__ void __launch_bounds__(256) k_det_reduce(int nrec, const int64_t *__restrict__ recStart,
+                                                    const int32_t *__restrict__ pos, const double *__restrict__ part,
+                                                    double *__restrict__ adj, int add) {
+    const int lane = threadIdx.x & 63;
+    const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= nrec) return;  // wave-uniform
+    const int64_t s = recStart[r], e = recStart[r + 1];
+    double acc[12];
+#pragma unroll
+    for (int m = 0; m < 12; ++m) acc[m] = 0.0;
+    for (int64_t i = s + lane; i < e; i += 64) {
+        const double *p = part + (int64_t)pos[i] * 12;
+#pragma unroll
+        for (int m = 0; m < 12; ++m) acc[m] += p[m];
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int m = 0; m < 12; ++m) acc[m] += __shfl_down(acc[m], off, 64);
+    if (lane == 0) {
+        double *o = adj + (int64_t)r * ADJ_STRIDE;
+#pragma unroll
+        for (int m = 0; m < 12; ++m) o[m] = add ? o[m] + acc[m] : acc[m];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -2406,7 +2471,7 @@ static void rev_pair_launch(hipStream_t st, const DevSurface &S, const DevVolume
 template <int EXPM>
 static void reverse_impl(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, const Mirror &M,
                          int fast, int prec, double *partial, int nchunk, const double *dxv, const DevPlan *plan,
-                         const Tuning &T) {
+                         const Tuning &T, double *detPart) {
     if (!fast) {
         launch_prep_numb(st, V, M, dxv);
         const double4 *pos = M.mask ? V.volm : V.vol;
@@ -2421,10 +2486,11 @@ static void reverse_impl(hipStream_t st, const DevSurface &S, const DevVolume &V
                                                                               M.mask != 0);
     } else {
         const unsigned g = xcd_grid(V.n);
-        if (prec == IDW_PREC_FP64)
+        // the deterministic reverse takes the per-wave path with FP64 weights in either precision
+        if (prec == IDW_PREC_FP64 || detPart)
             k_fast_rev<IDW_PREC_FP64, EXPM><<<g, 256, 0, st>>>(
                 (int)V.n, V.vol, V.perm, dxv, V.den, plan->entries, plan->waveOff, plan->waveSplit, S.nnodes,
-                S.nnodes + S.n, S.rec, P, M, S.adj);
+                S.nnodes + S.n, S.rec, P, M, S.adj, detPart);
         else if (T.mq == 8)
             rev_pair_launch<EXPM, 8>(st, S, V, P, M, dxv, plan);
         else if (T.mq == 16)
@@ -2451,9 +2517,22 @@ void launch_forward(hipStream_t st, const DevSurface &S, const DevVolume &V, con
 
 void launch_reverse(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, const Mirror &M,
                     int fast, int prec, double *partial, int nchunk, const double *dxv, const DevPlan *plan,
-                    const Tuning &T) {
-    if (P.expMode == EXP_35) reverse_impl<EXP_35>(st, S, V, P, M, fast, prec, partial, nchunk, dxv, plan, T);
-    else reverse_impl<EXP_INT>(st, S, V, P, M, fast, prec, partial, nchunk, dxv, plan, T);
+                    const Tuning &T, double *detPart) {
+    if (P.expMode == EXP_35) reverse_impl<EXP_35>(st, S, V, P, M, fast, prec, partial, nchunk, dxv, plan, T, detPart);
+    else reverse_impl<EXP_INT>(st, S, V, P, M, fast, prec, partial, nchunk, dxv, plan, T, detPart);
+}
+
+void launch_plan_recs(hipStream_t st, const DevPlan &plan, int nrec, unsigned *keys, int32_t *vals) {
+    if (plan.total > 0) k_plan_recs<<<grid_for(plan.total, 256), 256, 0, st>>>(plan.total, plan.entries, keys, vals, nrec);
+}
+
+void launch_rec_bounds(hipStream_t st, int nrec, int64_t total, const unsigned *keys, int64_t *recStart) {
+    k_rec_bounds<<<grid_for((int64_t)nrec + 1, 256), 256, 0, st>>>(nrec, total, keys, recStart);
+}
+
+void launch_det_reduce(hipStream_t st, const DevSurface &S, const DevPlan &plan, const double *part, int add) {
+    const int nrec = S.nnodes + S.n;
+    k_det_reduce<<<grid_for(nrec, 4), 256, 0, st>>>(nrec, plan.recStart, plan.detPos, part, S.adj, add);
 }
 
 void launch_nodal_reverse(hipStream_t st, const DevSurface &S, const double *xs, int fast, int useRot, double *dxs,

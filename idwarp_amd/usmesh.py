@@ -46,6 +46,7 @@ _DEFAULTS = {
     "nearWallFactor": [float, 1e-6],
     "reusePlan": [bool, False],
     "autotune": [bool, False],
+    "deterministic": [bool, False],
     "device": [int, 0],
 }
 
@@ -142,6 +143,7 @@ class USMesh:
         o.precision = _lib.PREC_MIXED if g("precision") == "mixed" else _lib.PREC_FP64
         o.nearWallFactor = g("nearWallFactor")
         o.reusePlan = int(bool(g("reusePlan")))
+        o.deterministic = int(bool(g("deterministic")))
         return o
 
     def _setMeshOptions(self):
@@ -264,7 +266,9 @@ class USMesh:
         """Warp the volume to the surface set by setSurfaceCoordinates()."""
         self._setInternalSurface()
         self._setMeshOptions()
-        if self.getOption("autotune"):
+        # a search picks parameters by timing, and the sums' order follows
+        # them: a deterministic mesh keeps its parameters unless tune() is called
+        if self.getOption("autotune") and not self.getOption("deterministic"):
             _lib.check(self._L.idw_tune(self._ctx, 0))
         _lib.check(self._L.idw_warp_mesh(self._ctx))
 

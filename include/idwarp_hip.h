@@ -64,6 +64,13 @@ typedef struct idw_options {
                                q^aExp and alpha^bExp q^bExp below 1e32 for q = 1/nearWallFactor
                                (9.95e-8 for the defaults), and nearWallFactor*Ldef >= 1e-12;
                                smaller values are rejected with IDW_EINVAL */
+    int32_t deterministic;  /* fast mode: 1 = the reverse product is bit-for-bit the same on
+                               every run (the reference sums serially, kd_tree.F90:942-1051):
+                               each plan entry's sums go to a slot of their own and are added
+                               per term in plan order, with FP64 weights in either precision;
+                               0 (default) = FP64 atomics, equal to rounding only.  The bits
+                               hold for fixed tile parameters (the forward's chunking orders
+                               its sums), so batches do not search them for such a context */
 } idw_options;
 
 typedef struct idw_ctx idw_ctx;
@@ -264,6 +271,10 @@ int idw_get_tuning(idw_ctx *ctx, int32_t *params5, double *ms, int64_t nms);
  * a search; the context counts as untuned afterwards, and batches keep these
  * parameters instead of searching (until idw_tune is called). */
 int idw_set_tuning(idw_ctx *ctx, const int32_t *params5);
+/* The deterministic reverse also needs memory: per plan entry 96 bytes of
+ * sums, 4 of transpose and 12 for its sort (with the sort's temporary
+ * storage), and 8 bytes per term record; the first reverse after a plan
+ * fill sorts the plan's entries. */
 
 /* Batches (no counterpart in the reference, which warps one surface per
  * call): many warps and reverse products over one or more contexts on one

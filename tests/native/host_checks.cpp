@@ -146,6 +146,11 @@ int api() {
     bad.nearWallFactor = 1e-6;
     expect(idw_create(&c, 0, &bad) == IDW_ENODEV, "mixed precision, nearWallFactor 1e-6 (valid options)");
     bad = o;
+    bad.deterministic = 2;
+    expect(idw_create(&c, 0, &bad) == IDW_EINVAL, "deterministic 2");
+    bad.deterministic = 1;
+    expect(idw_create(&c, 0, &bad) == IDW_ENODEV, "deterministic 1 (valid options)");
+    bad = o;
     bad.errTol = NAN;
     expect(idw_create(&c, 0, &bad) == IDW_EINVAL, "NaN errTol");
     expect(idw_destroy(nullptr) == IDW_OK, "destroy null");
