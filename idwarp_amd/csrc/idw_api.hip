@@ -115,6 +115,10 @@ struct idw_ctx {
     unsigned char *detTmp = nullptr;                   // and the sort's temporary storage
     int64_t detSortCap = 0;
     size_t detTmpBytes = 0;
+    int64_t *detCount = nullptr;      // chunks per record (nrec + 1, the last 0) before their scan
+    int64_t detCountCap = 0;
+    double *detChunkPart = nullptr;   // 12 partial sums per chunk
+    int64_t detChunkCap = 0;
     hipEvent_t evOrder = nullptr;  // orders the context stream with a caller's stream (allocation-free entry point)
     double *scratch = nullptr;     // this device's copy of the allocation-free reverse's scratch array
     uint64_t gen = 0;        // bumped whenever device buffers, plans or options change (recorded graphs check it)
@@ -296,6 +300,9 @@ void drop_plan(idw_ctx *c) {
         dfree(c, P.entries);
         dfree(c, P.detPos);
         dfree(c, P.recStart);
+        dfree(c, P.chunkOff);
+        dfree(c, P.chunkRec);
+        P.chunkCap = P.nchunk = 0;
         dfree(c, P.waveOff);
         dfree(c, P.waveSplit);
         dfree(c, P.waveCount);
@@ -574,28 +581,64 @@ int ensure_transposed(idw_ctx *c, idw::DevPlan &P) {
     }
     idw::launch_rec_bounds(c->st, nrec, P.total, c->detKeys2, P.recStart);
     HIPCHK(hipGetLastError());
+    // each record's range cut into chunks, and the record of every chunk
+    if (!P.chunkOff) TRY(dalloc(c, &P.chunkOff, (size_t)nrec + 1));
+    if (c->detCountCap < nrec + 1) {
+        dfree(c, c->detCount);
+        c->detCountCap = 0;
+        TRY(dalloc(c, &c->detCount, (size_t)nrec + 1));
+        c->detCountCap = nrec + 1;
+    }
+    HIPCHK(hipMemsetAsync(c->detCount + nrec, 0, sizeof(int64_t), c->st));
+    idw::launch_rec_chunks(c->st, nrec, P.recStart, c->detCount);
+    HIPCHK(hipGetLastError());
+    HIPCHK(idw::exclusive_scan(c->st, c->detCount, P.chunkOff, nrec + 1));
+    int64_t nch = 0;
+    HIPCHK(hipMemcpy(&nch, P.chunkOff + nrec, sizeof(int64_t), hipMemcpyDeviceToHost));
+    if (nch > P.chunkCap) {
+        dfree(c, P.chunkRec);
+        P.chunkCap = 0;
+        TRY(dalloc(c, &P.chunkRec, (size_t)nch));
+        P.chunkCap = nch;
+    }
+    P.nchunk = nch;
+    idw::launch_chunk_recs(c->st, nrec, P.chunkOff, P.chunkRec);
+    HIPCHK(hipGetLastError());
     P.transposed = true;
     return IDW_OK;
 }
 
 int ensure_det_part(idw_ctx *c) {
-    int64_t need = 1;
-    for (const idw::DevPlan &P : c->plans) need = std::max(need, P.total);
-    if (c->detPart && c->detPartCap >= need) return IDW_OK;
-    if (c->capturing) return fail(IDW_ESTATE, "a recorded deterministic reverse needs an earlier one of the same plan");
-    dfree(c, c->detPart);
-    TRY(dalloc(c, &c->detPart, (size_t)need * 12));
-    c->detPartCap = need;
+    int64_t need = 1, nch = 1;
+    for (const idw::DevPlan &P : c->plans) {
+        need = std::max(need, P.total);
+        nch = std::max(nch, P.nchunk);
+    }
+    if (c->capturing && !(c->detPart && c->detPartCap >= need && c->detChunkPart && c->detChunkCap >= nch))
+        return fail(IDW_ESTATE, "a recorded deterministic reverse needs an earlier one of the same plan");
+    if (!(c->detPart && c->detPartCap >= need)) {
+        dfree(c, c->detPart);
+        c->detPartCap = 0;
+        TRY(dalloc(c, &c->detPart, (size_t)need * 12));
+        c->detPartCap = need;
+    }
+    if (!(c->detChunkPart && c->detChunkCap >= nch)) {
+        dfree(c, c->detChunkPart);
+        c->detChunkCap = 0;
+        TRY(dalloc(c, &c->detChunkPart, (size_t)nch * 12));
+        c->detChunkCap = nch;
+    }
     return IDW_OK;
 }
 
 bool det_ready(const idw_ctx *c) {
-    int64_t need = 1;
+    int64_t need = 1, nch = 1;
     for (const idw::DevPlan &P : c->plans) {
         if (!P.transposed) return false;
         need = std::max(need, P.total);
+        nch = std::max(nch, P.nchunk);
     }
-    return c->detPart && c->detPartCap >= need;
+    return c->detPart && c->detPartCap >= need && c->detChunkPart && c->detChunkCap >= nch;
 }
 
 // The kernels of a reverse product, given the exact mode's chunk partials
@@ -614,7 +657,7 @@ int deriv_kernels(idw_ctx *c, const double *xs, const double *dxv, double *dxs, 
                             partial, nchunk, dxv, fast ? &c->plans[kk] : nullptr, c->tune,
                             det ? c->detPart : nullptr);
         // per term, the slots of this pass in plan order (stored by the first pass, added by later ones)
-        if (det) idw::launch_det_reduce(c->st, c->S, c->plans[kk], c->detPart, kk > 0);
+        if (det) idw::launch_det_reduce(c->st, c->S, c->plans[kk], c->detPart, c->detChunkPart, kk > 0);
     }
     HIPCHK(hipGetLastError());
     REC(c, 6);

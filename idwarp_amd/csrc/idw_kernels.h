@@ -93,7 +93,12 @@ struct DevPlan {
     // each record's first sorted position (recStart, nrec + 1)
     int32_t *detPos = nullptr;
     int64_t *recStart = nullptr;
-    bool transposed = false;  // detPos / recStart describe the current entries
+    // each record's sorted range cut into chunks of DET_CHUNK slots: chunk
+    // offsets per record (nrec + 1) and the record of every chunk
+    int64_t *chunkOff = nullptr;
+    int32_t *chunkRec = nullptr;
+    int64_t nchunk = 0, chunkCap = 0;
+    bool transposed = false;  // detPos / recStart / chunks describe the current entries
     int64_t nwave = 0, total = 0, capacity = 0;
     double errTol = -1.0, nearT2 = -1.0;  // key of the filled plan
     double orderTol = -1.0, orderT2 = -1.0;  // key of the near-wall ordering of the volume nodes
@@ -163,7 +168,11 @@ void launch_reverse(hipStream_t st, const DevSurface &S, const DevVolume &V, con
 // the deterministic reverse's transpose of a plan and its reduction into the adjoint records
 void launch_plan_recs(hipStream_t st, const DevPlan &plan, int nrec, unsigned *keys, int32_t *vals);
 void launch_rec_bounds(hipStream_t st, int nrec, int64_t total, const unsigned *keys, int64_t *recStart);
-void launch_det_reduce(hipStream_t st, const DevSurface &S, const DevPlan &plan, const double *part, int add);
+constexpr int DET_CHUNK = 512;  // slots of a record summed by one wave
+void launch_rec_chunks(hipStream_t st, int nrec, const int64_t *recStart, int64_t *count);
+void launch_chunk_recs(hipStream_t st, int nrec, const int64_t *chunkOff, int32_t *chunkRec);
+void launch_det_reduce(hipStream_t st, const DevSurface &S, const DevPlan &plan, const double *part,
+                       double *chunkPart, int add);
 hipError_t exclusive_scan(hipStream_t st, const int64_t *in, int64_t *out, int64_t n);
 void launch_reverse_prep(hipStream_t st, const DevSurface &S, int fast);
 void launch_fast_count(hipStream_t st, const DevSurface &S, const DevVolume &V, const Mirror &M, double errTol,

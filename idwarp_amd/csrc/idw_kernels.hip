@@ -1530,34 +1530,67 @@ __global__ void k_rec_bounds(int nrec, int64_t total, const unsigned *__restrict
     recStart[r] = lo;
 }
 
-// One wave per record: lane l sums the record's slots l, l + 64, ... in
-// order, then a fixed shuffle tree adds the lanes; the record's 12 sums are
-// stored (the first mirror pass) or added (later passes).  Every step has a
-// fixed order, so the result is the same on every run.
-__global__ void __launch_bounds__(256) k_det_reduce(int nrec, const int64_t *__restrict__ recStart,
+// Chunks of the records' sorted slot ranges: record r has
+// max(1, ceil(n_r / DET_CHUNK)) of them, so that a cluster accepted by most
+// waves is summed by many waves instead of one.
+__global__ void k_rec_chunks(int nrec, const int64_t *__restrict__ recStart, int64_t *__restrict__ count) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrec) return;
+    const int64_t n = recStart[r + 1] - recStart[r];
+    count[r] = n > DET_CHUNK ? (n + DET_CHUNK - 1) / DET_CHUNK : 1;
+}
+
+__global__ void k_chunk_recs(int nrec, const int64_t *__restrict__ chunkOff, int32_t *__restrict__ chunkRec) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrec) return;
+    for (int64_t c = chunkOff[r]; c < chunkOff[r + 1]; ++c) chunkRec[c] = r;
+}
+
+// One wave per chunk: lane group g (12 lanes, 5 groups) reads slots
+// g, g + 5, ... of the chunk, each slot's 12 sums as one 96-byte access, and
+// lane m of a group adds sum m; the groups are then added in order 0..4.
+__global__ void __launch_bounds__(256) k_det_chunks(int64_t nchunk, const int32_t *__restrict__ chunkRec,
+                                                    const int64_t *__restrict__ chunkOff,
+                                                    const int64_t *__restrict__ recStart,
                                                     const int32_t *__restrict__ pos, const double *__restrict__ part,
-                                                    double *__restrict__ adj, int add) {
-    const int lane = threadIdx.x & 63;
-    const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (r >= nrec) return;  // wave-uniform
-    const int64_t s = recStart[r], e = recStart[r + 1];
-    double acc[12];
-#pragma unroll
-    for (int m = 0; m < 12; ++m) acc[m] = 0.0;
-    for (int64_t i = s + lane; i < e; i += 64) {
-        const double *p = part + (int64_t)pos[i] * 12;
-#pragma unroll
-        for (int m = 0; m < 12; ++m) acc[m] += p[m];
+                                                    double *__restrict__ chunkPart) {
+    const int lane = threadIdx.x & 63, g = lane / 12, m = lane - 12 * g;
+    const int64_t c = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (c >= nchunk) return;  // wave-uniform
+    const int r = chunkRec[c];
+    const int64_t s = recStart[r] + (c - chunkOff[r]) * DET_CHUNK;
+    const int64_t e = min(recStart[r + 1], s + DET_CHUNK);
+    double acc = 0.0;
+    if (g < 5) {
+        int64_t i = s + g;
+        for (; i + 15 < e; i += 20) {  // four slots of the group in flight
+            const double a0 = part[(int64_t)pos[i] * 12 + m], a1 = part[(int64_t)pos[i + 5] * 12 + m];
+            const double a2 = part[(int64_t)pos[i + 10] * 12 + m], a3 = part[(int64_t)pos[i + 15] * 12 + m];
+            acc += a0;
+            acc += a1;
+            acc += a2;
+            acc += a3;
+        }
+        for (; i < e; i += 5) acc += part[(int64_t)pos[i] * 12 + m];
     }
+    double tot = acc;
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1)
-#pragma unroll
-        for (int m = 0; m < 12; ++m) acc[m] += __shfl_down(acc[m], off, 64);
-    if (lane == 0) {
-        double *o = adj + (int64_t)r * ADJ_STRIDE;
-#pragma unroll
-        for (int m = 0; m < 12; ++m) o[m] = add ? o[m] + acc[m] : acc[m];
-    }
+    for (int k = 1; k < 5; ++k) tot += __shfl(acc, 12 * k + (lane < 12 ? lane : 0), 64);
+    if (lane < 12) chunkPart[c * 12 + lane] = tot;
+}
+
+// Per record and sum: its chunks' partials added in chunk order, stored (the
+// first mirror pass) or added (later passes) into the adjoint record.  Every
+// step has a fixed order, so the result is the same on every run.
+__global__ void k_det_finish(int nrec, const int64_t *__restrict__ chunkOff, const double *__restrict__ chunkPart,
+                             double *__restrict__ adj, int add) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nrec * 12) return;
+    const int r = (int)(t / 12), m = (int)(t - 12 * (int64_t)r);
+    double acc = 0.0;
+    for (int64_t c = chunkOff[r]; c < chunkOff[r + 1]; ++c) acc += chunkPart[c * 12 + m];
+    double &o = adj[(int64_t)r * ADJ_STRIDE + m];
+    o = add ? o + acc : acc;
 }
 
 // ---------------------------------------------------------------------------
@@ -2530,9 +2563,21 @@ void launch_rec_bounds(hipStream_t st, int nrec, int64_t total, const unsigned *
     k_rec_bounds<<<grid_for((int64_t)nrec + 1, 256), 256, 0, st>>>(nrec, total, keys, recStart);
 }
 
-void launch_det_reduce(hipStream_t st, const DevSurface &S, const DevPlan &plan, const double *part, int add) {
+void launch_rec_chunks(hipStream_t st, int nrec, const int64_t *recStart, int64_t *count) {
+    k_rec_chunks<<<grid_for(nrec, 256), 256, 0, st>>>(nrec, recStart, count);
+}
+
+void launch_chunk_recs(hipStream_t st, int nrec, const int64_t *chunkOff, int32_t *chunkRec) {
+    k_chunk_recs<<<grid_for(nrec, 256), 256, 0, st>>>(nrec, chunkOff, chunkRec);
+}
+
+void launch_det_reduce(hipStream_t st, const DevSurface &S, const DevPlan &plan, const double *part,
+                       double *chunkPart, int add) {
     const int nrec = S.nnodes + S.n;
-    k_det_reduce<<<grid_for(nrec, 4), 256, 0, st>>>(nrec, plan.recStart, plan.detPos, part, S.adj, add);
+    if (plan.nchunk > 0)
+        k_det_chunks<<<grid_for(plan.nchunk, 4), 256, 0, st>>>(plan.nchunk, plan.chunkRec, plan.chunkOff,
+                                                              plan.recStart, plan.detPos, part, chunkPart);
+    k_det_finish<<<grid_for((int64_t)nrec * 12, 256), 256, 0, st>>>(nrec, plan.chunkOff, chunkPart, S.adj, add);
 }
 
 void launch_nodal_reverse(hipStream_t st, const DevSurface &S, const double *xs, int fast, int useRot, double *dxs,
