@@ -411,6 +411,7 @@ def ellipsoid_exact(args):
     res["adjoint_fd"] = {"directions": args.fd_directions, "h": 1e-6, "max_rel_diff": worst,
                          "seed": "dXv ~ N(0,1) on a random 10% of the volume nodes (seed 5)"}
     log("FD check: %s" % res["adjoint_fd"])
+    res["stored_weights"] = stored_weights(args)
     if not args.no_cpu:
         # the CPU baseline leg: the oracle on a sample of volume nodes, which
         # also checks the FP64 forward on it and the reverse for a seed carried
@@ -447,6 +448,58 @@ def ellipsoid_exact(args):
                                          "x%.0f" % (nvol, lw + ld, nvol / len(idx))}
         log("cpu baseline %.1fs" % (time.time() - t0))
     print(json.dumps(res), flush=True)
+
+
+def stored_weights(args):
+    """Recompute against stored weights (option storeWeights) for the
+    reverse product, in exact mode on the CPU-sized sphere O-mesh 49x33x33
+    (the 3.2M-node ellipsoid would need 836 GB of weights): device memory of
+    the context, the reverse kernels' time (HIP events) and the whole
+    warpDeriv, FP64, dXv ~ N(0,1) on a random 10% of the nodes (seed 5)."""
+    import torch
+    from idwarp_amd import USMesh, _lib, synthetic
+
+    m = synthetic.sphere_omesh(49, 33, 33)
+    xs = m["perturb"](m["pts"])
+    nvol, npts = len(m["xv0"]), len(m["pts"])
+    dev = torch.device("cuda:0")
+    d_xs = torch.tensor(xs.ravel(), dtype=torch.float64, device=dev)
+    d_dxv = torch.tensor(synthetic.random_dxv(nvol, seed=5, frac=0.1), dtype=torch.float64, device=dev)
+    sp = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+    out = {}
+    for mode in ("recompute", "stored"):
+        g = USMesh(options=dict(gridFile=m["xv0"], fileType="memory", evalMode="exact", precision="fp64",
+                                storeWeights=mode == "stored"))
+        g.setSurfaceDefinition(m["pts"], m["conn"], m["faceSizes"])
+        g._setMeshOptions()
+        d_xv = torch.empty(3 * nvol, dtype=torch.float64, device=dev)
+        d_dxs = torch.empty(3 * npts, dtype=torch.float64, device=dev)
+        _lib.check(g._L.idw_warp_mesh_dev(g.ctx, ctypes.c_void_p(d_xs.data_ptr()), ctypes.c_void_p(d_xv.data_ptr()),
+                                          sp))
+        reps = max(args.steps, 1) * 4
+        ker = 0.0
+        for r in range(reps + 2):
+            if r == 2:
+                torch.cuda.synchronize()
+                t1 = time.perf_counter()
+            _lib.check(g._L.idw_warp_deriv_dev(g.ctx, ctypes.c_void_p(d_xs.data_ptr()),
+                                               ctypes.c_void_p(d_dxv.data_ptr()), ctypes.c_void_p(d_dxs.data_ptr()),
+                                               sp))
+            if r >= 2:
+                ker += g.getTimings()[4]
+        torch.cuda.synchronize()
+        out[mode] = {"device_bytes": g.getDeviceMemory(),
+                     "warpDeriv_ms": round((time.perf_counter() - t1) * 1e3 / reps, 4),
+                     "reverse_kernels_ms": round(ker / reps, 4), "dxs": d_dxs.cpu().numpy()}
+    a, b = out["stored"], out["recompute"]
+    rec = {"workload": "sphere O-mesh 49x33x33: %d volume nodes, %d unique surface nodes, exact mode, FP64" %
+                       (nvol, _unique(g)),
+           "weights_bytes": a["device_bytes"] - b["device_bytes"],
+           "adjoint_rel_diff": _rel(a["dxs"], b["dxs"])}
+    for k in ("recompute", "stored"):
+        rec[k] = {kk: vv for kk, vv in out[k].items() if kk != "dxs"}
+    log("stored weights: %s" % rec)
+    return rec
 
 
 def batched(args):

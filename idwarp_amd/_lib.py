@@ -25,7 +25,7 @@ EXPORTS = (
     "idw_get_denominator0", "idw_get_timings", "idw_fast_counts", "idw_warp_deriv_fwd", "idw_warp_deriv_fwd_dev",
     "idw_set_symmetry_planes", "idw_batch_create", "idw_batch_destroy", "idw_batch_add_warp",
     "idw_batch_add_warp_deriv", "idw_batch_run", "idw_batch_info", "idw_set_restart_file",
-    "idw_get_restart_info", "idw_mixed_near_wall_floor", "idw_batch_stats",
+    "idw_get_restart_info", "idw_mixed_near_wall_floor", "idw_batch_stats", "idw_device_bytes",
     "idw_warp_deriv_dev_noalloc", "idw_warp_mesh_async", "idw_warp_deriv_async", "idw_wait",
     "idw_tune", "idw_get_tuning", "idw_set_tuning",
 )
@@ -47,6 +47,7 @@ class IdwOptions(ctypes.Structure):
         ("reusePlan", ctypes.c_int32),
         ("nearWallFactor", ctypes.c_double),
         ("deterministic", ctypes.c_int32),
+        ("storeWeights", ctypes.c_int32),
     ]
 
 
@@ -118,6 +119,7 @@ def load():
         "idw_get_restart_info": (i32, [vp, P(i32), vp, i64]),
         "idw_mixed_near_wall_floor": (d, [d, d, d]),
         "idw_batch_stats": (i32, [vp, P(i64), P(i64)]),
+        "idw_device_bytes": (i32, [vp, P(i64)]),
         "idw_warp_deriv_dev_noalloc": (i32, [vp, vp, vp, vp, vp]),
         "idw_warp_mesh_async": (i32, [vp]),
         "idw_warp_deriv_async": (i32, [vp, vp, i64]),

@@ -57,8 +57,9 @@ int64_t first_non_finite(const double *a, int64_t n) {
 }
 
 bool same_numeric_options(const idw_options &a, const idw_options &b) {
-    return a.LdefFact == b.LdefFact && a.alpha == b.alpha && a.bExp == b.bExp && a.errTol == b.errTol &&
-           a.evalMode == b.evalMode && a.precision == b.precision && a.nearWallFactor == b.nearWallFactor;
+    return a.LdefFact == b.LdefFact && a.alpha == b.alpha && a.aExp == b.aExp && a.bExp == b.bExp &&
+           a.errTol == b.errTol && a.evalMode == b.evalMode && a.precision == b.precision &&
+           a.nearWallFactor == b.nearWallFactor;
 }
 
 double plan_near_t2(const idw_ctx *c);
@@ -87,6 +88,11 @@ struct idw_ctx {
     int nchunk = 0;
     double bboxLo[3] = {0, 0, 0}, bboxScale[3] = {0, 0, 0};  // Morton quantisation of the volume box
     std::vector<void *> allocs;
+    std::vector<size_t> allocBytes;  // the size of each of allocs
+    int64_t devBytes = 0;            // their sum
+    double *W = nullptr;   // exact mode, storeWeights: every pair's weight, [pass][surface point][sorted volume node]
+    int64_t wCount = 0;    // weights it holds
+    bool wValid = false;   // written by the last warp at the current options
     std::vector<int32_t> solverInd;
     hipEvent_t ev[8] = {};
     double ms[6] = {0, 0, 0, 0, 0, 0};
@@ -139,8 +145,14 @@ int dalloc(idw_ctx *c, T **p, size_t count) {
     ++c->nAllocs;
     void *q = nullptr;
     hipError_t e = hipMalloc(&q, count * sizeof(T));
-    if (e != hipSuccess) return fail(IDW_ENOMEM, "hipMalloc(%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
+    if (e != hipSuccess) {
+        // reported here: clear it, or the next launch check would report it again
+        (void)hipGetLastError();
+        return fail(IDW_ENOMEM, "hipMalloc(%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
+    }
     c->allocs.push_back(q);
+    c->allocBytes.push_back(count * sizeof(T));
+    c->devBytes += (int64_t)(count * sizeof(T));
     *p = static_cast<T *>(q);
     ++c->gen;
     return IDW_OK;
@@ -150,7 +162,12 @@ template <class T>
 void dfree(idw_ctx *c, T *&p) {
     if (!p) return;
     auto it = std::find(c->allocs.begin(), c->allocs.end(), (void *)p);
-    if (it != c->allocs.end()) c->allocs.erase(it);
+    if (it != c->allocs.end()) {
+        const size_t at = (size_t)(it - c->allocs.begin());
+        c->devBytes -= (int64_t)c->allocBytes[at];
+        c->allocBytes.erase(c->allocBytes.begin() + (std::ptrdiff_t)at);
+        c->allocs.erase(it);
+    }
     (void)hipFree(p);
     p = nullptr;
     ++c->gen;
@@ -476,6 +493,24 @@ int run_warp(idw_ctx *c, const double *xs, double *xv) {
     REC(c, 0);
     TRY(surface_pass(c, xs, fast));
     REC(c, 1);
+    // exact mode, storeWeights: room for every pair's weight of every pass
+    // (the weights follow from the undeformed surface and volume and the
+    // options, so any warp at the same options writes the same ones)
+    const bool storeW = !fast && c->opt.storeWeights && c->V.n;
+    c->wValid = false;
+    if (storeW) {
+        const int64_t need = (int64_t)nloop(c) * idw::wrow(c->V.n) * c->S.n;
+        if (c->wCount != need) {
+            if (c->capturing) return fail(IDW_ESTATE, "a recorded warp with stored weights needs an earlier one");
+            dfree(c, c->W);
+            c->wCount = 0;
+            TRY(dalloc(c, &c->W, (size_t)need));
+            c->wCount = need;
+        }
+    } else if (c->W && !c->capturing) {
+        dfree(c, c->W);  // storeWeights is off: give the memory back
+        c->wCount = 0;
+    }
     std::vector<char> fill(nloop(c), 0);
     const double nt2 = fast ? plan_near_t2(c) : 0.0;
     if (fast) {
@@ -507,10 +542,12 @@ int run_warp(idw_ctx *c, const double *xs, double *xv) {
     for (int kk = 0; kk < nloop(c) && c->V.n; ++kk) {
         P.pass = pass_of(c, kk);
         idw::launch_forward(c->st, c->S, c->V, P, mirror_of(c, kk), fast, c->opt.precision, xv,
-                            fast ? &c->plans[kk] : nullptr, c->tune);
+                            fast ? &c->plans[kk] : nullptr, c->tune, 0, -1,
+                            storeW ? c->W + (int64_t)kk * idw::wrow(c->V.n) * c->S.n : nullptr);
     }
     HIPCHK(hipGetLastError());
     REC(c, 3);
+    c->wValid = storeW;
     c->denOpt = c->opt;
     c->denValid = true;
     c->planValid = fast;
@@ -652,10 +689,13 @@ int deriv_kernels(idw_ctx *c, const double *xs, const double *dxv, double *dxs, 
     REC(c, 5);
     const bool det = fast && c->opt.deterministic && c->V.n;
     if (det && !det_ready(c)) return fail(IDW_ESTATE, "the deterministic reverse's plan transpose is not built");
+    // exact mode, storeWeights: the last warp's weights (at these options) instead of recomputed ones
+    const bool useW = !fast && c->opt.storeWeights && c->wValid && c->V.n;
     for (int kk = 0; kk < nloop(c) && c->V.n; ++kk) {
         idw::launch_reverse(c->st, c->S, c->V, make_params(c, false), mirror_of(c, kk), fast, c->opt.precision,
                             partial, nchunk, dxv, fast ? &c->plans[kk] : nullptr, c->tune,
-                            det ? c->detPart : nullptr);
+                            det ? c->detPart : nullptr,
+                            useW ? c->W + (int64_t)kk * idw::wrow(c->V.n) * c->S.n : nullptr);
         // per term, the slots of this pass in plan order (stored by the first pass, added by later ones)
         if (det) idw::launch_det_reduce(c->st, c->S, c->plans[kk], c->detPart, c->detChunkPart, kk > 0);
     }
@@ -671,7 +711,8 @@ int deriv_kernels(idw_ctx *c, const double *xs, const double *dxv, double *dxs, 
 bool deriv_needs_warp(const idw_ctx *c) {
     // the denominator (and the plan) depend on the options, not on the surface coordinates
     return !c->denValid || !same_numeric_options(c->denOpt, c->opt) ||
-           (c->opt.evalMode == IDW_EVAL_FAST && !c->planValid);
+           (c->opt.evalMode == IDW_EVAL_FAST && !c->planValid) ||
+           (c->opt.evalMode == IDW_EVAL_EXACT && c->opt.storeWeights && !c->wValid);
 }
 
 int run_deriv(idw_ctx *c, const double *xs, const double *dxv, double *dxs) {
@@ -874,6 +915,8 @@ const char *invalid_opts(const idw_options *o) {
     if (!(o->precision == 0 || o->precision == 1)) return "precision must be fp64 (0) or mixed (1)";
     if (!(o->reusePlan == 0 || o->reusePlan == 1)) return "reusePlan must be 0 or 1";
     if (!(o->deterministic == 0 || o->deterministic == 1)) return "deterministic must be 0 or 1";
+    if (!(o->storeWeights == 0 || o->storeWeights == 1)) return "storeWeights must be 0 or 1";
+    if (o->storeWeights && o->evalMode != IDW_EVAL_EXACT) return "storeWeights needs evalMode exact";
     if (!std::isfinite(o->aExp) || !std::isfinite(o->bExp)) return "aExp and bExp must be finite";
     if (!(o->LdefFact > 0)) return "LdefFact must be positive";
     if (!(o->errTol >= 0)) return "errTol must be non-negative";
@@ -893,7 +936,10 @@ hipError_t sort_pairs(hipStream_t st, int64_t n, uint64_t *keys, int32_t *ids, u
     if (e != hipSuccess) return e;
     void *tmp = nullptr;
     e = hipMalloc(&tmp, bytes);
-    if (e != hipSuccess) return e;
+    if (e != hipSuccess) {
+        (void)hipGetLastError();  // returned to the caller, which reports it
+        return e;
+    }
     e = hipcub::DeviceRadixSort::SortPairs(tmp, bytes, keys, keys_out, ids, ids_out, (int)n, 0, 63, st);
     hipError_t e2 = hipStreamSynchronize(st);
     (void)hipFree(tmp);
@@ -913,7 +959,10 @@ hipError_t exclusive_scan(hipStream_t st, const int64_t *in, int64_t *out, int64
     if (e != hipSuccess) return e;
     void *tmp = nullptr;
     e = hipMalloc(&tmp, bytes);
-    if (e != hipSuccess) return e;
+    if (e != hipSuccess) {
+        (void)hipGetLastError();  // returned to the caller, which reports it
+        return e;
+    }
     e = hipcub::DeviceScan::ExclusiveSum(tmp, bytes, in, out, (int)n, st);
     hipError_t e2 = hipStreamSynchronize(st);
     (void)hipFree(tmp);
@@ -1050,6 +1099,8 @@ int idw_set_volume_nodes(idw_ctx *c, const double *xv0, int64_t ndof) {
     dfree(c, c->V.vol); dfree(c, c->V.perm); dfree(c, c->V.den); dfree(c, c->V.numb);
     dfree(c, c->V.acc); dfree(c, c->V.volm);
     dfree(c, c->partial);
+    dfree(c, c->W);
+    c->wCount = 0;
     c->volSet = false;
     c->denValid = false;
     drop_plan(c);
@@ -1176,6 +1227,9 @@ int idw_initialize_warping(idw_ctx *c, const double *pts, int64_t n_pts, const d
     }
     S = idw::DevSurface();
     c->xs = c->dxs = c->dxsIn = c->partial = nullptr;
+    dfree(c, c->W);
+    c->wCount = 0;
+    c->wValid = false;
     c->surfInit = false;
     c->denValid = false;
     drop_plan(c);
@@ -1843,6 +1897,12 @@ int idw_batch_run(idw_batch *b, void *hip_stream) {
     ++b->launches;
     HIPCHK(hipEventRecord(b->out, b->origin));
     HIPCHK(hipStreamWaitEvent(user, b->out, 0));
+    return IDW_OK;
+}
+
+int idw_device_bytes(idw_ctx *c, int64_t *bytes) {
+    if (!c || !bytes) return fail(IDW_EINVAL, "null argument");
+    *bytes = c->devBytes;
     return IDW_OK;
 }
 

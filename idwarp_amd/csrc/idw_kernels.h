@@ -152,19 +152,23 @@ void launch_permute_vol(hipStream_t st, int64_t n, const int32_t *order, const d
 void launch_fill(hipStream_t st, double *a, int64_t n, double v);
 // zero n counters with a kernel (safe inside graph capture, unlike a memset node)
 void launch_zero_u32(hipStream_t st, unsigned int *a, int64_t n);
-// fast mode: blocks [blk0, blk0 + nblk) of 256 volume nodes (nblk < 0: to the end)
+// Row length of the stored weights: n_vol rounded up to 4.
+__host__ __device__ inline int64_t wrow(int64_t nvol) { return (nvol + 3) & ~(int64_t)3; }
+// fast mode: blocks [blk0, blk0 + nblk) of 256 volume nodes (nblk < 0: to the end);
+// exact mode with wout: every pair's weight stored at wout[i wrow(n_vol) + k]
 void launch_forward(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, const Mirror &M,
                     int fast, int prec, double *xv, const DevPlan *plan, const Tuning &T, int64_t blk0 = 0,
-                    int64_t nblk = -1);
+                    int64_t nblk = -1, double *wout = nullptr);
 void launch_prep_numb(hipStream_t st, const DevVolume &V, const Mirror &M, const double *dxv);
 void launch_plan_count(hipStream_t st, const DevSurface &S, const DevVolume &V, const Mirror &M, double errTol,
                        int64_t *waveCount);
 void launch_plan_fill(hipStream_t st, const DevSurface &S, const DevVolume &V, const Mirror &M, double errTol,
                       double nearT2, const DevPlan &plan, const Tuning &T, int64_t blk0 = 0, int64_t nblk = -1);
-// detPart (fast mode, deterministic): per-slot sums instead of atomics
+// detPart (fast mode, deterministic): per-slot sums instead of atomics;
+// wst (exact mode): the forward's stored weights instead of recomputed ones
 void launch_reverse(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, const Mirror &M,
                     int fast, int prec, double *partial, int nchunk, const double *dxv, const DevPlan *plan,
-                    const Tuning &T, double *detPart = nullptr);
+                    const Tuning &T, double *detPart = nullptr, const double *wst = nullptr);
 // the deterministic reverse's transpose of a plan and its reduction into the adjoint records
 void launch_plan_recs(hipStream_t st, const DevPlan &plan, int nrec, unsigned *keys, int32_t *vals);
 void launch_rec_bounds(hipStream_t st, int nrec, int64_t total, const unsigned *keys, int64_t *recStart);

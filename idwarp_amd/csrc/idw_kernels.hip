@@ -555,14 +555,16 @@ __global__ void __launch_bounds__(256) k_node_sums(int n, const int32_t *__restr
 // Exact mode (evalNodeExact on the root, kd_tree.F90:664-690)
 // ---------------------------------------------------------------------------
 // G: surface points per FP32 partial sum in mixed precision (4, 8 or 16, a
-// tuning parameter).
-template <int PREC, int EXPM, int G>
+// tuning parameter).  STORE: every pair's weight, as used, is also written to
+// wout[i ld + k] (surface point i, sorted volume node k; rows padded to
+// ld = wrow(nvol), a multiple of 4) for the reverse.
+template <int PREC, int EXPM, int G, bool STORE = false>
 __global__ void __launch_bounds__(256) k_exact_fwd(int nvol, const double4 *__restrict__ vol,
                                                    const int32_t *__restrict__ perm, int n,
                                                    const double4 *__restrict__ ps, const Rec64 *__restrict__ pd,
                                                    const Rec32 *__restrict__ pd32, WParams P, Mirror M,
                                                    double *__restrict__ xv, double *__restrict__ den,
-                                                   double4 *__restrict__ acc) {
+                                                   double4 *__restrict__ acc, double *__restrict__ wout = nullptr) {
     const int nb = (nvol + 255) / 256;
     const int blk = xcd_block(nb);
     if (blk >= nb) return;
@@ -579,6 +581,7 @@ __global__ void __launch_bounds__(256) k_exact_fwd(int nvol, const double4 *__re
             const double rx = v.x - s.x, ry = v.y - s.y, rz = v.z - s.z;
             const double lo = P.Ldef * rsqrt_nr(rx * rx + ry * ry + rz * rz + EPS_W);
             const double w = s.w * wfun<EXPM>(lo, P);
+            if (STORE && valid) wout[(int64_t)i * wrow(nvol) + k] = w;
             n0 += w * (q.K[0] * rx + q.K[1] * ry + q.K[2] * rz + q.d[0]);
             n1 += w * (q.K[3] * rx + q.K[4] * ry + q.K[5] * rz + q.d[1]);
             n2 += w * (q.K[6] * rx + q.K[7] * ry + q.K[8] * rz + q.d[2]);
@@ -600,6 +603,7 @@ __global__ void __launch_bounds__(256) k_exact_fwd(int nvol, const double4 *__re
                     const Rec64 &q = pd[i];
                     const double lo = P.Ldef * rsqrt_nr(d2 + EPS_W);
                     const double w = s.w * wfun<EXPM>(lo, P);
+                    if (STORE && valid) wout[(int64_t)i * wrow(nvol) + k] = w;
                     n0 += w * (q.K[0] * rx + q.K[1] * ry + q.K[2] * rz + q.d[0]);
                     n1 += w * (q.K[3] * rx + q.K[4] * ry + q.K[5] * rz + q.d[1]);
                     n2 += w * (q.K[6] * rx + q.K[7] * ry + q.K[8] * rz + q.d[2]);
@@ -610,6 +614,7 @@ __global__ void __launch_bounds__(256) k_exact_fwd(int nvol, const double4 *__re
                     // |r|^2 + eps (evalNodeExact's eps) in FP64, rounded once
                     const float lo = Lf * __frsqrt_rn((float)(d2 + EPS_W));
                     const float w = (float)s.w * wfun<EXPM>(lo, P);
+                    if (STORE && valid) wout[(int64_t)i * wrow(nvol) + k] = (double)w;
                     a0 += w * (q.K[0] * fx + q.K[1] * fy + q.K[2] * fz + q.d[0]);
                     a1 += w * (q.K[3] * fx + q.K[4] * fy + q.K[5] * fz + q.d[1]);
                     a2 += w * (q.K[6] * fx + q.K[7] * fy + q.K[8] * fz + q.d[2]);
@@ -681,6 +686,46 @@ __global__ void __launch_bounds__(256) k_exact_rev(int nvol, int chunk, const do
         acc[3] += s1 * rx; acc[4] += s1 * ry; acc[5] += s1 * rz;
         acc[6] += s2 * rx; acc[7] += s2 * ry; acc[8] += s2 * rz;
         acc[9] += s0; acc[10] += s1; acc[11] += s2;
+    }
+    if (valid) {
+        double *o = partial + ((int64_t)c * n + i) * 12;
+        for (int k = 0; k < 12; ++k) o[k] = acc[k];
+    }
+}
+
+// k_exact_rev with the forward's stored weights (w[i ld + j]) in place of
+// recomputed ones: lane i streams its own row of the weights, four at a time
+// (chunk is a multiple of 4, so every load is 32-byte aligned; the row's
+// padding is loaded, never used).
+__global__ void __launch_bounds__(256) k_exact_rev_w(int nvol, int chunk, const double4 *__restrict__ vol,
+                                                     const double4 *__restrict__ numb, int n,
+                                                     const double4 *__restrict__ ps, const double *__restrict__ wst,
+                                                     double *__restrict__ partial) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int c = blockIdx.y;
+    const int j0 = c * chunk, j1 = min(nvol, j0 + chunk);
+    const bool valid = i < n;
+    const double4 s = valid ? ps[i] : make_double4(0.0, 0.0, 0.0, 0.0);
+    const double *wr = wst + (int64_t)(valid ? i : 0) * wrow(nvol);
+    double acc[12];
+    for (int k = 0; k < 12; ++k) acc[k] = 0.0;
+    for (int j4 = j0; j4 < j1; j4 += 4) {
+        const double4 w4 = *reinterpret_cast<const double4 *>(wr + j4);
+        const double ws[4] = {w4.x, w4.y, w4.z, w4.w};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int j = j4 + q;
+            if (j >= j1) break;  // uniform: the chunk's end
+            const double4 v = vol[j];
+            const double4 nb = numb[j];
+            const double rx = v.x - s.x, ry = v.y - s.y, rz = v.z - s.z;
+            const double w = ws[q];
+            const double s0 = w * nb.x, s1 = w * nb.y, s2 = w * nb.z;
+            acc[0] += s0 * rx; acc[1] += s0 * ry; acc[2] += s0 * rz;
+            acc[3] += s1 * rx; acc[4] += s1 * ry; acc[5] += s1 * rz;
+            acc[6] += s2 * rx; acc[7] += s2 * ry; acc[8] += s2 * rz;
+            acc[9] += s0; acc[10] += s1; acc[11] += s2;
+        }
     }
     if (valid) {
         double *o = partial + ((int64_t)c * n + i) * 12;
@@ -2450,10 +2495,18 @@ static void exact_fwd_g(hipStream_t st, const DevSurface &S, const DevVolume &V,
 template <int EXPM>
 static void forward_impl(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, const Mirror &M,
                          int fast, int prec, double *xv, const DevPlan *plan, int64_t blk0, int64_t nblk,
-                         const Tuning &T) {
+                         const Tuning &T, double *wout) {
     if (nblk < 0) nblk = grid_for(V.n, 256) - blk0;
     if (fast && nblk <= 0) return;
-    if (!fast) {
+    if (!fast && wout) {
+        // stored weights: G = 8 in mixed precision
+        if (prec == IDW_PREC_FP64)
+            k_exact_fwd<IDW_PREC_FP64, EXPM, 8, true><<<xcd_grid(V.n), 256, 0, st>>>(
+                (int)V.n, V.vol, V.perm, S.n, S.ps, S.pd, S.pd32, P, M, xv, V.den, V.acc, wout);
+        else
+            k_exact_fwd<IDW_PREC_MIXED, EXPM, 8, true><<<xcd_grid(V.n), 256, 0, st>>>(
+                (int)V.n, V.vol, V.perm, S.n, S.ps, S.pd, S.pd32, P, M, xv, V.den, V.acc, wout);
+    } else if (!fast) {
         // G only shapes the mixed-precision kernel
         if (prec == IDW_PREC_FP64)
             k_exact_fwd<IDW_PREC_FP64, EXPM, 8><<<xcd_grid(V.n), 256, 0, st>>>((int)V.n, V.vol, V.perm, S.n, S.ps, S.pd,
@@ -2504,13 +2557,15 @@ static void rev_pair_launch(hipStream_t st, const DevSurface &S, const DevVolume
 template <int EXPM>
 static void reverse_impl(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, const Mirror &M,
                          int fast, int prec, double *partial, int nchunk, const double *dxv, const DevPlan *plan,
-                         const Tuning &T, double *detPart) {
+                         const Tuning &T, double *detPart, const double *wst) {
     if (!fast) {
         launch_prep_numb(st, V, M, dxv);
         const double4 *pos = M.mask ? V.volm : V.vol;
         const int chunk = (int)((V.n + nchunk - 1) / nchunk);
         dim3 g(grid_for(S.n, 256), nchunk);
-        if (prec == IDW_PREC_FP64)
+        if (wst)  // chunks rounded to 4 nodes: the last one may be shorter or empty
+            k_exact_rev_w<<<g, 256, 0, st>>>((int)V.n, (chunk + 3) & ~3, pos, V.numb, S.n, S.ps, wst, partial);
+        else if (prec == IDW_PREC_FP64)
             k_exact_rev<IDW_PREC_FP64, EXPM><<<g, 256, 0, st>>>((int)V.n, chunk, pos, V.numb, S.n, S.ps, P, partial);
         else
             k_exact_rev<IDW_PREC_MIXED, EXPM><<<g, 256, 0, st>>>((int)V.n, chunk, pos, V.numb, S.n, S.ps, P, partial);
@@ -2543,16 +2598,18 @@ void launch_fast_count(hipStream_t st, const DevSurface &S, const DevVolume &V, 
 }
 
 void launch_forward(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, const Mirror &M,
-                    int fast, int prec, double *xv, const DevPlan *plan, const Tuning &T, int64_t blk0, int64_t nblk) {
-    if (P.expMode == EXP_35) forward_impl<EXP_35>(st, S, V, P, M, fast, prec, xv, plan, blk0, nblk, T);
-    else forward_impl<EXP_INT>(st, S, V, P, M, fast, prec, xv, plan, blk0, nblk, T);
+                    int fast, int prec, double *xv, const DevPlan *plan, const Tuning &T, int64_t blk0, int64_t nblk,
+                    double *wout) {
+    if (P.expMode == EXP_35) forward_impl<EXP_35>(st, S, V, P, M, fast, prec, xv, plan, blk0, nblk, T, wout);
+    else forward_impl<EXP_INT>(st, S, V, P, M, fast, prec, xv, plan, blk0, nblk, T, wout);
 }
 
 void launch_reverse(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, const Mirror &M,
                     int fast, int prec, double *partial, int nchunk, const double *dxv, const DevPlan *plan,
-                    const Tuning &T, double *detPart) {
-    if (P.expMode == EXP_35) reverse_impl<EXP_35>(st, S, V, P, M, fast, prec, partial, nchunk, dxv, plan, T, detPart);
-    else reverse_impl<EXP_INT>(st, S, V, P, M, fast, prec, partial, nchunk, dxv, plan, T, detPart);
+                    const Tuning &T, double *detPart, const double *wst) {
+    if (P.expMode == EXP_35)
+        reverse_impl<EXP_35>(st, S, V, P, M, fast, prec, partial, nchunk, dxv, plan, T, detPart, wst);
+    else reverse_impl<EXP_INT>(st, S, V, P, M, fast, prec, partial, nchunk, dxv, plan, T, detPart, wst);
 }
 
 void launch_plan_recs(hipStream_t st, const DevPlan &plan, int nrec, unsigned *keys, int32_t *vals) {

@@ -47,6 +47,7 @@ _DEFAULTS = {
     "reusePlan": [bool, False],
     "autotune": [bool, False],
     "deterministic": [bool, False],
+    "storeWeights": [bool, False],
     "device": [int, 0],
 }
 
@@ -144,6 +145,7 @@ class USMesh:
         o.nearWallFactor = g("nearWallFactor")
         o.reusePlan = int(bool(g("reusePlan")))
         o.deterministic = int(bool(g("deterministic")))
+        o.storeWeights = int(bool(g("storeWeights")))
         return o
 
     def _setMeshOptions(self):
@@ -469,6 +471,14 @@ class USMesh:
         t = np.zeros(6)
         _lib.check(self._L.idw_get_timings(self._ctx, _lib.ptr(t)))
         return t
+
+    def getDeviceMemory(self):
+        """Bytes of device memory this mesh holds (idw_device_bytes): its
+        surface, volume and plan buffers and, with storeWeights, the stored
+        weights."""
+        b = ctypes.c_int64()
+        _lib.check(self._L.idw_device_bytes(self._ctx, ctypes.byref(b)))
+        return int(b.value)
 
     def getFastCounts(self):
         """Fast-mode work: per-node sums of visits, accepts, leaf pairs, error

@@ -71,6 +71,12 @@ typedef struct idw_options {
                                0 (default) = FP64 atomics, equal to rounding only.  The bits
                                hold for fixed tile parameters (the forward's chunking orders
                                its sums), so batches do not search them for such a context */
+    int32_t storeWeights;   /* exact mode: 1 = the warp keeps every (volume, surface) pair's
+                               weight, as used, for the reverse products that follow at the
+                               same options (passes x n_unique x n_vol x 8 bytes of device
+                               memory, n_vol rounded up to 4; the reference recomputes them,
+                               kd_tree.F90:984-1013);
+                               0 (default) = recompute.  Fast mode rejects 1 */
 } idw_options;
 
 typedef struct idw_ctx idw_ctx;
@@ -303,6 +309,12 @@ int idw_batch_info(idw_batch *batch, int *layout, double *ms2);
 /* How often the batch was (re)recorded (warm-up run, layouts timed, graph
  * recorded) and how many graph launches its runs made. */
 int idw_batch_stats(idw_batch *batch, int64_t *recordings, int64_t *launches);
+
+/* Device memory the context holds, in bytes: its surface, volume, plan and
+ * stored-weight buffers (not the allocation-free reverse's scratch array,
+ * which every context on a device shares).  No counterpart in the reference,
+ * whose arrays live in host memory. */
+int idw_device_bytes(idw_ctx *ctx, int64_t *bytes);
 
 #ifdef __cplusplus
 }

@@ -151,6 +151,13 @@ int api() {
     bad.deterministic = 1;
     expect(idw_create(&c, 0, &bad) == IDW_ENODEV, "deterministic 1 (valid options)");
     bad = o;
+    bad.storeWeights = 1;  // the default evalMode is fast
+    expect(idw_create(&c, 0, &bad) == IDW_EINVAL, "storeWeights in fast mode");
+    bad.evalMode = IDW_EVAL_EXACT;
+    expect(idw_create(&c, 0, &bad) == IDW_ENODEV, "storeWeights in exact mode (valid options)");
+    bad.storeWeights = 2;
+    expect(idw_create(&c, 0, &bad) == IDW_EINVAL, "storeWeights 2");
+    bad = o;
     bad.errTol = NAN;
     expect(idw_create(&c, 0, &bad) == IDW_EINVAL, "NaN errTol");
     expect(idw_destroy(nullptr) == IDW_OK, "destroy null");
@@ -173,6 +180,7 @@ int api() {
     expect(idw_wait(nullptr) == IDW_EINVAL, "wait");
     expect(idw_warp_deriv_dev_noalloc(nullptr, d, d, d, nullptr) == IDW_EINVAL, "warp_deriv_dev_noalloc");
     expect(idw_batch_stats(nullptr, &n64, &n64) == IDW_EINVAL, "batch_stats");
+    expect(idw_device_bytes(nullptr, &n64) == IDW_EINVAL, "device_bytes");
     expect(idw_get_volume_coordinates(nullptr, d, 3) == IDW_EINVAL, "get_volume_coordinates");
     expect(idw_set_external_mesh_indices(nullptr, i, 1) == IDW_EINVAL, "set_external_mesh_indices");
     expect(idw_get_solver_dof(nullptr, &n64) == IDW_EINVAL, "get_solver_dof");

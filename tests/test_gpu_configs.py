@@ -262,6 +262,19 @@ def test_c2_ellipsoid_exact_full_size_and_c4_finite_differences():
         print("C4 dot product, rotations %s: rel %.2e" % ("on" if rot else "off", abs(lhs - rhs) / abs(lhs)),
               flush=True)
         assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
+    # stored weights at this size would take 3,215,841 x 32,514 x 8 bytes
+    # (836 GB): the warp reports the failed allocation, and the mesh warps
+    # again once the option is off
+    g.setOption("useRotations", True)
+    g.setOption("storeWeights", True)
+    g.setSurfaceCoordinates(xs)
+    with pytest.raises(_lib.IdwError) as e:
+        g.warpMesh()
+    assert e.value.code == _lib.IDW_ENOMEM, e.value
+    g.setOption("storeWeights", False)
+    g.warpMesh()
+    g.warpDeriv(full, solverVec=False)
+    assert _rel(g.getdXs().ravel(), dref) <= 1e-10
 
 
 def test_split_window_surface_fast_parity():
