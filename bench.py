@@ -243,7 +243,9 @@ def main():
     # kernel names as rocprofv3 reports them; the template argument 0 is the
     # exponent mode of the default exponents (3, 5), compiled as products
     if head_prec == "mixed":
-        kname, peak = "idw::k_fast_rev_pair<0>", FP32_PEAK_TFLOPS
+        # the kernel is instantiated per merge window, the tuned one (32 by default)
+        mq = tuning.get("mixed", {}).get("chosen", {}).get("mergeWindow", 32)
+        kname, peak = "idw::k_fast_rev_pair<0, %d>" % mq, FP32_PEAK_TFLOPS
         compute = ("FP32 terms (the bulk): FP32 weights on VALU and the sums over the 128 volume nodes of a pair of "
                    "walk waves on v_mfma_f32_16x16x4f32, each term shared by the pair added once; FP64 terms near "
                    "the wall: FP64 weights and v_mfma_f64_16x16x4f64 per wave.  Priced against the 157.3 TFLOP/s "
