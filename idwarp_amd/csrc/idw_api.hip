@@ -63,6 +63,7 @@ bool same_numeric_options(const idw_options &a, const idw_options &b) {
 }
 
 double plan_near_t2(const idw_ctx *c);
+double plan_far_t2(const idw_ctx *c);
 
 }  // namespace
 
@@ -341,6 +342,14 @@ double plan_near_t2(const idw_ctx *c) {
     return t * t;
 }
 
+// Squared distance beyond which mixed precision keeps a term in FP64 (0: no
+// such test, FP64 precision).
+double plan_far_t2(const idw_ctx *c) {
+    if (c->opt.precision != IDW_PREC_MIXED) return 0.0;
+    const double t = idw::FAR_FACTOR * c->hs.ldef0 * c->opt.LdefFact;
+    return t * t;
+}
+
 // Mixed precision: move the volume nodes that need FP64 buckets behind the
 // others (Morton order kept within both groups), so that the other waves
 // run FP32 throughout.  Redone when errTol or the near-wall distance change.
@@ -512,12 +521,13 @@ int run_warp(idw_ctx *c, const double *xs, double *xv) {
         c->wCount = 0;
     }
     std::vector<char> fill(nloop(c), 0);
-    const double nt2 = fast ? plan_near_t2(c) : 0.0;
+    const double nt2 = fast ? plan_near_t2(c) : 0.0, ft2 = fast ? plan_far_t2(c) : 0.0;
     if (fast) {
         for (int kk = 0; kk < nloop(c); ++kk) {
             idw::DevPlan &pl = c->plans[kk];
-            if (!(c->opt.reusePlan || c->batchPlan) || !pl.filled || pl.errTol != c->opt.errTol || pl.nearT2 != nt2) {
-                if (c->capturing && !(pl.filled && pl.errTol == c->opt.errTol && pl.nearT2 == nt2))
+            const bool current = pl.filled && pl.errTol == c->opt.errTol && pl.nearT2 == nt2 && pl.farT2 == ft2;
+            if (!(c->opt.reusePlan || c->batchPlan) || !current) {
+                if (c->capturing && !current)
                     return fail(IDW_ESTATE, "a captured warp needs the plan of a previous warp at the same options");
                 fill[kk] = 1;
             }
@@ -527,7 +537,7 @@ int run_warp(idw_ctx *c, const double *xs, double *xv) {
         if (!fill[kk]) continue;
         idw::DevPlan &pl = c->plans[kk];
         idw::launch_zero_u32(c->st, pl.mismatch, 1);
-        if (c->V.n) idw::launch_plan_fill(c->st, c->S, c->V, mirror_of(c, kk), c->opt.errTol, nt2, pl, c->tune);
+        if (c->V.n) idw::launch_plan_fill(c->st, c->S, c->V, mirror_of(c, kk), c->opt.errTol, nt2, ft2, pl, c->tune);
         HIPCHK(hipGetLastError());
         if (!c->capturing && kk < (1 << idw::MAX_PLANES)) {
             HIPCHK(hipMemcpyAsync(c->mmHost + kk, pl.mismatch, sizeof(unsigned int), hipMemcpyDeviceToHost, c->st));
@@ -536,6 +546,7 @@ int run_warp(idw_ctx *c, const double *xs, double *xv) {
         pl.filled = true;
         pl.transposed = false;
         pl.nearT2 = nt2;
+        pl.farT2 = ft2;
     }
     if (c->mmPending && !c->capturing) HIPCHK(hipEventRecord(c->mmEvent, c->st));
     REC(c, 2);
@@ -836,12 +847,13 @@ int run_tune(idw_ctx *c) {
         }
     };
     if (fast) {
-        const double nt2 = plan_near_t2(c);
+        const double nt2 = plan_near_t2(c), ft2 = plan_far_t2(c);
         static const int wbufs[3] = {64, 128, 256}, fchs[3] = {16, 32, 64}, mqs[3] = {8, 16, 32};
         TRY(pick(idw::TUNE_WBUF, wbufs, 3, c->tune.wbuf, [&]() {
             for (int kk = 0; kk < nloop(c); ++kk) {
                 idw::launch_zero_u32(c->st, c->plans[kk].mismatch, 1);
-                idw::launch_plan_fill(c->st, c->S, c->V, mirror_of(c, kk), c->opt.errTol, nt2, c->plans[kk], c->tune);
+                idw::launch_plan_fill(c->st, c->S, c->V, mirror_of(c, kk), c->opt.errTol, nt2, ft2, c->plans[kk],
+                                      c->tune);
             }
         }));
         HIPCHK(hipStreamSynchronize(c->st));

@@ -15,6 +15,10 @@ enum { EXP_35 = 0, EXP_INT = 1, EXP_REAL = 2 };
 
 constexpr int MAX_PLANES = 6;
 
+// Mixed precision keeps in FP64 every fast-mode term farther than
+// FAR_FACTOR * Ldef from an active lane of the wave.
+constexpr double FAR_FACTOR = 2.0;
+
 // One pass of the symmetry-plane loop (warpMesh.F90:46-64): plane j (0-based)
 // reflects the volume node when bit j of mask is set, i.e. the reference's
 // getMirrorPt test mod(kk-1, 2^(j+1)) >= 2^j with mask = kk-1.
@@ -100,7 +104,7 @@ struct DevPlan {
     int64_t nchunk = 0, chunkCap = 0;
     bool transposed = false;  // detPos / recStart / chunks describe the current entries
     int64_t nwave = 0, total = 0, capacity = 0;
-    double errTol = -1.0, nearT2 = -1.0;  // key of the filled plan
+    double errTol = -1.0, nearT2 = -1.0, farT2 = -1.0;  // key of the filled plan
     double orderTol = -1.0, orderT2 = -1.0;  // key of the near-wall ordering of the volume nodes
     bool sized = false, filled = false;
 };
@@ -163,7 +167,8 @@ void launch_prep_numb(hipStream_t st, const DevVolume &V, const Mirror &M, const
 void launch_plan_count(hipStream_t st, const DevSurface &S, const DevVolume &V, const Mirror &M, double errTol,
                        int64_t *waveCount);
 void launch_plan_fill(hipStream_t st, const DevSurface &S, const DevVolume &V, const Mirror &M, double errTol,
-                      double nearT2, const DevPlan &plan, const Tuning &T, int64_t blk0 = 0, int64_t nblk = -1);
+                      double nearT2, double farT2, const DevPlan &plan, const Tuning &T, int64_t blk0 = 0,
+                      int64_t nblk = -1);
 // detPart (fast mode, deterministic): per-slot sums instead of atomics;
 // wst (exact mode): the forward's stored weights instead of recomputed ones
 void launch_reverse(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, const Mirror &M,

@@ -854,11 +854,13 @@ struct PlanWriter {
         }
         c += n;
     }
-    __device__ __forceinline__ void finish(int64_t *split, int wave) {
+    // allFp64: the wave's FP32 terms are evaluated in FP64 as well (its
+    // split moves to the front of its range)
+    __device__ __forceinline__ void finish(int64_t *split, int wave, bool allFp64) {
         flush();
         flush64();
         if ((threadIdx.x & 63) == 0) {
-            split[wave] = off + nf;
+            split[wave] = allFp64 ? off : off + nf;
             if (nf + nb != end - off) atomicAdd(mismatch, 1u);
         }
     }
@@ -933,14 +935,20 @@ __device__ __forceinline__ int walk_visit(const WalkHdr &h, int node, const doub
 // and every cluster some lane accepts (evalNodeApprox, :692-713), with the
 // lane masks, in depth-first order.  The decisions use only init-time data
 // (centroids, radii, error tables, denominator0) and errTol.  COUNT: terms
-// per wave only.  A bucket's points are FP64 terms when an active lane lies
-// within sqrt(nearT2) of the bucket's bounding sphere (mixed precision keeps
-// those pairs in FP64).
+// per wave only.  Mixed precision (nearT2 > 0) keeps some terms in FP64:
+// the points of a bucket when an active lane lies within sqrt(nearT2) of
+// the bucket's bounding sphere, and every term of a wave with a node more
+// than sqrt(farT2) outside the root's bounding sphere (the whole surface).
+// Far from the body the terms' displacements at the node, K (v - x) + d,
+// grow with the distance while their weighted mean stays small, so FP32
+// weights and sums would lose against the cancellation; such waves
+// evaluate few terms.
 template <bool COUNT, int WBUF>
 __global__ void __launch_bounds__(256) k_plan_walk(int nvol, int blk0, int nbc, const double4 *__restrict__ vol,
                                                    int nnodes, const WalkHdr *__restrict__ whdr,
                                                    const NodeRec *__restrict__ nodes, double errTol, double nearT2,
-                                                   int64_t *__restrict__ waveCount, PlanEntry *__restrict__ plan,
+                                                   double farT2, int64_t *__restrict__ waveCount,
+                                                   PlanEntry *__restrict__ plan,
                                                    const int64_t *__restrict__ waveOff,
                                                    int64_t *__restrict__ waveSplit,
                                                    unsigned int *__restrict__ mismatch, Mirror M) {
@@ -976,6 +984,14 @@ __global__ void __launch_bounds__(256) k_plan_walk(int nvol, int blk0, int nbc, 
     constexpr unsigned HB = sizeof(WalkHdr);
     int node = 0;
     WalkHdr h = load_whdr(whdr, 0);
+    // mixed precision: a wave with a node more than sqrt(farT2) outside the
+    // root's bounding sphere keeps every term in FP64
+    bool farWave = false;
+    if (!COUNT && farT2 > 0.0) {
+        const double R = h.skip == 1 ? h.r : 1.0 / h.r;  // the root's radius (bucket: rad, else 1 / orad)
+        const double cx = v.x - h.X[0], cy = v.y - h.X[1], cz = v.z - h.X[2], t = R + sqrt(farT2);
+        farWave = __ballot(valid && cx * cx + cy * cy + cz * cz > t * t) != 0ull;
+    }
     unsigned touched = 0;
     for (;;) {
         const unsigned *pa =
@@ -997,7 +1013,7 @@ __global__ void __launch_bounds__(256) k_plan_walk(int nvol, int blk0, int nbc, 
     if (COUNT) {
         if ((threadIdx.x & 63) == 0) waveCount[wave] = cnt;
     } else {
-        pw.finish(waveSplit, wave);
+        pw.finish(waveSplit, wave, farWave);
     }
 }
 
@@ -1100,24 +1116,72 @@ __device__ __forceinline__ double first_lane(double x) {
 // B[l/16][l%16]; register j of D holds D[4j + l/16][l%16].
 
 // Terms staged per chunk in the forward (FCH: 16, 32 or 64, a tuning
-// parameter; 64 / FCH lanes stage each term).
+// parameter; 64 / FCH lanes stage each term).  FP32 terms (mixed precision)
+// and FP64 terms have chunk layouts of their own, so that the FP32 one, which
+// carries the bulk, holds no FP64 B rows: at 64 terms a wave's chunk takes
+// 7.4 KB and four workgroups of four waves fit a CU's LDS alongside.
 constexpr int BROW = 17;  // padded LDS row (doubles) of a staged B row
 
 constexpr int BROWF = 17;  // padded LDS row (floats) of a staged FP32 B row
 
+// Per-term weight factors.  The default exponents (3, 5) give
+// w = A L^3 p^3 + A L^3 alpha^5 L^2 p^5 with p = 1/|r|, evaluated as
+// p^3 (f0 + f1 p^2) from f0 = A L^3 and f1 = A L^3 alpha^5 L^2; other
+// exponents use f0 = A and the general form A wfun(L p).
+template <int EXPM>
+__device__ __forceinline__ double2 weight_factors(double A, const WParams &P) {
+    if (EXPM == EXP_35) {
+        const double L3 = P.Ldef * P.Ldef * P.Ldef;
+        return make_double2(A * L3, A * L3 * (P.a5 * P.Ldef * P.Ldef));
+    }
+    return make_double2(A, 0.0);
+}
+
+// FP32 weight of one pair from |r|^2 (mixed precision) and the term's factors.
+template <int EXPM>
+__device__ __forceinline__ float weight_f32(float d2, float2 f, float Lf, const WParams &P) {
+    if (EXPM == EXP_35) {
+        const float p = __frsqrt_rn(d2), p2 = p * p;
+        return (p * p2) * fmaf(f.y, p2, f.x);
+    }
+    return f.x * wfun<EXPM>(Lf * __frsqrt_rn(d2), P);
+}
+
+// FP64 weight of one pair from |r|^2 and the term's factors.
+template <int EXPM>
+__device__ __forceinline__ double weight_f64(double d2, double2 f, const WParams &P) {
+    if (EXPM == EXP_35) {
+        const double p = rsqrt_nr(d2), p2 = p * p;
+        return (p * p2) * fma(f.y, p2, f.x);
+    }
+    return f.x * wfun<EXPM>(P.Ldef * rsqrt_nr(d2), P);
+}
+
+// A chunk of staged terms: B rows [K (9), c (3), 1, 0, 0, 0], positions x
+// with the eps added to |r|^2 (EPS_W for points, evalNodeExact; 0 for
+// clusters, evalNodeApprox), the lane masks and the weight factors.
 template <int FCH>
-struct FwdChunk {
-    union {
-        double B[FCH][BROW];    // K (9), c (3), 1, 0, 0, 0
-        float Bf[FCH][BROWF];   // the same in FP32 (mixed precision, FP32 terms)
-    };
-    double X[FCH][4];     // x (3), A
-    double eps[FCH];      // added to |r|^2: EPS_W for points (evalNodeExact), 0 for clusters (evalNodeApprox)
+struct FwdChunk32 {
+    float B[FCH][BROWF];
+    double X[FCH][4];  // x (3), eps
     unsigned long long mask[FCH];
+    float2 F[FCH];
 };
 template <int FCH>
+struct FwdChunk64 {
+    double B[FCH][BROW];
+    double X[FCH][4];
+    unsigned long long mask[FCH];
+    double2 F[FCH];
+};
+// FP64 terms per chunk: FCH in FP64 precision; in mixed precision (a short
+// tail of FP64 terms per near-wall wave) at most 32, within the FP32 chunk's
+// footprint.
+constexpr int fch64(int prec, int fch) { return prec == IDW_PREC_MIXED && fch > 32 ? 32 : fch; }
+template <int PREC, int FCH>
 union FwdLds {
-    FwdChunk<FCH> c;
+    FwdChunk32<FCH> c32;
+    FwdChunk64<fch64(PREC, FCH)> c64;
     double acc[32][BROW];  // half of the wave's 64 x 16 sums
 };
 
@@ -1136,28 +1200,16 @@ __device__ __forceinline__ PlanEntry plan_entry(const PlanEntry *__restrict__ pl
     return pe;
 }
 
-// FP32 weight of one pair from |r|^2 (mixed precision).  The default
-// exponents (3, 5) use w = A L^3 p^3 (1 + alpha^5 L^2 p^2) with p = 1/|r|,
-// which is A (q^3 + alpha^5 q^5) for q = L p; AL3 = A L^3 per term and
-// a5L2 = alpha^5 L^2 per launch.
-template <int EXPM>
-__device__ __forceinline__ float weight_f32(float d2, float A, float AL3, float Lf, float a5L2, const WParams &P) {
-    if (EXPM == EXP_35) {
-        const float p = __frsqrt_rn(d2), p2 = p * p;
-        return (AL3 * (p2 * p)) * fmaf(a5L2, p2, 1.0f);
-    }
-    return A * wfun<EXPM>(Lf * __frsqrt_rn(d2), P);
-}
-
 // Stage FCH terms of the plan in LDS, 64 / FCH lanes per term: term r is
 // staged by lanes r + FCH p.  Part p = 0 writes row 0 of K and c (all rows
-// when one lane stages a term), x, A, eps and the mask; the other parts the
-// remaining rows; the last part the constant column.  pe: this lane's entry
-// (record -1: padding row).  W32 reads the FP32 records (mixed precision).
-template <bool W32, int FCH>
-__device__ __forceinline__ void stage_terms(FwdChunk<FCH> &C, PlanEntry pe, int nnodes, int nrec,
+// when one lane stages a term), x, eps, the mask and the weight factors; the
+// other parts the remaining rows; the last part the constant column.  pe:
+// this lane's entry (record -1: padding row).  W32 reads the FP32 records
+// (mixed precision) into an FP32 chunk.
+template <bool W32, int EXPM, int FCH, typename Chunk>
+__device__ __forceinline__ void stage_terms(Chunk &C, PlanEntry pe, int nnodes, int nrec,
                                             const Rec64 *__restrict__ rec, const Rec32 *__restrict__ rec32,
-                                            double ox, double oy, double oz) {
+                                            const WParams &P, double ox, double oy, double oz) {
     static_assert(FCH == 16 || FCH == 32 || FCH == 64, "16, 32 or 64 terms per chunk");
     constexpr int LPT = 64 / FCH;  // lanes per term
     const int lane = threadIdx.x & 63, r = lane & (FCH - 1), part = lane / FCH;
@@ -1181,43 +1233,32 @@ __device__ __forceinline__ void stage_terms(FwdChunk<FCH> &C, PlanEntry pe, int 
                 K0 = q.K[3 * a]; K1 = q.K[3 * a + 1]; K2 = q.K[3 * a + 2]; d = q.d[a];
             }
             const double c = d - (K0 * y0 + K1 * y1 + K2 * y2);
-            if (W32) {
-                C.Bf[r][3 * a] = (float)K0;
-                C.Bf[r][3 * a + 1] = (float)K1;
-                C.Bf[r][3 * a + 2] = (float)K2;
-                C.Bf[r][9 + a] = (float)c;
-            } else {
-                C.B[r][3 * a] = K0;
-                C.B[r][3 * a + 1] = K1;
-                C.B[r][3 * a + 2] = K2;
-                C.B[r][9 + a] = c;
-            }
+            C.B[r][3 * a] = K0;
+            C.B[r][3 * a + 1] = K1;
+            C.B[r][3 * a + 2] = K2;
+            C.B[r][9 + a] = c;
         }
         if (hi) {
-            if (W32) {
-                C.Bf[r][12] = 1.0f;
-                C.Bf[r][13] = C.Bf[r][14] = C.Bf[r][15] = 0.0f;
-            } else {
-                C.B[r][12] = 1.0;
-                C.B[r][13] = C.B[r][14] = C.B[r][15] = 0.0;
-            }
+            C.B[r][12] = 1.0f;
+            C.B[r][13] = C.B[r][14] = C.B[r][15] = 0.0f;
         }
         if (lo) {
             for (int i = 0; i < 3; ++i) C.X[r][i] = x[i];
-            C.X[r][3] = W32 ? (double)rec32[pe.rec].A : rec[pe.rec].A;
-            C.eps[r] = pe.rec < nnodes ? 0.0 : EPS_W;
+            C.X[r][3] = pe.rec < nnodes ? 0.0 : EPS_W;
             C.mask[r] = pe.mask;
+            const double2 f = weight_factors<EXPM>(W32 ? (double)rec32[pe.rec].A : rec[pe.rec].A, P);
+            if constexpr (W32) C.F[r] = make_float2((float)f.x, (float)f.y);
+            else C.F[r] = f;
         }
     } else {
         constexpr int W = 16 / LPT;  // B columns each part clears
-        for (int j = part * W; j < (part + 1) * W; ++j) {
-            if (W32) C.Bf[r][j] = 0.0f;
-            else C.B[r][j] = 0.0;
-        }
+        for (int j = part * W; j < (part + 1) * W; ++j) C.B[r][j] = 0.0f;
         if (lo) {
-            for (int i = 0; i < 4; ++i) C.X[r][i] = 0.0;
-            C.eps[r] = 1.0;
+            for (int i = 0; i < 3; ++i) C.X[r][i] = 0.0;
+            C.X[r][3] = 1.0;
             C.mask[r] = 0ull;
+            if constexpr (W32) C.F[r] = make_float2(0.0f, 0.0f);
+            else C.F[r] = make_double2(0.0, 0.0);
         }
     }
 }
@@ -1258,36 +1299,34 @@ __device__ __forceinline__ void add_f32_sums(dvec4 &D, fvec4 F) {
 // The entries of the next chunk are requested while this one is summed.
 // W32 (mixed precision, FP32 terms): FP32 weights and B rows on
 // v_mfma_f32_16x16x4f32, each chunk's FP32 sums added to the FP64 sums.
-template <bool W32, int EXPM, int FCH>
+template <bool W32, int EXPM, int FCH, typename Chunk>
 __device__ __forceinline__ void fwd_terms(int64_t a, int64_t b, const PlanEntry *__restrict__ plan, int nnodes,
                                           int nrec, const Rec64 *__restrict__ rec, const Rec32 *__restrict__ rec32,
-                                          const WParams &P, FwdChunk<FCH> &C, double ox, double oy, double oz,
+                                          const WParams &P, Chunk &C, double ox, double oy, double oz,
                                           const double (&vx)[4], const double (&vy)[4], const double (&vz)[4],
                                           dvec4 (&D)[4]) {
     const int lane = threadIdx.x & 63, n = lane & 15, R = lane >> 4, r = lane & (FCH - 1);
-    const float Lf = (float)P.Ldef, a5L2 = (float)(P.a5 * P.Ldef * P.Ldef);
-    const double L3 = P.Ldef * P.Ldef * P.Ldef;
+    const float Lf = (float)P.Ldef;
     if (a >= b) return;
     PlanEntry pe = plan_entry(plan, a + r, b);
     for (int64_t c0 = a; c0 < b; c0 += FCH) {
         const int cnt = (int)min((int64_t)FCH, b - c0);
         wave_sync();  // the previous chunk is consumed
-        stage_terms<W32, FCH>(C, pe, nnodes, nrec, rec, rec32, ox, oy, oz);
+        stage_terms<W32, EXPM, FCH>(C, pe, nnodes, nrec, rec, rec32, P, ox, oy, oz);
         pe = plan_entry(plan, c0 + FCH + r, b);  // next chunk
         wave_sync();
         const int ng = (cnt + 3) >> 2;
-        if (W32) {
+        if constexpr (W32) {
             fvec4 F[4];
 #pragma unroll
             for (int t = 0; t < 4; ++t) F[t] = fvec4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll 1
             for (int q = 0; q < ng; ++q) {
                 const int tr = 4 * q + R;
-                const float Bv = C.Bf[tr][n];
+                const float Bv = C.B[tr][n];
                 const unsigned long long m = C.mask[tr] >> n;
-                const double x0 = C.X[tr][0], x1 = C.X[tr][1], x2 = C.X[tr][2], A = C.X[tr][3];
-                const double eps = C.eps[tr];
-                const float Af = (float)A, AL3 = (float)(A * L3);
+                const double x0 = C.X[tr][0], x1 = C.X[tr][1], x2 = C.X[tr][2], eps = C.X[tr][3];
+                const float2 f = C.F[tr];
 #pragma unroll
                 for (int t = 0; t < 4; ++t) {
                     const unsigned long long on = __ballot((m >> (16 * t)) & 1ull);
@@ -1297,8 +1336,7 @@ __device__ __forceinline__ void fwd_terms(int64_t a, int64_t b, const PlanEntry 
                     const double rx = vx[t] - x0, ry = vy[t] - x1, rz = vz[t] - x2;
                     // |r|^2 + eps in FP64 (eps: points only; folded into the
                     // first product, so it costs no instruction), rounded once
-                    const float w = weight_f32<EXPM>((float)fma(rz, rz, fma(ry, ry, fma(rx, rx, eps))), Af, AL3, Lf,
-                                                     a5L2, P);
+                    const float w = weight_f32<EXPM>((float)fma(rz, rz, fma(ry, ry, fma(rx, rx, eps))), f, Lf, P);
                     F[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(lane_in(on) ? w : 0.0f, Bv, F[t], 0, 0, 0);
                 }
             }
@@ -1310,14 +1348,14 @@ __device__ __forceinline__ void fwd_terms(int64_t a, int64_t b, const PlanEntry 
                 const int tr = 4 * q + R;
                 const double Bv = C.B[tr][n];
                 const unsigned long long m = C.mask[tr] >> n;
-                const double x0 = C.X[tr][0], x1 = C.X[tr][1], x2 = C.X[tr][2], A = C.X[tr][3];
-                const double eps = C.eps[tr];
+                const double x0 = C.X[tr][0], x1 = C.X[tr][1], x2 = C.X[tr][2], eps = C.X[tr][3];
+                const double2 f = C.F[tr];
 #pragma unroll
                 for (int t = 0; t < 4; ++t) {
                     const unsigned long long on = __ballot((m >> (16 * t)) & 1ull);
                     if (!on) continue;  // node block t idle for all four terms
                     const double rx = vx[t] - x0, ry = vy[t] - x1, rz = vz[t] - x2;
-                    const double w = A * wfun<EXPM>(P.Ldef * rsqrt_nr(rx * rx + ry * ry + rz * rz + eps), P);
+                    const double w = weight_f64<EXPM>(rx * rx + ry * ry + rz * rz + eps, f, P);
                     D[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(lane_in(on) ? w : 0.0, Bv, D[t], 0, 0, 0);
                 }
             }
@@ -1339,7 +1377,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
                                                   const Rec64 *__restrict__ rec, const Rec32 *__restrict__ rec32,
                                                   WParams P, Mirror M, double *__restrict__ xv,
                                                   double *__restrict__ den, double4 *__restrict__ acc) {
-    __shared__ FwdLds<FCH> lds[4];
+    __shared__ FwdLds<PREC, FCH> lds[4];
     const int lb = xcd_block(nbc);
     if (lb >= nbc) return;
     const int blk = blk0 + lb;
@@ -1362,14 +1400,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     dvec4 D[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) D[t] = dvec4{0.0, 0.0, 0.0, 0.0};
-    FwdLds<FCH> &L = lds[wv];
+    FwdLds<PREC, FCH> &L = lds[wv];
+    constexpr int F64 = fch64(PREC, FCH);
     const int64_t e0 = waveOff[wave], e1 = waveOff[wave + 1];
     if (PREC == IDW_PREC_MIXED) {
         const int64_t es = min(max(waveSplit[wave], e0), e1);
-        fwd_terms<true, EXPM, FCH>(e0, es, plan, nnodes, nrec, rec, rec32, P, L.c, ox, oy, oz, vx, vy, vz, D);
-        fwd_terms<false, EXPM, FCH>(es, e1, plan, nnodes, nrec, rec, rec32, P, L.c, ox, oy, oz, vx, vy, vz, D);
+        fwd_terms<true, EXPM, FCH>(e0, es, plan, nnodes, nrec, rec, rec32, P, L.c32, ox, oy, oz, vx, vy, vz, D);
+        fwd_terms<false, EXPM, F64>(es, e1, plan, nnodes, nrec, rec, rec32, P, L.c64, ox, oy, oz, vx, vy, vz, D);
     } else {
-        fwd_terms<false, EXPM, FCH>(e0, e1, plan, nnodes, nrec, rec, rec32, P, L.c, ox, oy, oz, vx, vy, vz, D);
+        fwd_terms<false, EXPM, F64>(e0, e1, plan, nnodes, nrec, rec, rec32, P, L.c64, ox, oy, oz, vx, vy, vz, D);
     }
     // D[t] register j: node 16t + 4j + R, column n.  Through LDS, 32 nodes
     // at a time, each lane gathers its node's 13 sums.  The lane's own
@@ -1451,6 +1490,7 @@ __device__ __forceinline__ void rev_terms(int64_t a, int64_t b, const PlanEntry 
             idxs[m] = idx;
         }
         const unsigned long long mR = pe.mask >> R;
+        const double2 f = weight_factors<EXPM>(xa.w, P);
         dvec4 D0 = {0.0, 0.0, 0.0, 0.0}, D1 = D0;
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
@@ -1458,7 +1498,7 @@ __device__ __forceinline__ void rev_terms(int64_t a, int64_t b, const PlanEntry 
             if (!on) continue;  // nodes 4q..4q+3 idle for all the tile's terms
             const double4 c = nodes[4 * q + R];
             const double rx = c.x - xa.x, ry = c.y - xa.y, rz = c.z - xa.z;
-            double w = xa.w * wfun<EXPM>(P.Ldef * rsqrt_nr(rx * rx + ry * ry + rz * rz + eps), P);
+            double w = weight_f64<EXPM>(rx * rx + ry * ry + rz * rz + eps, f, P);
             w = lane_in(on) ? w : 0.0;
             if (q & 1) D1 = __builtin_amdgcn_mfma_f64_16x16x4f64(w, B[q], D1, 0, 0, 0);
             else D0 = __builtin_amdgcn_mfma_f64_16x16x4f64(w, B[q], D0, 0, 0, 0);
@@ -1760,8 +1800,7 @@ k_fast_rev_pair(int nvol, int nwave, const double4 *__restrict__ vol, const int3
         GB[q] = (float)g_entry(64 + 4 * q + R);
         __builtin_amdgcn_sched_barrier(0);
     }
-    const float Lf = (float)P.Ldef, a5L2 = (float)(P.a5 * P.Ldef * P.Ldef);
-    const double L3 = P.Ldef * P.Ldef * P.Ldef;
+    const float Lf = (float)P.Ldef;
     const int src9 = (lane & ~15) | (9 + (m < 9 ? m / 3 : 0));
     const int wA = 2 * pair, wB = min(2 * pair + 1, nwave);  // wave B == nwave: an empty range
     const int64_t a0 = waveOff[wA], a1 = waveOff[wA + 1];
@@ -1919,7 +1958,8 @@ k_fast_rev_pair(int nvol, int nwave, const double4 *__restrict__ vol, const int3
             L.Y[m][2] = T.xa.z - oz;
             L.idx[m] = idx;
         }
-        const float Af = (float)T.xa.w, AL3 = (float)(T.xa.w * L3);
+        const double2 fd = weight_factors<EXPM>(T.xa.w, P);
+        const float2 f = make_float2((float)fd.x, (float)fd.y);
         const double epsT = T.rec >= nnodes ? EPS_W : 0.0;  // points carry evalNodeExact's eps, clusters none
         F0 = fvec4{0.0f, 0.0f, 0.0f, 0.0f};
         F1 = F0;
@@ -1933,7 +1973,7 @@ k_fast_rev_pair(int nvol, int nwave, const double4 *__restrict__ vol, const int3
                 const bool on = (mR >> (4 * q)) & 1ull;
                 const double rx = c.x - T.xa.x, ry = c.y - T.xa.y, rz = c.z - T.xa.z;
                 const float wf =
-                    weight_f32<EXPM>((float)fma(rz, rz, fma(ry, ry, fma(rx, rx, epsT))), Af, AL3, Lf, a5L2, P);
+                    weight_f32<EXPM>((float)fma(rz, rz, fma(ry, ry, fma(rx, rx, epsT))), f, Lf, P);
                 const float w = on ? wf : 0.0f;
                 const float g = h == 0 ? GA[q] : GB[q];
                 if (q & 1) F1 = __builtin_amdgcn_mfma_f32_16x16x4f32(w, g, F1, 0, 0, 0);
@@ -2525,25 +2565,25 @@ void launch_prep_numb(hipStream_t st, const DevVolume &V, const Mirror &M, const
 void launch_plan_count(hipStream_t st, const DevSurface &S, const DevVolume &V, const Mirror &M, double errTol,
                        int64_t *waveCount) {
     const int nb = (int)grid_for(V.n, 256);
-    k_plan_walk<true, 128><<<nb, 256, 0, st>>>((int)V.n, 0, nb, V.vol, S.nnodes, S.whdr, S.nodes, errTol, 0.0, waveCount,
-                                               nullptr, nullptr, nullptr, nullptr, M);
+    k_plan_walk<true, 128><<<nb, 256, 0, st>>>((int)V.n, 0, nb, V.vol, S.nnodes, S.whdr, S.nodes, errTol, 0.0, 0.0,
+                                               waveCount, nullptr, nullptr, nullptr, nullptr, M);
 }
 
 template <int WBUF>
 static void plan_fill_launch(hipStream_t st, const DevSurface &S, const DevVolume &V, const Mirror &M, double errTol,
-                             double nearT2, const DevPlan &plan, int64_t blk0, int64_t nblk) {
+                             double nearT2, double farT2, const DevPlan &plan, int64_t blk0, int64_t nblk) {
     k_plan_walk<false, WBUF><<<xcd_blocks(nblk), 256, 0, st>>>((int)V.n, (int)blk0, (int)nblk, V.vol, S.nnodes, S.whdr,
-                                                               S.nodes, errTol, nearT2, nullptr, plan.entries,
+                                                               S.nodes, errTol, nearT2, farT2, nullptr, plan.entries,
                                                                plan.waveOff, plan.waveSplit, plan.mismatch, M);
 }
 
 void launch_plan_fill(hipStream_t st, const DevSurface &S, const DevVolume &V, const Mirror &M, double errTol,
-                      double nearT2, const DevPlan &plan, const Tuning &T, int64_t blk0, int64_t nblk) {
+                      double nearT2, double farT2, const DevPlan &plan, const Tuning &T, int64_t blk0, int64_t nblk) {
     if (nblk < 0) nblk = grid_for(V.n, 256) - blk0;
     if (nblk <= 0) return;
-    if (T.wbuf == 64) plan_fill_launch<64>(st, S, V, M, errTol, nearT2, plan, blk0, nblk);
-    else if (T.wbuf == 256) plan_fill_launch<256>(st, S, V, M, errTol, nearT2, plan, blk0, nblk);
-    else plan_fill_launch<128>(st, S, V, M, errTol, nearT2, plan, blk0, nblk);
+    if (T.wbuf == 64) plan_fill_launch<64>(st, S, V, M, errTol, nearT2, farT2, plan, blk0, nblk);
+    else if (T.wbuf == 256) plan_fill_launch<256>(st, S, V, M, errTol, nearT2, farT2, plan, blk0, nblk);
+    else plan_fill_launch<128>(st, S, V, M, errTol, nearT2, farT2, plan, blk0, nblk);
 }
 
 template <int EXPM, int MQ>

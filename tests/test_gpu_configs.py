@@ -7,6 +7,8 @@ MI355X), and the library's failure paths.
     sample), and the graph reused after eager calls;
   * the 20M-node configuration's recipe at 864k nodes in mixed precision at
     errTol 1e-3 and 1e-5 (and FP64), sampled against the oracle;
+  * mixed precision far from the body (the ellipsoid O-mesh's outer field):
+    the waves there keep FP64 terms and match the oracle to the FP64 bar;
   * the 3.2M-node ellipsoid O-mesh in exact mode, FP64 and mixed, sampled
     against the oracle, and its adjoint against central differences along 20
     random unit surface directions (the reference's check,
@@ -184,6 +186,43 @@ def test_c3_recipe_mixed_errtol_sweep_against_oracle(tol):
         assert err <= (FP64_TOL if prec == "fp64" else MIXED_TOL)
         assert r <= (1e-10 if prec == "fp64" else MIXED_TOL)
         del g
+
+
+def test_mixed_far_field_waves_evaluate_in_fp64():
+    """The 10:1:1 ellipsoid O-mesh at 65x33x25 (its outer boundary 30
+    semi-major lengths out), fast mode, mixed precision, four surfaces.
+    Far from the body the terms' displacements at a node grow with the
+    distance (rotations times up to 30) while their weighted mean stays
+    small: FP32 weights and sums left errors near 1e-6 Ldef there.  A wave
+    with a node more than 2 Ldef outside the surface's bounding sphere now
+    keeps every term in FP64, so every sampled node that far out matches the
+    oracle to the FP64 bar, and the whole sample stays far inside the mixed
+    bar."""
+    m = synthetic.ellipsoid_omesh(65, 33, 25)
+    nvol = len(m["xv0"])
+    idx = np.sort(np.random.default_rng(7).choice(nvol, 3000, replace=False))
+    opts = dict(evalMode="fast", errTol=1e-4)
+    ora = _oracle(m, idx, **opts)
+    st = ora.surface_state()
+    centre = st["Xu0"].mean(0)
+    R = np.linalg.norm(st["Xu0"] - centre, axis=1).max()
+    far = np.linalg.norm(m["xv0"][idx] - centre, axis=1) > R + 2.0 * ora.Ldef0 * (1 + 1e-9)
+    assert 500 < far.sum() < len(idx) - 500
+    g = _usmesh(m, precision="mixed", **opts)
+    base = m["pts"]
+    delta = m["perturb"](base) - base
+    worst_far = worst = 0.0
+    for p in range(4):
+        xs = base + np.random.default_rng(100 + p).uniform(0.5, 1.5) * delta
+        ora.setSurfaceCoordinates(xs)
+        ref = ora.warpMesh().reshape(-1, 3)
+        g.setSurfaceCoordinates(xs)
+        g.warpMesh()
+        e = np.abs(g.getWarpGrid().reshape(-1, 3)[idx] - ref).max(1) / ora.Ldef0
+        worst_far, worst = max(worst_far, e[far].max()), max(worst, e.max())
+    print("far nodes %d: max |dXv|/Ldef %.2e; all %d: %.2e" % (far.sum(), worst_far, len(idx), worst), flush=True)
+    assert worst_far <= FP64_TOL
+    assert worst <= 0.2 * MIXED_TOL
 
 
 def test_c2_ellipsoid_exact_full_size_and_c4_finite_differences():
