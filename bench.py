@@ -144,8 +144,19 @@ def main():
     log("init (tree, error tables, denominator estimate): %.1fs" % (time.time() - t0))
     allc = mesh.getFastCounts()
     counts = allc[:4]  # V, A, L, E summed over volume nodes
+    # The reverse adds nothing for a node whose seed dXv is zero, and skips
+    # such nodes: its algorithmic work is that of the seeded nodes, whose
+    # counts come from a mesh of those nodes alone (a node's counts depend
+    # only on its position, the tree and errTol).
+    seeded = np.flatnonzero(np.abs(dxv.reshape(-1, 3)).sum(1) != 0.0)
+    ms_ = USMesh(options=dict(gridFile=m["xv0"][seeded], evalMode="fast", errTol=args.errtol, precision="fp64"))
+    ms_.setSurfaceDefinition(m["pts"], m["conn"], m["faceSizes"])
+    scounts = ms_.getFastCounts()[:4]
+    del ms_
     kflops = {"walk": float(np.dot(WALK_FLOPS, counts)), "fwd": float(np.dot(FWD_FLOPS, counts)),
-              "rev": float(np.dot(REV_FLOPS, counts))}
+              "rev": float(np.dot(REV_FLOPS, scounts)), "rev_dense": float(np.dot(REV_FLOPS, counts))}
+    log("seeded nodes %d of %d; reverse flops %.4g (dense seed %.4g)" % (len(seeded), nvol, kflops["rev"],
+                                                                          kflops["rev_dense"]))
     util = {"internal_visits": counts[0] / (64 * allc[4]), "accepts": counts[1] / (64 * allc[5]),
             "bucket_pairs": counts[2] / (64 * allc[7])}
     log("work: visits %.3g accepts %.3g leaf pairs %.3g checks %.3g; wave-level %.3g %.3g %.3g %.3g" % tuple(allc))
@@ -154,6 +165,9 @@ def main():
     dev = torch.device("cuda:0")
     d_xs = torch.tensor(xs.ravel(), dtype=torch.float64, device=dev)
     d_dxv = torch.tensor(dxv, dtype=torch.float64, device=dev)
+    # the same step with a dense seed (dXv ~ N(0,1) on every node), for the
+    # reverse's cost when no node can be skipped
+    d_dxv_dense = torch.tensor(synthetic.random_dxv(nvol, seed=5, frac=1.0), dtype=torch.float64, device=dev)
     stream = torch.cuda.current_stream(dev)
     L, ctx = mesh._L, mesh.ctx
 
@@ -161,7 +175,8 @@ def main():
         return ctypes.c_void_p(t.data_ptr())
 
     out, tuning = {}, {}
-    legs = {"both": ["fp64", "mixed", "mixed_reuse", "mixed_det"], "fp64": ["fp64"], "mixed": ["mixed"]}[args.only]
+    legs = {"both": ["fp64", "mixed", "mixed_reuse", "mixed_det", "mixed_dense"], "fp64": ["fp64"],
+            "mixed": ["mixed"]}[args.only]
 
     def set_leg(leg):
         mesh.setOption("precision", leg.split("_")[0])
@@ -173,7 +188,8 @@ def main():
     for leg in legs:
         prec = leg.split("_")[0]
         set_leg(leg)
-        if not args.no_tune and not leg.endswith("_det"):  # the deterministic leg keeps the mixed leg's choice
+        # the deterministic and dense-seed legs keep the mixed leg's choice
+        if not args.no_tune and not leg.endswith("_det") and not leg.endswith("_dense"):
             # the tile parameters for this mesh and precision, searched once
             # before the timed steps (kept for the same options)
             t0 = time.time()
@@ -184,10 +200,11 @@ def main():
         d_xv = torch.empty(3 * nvol, dtype=torch.float64, device=dev)
         d_dxs = torch.empty(3 * npts, dtype=torch.float64, device=dev)
         sp = ctypes.c_void_p(stream.cuda_stream)
+        seed_t = d_dxv_dense if leg.endswith("_dense") else d_dxv
 
         def step():
             _lib.check(L.idw_warp_mesh_dev(ctx, vp(d_xs), vp(d_xv), sp))
-            _lib.check(L.idw_warp_deriv_dev(ctx, vp(d_xs), vp(d_dxv), vp(d_dxs), sp))
+            _lib.check(L.idw_warp_deriv_dev(ctx, vp(d_xs), vp(seed_t), vp(d_dxs), sp))
 
         for _ in range(args.warmup):
             step()
@@ -230,31 +247,42 @@ def main():
         res["mixed_plan_reused_ms_per_step"] = round(out["mixed_reuse"]["ms"], 3)
     if "mixed_det" in out:
         res["mixed_deterministic_ms_per_step"] = round(out["mixed_det"]["ms"], 3)
+    if "mixed_dense" in out:
+        res["mixed_dense_seed_ms_per_step"] = round(out["mixed_dense"]["ms"], 3)
     if "fp64" in out and "mixed" in out:
         a, b = out["mixed"], out["fp64"]
         res["mixed_speedup_vs_fp64"] = round(b["ms"] / a["ms"], 3)
         res["mixed_max_err_over_Ldef"] = float((a["xv"] - b["xv"]).abs().max().item() / ldef)
         res["mixed_adjoint_rel_err"] = float(((a["dxs"] - b["dxs"]).norm() / b["dxs"].norm()).item())
-    # roofline of the dominant kernel launch of the headline leg: a warp that
-    # writes its plan runs the walk, then the forward, back to back on the
-    # context stream; the reverse is the longest of the three launches of a
-    # step, bracketed by HIP events on that stream (phase 4).
+    # roofline of the dominant kernel launch of the headline leg, the longest
+    # of the walk (phase 1), the forward (phase 2) and the reverse (phase 4),
+    # each bracketed by HIP events on the context stream.  Kernel names as
+    # rocprofv3 reports them; the template argument 0 is the exponent mode of
+    # the default exponents (3, 5), compiled as products.
     ph = head["phases"]
-    # kernel names as rocprofv3 reports them; the template argument 0 is the
-    # exponent mode of the default exponents (3, 5), compiled as products
+    chosen = tuning.get(head_prec, {}).get("chosen", {})
     if head_prec == "mixed":
-        # the kernel is instantiated per merge window, the tuned one (32 by default)
-        mq = tuning.get("mixed", {}).get("chosen", {}).get("mergeWindow", 32)
-        kname, peak = "idw::k_fast_rev_pair<0, %d>" % mq, FP32_PEAK_TFLOPS
-        compute = ("FP32 terms (the bulk): FP32 weights on VALU and the sums over the 128 volume nodes of a pair of "
-                   "walk waves on v_mfma_f32_16x16x4f32, each term shared by the pair added once; FP64 terms near "
-                   "the wall: FP64 weights and v_mfma_f64_16x16x4f64 per wave.  Priced against the 157.3 TFLOP/s "
-                   "dense FP32 roof")
+        rev = ("idw::k_fast_rev_pair<0, %d>" % chosen.get("mergeWindow", 32), FP32_PEAK_TFLOPS,
+               "FP32 terms (the bulk): FP32 weights on VALU and the sums over the seeded volume nodes of a pair of "
+               "walk waves on v_mfma_f32_16x16x4f32, each term shared by the pair added once; FP64 terms near the "
+               "wall: FP64 weights and v_mfma_f64_16x16x4f64 per wave.  Priced against the 157.3 TFLOP/s dense "
+               "FP32 roof; algorithmic work of the seeded nodes only")
+        fwd = ("idw::k_fast_fwd<1, 0, %d>" % chosen.get("forwardChunk", 32), FP32_PEAK_TFLOPS,
+               "FP32 weights on VALU and the sums over each wave's 64 volume nodes on v_mfma_f32_16x16x4f32 (FP64 "
+               "near the wall and far out).  Priced against the 157.3 TFLOP/s dense FP32 roof")
     else:
-        kname, peak = "idw::k_fast_rev<0, 0>", FP64_PEAK_TFLOPS
-        compute = ("FP64 weights on VALU, the sums over each wave's 64 volume nodes on v_mfma_f64_16x16x4f64; the "
-                   "78.6 TFLOP/s FP64 vector roof equals the FP64 matrix roof")
-    kms, kfl = ph[4], kflops["rev"]
+        rev = ("idw::k_fast_rev<0, 0>", FP64_PEAK_TFLOPS,
+               "FP64 weights on VALU, the sums over each wave's seeded volume nodes on v_mfma_f64_16x16x4f64; the "
+               "78.6 TFLOP/s FP64 vector roof equals the FP64 matrix roof; algorithmic work of the seeded nodes only")
+        fwd = ("idw::k_fast_fwd<0, 0, %d>" % chosen.get("forwardChunk", 32), FP64_PEAK_TFLOPS,
+               "FP64 weights on VALU, the sums on v_mfma_f64_16x16x4f64")
+    walk = ("idw::k_plan_walk<false, %d>" % chosen.get("walkBuffer", 128), FP64_PEAK_TFLOPS,
+            "the reference's accept/recurse tests in FP64 on VALU (distance, square root, error-table "
+            "interpolation), decisions bit for bit; priced against the 78.6 TFLOP/s FP64 vector roof")
+    cands = {1: walk + (kflops["walk"],), 2: fwd + (kflops["fwd"],), 4: rev + (kflops["rev"],)}
+    phase = max(cands, key=lambda i: ph[i])
+    kname, peak, compute, kfl = cands[phase]
+    kms = ph[phase]
     achieved = kfl / (kms * 1e-3) / 1e12
     res["roofline"] = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
                        "frac": round(achieved / peak, 4),
@@ -267,6 +295,18 @@ def main():
         res["roofline"]["traffic_detail"] = {"fetch_bytes": tr["fetch_bytes"], "write_bytes": tr["write_bytes"],
                                              "hbm_GBps": round(tr["bytes"] / (kms * 1e-3) / 1e9, 1),
                                              "source": "profiles/traffic.json"}
+    # every launch of the headline step against its roof, and the reverse with
+    # a dense seed (all nodes' work)
+    res["kernels"] = {n: {"kernel": c[0], "ms": round(float(ph[i]), 3), "algorithmic_flops": c[3],
+                          "TFLOPs": round(c[3] / (ph[i] * 1e-3) / 1e12, 3),
+                          "frac": round(c[3] / (ph[i] * 1e-3) / 1e12 / c[1], 4)}
+                      for n, i, c in (("walk", 1, cands[1]), ("fwd", 2, cands[2]), ("rev", 4, cands[4]))}
+    if "mixed_dense" in out and head_prec == "mixed":
+        dms = out["mixed_dense"]["phases"][4]
+        res["kernels"]["rev_dense_seed"] = {"kernel": rev[0], "ms": round(float(dms), 3),
+                                            "algorithmic_flops": kflops["rev_dense"],
+                                            "TFLOPs": round(kflops["rev_dense"] / (dms * 1e-3) / 1e12, 3),
+                                            "frac": round(kflops["rev_dense"] / (dms * 1e-3) / 1e12 / rev[1], 4)}
     if tuning:
         res["tuning"] = tuning
     res["phases_ms"] = {p: dict(zip(("nodal", "walk", "fwd", "rev_setup", "rev", "nodal_rev"),
@@ -284,7 +324,7 @@ def main():
         seed = torch.zeros(nvol, 3, dtype=torch.float64, device=dev)
         seed[idx] = d_dxv.view(-1, 3)[idx]
         vs = {}
-        for leg in out:
+        for leg in [k for k in out if not k.endswith("_dense")]:  # the dense-seed leg runs the mixed leg's kernels
             set_leg(leg)
             got = out[leg]["xv"].view(-1, 3)[idx].cpu().numpy()
             d_dxs = torch.empty(3 * npts, dtype=torch.float64, device=dev)

@@ -16,6 +16,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <type_traits>
 
 #include "idw_err_segment.h"
 #include "idw_internal.h"
@@ -1444,9 +1445,41 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
 
 constexpr int RT = 16;  // terms per reverse tile
 
+// A volume node whose seed s = dXv / den is zero adds nothing to the reverse
+// sums.  compact_nodes puts the wave's nodes with a nonzero seed first, in
+// lane order, with their lane index in the low word of w (the bit of the
+// plan masks that serves them), and pads them to a multiple of four with
+// rows that add exactly zero: seed 0 at position (1e30, 1e30, 1e30), whose
+// weights are finite (FP64) or 0 (FP32).  Returns the number of node quads
+// the reverse sums over (0 when every seed is zero).  cs: the compacted
+// seeds.
+__device__ __forceinline__ int compact_nodes(bool valid, const double4 &v, const double (&sd)[3], double4 *cnode,
+                                             double (*cs)[3]) {
+    const int lane = threadIdx.x & 63;
+    const bool nz = valid && (sd[0] != 0.0 || sd[1] != 0.0 || sd[2] != 0.0);
+    const unsigned long long mnz = __ballot(nz);
+    const int cnt = __popcll(mnz), cnt4 = (cnt + 3) & ~3;
+    if (nz) {
+        const int at = __popcll(mnz & ((1ull << lane) - 1ull));
+        cnode[at] = make_double4(v.x, v.y, v.z, __hiloint2double(0, lane));
+        for (int c = 0; c < 3; ++c) cs[at][c] = sd[c];
+    }
+    if (lane >= cnt && lane < cnt4) {
+        cnode[lane] = make_double4(1e30, 1e30, 1e30, __hiloint2double(0, 0));
+        for (int c = 0; c < 3; ++c) cs[lane][c] = 0.0;
+    }
+    return uniform(cnt4 >> 2);
+}
+
+// This lane's bit of a plan mask for a compacted node row (its lane index in
+// the low word of w).
+__device__ __forceinline__ bool serves(unsigned long long mask, const double4 &c) {
+    return (mask >> (__double2loint(c.w) & 63)) & 1ull;
+}
+
 struct RevLds {
-    double4 node[64];    // mirrored positions of the wave's nodes
-    double sg[64][6];    // seeds s (3) and g = v - o (3): B-operand set-up
+    double4 node[64];    // mirrored positions of the wave's nodes with a nonzero seed (compact_nodes)
+    double s[64][3];     // their seeds
     double Y[RT][4];     // x - o of the tile's terms
     int32_t idx[RT];     // their records (-1: padding)
 };
@@ -1455,8 +1488,9 @@ struct RevLds {
 // P(term, :) = sum over the wave's nodes of w(node, term) G(node, :) with
 // G = [s (x) g (9), s (3)] (B in registers), then
 // Mb'(a, c) += P(3a + c) - P(9 + a) (x - o)_c and Bb(a) += P(9 + a) into
-// the term's adjoint record.  Lane l computes the weight of term l%16 with
-// node 4q + l/16 for node group q, in FP64.  The next tile's record and the
+// the term's adjoint record.  nodes: the nq quads of compacted node rows
+// (compact_nodes); B[q] their G rows.  Lane l computes the weight of term
+// l%16 with node row 4q + l/16, in FP64.  The next tile's record and the
 // entries of the tile after it are requested before this tile's sums and are
 // in registers before its atomics are issued.
 // With part (deterministic reverse) the 12 sums of the term at plan slot e
@@ -1464,7 +1498,7 @@ struct RevLds {
 template <int EXPM>
 __device__ __forceinline__ void rev_terms(int64_t a, int64_t b, const PlanEntry *__restrict__ plan, int nnodes,
                                           int nrec, const Rec64 *__restrict__ rec, const WParams &P,
-                                          const double4 *nodes, double (*Y)[4], int32_t *idxs,
+                                          const double4 *nodes, int nq, double (*Y)[4], int32_t *idxs,
                                           const double (&B)[16], double ox, double oy, double oz,
                                           double *__restrict__ adj, double *__restrict__ part = nullptr) {
     const int lane = threadIdx.x & 63, m = lane & 15, R = lane >> 4;
@@ -1489,20 +1523,28 @@ __device__ __forceinline__ void rev_terms(int64_t a, int64_t b, const PlanEntry 
             Y[m][2] = xa.z - oz;
             idxs[m] = idx;
         }
-        const unsigned long long mR = pe.mask >> R;
         const double2 f = weight_factors<EXPM>(xa.w, P);
         dvec4 D0 = {0.0, 0.0, 0.0, 0.0}, D1 = D0;
+        auto sums = [&](auto dense) {
+            // dense: all 64 seeds nonzero, the rows are the wave's nodes in
+            // lane order and row 4q + R is bit 4q + R of the masks
+            const unsigned long long mR = pe.mask >> R;
 #pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const unsigned long long on = __ballot((mR >> (4 * q)) & 1ull);
-            if (!on) continue;  // nodes 4q..4q+3 idle for all the tile's terms
-            const double4 c = nodes[4 * q + R];
-            const double rx = c.x - xa.x, ry = c.y - xa.y, rz = c.z - xa.z;
-            double w = weight_f64<EXPM>(rx * rx + ry * ry + rz * rz + eps, f, P);
-            w = lane_in(on) ? w : 0.0;
-            if (q & 1) D1 = __builtin_amdgcn_mfma_f64_16x16x4f64(w, B[q], D1, 0, 0, 0);
-            else D0 = __builtin_amdgcn_mfma_f64_16x16x4f64(w, B[q], D0, 0, 0, 0);
-        }
+            for (int q = 0; q < 16; ++q) {
+                if (!dense && q >= nq) continue;  // rows past 4 nq have zero seeds
+                const double4 c = nodes[4 * q + R];
+                const bool bit = dense ? ((mR >> (4 * q)) & 1ull) != 0 : serves(pe.mask, c);
+                const unsigned long long on = __ballot(bit);
+                if (!on) continue;  // node rows 4q..4q+3 idle for all the tile's terms
+                const double rx = c.x - xa.x, ry = c.y - xa.y, rz = c.z - xa.z;
+                double w = weight_f64<EXPM>(rx * rx + ry * ry + rz * rz + eps, f, P);
+                w = bit ? w : 0.0;
+                if (q & 1) D1 = __builtin_amdgcn_mfma_f64_16x16x4f64(w, B[q], D1, 0, 0, 0);
+                else D0 = __builtin_amdgcn_mfma_f64_16x16x4f64(w, B[q], D0, 0, 0, 0);
+            }
+        };
+        if (nq == 16) sums(std::true_type{});
+        else sums(std::false_type{});
         const dvec4 Ds = D0 + D1;
         wait_vmem();  // the prefetches are in before this tile's atomics go out
         wave_sync();  // Y / idx of this tile are written
@@ -1566,24 +1608,23 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     // wave origin: lane 0 is valid whenever any lane of the wave is
     const double ox = first_lane(v.x), oy = first_lane(v.y), oz = first_lane(v.z);
     RevLds &L = lds[wv];
-    L.node[lane] = make_double4(v.x, v.y, v.z, 0.0);
-    L.sg[lane][0] = s0;
-    L.sg[lane][1] = s1;
-    L.sg[lane][2] = s2;
-    L.sg[lane][3] = v.x - ox;
-    L.sg[lane][4] = v.y - oy;
-    L.sg[lane][5] = v.z - oz;
+    const double sd[3] = {s0, s1, s2};
+    const int nq = compact_nodes(valid, v, sd, L.node, L.s);
+    // no seed in the wave: nothing to add (the deterministic reverse still
+    // writes its slots' zeros)
+    if (nq == 0 && !part) return;
     wave_sync();
-    // B operand of node group q: G(node 4q + R, column n)
-    const int sa = n < 9 ? n / 3 : (n < 12 ? n - 9 : 0), gb = 3 + n % 3;
+    // B operand of node row group q: G(row 4q + R, column n)
+    const int sa = n < 9 ? n / 3 : (n < 12 ? n - 9 : 0), gb = n % 3;
+    const double og = gb == 0 ? ox : (gb == 1 ? oy : oz);
     double B[16];
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
-        const double sv = L.sg[4 * q + R][sa], gv = L.sg[4 * q + R][gb];
-        B[q] = n < 9 ? sv * gv : (n < 12 ? sv : 0.0);
+        const double sv = L.s[4 * q + R][sa], gv = reinterpret_cast<const double *>(&L.node[4 * q + R])[gb] - og;
+        B[q] = q < nq ? (n < 9 ? sv * gv : (n < 12 ? sv : 0.0)) : 0.0;
     }
     const int64_t e0 = waveOff[wave], e1 = waveOff[wave + 1];
-    rev_terms<EXPM>(e0, e1, plan, nnodes, nrec, rec, P, L.node, L.Y, L.idx, B, ox, oy, oz, adj, part);
+    rev_terms<EXPM>(e0, e1, plan, nnodes, nrec, rec, P, L.node, nq, L.Y, L.idx, B, ox, oy, oz, adj, part);
 }
 
 // ---------------------------------------------------------------------------
@@ -1705,7 +1746,7 @@ struct PairMergeLds {
 
 template <int MQ>
 struct PairLds {
-    double4 node[128];  // mirrored positions: wave A's nodes, then wave B's
+    double4 node[128];  // mirrored positions of the nodes with a nonzero seed: wave A's at 0, wave B's at 64
     union {
         PairMergeLds<MQ> q;  // during the merge
         double s[128][3];    // seeds, while the B operands are built
@@ -1772,20 +1813,28 @@ k_fast_rev_pair(int nvol, int nwave, const double4 *__restrict__ vol, const int3
         }
         v.w = 0.0;
     };
-    double vx0;
+    // each wave's nodes with a nonzero seed, compacted: rows 64 h + i of the
+    // node array, nqA / nqB quads of rows (the rest add nothing)
+    double ox, oy, oz;
+    int nq[2];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         double4 v;
         double sd[3];
-        node_seed(pair * 128 + 64 * h + lane, v, sd);
-        if (h == 0) vx0 = v.x;
-        L.node[64 * h + lane] = v;
-        for (int c = 0; c < 3; ++c) L.s[64 * h + lane][c] = sd[c];
+        const int k = pair * 128 + 64 * h + lane;
+        node_seed(k, v, sd);
+        if (h == 0) {
+            // pair origin: lane 0's node of wave A (valid whenever any node of the pair is)
+            ox = first_lane(v.x);
+            oy = first_lane(v.y);
+            oz = first_lane(v.z);
+        }
+        nq[h] = compact_nodes(k < nvol, v, sd, L.node + 64 * h, L.s + 64 * h);
     }
-    // pair origin: lane 0's node of wave A (valid whenever any node of the pair is)
-    const double ox = first_lane(vx0), oy = first_lane(L.node[lane].y), oz = first_lane(L.node[lane].z);
+    const int nqA = nq[0], nqB = nq[1];
+    if (nqA == 0 && nqB == 0) return;  // no seed in the pair: nothing to add
     wave_sync();
-    // G(node nd, column m) = s_a g_b (9), s_a (3), 0 with g = v - o, from LDS
+    // G(row nd, column m) = s_a g_b (9), s_a (3), 0 with g = v - o, from LDS
     const int sa = m < 9 ? m / 3 : (m < 12 ? m - 9 : 0), gb = m % 3;
     const double og = gb == 0 ? ox : (gb == 1 ? oy : oz);
     auto g_entry = [&](int nd) {
@@ -1796,8 +1845,8 @@ k_fast_rev_pair(int nvol, int nwave, const double4 *__restrict__ vol, const int3
     float GA[16], GB[16];
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
-        GA[q] = (float)g_entry(4 * q + R);
-        GB[q] = (float)g_entry(64 + 4 * q + R);
+        GA[q] = q < nqA ? (float)g_entry(4 * q + R) : 0.0f;
+        GB[q] = q < nqB ? (float)g_entry(64 + 4 * q + R) : 0.0f;
         __builtin_amdgcn_sched_barrier(0);
     }
     const float Lf = (float)P.Ldef;
@@ -1966,43 +2015,56 @@ k_fast_rev_pair(int nvol, int nwave, const double4 *__restrict__ vol, const int3
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             if (h == 0 ? T.kind == 2 : T.kind == 1) continue;  // wave-uniform: the half has no lanes
-            const unsigned long long mR = (h == 0 ? T.mA : T.mB) >> R;
+            const int nqh = h == 0 ? nqA : nqB;
+            const unsigned long long mh = h == 0 ? T.mA : T.mB;
+            auto sums = [&](auto dense) {
+                // dense: all 64 seeds nonzero, the rows are the wave's nodes
+                // in lane order and row 4q + R is bit 4q + R of the masks
+                const unsigned long long mR = mh >> R;
 #pragma unroll
-            for (int q = 0; q < 16; ++q) {
-                const double4 c = L.node[64 * h + 4 * q + R];
-                const bool on = (mR >> (4 * q)) & 1ull;
-                const double rx = c.x - T.xa.x, ry = c.y - T.xa.y, rz = c.z - T.xa.z;
-                const float wf =
-                    weight_f32<EXPM>((float)fma(rz, rz, fma(ry, ry, fma(rx, rx, epsT))), f, Lf, P);
-                const float w = on ? wf : 0.0f;
-                const float g = h == 0 ? GA[q] : GB[q];
-                if (q & 1) F1 = __builtin_amdgcn_mfma_f32_16x16x4f32(w, g, F1, 0, 0, 0);
-                else F0 = __builtin_amdgcn_mfma_f32_16x16x4f32(w, g, F0, 0, 0, 0);
-            }
+                for (int q = 0; q < 16; ++q) {
+                    if (dense || q < nqh) {  // rows past 4 nqh have zero seeds
+                        const double4 c = L.node[64 * h + 4 * q + R];
+                        const bool on = dense ? ((mR >> (4 * q)) & 1ull) != 0 : serves(mh, c);
+                        const double rx = c.x - T.xa.x, ry = c.y - T.xa.y, rz = c.z - T.xa.z;
+                        const float wf =
+                            weight_f32<EXPM>((float)fma(rz, rz, fma(ry, ry, fma(rx, rx, epsT))), f, Lf, P);
+                        const float w = on ? wf : 0.0f;
+                        const float g = h == 0 ? GA[q] : GB[q];
+                        if (q & 1) F1 = __builtin_amdgcn_mfma_f32_16x16x4f32(w, g, F1, 0, 0, 0);
+                        else F0 = __builtin_amdgcn_mfma_f32_16x16x4f32(w, g, F0, 0, 0, 0);
+                    }
+                }
+            };
+            if (nqh == 16) sums(std::true_type{});
+            else sums(std::false_type{});
         }
     }
-    // the FP64 terms, per wave, with FP64 G (the merge buffers are free again)
+    // the FP64 terms, per wave, with FP64 G (the merge buffers are free
+    // again: the compacted seeds are written once more, to the same rows)
     if (asp >= a1 && bsp >= b1) return;
     wave_sync();
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         double4 v;
         double sd[3];
-        node_seed(pair * 128 + 64 * h + lane, v, sd);
-        for (int c = 0; c < 3; ++c) L.s[64 * h + lane][c] = sd[c];
+        const int k = pair * 128 + 64 * h + lane;
+        node_seed(k, v, sd);
+        compact_nodes(k < nvol, v, sd, L.node + 64 * h, L.s + 64 * h);
     }
     wave_sync();
 #pragma unroll 1
     for (int h = 0; h < 2; ++h) {
         const int64_t e0 = h == 0 ? asp : bsp, e1 = h == 0 ? a1 : b1;
-        if (e0 >= e1) continue;
+        const int nqh = h == 0 ? nqA : nqB;
+        if (e0 >= e1 || nqh == 0) continue;
         double B[16];
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
-            B[q] = g_entry(64 * h + 4 * q + R);
+            B[q] = q < nqh ? g_entry(64 * h + 4 * q + R) : 0.0;
             __builtin_amdgcn_sched_barrier(0);
         }
-        rev_terms<EXPM>(e0, e1, plan, nnodes, nrec, rec, P, L.node + 64 * h, L.Y, L.idx, B, ox, oy, oz, adj);
+        rev_terms<EXPM>(e0, e1, plan, nnodes, nrec, rec, P, L.node + 64 * h, nqh, L.Y, L.idx, B, ox, oy, oz, adj);
     }
 }
 
