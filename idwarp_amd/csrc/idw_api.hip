@@ -324,6 +324,7 @@ void drop_plan(idw_ctx *c) {
         dfree(c, P.waveOff);
         dfree(c, P.waveSplit);
         dfree(c, P.waveCount);
+        dfree(c, P.groupSparse);
         dfree(c, P.mismatch);
     }
     c->plans.assign(nloop(c), idw::DevPlan());
@@ -406,10 +407,12 @@ int size_one_plan(idw_ctx *c, int kk) {
         dfree(c, P.waveOff);
         dfree(c, P.waveSplit);
         dfree(c, P.waveCount);
+        dfree(c, P.groupSparse);
         if (!P.mismatch) TRY(dalloc(c, &P.mismatch, 1));
         TRY(dalloc(c, &P.waveOff, nwave + 1));
         TRY(dalloc(c, &P.waveSplit, nwave));
         TRY(dalloc(c, &P.waveCount, nwave + 1));
+        TRY(dalloc(c, &P.groupSparse, nwave));
         P.nwave = nwave;
     }
     HIPCHK(hipMemsetAsync(P.waveCount, 0, sizeof(int64_t) * (nwave + 1), c->st));

@@ -93,6 +93,9 @@ struct DevPlan {
     int64_t *waveSplit = nullptr;
     int64_t *waveCount = nullptr;
     unsigned int *mismatch = nullptr;
+    // per group of walk waves of the mixed reverse: 1 when its seeds were
+    // sparse and the group reverse took it (written by each reverse)
+    uint8_t *groupSparse = nullptr;
     // deterministic reverse: the plan's slots sorted by record (detPos) and
     // each record's first sorted position (recStart, nrec + 1)
     int32_t *detPos = nullptr;

@@ -1444,6 +1444,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
 }
 
 constexpr int RT = 16;  // terms per reverse tile
+constexpr int GROUP_NS = 4;  // walk waves per group of the sparse-seed reverse (k_fast_rev_group)
 
 // A volume node whose seed s = dXv / den is zero adds nothing to the reverse
 // sums.  compact_nodes puts the wave's nodes with a nonzero seed first, in
@@ -1784,7 +1785,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 k_fast_rev_pair(int nvol, int nwave, const double4 *__restrict__ vol, const int32_t *__restrict__ perm,
                 const double *__restrict__ dxv, const double *__restrict__ den, const PlanEntry *__restrict__ plan,
                 const int64_t *__restrict__ waveOff, const int64_t *__restrict__ waveSplit, int nnodes, int nrec,
-                const Rec64 *__restrict__ rec, WParams P, Mirror M, double *__restrict__ adj) {
+                const Rec64 *__restrict__ rec, WParams P, Mirror M, double *__restrict__ adj,
+                const uint8_t *__restrict__ groupSparse) {
     static_assert(MQ == 8 || MQ == 16 || MQ == 32, "both merge buffers are filled by one load per lane");
     __shared__ PairLds<MQ> lds[4];
     const int npair = (nwave + 1) >> 1;
@@ -1794,6 +1796,8 @@ k_fast_rev_pair(int nvol, int nwave, const double4 *__restrict__ vol, const int3
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, m = lane & 15, R = lane >> 4;
     const int pair = uniform(blk * 4 + wv);
     if (pair >= npair) return;  // wave-uniform
+    // a group of walk waves with sparse seeds is the group reverse's
+    if (groupSparse && groupSparse[pair / (GROUP_NS / 2)]) return;
     PairLds<MQ> &L = lds[wv];
     // the pair's nodes: lane j holds node j of wave A and node j of wave B
     auto node_seed = [&](int k, double4 &v, double (&sd)[3]) {  // mirrored position and seed s = dXv / den
@@ -2065,6 +2069,314 @@ k_fast_rev_pair(int nvol, int nwave, const double4 *__restrict__ vol, const int3
             __builtin_amdgcn_sched_barrier(0);
         }
         rev_terms<EXPM>(e0, e1, plan, nnodes, nrec, rec, P, L.node + 64 * h, nqh, L.Y, L.idx, B, ox, oy, oz, adj);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Reverse over groups of walk waves with sparse seeds (mixed precision)
+// ---------------------------------------------------------------------------
+// With few seeded nodes the pair reverse's time is its memory-side FP64
+// atomics: one set of 12 per term of a pair's union, whatever the number of
+// seeded nodes behind it.  A record appears in 1.71 of two neighbouring walk
+// waves' plans, 2.88 of four and 8.0 of sixteen, so one set per term of the
+// union of NS > 2 walk waves takes fewer atomics.  One wave merges the NS
+// depth-first FP32 ranges on the fly (64 / NS lanes and buffered entries per
+// range), queues each union term once with its NS lane masks, and sums a
+// tile of 16 union terms over the seeded node quads of every walk wave that
+// uses it.  A group goes here when at most 12 of each 64 of its nodes carry
+// a seed, and the rows then fit a fixed LDS budget; this kernel, launched
+// first, flags the groups it takes and the pair reverse leaves them.
+
+// A seed: a nonzero component of dXv (node k in traversal order).
+__device__ __forceinline__ bool seeded(int nvol, int64_t k, const int32_t *__restrict__ perm,
+                                       const double *__restrict__ dxv) {
+    if (k >= nvol) return false;
+    const int64_t j = perm[k];
+    return dxv[3 * j] != 0.0 || dxv[3 * j + 1] != 0.0 || dxv[3 * j + 2] != 0.0;
+}
+
+template <int NS>
+struct GroupLds {
+    static constexpr int MQ = 64 / NS;             // buffered plan entries per walk wave
+    static constexpr int MAXS = 12 * NS;           // seeded nodes of a sparse group
+    static constexpr int ROWS = MAXS + 3 * NS;     // their rows, each walk wave's padded to quads
+    static constexpr int QCAP = RT + 64;           // union queue: < RT items, then at most 64 per merge round
+    double4 node[ROWS];    // mirrored positions (w: the node's lane in its walk wave, low word)
+    float G[ROWS][12];     // G rows in FP32: s (x) g (9), s (3) with g = v - o; columns 12-15 are 0
+    unsigned long long key[NS][MQ], msk[NS][MQ];
+    int32_t qrec[QCAP];
+    unsigned long long qm[NS][QCAP];
+    double Y[RT][4];  // x - o of the tile's terms
+    int32_t idx[RT];  // their records (-1: padding)
+    int32_t nq[NS], base[NS];  // each walk wave's node quads and first row
+};
+
+template <int EXPM, int NS>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_fast_rev_group(int nvol, int nwave, const double4 *__restrict__ vol,
+                                                       const int32_t *__restrict__ perm, const double *__restrict__ dxv,
+                                                       const double *__restrict__ den,
+                                                       const PlanEntry *__restrict__ plan,
+                                                       const int64_t *__restrict__ waveOff,
+                                                       const int64_t *__restrict__ waveSplit, int nnodes, int nrec,
+                                                       const Rec64 *__restrict__ rec, WParams P, Mirror M,
+                                                       double *__restrict__ adj, uint8_t *__restrict__ groupSparse) {
+    static_assert(NS == 2 || NS == 4 || NS == 8, "the lanes split evenly over the walk waves");
+    using Lds = GroupLds<NS>;
+    constexpr int MQ = Lds::MQ;
+    __shared__ Lds L;
+    const int ngroup = (nwave + NS - 1) / NS;
+    const int grp = xcd_block(ngroup);
+    if (grp >= ngroup) return;
+    const int lane = threadIdx.x & 63, m = lane & 15, R = lane >> 4;
+    const int w0 = grp * NS;
+    const int64_t node0 = (int64_t)w0 * 64;
+    // the group's seeded nodes per walk wave
+    unsigned long long nzm[NS];
+    int tot = 0;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        nzm[s] = __ballot(seeded(nvol, node0 + 64 * s + lane, perm, dxv));
+        tot += __popcll(nzm[s]);
+    }
+    const bool sparse = tot <= Lds::MAXS;
+    if (lane == 0) groupSparse[grp] = sparse ? 1 : 0;
+    if (!sparse || tot == 0) return;  // the pair reverse takes a dense group; no seed: nothing to add
+    // origin: the group's first node (valid: some node of the group is)
+    double4 o = vol[node0];
+    if (M.mask) mirror_pt(M, o.x, o.y, o.z);
+    const double ox = first_lane(o.x), oy = first_lane(o.y), oz = first_lane(o.z);
+    // rows: walk wave s's seeded nodes at base[s] .. base[s] + 4 nq[s], in lane order, padded with rows
+    // that add exactly zero (seed 0 at 1e30)
+    int nq[NS], base[NS];
+    int at = 0;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        const int cnt = __popcll(nzm[s]);
+        nq[s] = (cnt + 3) >> 2;
+        base[s] = at;
+        const int64_t k = node0 + 64 * s + lane;
+        if ((nzm[s] >> lane) & 1ull) {
+            double4 v = vol[k];
+            const int64_t j = perm[k];
+            const double oden = 1.0 / den[k];
+            double sd[3] = {dxv[3 * j] * oden, dxv[3 * j + 1] * oden, dxv[3 * j + 2] * oden};
+            if (M.mask) {
+                mirror_pt(M, v.x, v.y, v.z);
+                mirror_vec_b(M, sd[0], sd[1], sd[2]);
+            }
+            const int row = at + __popcll(nzm[s] & ((1ull << lane) - 1ull));
+            L.node[row] = make_double4(v.x, v.y, v.z, __hiloint2double(0, lane));
+            const double g[3] = {v.x - ox, v.y - oy, v.z - oz};
+            float *Gr = L.G[row];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+#pragma unroll
+                for (int b = 0; b < 3; ++b) Gr[3 * a + b] = (float)(sd[a] * g[b]);
+                Gr[9 + a] = (float)sd[a];
+            }
+        }
+        if (lane >= cnt && lane < 4 * nq[s]) {
+            const int row = at + lane;
+            L.node[row] = make_double4(1e30, 1e30, 1e30, __hiloint2double(0, 0));
+#pragma unroll
+            for (int c = 0; c < 12; ++c) L.G[row][c] = 0.0f;
+        }
+        at += 4 * nq[s];
+    }
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+        if (lane == s) {
+            L.nq[s] = nq[s];
+            L.base[s] = base[s];
+        }
+    // merge state of walk wave sid = lane / MQ, held by its MQ lanes: next
+    // unread entry ia, end ea of its FP32 range, entries buffered na
+    const int sid = lane / MQ, i = lane % MQ;
+    const int ws = w0 + sid;
+    int64_t ia = 0, ea = 0;
+    if (ws < nwave) {
+        ia = waveOff[ws];
+        ea = min(max(waveSplit[ws], ia), waveOff[ws + 1]);
+    }
+    int na = 0, qn = 0;
+    PlanEntry pre{};
+    if (ia + i < ea) pre = plan[ia + i];
+    auto record = [&](int r) {  // x, A of a record (zero for padding); unconditional load
+        const double4 v = *reinterpret_cast<const double4 *>(rec + min(max(r, 0), nrec - 1));
+        return r >= 0 ? v : make_double4(0.0, 0.0, 0.0, 0.0);
+    };
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const unsigned long long mine = (MQ == 64 ? ~0ull : ((1ull << MQ) - 1ull)) << (sid * MQ);
+    const float Lf = (float)P.Ldef;
+    const int src9 = (lane & ~15) | (9 + (m < 9 ? m / 3 : 0));
+    wave_sync();
+    // tile state: record, x and A, and the NS masks of term m
+    int trec = -1, nrec_t = -1;
+    double4 txa = make_double4(0.0, 0.0, 0.0, 0.0), nxa = txa;
+    unsigned long long tm[NS], nm[NS];
+    bool have = false;
+    fvec4 F0 = {0.0f, 0.0f, 0.0f, 0.0f}, F1 = F0;
+    for (;;) {
+        // next tile: a full queue, or at the end what is left, merging as needed
+        bool next = false;
+        for (;;) {
+            const bool done = __ballot(ia < ea || na > 0) == 0ull;
+            if (qn >= RT || (done && qn > 0)) {
+                const int cnt = min(qn, RT), b = qn - cnt;
+                const bool ok = m < cnt;
+                nrec_t = ok ? L.qrec[b + m] : -1;
+#pragma unroll
+                for (int t = 0; t < NS; ++t) nm[t] = ok ? L.qm[t][b + m] : 0ull;
+                qn = uniform(b);
+                nxa = record(nrec_t);
+                next = true;
+                break;
+            }
+            if (done) break;
+            // One merge round: top up every buffer to MQ entries, then move every entry whose key
+            // is at most K = min(last key loaded of each range not read to its end) to the queue,
+            // once per distinct key with the masks of every range holding it.  No key past K can
+            // precede one still unread, and at least one buffer empties, so the merge ends.
+            const int la = (int)min((int64_t)(MQ - na), ea - ia);
+            wave_sync();
+            if (i < la) {
+                L.key[sid][na + i] = plan_key(pre);
+                L.msk[sid][na + i] = pre.mask;
+            }
+            ia += la;
+            na += la;
+            pre = PlanEntry{};
+            if (ia + i < ea) pre = plan[ia + i];
+            wave_sync();
+            const unsigned long long last = ia < ea ? L.key[sid][na - 1] : ~0ull;
+            unsigned long long K = ~0ull;
+            int n[NS];
+#pragma unroll
+            for (int t = 0; t < NS; ++t) {
+                const unsigned lo = __builtin_amdgcn_readlane((unsigned)last, t * MQ);
+                const unsigned hi = __builtin_amdgcn_readlane((unsigned)(last >> 32), t * MQ);
+                const unsigned long long kt = ((unsigned long long)hi << 32) | lo;
+                K = kt < K ? kt : K;
+                n[t] = __builtin_amdgcn_readlane(na, t * MQ);
+            }
+            const bool hv = i < na;
+            const unsigned long long k = hv ? L.key[sid][i] : ~0ull;
+            const unsigned long long mk = hv ? L.msk[sid][i] : 0ull;
+            const bool emit = hv && k <= K;
+            // the same key in the other buffers (a key is unique within a range)
+            unsigned long long mt[NS];
+            bool earlier = false;  // a lower-numbered range holds the key: it queues it
+#pragma unroll
+            for (int t = 0; t < NS; ++t) {
+                const int c = count_below(L.key[t], n[t], k);
+                const bool twin = emit && t != sid && c < n[t] && L.key[t][c] == k;
+                mt[t] = t == sid ? mk : (twin ? L.msk[t][c] : 0ull);
+                earlier = earlier || (twin && t < sid);
+            }
+            const bool owner = emit && !earlier;
+            const unsigned long long mo = __ballot(owner);
+            if (owner) {
+                const int slot = qn + __popcll(mo & below);
+                L.qrec[slot] = (int32_t)(unsigned)k;
+#pragma unroll
+                for (int t = 0; t < NS; ++t) L.qm[t][slot] = mt[t];
+            }
+            qn = uniform(qn + __popcll(mo));
+            // the consumed entries are a prefix of each buffer: move the rest to the front
+            const int e_own = __popcll(__ballot(emit) & mine);
+            const bool mv = i + e_own < na;
+            const unsigned long long k2 = mv ? L.key[sid][i + e_own] : 0ull;
+            const unsigned long long m2 = mv ? L.msk[sid][i + e_own] : 0ull;
+            wave_sync();
+            if (mv) {
+                L.key[sid][i] = k2;
+                L.msk[sid][i] = m2;
+            }
+            na -= e_own;
+        }
+        if (have) {
+            wave_sync();  // Y / idx of this tile are written
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int tr = 4 * R + t;  // f32 layout: register t holds term row 4 (l/16) + t
+                double val = (double)F0[t] + (double)F1[t];
+                const double pb = __shfl(val, src9, 64);
+                if (m < 9) val -= pb * L.Y[tr][m % 3];
+                const int id = L.idx[tr];
+                if (id >= 0 && m < 12 && val != 0.0) atomicAdd(adj + (int64_t)id * ADJ_STRIDE + m, val);
+            }
+        }
+        if (!next) break;
+        have = true;
+        trec = nrec_t;
+        txa = nxa;
+#pragma unroll
+        for (int t = 0; t < NS; ++t) tm[t] = nm[t];
+        const int idx = trec >= 0 ? min(trec, nrec - 1) : -1;
+        wave_sync();  // the previous tile's reads of Y / idx are done
+        if (R == 0) {
+            L.Y[m][0] = txa.x - ox;
+            L.Y[m][1] = txa.y - oy;
+            L.Y[m][2] = txa.z - oz;
+            L.idx[m] = idx;
+        }
+        const double2 fd = weight_factors<EXPM>(txa.w, P);
+        const float2 f = make_float2((float)fd.x, (float)fd.y);
+        const double epsT = trec >= nnodes ? EPS_W : 0.0;  // points carry evalNodeExact's eps, clusters none
+        F0 = fvec4{0.0f, 0.0f, 0.0f, 0.0f};
+        F1 = F0;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            // walk wave s: its seeded node quads, when a term of the tile serves it
+            if (nq[s] == 0 || __ballot(tm[s] != 0ull) == 0ull) continue;
+            const unsigned long long ms = tm[s];
+#pragma unroll 2
+            for (int q = 0; q < nq[s]; ++q) {
+                const int row = base[s] + 4 * q + R;
+                const double4 c = L.node[row];
+                const bool on = serves(ms, c);
+                const double rx = c.x - txa.x, ry = c.y - txa.y, rz = c.z - txa.z;
+                const float wf = weight_f32<EXPM>((float)fma(rz, rz, fma(ry, ry, fma(rx, rx, epsT))), f, Lf, P);
+                const float w = on ? wf : 0.0f;
+                const float g = m < 12 ? L.G[row][m] : 0.0f;
+                if (q & 1) F1 = __builtin_amdgcn_mfma_f32_16x16x4f32(w, g, F1, 0, 0, 0);
+                else F0 = __builtin_amdgcn_mfma_f32_16x16x4f32(w, g, F0, 0, 0, 0);
+            }
+        }
+    }
+    // the FP64 terms, per walk wave, with FP64 G rows (their seeds read
+    // again: a padding row's lane holds no seeded node)
+    const int sa = m < 9 ? m / 3 : (m < 12 ? m - 9 : 0), gb = m % 3;
+    const double og = gb == 0 ? ox : (gb == 1 ? oy : oz);
+#pragma unroll 1
+    for (int s = 0; s < NS; ++s) {
+        const int w = w0 + s, nqs = uniform(L.nq[s]), bs = uniform(L.base[s]);
+        if (w >= nwave || nqs == 0) continue;
+        const int64_t e1 = waveOff[w + 1], e0 = min(max(waveSplit[w], waveOff[w]), e1);
+        if (e0 >= e1) continue;
+        const unsigned long long nzs = __ballot(seeded(nvol, node0 + 64 * s + lane, perm, dxv));
+        double B[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            double gv = 0.0;
+            if (q < nqs) {
+                const int row = bs + 4 * q + R;
+                const double4 c = L.node[row];
+                const int ln = __double2loint(c.w) & 63;
+                if ((nzs >> ln) & 1ull && 4 * q + R < __popcll(nzs)) {
+                    const int64_t kk = node0 + 64 * s + ln;
+                    const int64_t j = perm[kk];
+                    const double oden = 1.0 / den[kk];
+                    double sd[3] = {dxv[3 * j] * oden, dxv[3 * j + 1] * oden, dxv[3 * j + 2] * oden};
+                    if (M.mask) mirror_vec_b(M, sd[0], sd[1], sd[2]);
+                    const double sv = sa == 0 ? sd[0] : (sa == 1 ? sd[1] : sd[2]);
+                    const double pv = (gb == 0 ? c.x : (gb == 1 ? c.y : c.z)) - og;
+                    gv = m < 9 ? sv * pv : (m < 12 ? sv : 0.0);
+                }
+            }
+            B[q] = gv;
+        }
+        rev_terms<EXPM>(e0, e1, plan, nnodes, nrec, rec, P, L.node + bs, nqs, L.Y, L.idx, B, ox, oy, oz, adj);
     }
 }
 
@@ -2648,12 +2960,19 @@ void launch_plan_fill(hipStream_t st, const DevSurface &S, const DevVolume &V, c
     else plan_fill_launch<128>(st, S, V, M, errTol, nearT2, farT2, plan, blk0, nblk);
 }
 
+// Mixed-precision fast reverse: the groups of GROUP_NS walk waves with
+// sparse seeds in k_fast_rev_group (which flags them), then the other
+// groups' pairs of walk waves in k_fast_rev_pair.
 template <int EXPM, int MQ>
 static void rev_pair_launch(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, const Mirror &M,
                             const double *dxv, const DevPlan *plan) {
+    const int64_t ngroup = (plan->nwave + GROUP_NS - 1) / GROUP_NS;
+    k_fast_rev_group<EXPM, GROUP_NS><<<xcd_blocks(ngroup), 64, 0, st>>>(
+        (int)V.n, (int)plan->nwave, V.vol, V.perm, dxv, V.den, plan->entries, plan->waveOff, plan->waveSplit, S.nnodes,
+        S.nnodes + S.n, S.rec, P, M, S.adj, plan->groupSparse);
     k_fast_rev_pair<EXPM, MQ><<<xcd_blocks(((plan->nwave + 1) / 2 + 3) / 4), 256, 0, st>>>(
         (int)V.n, (int)plan->nwave, V.vol, V.perm, dxv, V.den, plan->entries, plan->waveOff, plan->waveSplit, S.nnodes,
-        S.nnodes + S.n, S.rec, P, M, S.adj);
+        S.nnodes + S.n, S.rec, P, M, S.adj, plan->groupSparse);
 }
 
 template <int EXPM>

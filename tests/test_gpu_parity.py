@@ -706,6 +706,58 @@ def test_mixed_adjoint_of_wave_pairs(planes):
     assert abs(dxv @ jp - jtv @ p) <= 1e-6 * abs(dxv @ jp)
 
 
+@pytest.mark.parametrize("planes", [False, True])
+def test_mixed_adjoint_sparse_and_dense_seed_groups(planes):
+    """The mixed reverse compacts away the volume nodes whose seed is zero and
+    takes groups of four walk waves with at most 12 seeds per 64 nodes in the
+    group reverse (one atomic set per term of the four waves' union), the
+    others in the pair reverse.  Same star body as above: seeds dense where
+    x > 0.2 and on 5% of the nodes elsewhere, so one launch runs both; then
+    a seed on 1% of the nodes, and a zero seed.  The adjoint must equal the
+    FP64 reverse and the oracle within the mixed bar, and the tangent's dot
+    product must hold."""
+    m = synthetic.star_shells(n=31, nshell=11)
+    xv0 = m["xv0"]
+    sym = [([0.0, 0.0, 0.0], [0.0, 1.0, 0.0])] if planes else None
+    opts = dict(evalMode="fast", errTol=1e-4)
+    rng = np.random.default_rng(11)
+    dense = xv0[:, 0] > 0.2
+    mixed_seed = rng.standard_normal(xv0.shape)
+    mixed_seed[~dense & (rng.random(len(xv0)) >= 0.05)] = 0.0
+    assert dense.sum() > 5000 and (~dense).sum() > 5000
+    seeds = {"half dense": mixed_seed.ravel(), "1%": synthetic.random_dxv(len(xv0), seed=12, frac=0.01),
+             "zero": np.zeros(3 * len(xv0))}
+    ora = O.OracleMesh(m["pts"], m["conn"], m["faceSizes"], xv0, symmetryPlanes=sym, **opts)
+    xs = m["perturb"](m["pts"])
+    ora.setSurfaceCoordinates(xs)
+    ora.warpMesh()
+    gpus = {}
+    for prec in ("fp64", "mixed"):
+        g = USMesh(options=dict(gridFile=xv0, fileType="memory", precision=prec, symmetryPlanes=sym, **opts))
+        g.setSurfaceDefinition(m["pts"], m["conn"], m["faceSizes"])
+        g.setSurfaceCoordinates(xs)
+        g.warpMesh()
+        gpus[prec] = g
+    for name, dxv in seeds.items():
+        dref = ora.warpDeriv(dxv)
+        got = {}
+        for prec, g in gpus.items():
+            g.warpDeriv(dxv, solverVec=False)
+            got[prec] = g.getdXs().ravel().copy()
+        if name == "zero":
+            assert not np.any(got["mixed"]) and not np.any(got["fp64"]) and not np.any(dref)
+            continue
+        r64, rmix, rora = _rel(got["fp64"], dref), _rel(got["mixed"], got["fp64"]), _rel(got["mixed"], dref)
+        print("planes", planes, name, "fp64 vs oracle %.2e, mixed vs fp64 %.2e, mixed vs oracle %.2e" %
+              (r64, rmix, rora), flush=True)
+        assert r64 <= 1e-10
+        assert rmix <= 1e-6 and rora <= 1e-6
+        assert rmix >= 1e-12  # the FP32 paths ran
+        p = np.random.default_rng(13).standard_normal(got["mixed"].size)
+        jp = gpus["mixed"].warpDerivFwd(p.reshape(-1, 3), solverVec=False)
+        assert abs(dxv @ jp - got["mixed"] @ p) <= 1e-6 * abs(dxv @ jp)
+
+
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_single_bucket_tree(prec):
     """A surface of 8 points (a cube, 6 quads) is one bucket: the tree has no
