@@ -262,15 +262,19 @@ def main():
     ph = head["phases"]
     chosen = tuning.get(head_prec, {}).get("chosen", {})
     if head_prec == "mixed":
-        rev = ("idw::k_fast_rev_pair<0, %d>" % chosen.get("mergeWindow", 32), FP32_PEAK_TFLOPS,
-               "FP32 terms (the bulk): FP32 weights on VALU and the sums over the seeded volume nodes of a pair of "
-               "walk waves on v_mfma_f32_16x16x4f32, each term shared by the pair added once; FP64 terms near the "
-               "wall: FP64 weights and v_mfma_f64_16x16x4f64 per wave.  Priced against the 157.3 TFLOP/s dense "
-               "FP32 roof; algorithmic work of the seeded nodes only")
+        # the mixed reverse phase is two launches: k_fast_rev_group takes the groups of four walk waves with
+        # sparse seeds (all of them with this seed), k_fast_rev_pair the others' pairs (here it returns at once)
+        rev = ("idw::k_fast_rev_group<0, 4>", FP32_PEAK_TFLOPS,
+               "FP32 terms (the bulk): FP32 weights on VALU and the sums over the seeded volume nodes of four walk "
+               "waves on v_mfma_f32_16x16x4f32, each term of their union added once with FP64 atomics; FP64 terms "
+               "near the wall: FP64 weights and v_mfma_f64_16x16x4f64 per wave.  Priced against the 157.3 TFLOP/s "
+               "dense FP32 roof; algorithmic work of the seeded nodes only")
+        rev_dense = "idw::k_fast_rev_pair<0, %d>" % chosen.get("mergeWindow", 32)
         fwd = ("idw::k_fast_fwd<1, 0, %d>" % chosen.get("forwardChunk", 32), FP32_PEAK_TFLOPS,
                "FP32 weights on VALU and the sums over each wave's 64 volume nodes on v_mfma_f32_16x16x4f32 (FP64 "
                "near the wall and far out).  Priced against the 157.3 TFLOP/s dense FP32 roof")
     else:
+        rev_dense = "idw::k_fast_rev<0, 0>"
         rev = ("idw::k_fast_rev<0, 0>", FP64_PEAK_TFLOPS,
                "FP64 weights on VALU, the sums over each wave's seeded volume nodes on v_mfma_f64_16x16x4f64; the "
                "78.6 TFLOP/s FP64 vector roof equals the FP64 matrix roof; algorithmic work of the seeded nodes only")
@@ -303,7 +307,7 @@ def main():
                       for n, i, c in (("walk", 1, cands[1]), ("fwd", 2, cands[2]), ("rev", 4, cands[4]))}
     if "mixed_dense" in out and head_prec == "mixed":
         dms = out["mixed_dense"]["phases"][4]
-        res["kernels"]["rev_dense_seed"] = {"kernel": rev[0], "ms": round(float(dms), 3),
+        res["kernels"]["rev_dense_seed"] = {"kernel": rev_dense, "ms": round(float(dms), 3),
                                             "algorithmic_flops": kflops["rev_dense"],
                                             "TFLOPs": round(kflops["rev_dense"] / (dms * 1e-3) / 1e12, 3),
                                             "frac": round(kflops["rev_dense"] / (dms * 1e-3) / 1e12 / rev[1], 4)}
