@@ -236,8 +236,14 @@ int pass_of(const idw_ctx *c, int kk) {
     return kk == 0 ? idw::PASS_FIRST : (kk == nloop(c) - 1 ? idw::PASS_LAST : idw::PASS_MID);
 }
 
-// Accumulators of the mirror passes (only with symmetry planes).
+// Accumulators of the mirror passes (only with symmetry planes), and in
+// exact mode the packed seeded nodes of the reverse (positions, and per
+// block of 256 nodes its offset, the total last).
 int ensure_mirror_buffers(idw_ctx *c) {
+    if (c->opt.evalMode == IDW_EVAL_EXACT && c->V.n && !c->V.cpos) {
+        TRY(dalloc(c, &c->V.cpos, c->V.n));
+        TRY(dalloc(c, &c->V.seedOff, (size_t)((c->V.n + 255) / 256) + 1));
+    }
     if (nloop(c) == 1 || c->V.acc) return IDW_OK;
     TRY(dalloc(c, &c->V.acc, c->V.n));
     TRY(dalloc(c, &c->V.volm, c->V.n));
@@ -1112,7 +1118,7 @@ int idw_set_volume_nodes(idw_ctx *c, const double *xv0, int64_t ndof) {
     TRY(ensure_device(c));
     dfree(c, c->xv0w); dfree(c, c->xv); dfree(c, c->dxv);
     dfree(c, c->V.vol); dfree(c, c->V.perm); dfree(c, c->V.den); dfree(c, c->V.numb);
-    dfree(c, c->V.acc); dfree(c, c->V.volm);
+    dfree(c, c->V.acc); dfree(c, c->V.volm); dfree(c, c->V.cpos); dfree(c, c->V.seedOff);
     dfree(c, c->partial);
     dfree(c, c->W);
     c->wCount = 0;
@@ -1539,6 +1545,7 @@ int idw_warp_deriv_dev_noalloc(idw_ctx *c, const double *d_xs, const double *d_d
     if (deriv_needs_warp(c))
         return fail(IDW_ESTATE, "the allocation-free reverse needs a warp of this context at the current options first");
     if (nloop(c) > 1 && !c->V.acc) return fail(IDW_ESTATE, "the allocation-free reverse needs a warp with the planes first");
+    if (!fast && c->V.n && !c->V.cpos) return fail(IDW_ESTATE, "the allocation-free reverse needs an exact-mode warp first");
     if (fast && c->opt.deterministic && c->V.n && !det_ready(c))
         return fail(IDW_ESTATE, "the allocation-free deterministic reverse needs an allocating one of this plan first");
     TRY(ensure_device(c));

@@ -134,7 +134,11 @@ struct DevVolume {
     double *den = nullptr;    // denominator of the last warp, traversal order
     double4 *numb = nullptr;  // reverse seed dXv/den, traversal order
     double4 *acc = nullptr;   // numerator and denominator summed over mirror passes (symmetry planes only)
-    double4 *volm = nullptr;  // mirrored positions of one pass for the exact reverse (symmetry planes only)
+    double4 *volm = nullptr;  // mirrored positions of one pass for the stored-weight exact reverse (symmetry planes only)
+    // exact reverse: positions of the seeded nodes (mirrored for the pass) packed in traversal order, and per
+    // block of 256 nodes the offset of its first (grid_for(n, 256) + 1 entries, the total last)
+    double4 *cpos = nullptr;
+    int32_t *seedOff = nullptr;
 };
 
 void launch_face_props(hipStream_t st, const DevSurface &S, const double *xs);

@@ -648,18 +648,109 @@ __global__ void k_prep_numb(int nvol, const int32_t *__restrict__ perm, const do
     numb[k] = make_double4(a, b, c, 0.0);
 }
 
+// A seed: a nonzero component of dXv (node k in traversal order).
+__device__ __forceinline__ bool seeded(int nvol, int64_t k, const int32_t *__restrict__ perm,
+                                       const double *__restrict__ dxv) {
+    if (k >= nvol) return false;
+    const int64_t j = perm[k];
+    return dxv[3 * j] != 0.0 || dxv[3 * j + 1] != 0.0 || dxv[3 * j + 2] != 0.0;
+}
+
+// The exact reverse runs over the seeded nodes only: a node whose seed is
+// zero adds exactly zero to every sum.  They are packed in traversal order,
+// so each surface node's sums keep their order: per block of 256 nodes the
+// seeded count (k_seed_count), an exclusive scan of the counts
+// (k_scan_counts, in place, the total after the last), then the packed
+// seeds dXv/den and (mirrored) positions (k_pack_seeds).
+__global__ void __launch_bounds__(256) k_seed_count(int nvol, const int32_t *__restrict__ perm,
+                                                    const double *__restrict__ dxv, int32_t *__restrict__ cnt) {
+    __shared__ int wc[4];
+    const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const unsigned long long b = __ballot(seeded(nvol, k, perm, dxv));
+    if ((threadIdx.x & 63) == 0) wc[threadIdx.x >> 6] = __popcll(b);
+    __syncthreads();
+    if (threadIdx.x == 0) cnt[blockIdx.x] = wc[0] + wc[1] + wc[2] + wc[3];
+}
+
+// Exclusive scan of v[0..n) in place, the total to v[n]; one workgroup of 1024.
+__global__ void __launch_bounds__(1024) k_scan_counts(int n, int32_t *__restrict__ v) {
+    __shared__ int ws[16];
+    __shared__ int carry;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    if (t == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < n; base += 1024) {
+        const int i = base + t;
+        const int x = i < n ? v[i] : 0;
+        int s = x;  // inclusive scan over the wave
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int y = __shfl_up(s, d, 64);
+            if (lane >= d) s += y;
+        }
+        if (lane == 63) ws[w] = s;
+        __syncthreads();
+        if (w == 0) {  // inclusive scan of the 16 wave totals
+            int u = lane < 16 ? ws[lane] : 0;
+#pragma unroll
+            for (int d = 1; d < 16; d <<= 1) {
+                const int y = __shfl_up(u, d, 64);
+                if (lane >= d) u += y;
+            }
+            if (lane < 16) ws[lane] = u;
+        }
+        __syncthreads();
+        const int before = carry + (w > 0 ? ws[w - 1] : 0) + s - x;
+        if (i < n) v[i] = before;
+        __syncthreads();  // carry and ws are read
+        if (t == 1023) carry = before + x;
+        __syncthreads();
+    }
+    if (t == 0) v[n] = carry;
+}
+
+__global__ void __launch_bounds__(256) k_pack_seeds(int nvol, const int32_t *__restrict__ perm,
+                                                    const double *__restrict__ dxv, const double *__restrict__ den,
+                                                    Mirror M, const double4 *__restrict__ vol,
+                                                    const int32_t *__restrict__ off, double4 *__restrict__ numb,
+                                                    double4 *__restrict__ pos) {
+    __shared__ int wc[4];
+    const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const bool sd = seeded(nvol, k, perm, dxv);
+    const unsigned long long b = __ballot(sd);
+    if (lane == 0) wc[w] = __popcll(b);
+    __syncthreads();
+    if (!sd) return;
+    int r = off[blockIdx.x] + __popcll(b & ((1ull << lane) - 1ull));
+    for (int u = 0; u < w; ++u) r += wc[u];
+    const int64_t j = perm[k];
+    const double oden = 1.0 / den[k];
+    double a = dxv[3 * j] * oden, bb = dxv[3 * j + 1] * oden, cc = dxv[3 * j + 2] * oden;
+    double4 v = vol[k];
+    if (M.mask) {
+        mirror_vec_b(M, a, bb, cc);
+        mirror_pt(M, v.x, v.y, v.z);
+    }
+    numb[r] = make_double4(a, bb, cc, 0.0);
+    pos[r] = v;
+}
+
 // evalNodeExact_b (kd_tree.F90:984-1013) transposed: one lane per surface
-// node, a chunk of volume nodes per block row.  Accumulates
+// node, a chunk of the packed seeded nodes per block row (the nchunk chunks
+// split the *count nodes evenly).  Accumulates
 // Mb'(r,c) = sum s_r (x_c - Xu0_c) (the reverse's Mib - Xu0 (x) Bib of
 // COMPUTENODALPROPERTIES_B, kd_tree.F90:1115-1118) and Bb = sum s.
 template <int PREC, int EXPM>
-__global__ void __launch_bounds__(256) k_exact_rev(int nvol, int chunk, const double4 *__restrict__ vol,
+__global__ void __launch_bounds__(256) k_exact_rev(int nchunk, const int32_t *__restrict__ count,
+                                                   const double4 *__restrict__ vol,
                                                    const double4 *__restrict__ numb, int n,
                                                    const double4 *__restrict__ ps, WParams P,
                                                    double *__restrict__ partial) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     const int c = blockIdx.y;
-    const int j0 = c * chunk, j1 = min(nvol, j0 + chunk);
+    const int nv = *count, chunk = (nv + nchunk - 1) / nchunk;
+    const int j0 = c * chunk, j1 = min(nv, j0 + chunk);
     const bool valid = i < n;
     const double4 s = valid ? ps[i] : make_double4(0.0, 0.0, 0.0, 0.0);
     double acc[12];
@@ -1201,19 +1292,47 @@ __device__ __forceinline__ PlanEntry plan_entry(const PlanEntry *__restrict__ pl
     return pe;
 }
 
+// Row of this lane's term in a chunk of 64 (one term per lane) ordered by the
+// 16-node blocks its mask touches: terms that touch the same blocks sit side
+// by side, so the sums over four rows at a time find more blocks that none of
+// the four touches and skip them; empty entries (mask 0) go last.  A stable
+// radix sort of the 4-bit block signature, one bit per pass, each pass a
+// split of the 64 lanes moved with ds_permute; then each sorted position is
+// sent back to the term's lane.  All 64 lanes must be active.
+__device__ __forceinline__ int block_order_row(unsigned long long mask) {
+    const int lane = threadIdx.x & 63;
+    const unsigned lo = (unsigned)mask, hi = (unsigned)(mask >> 32);
+    const unsigned sig = ((lo & 0xffffu) ? 1u : 0u) | ((lo >> 16) ? 2u : 0u) | ((hi & 0xffffu) ? 4u : 0u) |
+                         ((hi >> 16) ? 8u : 0u);
+    unsigned v = ((15u - sig) << 8) | (unsigned)lane;  // key above the term's lane
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        const bool one = (v >> (8 + b)) & 1u;
+        const unsigned long long ones = __ballot(one);
+        const int below = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(ones >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo((unsigned)ones, 0u));
+        const int dst = one ? 64 - __popcll(ones) + below : lane - below;  // a permutation of 0..63
+        v = (unsigned)__builtin_amdgcn_ds_permute(dst << 2, (int)v);
+    }
+    return __builtin_amdgcn_ds_permute((int)((v & 63u) << 2), lane);
+}
+
 // Stage FCH terms of the plan in LDS, 64 / FCH lanes per term: term r is
 // staged by lanes r + FCH p.  Part p = 0 writes row 0 of K and c (all rows
 // when one lane stages a term), x, eps, the mask and the weight factors; the
 // other parts the remaining rows; the last part the constant column.  pe:
 // this lane's entry (record -1: padding row).  W32 reads the FP32 records
-// (mixed precision) into an FP32 chunk.
+// (mixed precision) into an FP32 chunk; with one lane per term its rows are
+// ordered by block_order_row (the padding rows last).  FP64 chunks keep plan
+// order, so that FP64 sums do not depend on the chunk size.
 template <bool W32, int EXPM, int FCH, typename Chunk>
 __device__ __forceinline__ void stage_terms(Chunk &C, PlanEntry pe, int nnodes, int nrec,
                                             const Rec64 *__restrict__ rec, const Rec32 *__restrict__ rec32,
                                             const WParams &P, double ox, double oy, double oz) {
     static_assert(FCH == 16 || FCH == 32 || FCH == 64, "16, 32 or 64 terms per chunk");
     constexpr int LPT = 64 / FCH;  // lanes per term
-    const int lane = threadIdx.x & 63, r = lane & (FCH - 1), part = lane / FCH;
+    const int lane = threadIdx.x & 63, part = lane / FCH;
+    const int r = W32 && LPT == 1 ? block_order_row(pe.mask) : lane & (FCH - 1);
     // K rows [a0, a1) of this lane: 1 lane {0,1,2}; 2 lanes {0}, {1,2}; 4 lanes {0}, {1}, {2}, {}
     const int a0 = LPT == 1 ? 0 : (LPT == 2 ? (part == 0 ? 0 : 1) : min(part, 3));
     const int a1 = LPT == 1 ? 3 : (LPT == 2 ? (part == 0 ? 1 : 3) : min(part + 1, 3));
@@ -2086,14 +2205,6 @@ k_fast_rev_pair(int nvol, int nwave, const double4 *__restrict__ vol, const int3
 // uses it.  A group goes here when at most 12 of each 64 of its nodes carry
 // a seed, and the rows then fit a fixed LDS budget; this kernel, launched
 // first, flags the groups it takes and the pair reverse leaves them.
-
-// A seed: a nonzero component of dXv (node k in traversal order).
-__device__ __forceinline__ bool seeded(int nvol, int64_t k, const int32_t *__restrict__ perm,
-                                       const double *__restrict__ dxv) {
-    if (k >= nvol) return false;
-    const int64_t j = perm[k];
-    return dxv[3 * j] != 0.0 || dxv[3 * j + 1] != 0.0 || dxv[3 * j + 2] != 0.0;
-}
 
 template <int NS>
 struct GroupLds {
@@ -2980,16 +3091,24 @@ static void reverse_impl(hipStream_t st, const DevSurface &S, const DevVolume &V
                          int fast, int prec, double *partial, int nchunk, const double *dxv, const DevPlan *plan,
                          const Tuning &T, double *detPart, const double *wst) {
     if (!fast) {
-        launch_prep_numb(st, V, M, dxv);
-        const double4 *pos = M.mask ? V.volm : V.vol;
-        const int chunk = (int)((V.n + nchunk - 1) / nchunk);
         dim3 g(grid_for(S.n, 256), nchunk);
-        if (wst)  // chunks rounded to 4 nodes: the last one may be shorter or empty
+        if (wst) {  // every node (its row of weights); chunks rounded to 4 nodes: the last may be short or empty
+            launch_prep_numb(st, V, M, dxv);
+            const double4 *pos = M.mask ? V.volm : V.vol;
+            const int chunk = (int)((V.n + nchunk - 1) / nchunk);
             k_exact_rev_w<<<g, 256, 0, st>>>((int)V.n, (chunk + 3) & ~3, pos, V.numb, S.n, S.ps, wst, partial);
-        else if (prec == IDW_PREC_FP64)
-            k_exact_rev<IDW_PREC_FP64, EXPM><<<g, 256, 0, st>>>((int)V.n, chunk, pos, V.numb, S.n, S.ps, P, partial);
-        else
-            k_exact_rev<IDW_PREC_MIXED, EXPM><<<g, 256, 0, st>>>((int)V.n, chunk, pos, V.numb, S.n, S.ps, P, partial);
+        } else {  // the seeded nodes, packed
+            const int nb = (int)grid_for(V.n, 256);
+            k_seed_count<<<nb, 256, 0, st>>>((int)V.n, V.perm, dxv, V.seedOff);
+            k_scan_counts<<<1, 1024, 0, st>>>(nb, V.seedOff);
+            k_pack_seeds<<<nb, 256, 0, st>>>((int)V.n, V.perm, dxv, V.den, M, V.vol, V.seedOff, V.numb, V.cpos);
+            const int32_t *cnt = V.seedOff + nb;
+            if (prec == IDW_PREC_FP64)
+                k_exact_rev<IDW_PREC_FP64, EXPM><<<g, 256, 0, st>>>(nchunk, cnt, V.cpos, V.numb, S.n, S.ps, P, partial);
+            else
+                k_exact_rev<IDW_PREC_MIXED, EXPM><<<g, 256, 0, st>>>(nchunk, cnt, V.cpos, V.numb, S.n, S.ps, P,
+                                                                     partial);
+        }
         // the first pass stores, later passes add
         k_exact_rev_reduce<<<grid_for((int64_t)S.n * 12, 256), 256, 0, st>>>(S.n, nchunk, partial, S.padj,
                                                                               M.mask != 0);

@@ -758,6 +758,49 @@ def test_mixed_adjoint_sparse_and_dense_seed_groups(planes):
         assert abs(dxv @ jp - got["mixed"] @ p) <= 1e-6 * abs(dxv @ jp)
 
 
+@pytest.mark.parametrize("planes", [False, True])
+def test_exact_adjoint_over_packed_seeded_nodes(planes):
+    """The exact reverse runs over the volume nodes with a nonzero seed only,
+    packed in traversal order by a count and an exclusive scan per block of
+    256 nodes.  On a 309k-node ellipsoid (1,207 blocks: the scan takes two
+    rounds of 1,024) a seed carried by 1,500 sampled nodes must give the
+    oracle's reverse over those nodes (the other nodes add nothing), in FP64
+    and mixed precision; so must a seed on the last node alone.  A zero seed
+    gives zero.  (Dense seeds: the exact cases of the tests above.)"""
+    m = synthetic.ellipsoid_omesh(97, 65, 49)
+    xv0 = m["xv0"]
+    n = len(xv0)
+    assert (n + 255) // 256 > 1024
+    sym = [([0.0, 0.0, 0.0], [0.0, 1.0, 0.0])] if planes else None
+    xs = m["perturb"](m["pts"])
+    rng = np.random.default_rng(31)
+    cases = {"sampled": np.sort(rng.choice(n, size=1500, replace=False)), "last node": np.array([n - 1])}
+    gpus = {}
+    for prec in ("fp64", "mixed"):
+        g = USMesh(options=dict(gridFile=xv0, fileType="memory", evalMode="exact", precision=prec,
+                                symmetryPlanes=sym))
+        g.setSurfaceDefinition(m["pts"], m["conn"], m["faceSizes"])
+        g.setSurfaceCoordinates(xs)
+        g.warpMesh()
+        gpus[prec] = g
+    for name, idx in cases.items():
+        ora = O.OracleMesh(m["pts"], m["conn"], m["faceSizes"], xv0[idx], symmetryPlanes=sym, evalMode="exact")
+        ora.setSurfaceCoordinates(xs)
+        ora.warpMesh()
+        g = np.random.default_rng(32).standard_normal((len(idx), 3))
+        dref = ora.warpDeriv(g.ravel())
+        seed = np.zeros((n, 3))
+        seed[idx] = g
+        for prec, gpu in gpus.items():
+            gpu.warpDeriv(seed.ravel(), solverVec=False)
+            r = _rel(gpu.getdXs().ravel(), dref)
+            print("planes", planes, name, prec, "adjoint rel err vs oracle %.2e" % r, flush=True)
+            assert r <= (1e-10 if prec == "fp64" else 1e-6)
+    for gpu in gpus.values():
+        gpu.warpDeriv(np.zeros(3 * n), solverVec=False)
+        assert not np.any(gpu.getdXs())
+
+
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_single_bucket_tree(prec):
     """A surface of 8 points (a cube, 6 quads) is one bucket: the tree has no
