@@ -1363,8 +1363,16 @@ __device__ __forceinline__ void stage_terms(Chunk &C, PlanEntry pe, int nnodes, 
             C.B[r][13] = C.B[r][14] = C.B[r][15] = 0.0f;
         }
         if (lo) {
-            for (int i = 0; i < 3; ++i) C.X[r][i] = x[i];
-            C.X[r][3] = pe.rec < nnodes ? 0.0 : EPS_W;
+            const double eps = pe.rec < nnodes ? 0.0 : EPS_W;
+            if constexpr (W32) {  // |v - x|^2 = |g|^2 + (|y|^2 + eps) - 2 g.y with g = v - o, y = x - o
+                C.X[r][0] = -2.0 * y0;
+                C.X[r][1] = -2.0 * y1;
+                C.X[r][2] = -2.0 * y2;
+                C.X[r][3] = fma(y0, y0, fma(y1, y1, fma(y2, y2, eps)));
+            } else {
+                for (int i = 0; i < 3; ++i) C.X[r][i] = x[i];
+                C.X[r][3] = eps;
+            }
             C.mask[r] = pe.mask;
             const double2 f = weight_factors<EXPM>(W32 ? (double)rec32[pe.rec].A : rec[pe.rec].A, P);
             if constexpr (W32) C.F[r] = make_float2((float)f.x, (float)f.y);
@@ -1418,13 +1426,18 @@ __device__ __forceinline__ void add_f32_sums(dvec4 &D, fvec4 F) {
 // D[t]); vx/vy/vz: positions of the lane's rows (nodes 16t + lane%16).
 // The entries of the next chunk are requested while this one is summed.
 // W32 (mixed precision, FP32 terms): FP32 weights and B rows on
-// v_mfma_f32_16x16x4f32, each chunk's FP32 sums added to the FP64 sums.
+// v_mfma_f32_16x16x4f32, each chunk's FP32 sums added to the FP64 sums;
+// vx/vy/vz are then the rows' positions g relative to the wave origin and
+// g2 = |g|^2, and |r|^2 is formed in FP64 as g2 + (|y|^2 + eps) - 2 g.y
+// (y = x - o staged with the term).  The cancellation error, about
+// 1e-16 (|g| + |y|)^2, stays below the FP32 rounding of |r|^2 for the
+// terms FP32 takes (none nearer than the near-wall threshold).
 template <bool W32, int EXPM, int FCH, typename Chunk>
 __device__ __forceinline__ void fwd_terms(int64_t a, int64_t b, const PlanEntry *__restrict__ plan, int nnodes,
                                           int nrec, const Rec64 *__restrict__ rec, const Rec32 *__restrict__ rec32,
                                           const WParams &P, Chunk &C, double ox, double oy, double oz,
                                           const double (&vx)[4], const double (&vy)[4], const double (&vz)[4],
-                                          dvec4 (&D)[4]) {
+                                          const double (&g2)[4], dvec4 (&D)[4]) {
     const int lane = threadIdx.x & 63, n = lane & 15, R = lane >> 4, r = lane & (FCH - 1);
     const float Lf = (float)P.Ldef;
     if (a >= b) return;
@@ -1453,10 +1466,9 @@ __device__ __forceinline__ void fwd_terms(int64_t a, int64_t b, const PlanEntry 
                     // node block t idle for all four terms: nothing to add
                     // (about one block in seven)
                     if (!on) continue;
-                    const double rx = vx[t] - x0, ry = vy[t] - x1, rz = vz[t] - x2;
-                    // |r|^2 + eps in FP64 (eps: points only; folded into the
-                    // first product, so it costs no instruction), rounded once
-                    const float w = weight_f32<EXPM>((float)fma(rz, rz, fma(ry, ry, fma(rx, rx, eps))), f, Lf, P);
+                    // |r|^2 + eps in FP64 (x0..x2 = -2 y, eps = |y|^2 + eps), rounded once
+                    const double d2 = fma(vx[t], x0, fma(vy[t], x1, fma(vz[t], x2, g2[t] + eps)));
+                    const float w = weight_f32<EXPM>((float)d2, f, Lf, P);
                     F[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(lane_in(on) ? w : 0.0f, Bv, F[t], 0, 0, 0);
                 }
             }
@@ -1505,31 +1517,51 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     const int k = blk * 256 + threadIdx.x;
     const bool valid = k < nvol;
     const int wave = uniform(blk * 4 + wv);
-    double4 v = valid ? vol[k] : make_double4(0.0, 0.0, 0.0, 1.0);
-    if (M.mask) mirror_pt(M, v.x, v.y, v.z);
+    // the lane's (mirrored) node, loaded again where needed rather than kept live
+    auto node_pos = [&]() {
+        double4 p = valid ? vol[k] : make_double4(0.0, 0.0, 0.0, 1.0);
+        if (M.mask) mirror_pt(M, p.x, p.y, p.z);
+        return p;
+    };
+    // positions of the lane's rows: nodes 16t + lane%16
+    auto rows_of = [&](const double4 &p, double (&px)[4], double (&py)[4], double (&pz)[4]) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int src = 16 * t + n;
+            px[t] = __shfl(p.x, src, 64);
+            py[t] = __shfl(p.y, src, 64);
+            pz[t] = __shfl(p.z, src, 64);
+        }
+    };
+    double4 v = node_pos();
     // wave origin: lane 0 is valid whenever any lane of the wave is
     const double ox = first_lane(v.x), oy = first_lane(v.y), oz = first_lane(v.z);
-    double vx[4], vy[4], vz[4];
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        const int src = 16 * t + n;
-        vx[t] = __shfl(v.x, src, 64);
-        vy[t] = __shfl(v.y, src, 64);
-        vz[t] = __shfl(v.z, src, 64);
-    }
     dvec4 D[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) D[t] = dvec4{0.0, 0.0, 0.0, 0.0};
     FwdLds<PREC, FCH> &L = lds[wv];
     constexpr int F64 = fch64(PREC, FCH);
     const int64_t e0 = waveOff[wave], e1 = waveOff[wave + 1];
+    double vx[4], vy[4], vz[4], g2[4];
+    int64_t es = e0;
     if (PREC == IDW_PREC_MIXED) {
-        const int64_t es = min(max(waveSplit[wave], e0), e1);
-        fwd_terms<true, EXPM, FCH>(e0, es, plan, nnodes, nrec, rec, rec32, P, L.c32, ox, oy, oz, vx, vy, vz, D);
-        fwd_terms<false, EXPM, F64>(es, e1, plan, nnodes, nrec, rec, rec32, P, L.c64, ox, oy, oz, vx, vy, vz, D);
-    } else {
-        fwd_terms<false, EXPM, F64>(e0, e1, plan, nnodes, nrec, rec, rec32, P, L.c64, ox, oy, oz, vx, vy, vz, D);
+        es = min(max(waveSplit[wave], e0), e1);
+        if (es > e0) {  // the FP32 terms, with the rows relative to the origin
+            rows_of(v, vx, vy, vz);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                vx[t] -= ox;
+                vy[t] -= oy;
+                vz[t] -= oz;
+                g2[t] = fma(vx[t], vx[t], fma(vy[t], vy[t], vz[t] * vz[t]));
+            }
+            fwd_terms<true, EXPM, FCH>(e0, es, plan, nnodes, nrec, rec, rec32, P, L.c32, ox, oy, oz, vx, vy, vz, g2,
+                                       D);
+            v = node_pos();
+        }
     }
+    rows_of(v, vx, vy, vz);
+    fwd_terms<false, EXPM, F64>(es, e1, plan, nnodes, nrec, rec, rec32, P, L.c64, ox, oy, oz, vx, vy, vz, g2, D);
     // D[t] register j: node 16t + 4j + R, column n.  Through LDS, 32 nodes
     // at a time, each lane gathers its node's 13 sums.  The lane's own
     // (mirrored) position is row R of its four rows.
