@@ -87,7 +87,7 @@ struct idw_ctx {
     double *dxsIn = nullptr;  // host-API tangent seed (3 * npts)
     double *partial = nullptr;
     int nchunk = 0;
-    double bboxLo[3] = {0, 0, 0}, bboxScale[3] = {0, 0, 0};  // Morton quantisation of the volume box
+    double bboxLo[3] = {0, 0, 0}, bboxScale[3] = {0, 0, 0};  // quantisation of the volume box for the curve keys
     std::vector<void *> allocs;
     std::vector<size_t> allocBytes;  // the size of each of allocs
     int64_t devBytes = 0;            // their sum
@@ -358,7 +358,7 @@ double plan_far_t2(const idw_ctx *c) {
 }
 
 // Mixed precision: move the volume nodes that need FP64 buckets behind the
-// others (Morton order kept within both groups), so that the other waves
+// others (Hilbert order kept within both groups), so that the other waves
 // run FP32 throughout.  Redone when errTol or the near-wall distance change.
 int order_near_wall(idw_ctx *c) {
     const double nt2 = plan_near_t2(c);
@@ -1136,7 +1136,7 @@ int idw_set_volume_nodes(idw_ctx *c, const double *xv0, int64_t ndof) {
     if (n) {
         HIPCHK(hipMemcpyAsync(c->xv0w, xv0, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->st));
         HIPCHK(hipMemcpyAsync(c->xv, c->xv0w, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, c->st));
-        // Morton order of the volume nodes over their bounding box
+        // Hilbert order of the volume nodes over their bounding box
         double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
         for (int64_t j = 0; j < n; ++j)
             for (int k = 0; k < 3; ++k) {
@@ -1156,7 +1156,7 @@ int idw_set_volume_nodes(idw_ctx *c, const double *xv0, int64_t ndof) {
         TRY(dalloc(c, &keys, n));
         TRY(dalloc(c, &keys2, n));
         TRY(dalloc(c, &ids, n));
-        idw::launch_morton(c->st, n, c->xv0w, lo, scale, keys, ids);
+        idw::launch_curve_keys(c->st, n, c->xv0w, lo, scale, keys, ids);
         HIPCHK(hipGetLastError());
         HIPCHK(idw::sort_pairs(c->st, n, keys, ids, keys2, c->V.perm));
         dfree(c, keys);

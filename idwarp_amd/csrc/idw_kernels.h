@@ -129,7 +129,7 @@ enum { TUNE_WBUF = 0, TUNE_FCH = 3, TUNE_MQ = 6, TUNE_EXACT_G = 9, TUNE_EXACT_CH
 
 struct DevVolume {
     int64_t n = 0;
-    double4 *vol = nullptr;   // x, y, z, denominator0 in traversal (Morton) order
+    double4 *vol = nullptr;   // x, y, z, denominator0 in traversal (Hilbert) order
     int32_t *perm = nullptr;  // traversal order -> warp order
     double *den = nullptr;    // denominator of the last warp, traversal order
     double4 *numb = nullptr;  // reverse seed dXv/den, traversal order
@@ -153,7 +153,7 @@ void launch_den0(hipStream_t st, const DevSurface &S, const DevVolume &V, const 
 // load-balancing costs (dryRun) of one mirror pass, added into cost (traversal order)
 void launch_dry_run(hipStream_t st, const DevSurface &S, const DevVolume &V, const Mirror &M, double errTol,
                     double *cost);
-void launch_morton(hipStream_t st, int64_t n, const double *xv0, double lo[3], double scale[3], uint64_t *keys,
+void launch_curve_keys(hipStream_t st, int64_t n, const double *xv0, double lo[3], double scale[3], uint64_t *keys,
                    int32_t *ids);
 void launch_build_vol(hipStream_t st, int64_t n, const double *xv0, const int32_t *perm, double4 *vol);
 void launch_near_keys(hipStream_t st, const DevSurface &S, const DevVolume &V, double errTol, double nearT2,
@@ -199,7 +199,7 @@ void launch_nodal_reverse(hipStream_t st, const DevSurface &S, const double *xs,
 // tmp null, sets bytes to the temporary storage it needs and sorts nothing.
 hipError_t sort_pairs32(hipStream_t st, void *tmp, size_t &bytes, int64_t n, unsigned *keys, int32_t *vals,
                         unsigned *keys_out, int32_t *vals_out, int end_bit);
-// Sort keys/ids pairs (Morton order); temp storage allocated inside.
+// Sort keys/ids pairs (curve order); temp storage allocated inside.
 hipError_t sort_pairs(hipStream_t st, int64_t n, uint64_t *keys, int32_t *ids, uint64_t *keys_out, int32_t *ids_out);
 
 }  // namespace idw

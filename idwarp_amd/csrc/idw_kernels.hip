@@ -1112,7 +1112,51 @@ __global__ void __launch_bounds__(256) k_plan_walk(int nvol, int blk0, int nbc, 
 // Mixed precision keeps FP64 for buckets that an active lane approaches
 // within sqrt(nearT2).  This walk flags those lanes (the wall layer and
 // anything closer than nearT) so that they can be given waves of their
-// own: key = near bit above the 63-bit Morton code of the node.
+// own: key = near bit above the node's 63-bit Hilbert key.
+// Bits 0..20 of x spread to every third bit.
+__device__ __forceinline__ uint64_t spread3(uint64_t x) {
+    x &= 0x1fffff;
+    x = (x | x << 32) & 0x1f00000000ffffull;
+    x = (x | x << 16) & 0x1f0000ff0000ffull;
+    x = (x | x << 8) & 0x100f00f00f00f00full;
+    x = (x | x << 4) & 0x10c30c30c30c30c3ull;
+    x = (x | x << 2) & 0x1249249249249249ull;
+    return x;
+}
+
+// Index of the cell (x, y, z) (21 bits each) along a 3-D Hilbert curve, as a
+// 63-bit key: Skilling's transform of the axes to the curve's transposed
+// index ("Programming the Hilbert curve", AIP Conf. Proc. 707, 2004), then
+// its bits interleaved.  Consecutive keys are neighbouring cells, where a
+// Morton order jumps between distant ones at its octant boundaries, so the
+// 64 nodes of a wave stay closer together.
+__device__ __forceinline__ uint64_t hilbert_key(uint32_t x, uint32_t y, uint32_t z) {
+    uint32_t X[3] = {x, y, z};
+    constexpr uint32_t M = 1u << 20;
+    for (uint32_t Q = M; Q > 1; Q >>= 1) {
+        const uint32_t P = Q - 1;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            if (X[i] & Q) {
+                X[0] ^= P;  // invert the low bits
+            } else {        // exchange the low bits of X[0] and X[i]
+                const uint32_t t = (X[0] ^ X[i]) & P;
+                X[0] ^= t;
+                X[i] ^= t;
+            }
+        }
+    }
+    X[1] ^= X[0];  // Gray encode
+    X[2] ^= X[1];
+    uint32_t t = 0;
+    for (uint32_t Q = M; Q > 1; Q >>= 1)
+        if (X[2] & Q) t ^= Q - 1;
+    X[0] ^= t;
+    X[1] ^= t;
+    X[2] ^= t;
+    return (spread3(X[0]) << 2) | (spread3(X[1]) << 1) | spread3(X[2]);
+}
+
 __global__ void __launch_bounds__(256) k_near_keys(int nvol, const double4 *__restrict__ vol, int nnodes,
                                                    const NodeRec *__restrict__ nodes, double errTol, double nearT2,
                                                    double3 lo, double3 scale, uint64_t *__restrict__ keys,
@@ -1158,18 +1202,9 @@ __global__ void __launch_bounds__(256) k_near_keys(int nvol, const double4 *__re
         node = next;
     }
     if (!valid) return;
-    auto spread = [](uint64_t x) {
-        x &= 0x1fffff;
-        x = (x | x << 32) & 0x1f00000000ffffull;
-        x = (x | x << 16) & 0x1f0000ff0000ffull;
-        x = (x | x << 8) & 0x100f00f00f00f00full;
-        x = (x | x << 4) & 0x10c30c30c30c30c3ull;
-        x = (x | x << 2) & 0x1249249249249249ull;
-        return x;
-    };
-    auto q = [](double t) { return (uint64_t)fmin(fmax(t, 0.0), 2097151.0); };
-    keys[k] = ((uint64_t)near << 63) | spread(q((v.x - lo.x) * scale.x)) | (spread(q((v.y - lo.y) * scale.y)) << 1) |
-              (spread(q((v.z - lo.z) * scale.z)) << 2);
+    auto q = [](double t) { return (uint32_t)fmin(fmax(t, 0.0), 2097151.0); };
+    keys[k] = ((uint64_t)near << 63) |
+              hilbert_key(q((v.x - lo.x) * scale.x), q((v.y - lo.y) * scale.y), q((v.z - lo.z) * scale.z));
     ids[k] = k;
 }
 
@@ -2902,26 +2937,13 @@ __global__ void __launch_bounds__(256) k_dry_run(int nvol, const double4 *__rest
     if (valid) cost[k] = (M.mask ? cost[k] : 0.0) + c / nsurf;
 }
 
-__global__ void k_morton(int64_t nvol, const double *__restrict__ xv0, double3 lo, double3 scale,
+__global__ void k_curve_keys(int64_t nvol, const double *__restrict__ xv0, double3 lo, double3 scale,
                          uint64_t *__restrict__ keys, int32_t *__restrict__ ids) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nvol) return;
-    auto spread = [](uint64_t x) {
-        x &= 0x1fffff;
-        x = (x | x << 32) & 0x1f00000000ffffull;
-        x = (x | x << 16) & 0x1f0000ff0000ffull;
-        x = (x | x << 8) & 0x100f00f00f00f00full;
-        x = (x | x << 4) & 0x10c30c30c30c30c3ull;
-        x = (x | x << 2) & 0x1249249249249249ull;
-        return x;
-    };
-    auto q = [](double t) {
-        const double c = fmin(fmax(t, 0.0), 2097151.0);
-        return (uint64_t)c;
-    };
-    const uint64_t x = q((xv0[3 * k] - lo.x) * scale.x), y = q((xv0[3 * k + 1] - lo.y) * scale.y),
-                   z = q((xv0[3 * k + 2] - lo.z) * scale.z);
-    keys[k] = spread(x) | (spread(y) << 1) | (spread(z) << 2);
+    auto q = [](double t) { return (uint32_t)fmin(fmax(t, 0.0), 2097151.0); };
+    keys[k] = hilbert_key(q((xv0[3 * k] - lo.x) * scale.x), q((xv0[3 * k + 1] - lo.y) * scale.y),
+                          q((xv0[3 * k + 2] - lo.z) * scale.z));
     ids[k] = (int32_t)k;
 }
 
@@ -2983,9 +3005,9 @@ void launch_dry_run(hipStream_t st, const DevSurface &S, const DevVolume &V, con
     k_dry_run<<<grid_for(V.n, 256), 256, 0, st>>>((int)V.n, V.vol, S.nnodes, S.nodes, errTol, S.n, M, cost);
 }
 
-void launch_morton(hipStream_t st, int64_t n, const double *xv0, double lo[3], double scale[3], uint64_t *keys,
+void launch_curve_keys(hipStream_t st, int64_t n, const double *xv0, double lo[3], double scale[3], uint64_t *keys,
                    int32_t *ids) {
-    k_morton<<<grid_for(n, 256), 256, 0, st>>>(n, xv0, make_double3(lo[0], lo[1], lo[2]),
+    k_curve_keys<<<grid_for(n, 256), 256, 0, st>>>(n, xv0, make_double3(lo[0], lo[1], lo[2]),
                                                make_double3(scale[0], scale[1], scale[2]), keys, ids);
 }
 
