@@ -444,8 +444,11 @@ def ellipsoid_exact(args):
     kms = {p: (out[p]["phases"][2], out[p]["phases"][4]) for p in out}
     fwd_tf = 42.0 * pairs / (kms["mixed"][0] * 1e-3) / 1e12
     rev_tf = 41.0 * rpairs / (kms["mixed"][1] * 1e-3) / 1e12
-    kname, kt = ("idw::k_exact_rev<1, 0>", rev_tf) if kms["mixed"][1] > kms["mixed"][0] else ("idw::k_exact_fwd<1, 0>",
-                                                                                                fwd_tf)
+    mesh.setOption("precision", "mixed")
+    mesh._setMeshOptions()
+    ech = mesh.getTuning()[0]["exactGroup"]  # points per chunk of the mixed warp
+    kname, kt = (("idw::k_exact_rev<1, 0>", rev_tf) if kms["mixed"][1] > kms["mixed"][0]
+                 else ("idw::k_exact_fwd_mc<0, %d>" % ech, fwd_tf))
     res["roofline"] = {"bound": "mfma", "achieved": round(kt, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                        "frac": round(kt / FP64_PEAK_TFLOPS, 4),
                        "traffic": _traffic_record(kname)["bytes"] if _traffic_record(kname) else None,
@@ -454,7 +457,9 @@ def ellipsoid_exact(args):
                                        "rev": round(41.0 * rpairs / (kms["fp64"][1] * 1e-3) / 1e12, 3)},
                        "mixed_TFLOPs": {"fwd": round(fwd_tf, 3), "rev": round(rev_tf, 3),
                                         "rev_dense_seed": round(41.0 * pairs / (rev_dense * 1e-3) / 1e12, 3)},
-                       "compute": "VALU (no MFMA): all-pairs weights and displacement sums"}
+                       "compute": "mixed warp: FP32 weights on VALU, displacement sums on v_mfma_f32_16x16x4f32 "
+                                  "(FP64 sums per chunk); FP64 warp and both reverses on VALU; priced against the "
+                                  "78.6 TFLOP/s FP64 vector roof"}
     # central differences of the FP64 warp on the device along unit surface
     # directions (the quotient formed from the difference of the two grids)
     mesh.setOption("precision", "fp64")

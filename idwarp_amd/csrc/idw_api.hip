@@ -879,7 +879,7 @@ int run_tune(idw_ctx *c) {
             }));
         }
     } else {
-        static const int gs[3] = {4, 8, 16}, shifts[3] = {-1, 0, 1};
+        static const int gs[3] = {16, 32, 64}, shifts[3] = {-1, 0, 1};
         if (mixed) TRY(pick(idw::TUNE_EXACT_G, gs, 3, c->tune.exactG, forward));
         // partial sums for the largest chunk count, freed afterwards
         double *partial = nullptr;
@@ -1646,8 +1646,8 @@ int idw_set_tuning(idw_ctx *c, const int32_t *params5) {
     if (!c || !params5) return fail(IDW_EINVAL, "null argument");
     auto in = [](int32_t v, std::initializer_list<int32_t> l) { return std::find(l.begin(), l.end(), v) != l.end(); };
     if (!in(params5[0], {16, 32, 64}) || !in(params5[1], {8, 16, 32}) || !in(params5[2], {64, 128, 256}) ||
-        !in(params5[3], {4, 8, 16}) || !in(params5[4], {-1, 0, 1}))
-        return fail(IDW_EINVAL, "tuning parameters outside their candidates (16/32/64, 8/16/32, 64/128/256, 4/8/16, "
+        !in(params5[3], {16, 32, 64}) || !in(params5[4], {-1, 0, 1}))
+        return fail(IDW_EINVAL, "tuning parameters outside their candidates (16/32/64, 8/16/32, 64/128/256, 16/32/64, "
                                 "-1/0/1)");
     c->tune.fch = params5[0];
     c->tune.mq = params5[1];

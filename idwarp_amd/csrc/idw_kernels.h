@@ -118,13 +118,13 @@ struct Tuning {
     int32_t fch = 32;         // fast forward: terms staged per chunk (16, 32, 64)
     int32_t mq = 32;          // mixed fast reverse: merge window per walk wave (8, 16, 32)
     int32_t wbuf = 128;       // walk: plan entries gathered in LDS per kind (64, 128, 256)
-    int32_t exactG = 8;       // exact mixed forward: surface points per FP32 partial sum (4, 8, 16)
+    int32_t exactG = 64;      // exact mixed forward: surface points staged per chunk (16, 32, 64)
     int32_t exactChunkShift = 0;  // exact reverse: volume chunks = the default count * 2^shift (-1, 0, 1)
 };
 
 // Slots of the autotuner's timing table (ms per launch of each candidate):
 // walk buffer 64/128/256, forward chunk 16/32/64, merge window 8/16/32, exact
-// FP32 group 4/8/16, exact reverse chunks x1/2, x1, x2.
+// forward chunk 16/32/64, exact reverse chunks x1/2, x1, x2.
 enum { TUNE_WBUF = 0, TUNE_FCH = 3, TUNE_MQ = 6, TUNE_EXACT_G = 9, TUNE_EXACT_CHUNKS = 12, TUNE_SLOTS = 15 };
 
 struct DevVolume {

@@ -555,8 +555,10 @@ __global__ void __launch_bounds__(256) k_node_sums(int n, const int32_t *__restr
 // ---------------------------------------------------------------------------
 // Exact mode (evalNodeExact on the root, kd_tree.F90:664-690)
 // ---------------------------------------------------------------------------
-// G: surface points per FP32 partial sum in mixed precision (4, 8 or 16, a
-// tuning parameter).  STORE: every pair's weight, as used, is also written to
+// The VALU form, one lane per volume node: the FP64 forward, and in mixed
+// precision the stored-weight forward (the mixed forward otherwise runs on
+// the matrix cores, k_exact_fwd_mc).  G: surface points per FP32 partial sum
+// in mixed precision.  STORE: every pair's weight, as used, is also written to
 // wout[i ld + k] (surface point i, sorted volume node k; rows padded to
 // ld = wrow(nvol), a multiple of 4) for the reverse.
 template <int PREC, int EXPM, int G, bool STORE = false>
@@ -1358,16 +1360,17 @@ __device__ __forceinline__ int block_order_row(unsigned long long mask) {
 // other parts the remaining rows; the last part the constant column.  pe:
 // this lane's entry (record -1: padding row).  W32 reads the FP32 records
 // (mixed precision) into an FP32 chunk; with one lane per term its rows are
-// ordered by block_order_row (the padding rows last).  FP64 chunks keep plan
-// order, so that FP64 sums do not depend on the chunk size.
-template <bool W32, int EXPM, int FCH, typename Chunk>
+// ordered by block_order_row (the padding rows last) unless ORDER is off.
+// FP64 chunks keep plan order, so that FP64 sums do not depend on the chunk
+// size.
+template <bool W32, int EXPM, int FCH, typename Chunk, bool ORDER = true>
 __device__ __forceinline__ void stage_terms(Chunk &C, PlanEntry pe, int nnodes, int nrec,
                                             const Rec64 *__restrict__ rec, const Rec32 *__restrict__ rec32,
                                             const WParams &P, double ox, double oy, double oz) {
     static_assert(FCH == 16 || FCH == 32 || FCH == 64, "16, 32 or 64 terms per chunk");
     constexpr int LPT = 64 / FCH;  // lanes per term
     const int lane = threadIdx.x & 63, part = lane / FCH;
-    const int r = W32 && LPT == 1 ? block_order_row(pe.mask) : lane & (FCH - 1);
+    const int r = W32 && ORDER && LPT == 1 ? block_order_row(pe.mask) : lane & (FCH - 1);
     // K rows [a0, a1) of this lane: 1 lane {0,1,2}; 2 lanes {0}, {1,2}; 4 lanes {0}, {1}, {2}, {}
     const int a0 = LPT == 1 ? 0 : (LPT == 2 ? (part == 0 ? 0 : 1) : min(part, 3));
     const int a1 = LPT == 1 ? 3 : (LPT == 2 ? (part == 0 ? 1 : 3) : min(part + 1, 3));
@@ -1464,9 +1467,13 @@ __device__ __forceinline__ void add_f32_sums(dvec4 &D, fvec4 F) {
 // v_mfma_f32_16x16x4f32, each chunk's FP32 sums added to the FP64 sums;
 // vx/vy/vz are then the rows' positions g relative to the wave origin and
 // g2 = |g|^2, and |r|^2 is formed in FP64 as g2 + (|y|^2 + eps) - 2 g.y
-// (y = x - o staged with the term).  The cancellation error, about
-// 1e-16 (|g| + |y|)^2, stays below the FP32 rounding of |r|^2 for the
-// terms FP32 takes (none nearer than the near-wall threshold).
+// (y = x - o staged with the term).  Its cancellation, about
+// 1e-16 ((|g| + |y|) / |r|)^2 relative to |r|^2, stays below FP32's rounding
+// while |r| > (|g| + |y|) / 4000.  The terms FP32 takes are accepted
+// clusters (at least two radii away) and the points of buckets whose
+// bounding spheres no active lane comes within the near-wall distance of, so
+// a node at the wall keeps its close points in FP64; a point just beyond
+// that distance from a node of a wide wave would lose digits here.
 template <bool W32, int EXPM, int FCH, typename Chunk>
 __device__ __forceinline__ void fwd_terms(int64_t a, int64_t b, const PlanEntry *__restrict__ plan, int nnodes,
                                           int nrec, const Rec64 *__restrict__ rec, const Rec32 *__restrict__ rec32,
@@ -1628,6 +1635,141 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         finish_fwd(P, M, k, perm[k], vb, n0, n1, n2, dd, xv, den, acc);
     }
 }
+
+// Exact-mode forward in mixed precision on the matrix cores
+// (evalNodeExact, kd_tree.F90:664-690, for every surface point): each wave
+// takes the surface points as terms of the fast forward's form (x = Xu0, A,
+// K = M - I, d), FCH at a time, with FP32 weights in the A-operand layout and
+// v_mfma_f32_16x16x4f32 sums added to FP64 sums per chunk.  r = g - y is
+// formed in FP64 from the origin-relative positions (y = x - o recovered
+// exactly from the staged -2 y): the near-wall rule here is per pair, so an
+// FP32 pair can lie just beyond the threshold while the wave spans far more,
+// where the fast forward's dot-product form of |r|^2 would cancel.  A (node, point) pair
+// nearer than the near-wall distance is left out of the FP32 sums; a chunk
+// with such a pair is scanned again after its sums, each lane testing its
+// own node against the chunk's points with the same arithmetic (so the same
+// decisions), and adds those pairs in FP64 (rsqrt_nr, FP64 displacement) as
+// the VALU kernel evaluates them.  Rows past the last node are summed like
+// the others and never written.
+template <int EXPM, int FCH>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_exact_fwd_mc(
+    int nvol, const double4 *__restrict__ vol, const int32_t *__restrict__ perm, int n, int nnodes, int nrec,
+    const Rec64 *__restrict__ rec, const Rec32 *__restrict__ rec32, const double4 *__restrict__ ps, WParams P,
+    Mirror M, double *__restrict__ xv, double *__restrict__ den, double4 *__restrict__ acc) {
+    __shared__ FwdLds<IDW_PREC_MIXED, FCH> lds[4];
+    const int nb = (nvol + 255) / 256;
+    const int blk = xcd_block(nb);
+    if (blk >= nb) return;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nn = lane & 15, R = lane >> 4;
+    const int k = blk * 256 + threadIdx.x;
+    const bool valid = k < nvol;
+    double4 v = valid ? vol[k] : make_double4(0.0, 0.0, 0.0, 1.0);
+    if (M.mask) mirror_pt(M, v.x, v.y, v.z);
+    // wave origin: lane 0 is valid whenever any lane of the wave is
+    const double ox = first_lane(v.x), oy = first_lane(v.y), oz = first_lane(v.z);
+    // the lane's rows: nodes 16t + lane%16, relative to the origin
+    double gx[4], gy[4], gz[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int src = 16 * t + nn;
+        gx[t] = __shfl(v.x, src, 64) - ox;
+        gy[t] = __shfl(v.y, src, 64) - oy;
+        gz[t] = __shfl(v.z, src, 64) - oz;
+    }
+    // the lane's own node: row R
+    const double ogx = R == 0 ? gx[0] : R == 1 ? gx[1] : R == 2 ? gx[2] : gx[3];
+    const double ogy = R == 0 ? gy[0] : R == 1 ? gy[1] : R == 2 ? gy[2] : gy[3];
+    const double ogz = R == 0 ? gz[0] : R == 1 ? gz[1] : R == 2 ? gz[2] : gz[3];
+    double e0 = 0.0, e1 = 0.0, e2 = 0.0, ed = 0.0;  // FP64 sums of the node's near pairs
+    dvec4 D[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) D[t] = dvec4{0.0, 0.0, 0.0, 0.0};
+    FwdLds<IDW_PREC_MIXED, FCH> &L = lds[wv];
+    FwdChunk32<FCH> &C = L.c32;
+    const float Lf = (float)P.Ldef;
+    const double T2 = P.nearT2;
+    const int r = lane & (FCH - 1);
+    const unsigned long long vmask = __ballot(valid);
+    for (int c0 = 0; c0 < n; c0 += FCH) {
+        const int cnt = min(FCH, n - c0);
+        PlanEntry pe;
+        pe.mask = r < cnt ? vmask : 0ull;
+        pe.rec = r < cnt ? nnodes + c0 + r : -1;
+        pe.node = 0;
+        wave_sync();  // the previous chunk is consumed
+        stage_terms<true, EXPM, FCH, FwdChunk32<FCH>, false>(C, pe, nnodes, nrec, rec, rec32, P, ox, oy, oz);
+        wave_sync();
+        const int ng = (cnt + 3) >> 2;
+        fvec4 F[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) F[t] = fvec4{0.0f, 0.0f, 0.0f, 0.0f};
+        unsigned long long nearBits = 0ull;  // lanes with a near pair, over the chunk
+#pragma unroll 1
+        for (int q = 0; q < ng; ++q) {
+            const int tr = 4 * q + R;
+            const bool live = tr < cnt;  // a padding row's weight must stay 0 (its r may vanish)
+            const float Bv = C.B[tr][nn];
+            const double x0 = C.X[tr][0], x1 = C.X[tr][1], x2 = C.X[tr][2];  // -2 y
+            const float2 f = C.F[tr];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                // r = g - y (one rounding each), |r|^2 + eps for a point; padding rows are at the origin
+                const double rx = fma(x0, 0.5, gx[t]), ry = fma(x1, 0.5, gy[t]), rz = fma(x2, 0.5, gz[t]);
+                const double d2 = fma(rz, rz, fma(ry, ry, fma(rx, rx, EPS_W)));
+                const bool nr = live && d2 < T2;
+                nearBits |= __ballot(nr);
+                const float w = weight_f32<EXPM>((float)d2, f, Lf, P);
+                F[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(live && !nr ? w : 0.0f, Bv, F[t], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) add_f32_sums(D[t], F[t]);
+        if (nearBits) {
+            // the chunk's near pairs in FP64, each lane for its own node
+            double4 p = valid ? vol[k] : make_double4(0.0, 0.0, 0.0, 1.0);
+            if (M.mask) mirror_pt(M, p.x, p.y, p.z);
+            for (int j = 0; j < cnt; ++j) {
+                const double rx = fma(C.X[j][0], 0.5, ogx), ry = fma(C.X[j][1], 0.5, ogy),
+                             rz = fma(C.X[j][2], 0.5, ogz);
+                if (!(fma(rz, rz, fma(ry, ry, fma(rx, rx, EPS_W))) < T2)) continue;
+                const int i = c0 + j;
+                const double4 sp = ps[i];
+                const Rec64 &pr = rec[nnodes + i];
+                const double ax = p.x - sp.x, ay = p.y - sp.y, az = p.z - sp.z;
+                const double lo = P.Ldef * rsqrt_nr(ax * ax + ay * ay + az * az + EPS_W);
+                const double w = sp.w * wfun<EXPM>(lo, P);
+                e0 += w * (pr.K[0] * ax + pr.K[1] * ay + pr.K[2] * az + pr.d[0]);
+                e1 += w * (pr.K[3] * ax + pr.K[4] * ay + pr.K[5] * az + pr.d[1]);
+                e2 += w * (pr.K[6] * ax + pr.K[7] * ay + pr.K[8] * az + pr.d[2]);
+                ed += w;
+            }
+        }
+    }
+    // D[t] register j: node 16t + 4j + R, column n.  Through LDS, 32 nodes
+    // at a time, each lane gathers its node's 13 sums (own node: row R).
+    double n0 = 0.0, n1 = 0.0, n2 = 0.0, dd = 0.0;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        wave_sync();
+#pragma unroll
+        for (int tt = 0; tt < 2; ++tt)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) L.acc[16 * tt + 4 * j + R][nn] = D[2 * h + tt][j];
+        wave_sync();
+        if ((lane >> 5) == h) {
+            const double *sm = L.acc[lane & 31];
+            n0 = sm[0] * ogx + sm[1] * ogy + sm[2] * ogz + sm[9];
+            n1 = sm[3] * ogx + sm[4] * ogy + sm[5] * ogz + sm[10];
+            n2 = sm[6] * ogx + sm[7] * ogy + sm[8] * ogz + sm[11];
+            dd = sm[12];
+        }
+    }
+    if (valid) {
+        const double4 vb = vol[k];
+        finish_fwd(P, M, k, perm[k], vb, n0 + e0, n1 + e1, n2 + e2, dd + ed, xv, den, acc);
+    }
+}
+
 
 constexpr int RT = 16;  // terms per reverse tile
 constexpr int GROUP_NS = 4;  // walk waves per group of the sparse-seed reverse (k_fast_rev_group)
@@ -3056,19 +3198,20 @@ static void fast_fwd_fch(hipStream_t st, const DevSurface &S, const DevVolume &V
     else fast_fwd_launch<PREC, EXPM, 32>(st, S, V, P, M, xv, plan, blk0, nblk);
 }
 
-template <int PREC, int EXPM>
-static void exact_fwd_g(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, const Mirror &M,
-                        double *xv, int g) {
+template <int EXPM>
+static void exact_fwd_mc(hipStream_t st, const DevSurface &S, const DevVolume &V, const WParams &P, const Mirror &M,
+                         double *xv, int g) {
     const unsigned grid = xcd_grid(V.n);
-    if (g == 4)
-        k_exact_fwd<PREC, EXPM, 4><<<grid, 256, 0, st>>>((int)V.n, V.vol, V.perm, S.n, S.ps, S.pd, S.pd32, P, M, xv,
-                                                        V.den, V.acc);
-    else if (g == 16)
-        k_exact_fwd<PREC, EXPM, 16><<<grid, 256, 0, st>>>((int)V.n, V.vol, V.perm, S.n, S.ps, S.pd, S.pd32, P, M, xv,
-                                                         V.den, V.acc);
+    const int nrec = S.nnodes + S.n;
+    if (g == 16)
+        k_exact_fwd_mc<EXPM, 16><<<grid, 256, 0, st>>>((int)V.n, V.vol, V.perm, S.n, S.nnodes, nrec, S.rec, S.rec32,
+                                                      S.ps, P, M, xv, V.den, V.acc);
+    else if (g == 32)
+        k_exact_fwd_mc<EXPM, 32><<<grid, 256, 0, st>>>((int)V.n, V.vol, V.perm, S.n, S.nnodes, nrec, S.rec, S.rec32,
+                                                      S.ps, P, M, xv, V.den, V.acc);
     else
-        k_exact_fwd<PREC, EXPM, 8><<<grid, 256, 0, st>>>((int)V.n, V.vol, V.perm, S.n, S.ps, S.pd, S.pd32, P, M, xv,
-                                                        V.den, V.acc);
+        k_exact_fwd_mc<EXPM, 64><<<grid, 256, 0, st>>>((int)V.n, V.vol, V.perm, S.n, S.nnodes, nrec, S.rec, S.rec32,
+                                                      S.ps, P, M, xv, V.den, V.acc);
 }
 
 template <int EXPM>
@@ -3090,7 +3233,7 @@ static void forward_impl(hipStream_t st, const DevSurface &S, const DevVolume &V
         if (prec == IDW_PREC_FP64)
             k_exact_fwd<IDW_PREC_FP64, EXPM, 8><<<xcd_grid(V.n), 256, 0, st>>>((int)V.n, V.vol, V.perm, S.n, S.ps, S.pd,
                                                                               S.pd32, P, M, xv, V.den, V.acc);
-        else exact_fwd_g<IDW_PREC_MIXED, EXPM>(st, S, V, P, M, xv, T.exactG);
+        else exact_fwd_mc<EXPM>(st, S, V, P, M, xv, T.exactG);
     } else {
         if (prec == IDW_PREC_FP64) fast_fwd_fch<IDW_PREC_FP64, EXPM>(st, S, V, P, M, xv, plan, blk0, nblk, T.fch);
         else fast_fwd_fch<IDW_PREC_MIXED, EXPM>(st, S, V, P, M, xv, plan, blk0, nblk, T.fch);

@@ -284,7 +284,7 @@ class USMesh:
 
     TUNING_NAMES = ("forwardChunk", "mergeWindow", "walkBuffer", "exactGroup", "exactChunkShift")
     TUNING_SLOTS = (("walkBuffer", (64, 128, 256)), ("forwardChunk", (16, 32, 64)), ("mergeWindow", (8, 16, 32)),
-                    ("exactGroup", (4, 8, 16)), ("exactChunkShift", (-1, 0, 1)))
+                    ("exactGroup", (16, 32, 64)), ("exactChunkShift", (-1, 0, 1)))
 
     def setTuning(self, **params):
         """Set tile parameters by name (forwardChunk, mergeWindow, walkBuffer,

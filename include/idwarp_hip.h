@@ -267,9 +267,10 @@ int idw_fast_counts(idw_ctx *ctx, double *counts8);
  * calls it for every context it records. */
 int idw_tune(idw_ctx *ctx, int force);
 /* The chosen parameters {forward chunk, merge window, walk buffer, exact
- * FP32 group, exact chunk shift (-1, 0, 1)} and, if ms is not NULL, the 15
+ * forward chunk (surface points staged per chunk by the mixed exact
+ * forward: 16, 32, 64), exact chunk shift (-1, 0, 1)} and, if ms is not NULL, the 15
  * timings of the last search in ms per launch (walk buffer 64/128/256,
- * forward chunk 16/32/64, merge window 8/16/32, exact group 4/8/16, exact
+ * forward chunk 16/32/64, merge window 8/16/32, exact chunk 16/32/64, exact
  * chunks x1/2, x1, x2; -1 where not applicable).  Returns 0 if never tuned,
  * 1 if tuned for the current options and sizes, 2 if tuned for others. */
 int idw_get_tuning(idw_ctx *ctx, int32_t *params5, double *ms, int64_t nms);

@@ -524,7 +524,7 @@ def test_autotuner_candidates_and_choice(case):
             g.warpDeriv(v, solverVec=False)
             r = _rel(g.getdXs().ravel(), dref)
             assert err <= tol and r <= atol, (name, val, err, r)
-        g.setTuning(**{name: dict(forwardChunk=32, mergeWindow=32, walkBuffer=128, exactGroup=8,
+        g.setTuning(**{name: dict(forwardChunk=32, mergeWindow=32, walkBuffer=128, exactGroup=64,
                                   exactChunkShift=0)[name]})
     g.tune()
     chosen, table, state = g.getTuning()
