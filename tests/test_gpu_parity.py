@@ -758,6 +758,43 @@ def test_mixed_adjoint_sparse_and_dense_seed_groups(planes):
         assert abs(dxv @ jp - got["mixed"] @ p) <= 1e-6 * abs(dxv @ jp)
 
 
+@pytest.mark.parametrize("nplanes", [0, 1, 2])
+def test_exact_mixed_warp_on_the_matrix_cores(nplanes):
+    """The mixed exact warp sums every surface point on the matrix cores, a
+    chunk at a time, and re-scans a chunk holding a pair nearer than the
+    near-wall distance to add that pair in FP64.  Volume nodes are placed on
+    every surface point (d = 0: near pairs in every pass) and 2e-6 off them,
+    beyond the default near-wall distance (FP32 pairs whose r is tiny against
+    the wave's extent).  For 0, 1 and 2 symmetry planes and each chunk size
+    the warp and the reverse must stay within the mixed bar of the oracle."""
+    m, planes = _symmetric(nplanes)
+    m = dict(m)
+    rng = np.random.default_rng(41)
+    d = rng.standard_normal(m["pts"].shape)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    m["xv0"] = np.concatenate([m["xv0"], m["pts"], m["pts"] + 2.0e-6 * d])
+    sym = [[list(p), list(n)] for p, n in planes] or None
+    ora = O.OracleMesh(m["pts"], m["conn"], m["faceSizes"], m["xv0"], evalMode="exact", symmetryPlanes=planes or None)
+    xs = m["perturb"](m["pts"])
+    ora.setSurfaceCoordinates(xs)
+    ref = ora.warpMesh()
+    v = synthetic.random_dxv(len(m["xv0"]), seed=42, frac=0.5)
+    dref = ora.warpDeriv(v)
+    gpu = USMesh(options=dict(gridFile=m["xv0"], fileType="memory", evalMode="exact", precision="mixed",
+                              symmetryPlanes=sym))
+    gpu.setSurfaceDefinition(m["pts"], m["conn"], m["faceSizes"])
+    gpu.setSurfaceCoordinates(xs)
+    for ch in (16, 32, 64):
+        gpu.setTuning(exactGroup=ch)
+        gpu.warpMesh()
+        got = gpu.getWarpGrid()
+        err = np.abs(got - ref).max() / ora.Ldef0
+        gpu.warpDeriv(v, solverVec=False)
+        r = _rel(gpu.getdXs().ravel(), dref)
+        print("planes", nplanes, "chunk", ch, "max |dXv|/Ldef %.2e, adjoint rel %.2e" % (err, r), flush=True)
+        assert np.isfinite(got).all() and err <= MIXED_TOL and r <= MIXED_TOL
+
+
 @pytest.mark.parametrize("planes", [False, True])
 def test_exact_adjoint_over_packed_seeded_nodes(planes):
     """The exact reverse runs over the volume nodes with a nonzero seed only,
