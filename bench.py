@@ -209,13 +209,18 @@ def main():
         for _ in range(args.warmup):
             step()
         torch.cuda.synchronize()
-        phases = np.zeros(6)
         t1 = time.perf_counter()
         for _ in range(args.steps):
             step()
-            phases += mesh.getTimings()  # HIP events on the context stream (synchronises it)
         torch.cuda.synchronize()
         wall = (time.perf_counter() - t1) * 1e3 / max(args.steps, 1)
+        # the phases over as many steps again: reading a step's HIP events
+        # waits for the step, which the timed loop above must not do
+        phases = np.zeros(6)
+        for _ in range(args.steps):
+            step()
+            phases += mesh.getTimings()
+        torch.cuda.synchronize()
         phases /= max(args.steps, 1)
         out[leg] = dict(ms=wall, phases=phases, xv=d_xv, dxs=d_dxs)
         log("%s: %.3f ms/step  [nodal %.3f | walk %.3f | fwd %.3f | rev setup %.3f | rev %.3f | nodal rev %.3f]" %
@@ -391,13 +396,16 @@ def ellipsoid_exact(args):
         for _ in range(args.warmup):
             step()
         torch.cuda.synchronize()
-        phases = np.zeros(6)
         t1 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t1) * 1e3 / max(args.steps, 1)
+        phases = np.zeros(6)  # over as many steps again (reading the events waits for each step)
         for _ in range(args.steps):
             step()
             phases += mesh.getTimings()
         torch.cuda.synchronize()
-        wall = (time.perf_counter() - t1) * 1e3 / max(args.steps, 1)
         phases /= max(args.steps, 1)
         out[prec] = dict(ms=wall, phases=phases, xv=d_xv.clone(), dxs=d_dxs.clone())
         log("%s: %.3f ms/step [nodal %.3f | fwd %.3f | rev setup %.3f | rev %.3f | nodal rev %.3f]" %

@@ -1239,7 +1239,7 @@ int idw_initialize_warping(idw_ctx *c, const double *pts, int64_t n_pts, const d
                     (long long)hs.nodes.size(), (long long)idw::MAX_WALK_NODES);
     // free a previous surface
     idw::DevSurface &S = c->S;
-    void *olds[] = {S.nodes, S.whdr, S.rec, S.rec32, S.internal, S.sumOrder, S.csum, S.ps, S.normals0, S.nprops, S.corner, S.xuInd,
+    void *olds[] = {S.nodes, S.whdr, S.rec, S.rec32, S.internal, S.cut, S.topOrder, S.csum, S.ps, S.normals0, S.nprops, S.corner, S.xuInd,
                     S.facePtr, S.faceConn, S.n2eStart, S.n2e, S.slotStart, S.slots, S.fprops, S.adj,
                     S.ntmp, S.pb, S.fpropsd, c->xs, c->dxs, c->dxsIn, c->partial};
     for (void *p : olds) {
@@ -1269,7 +1269,6 @@ int idw_initialize_warping(idw_ctx *c, const double *pts, int64_t n_pts, const d
     S.pd = S.rec + S.nnodes;
     S.pd32 = S.rec32 + S.nnodes;
     TRY(dalloc(c, &S.internal, S.nInternal));
-    TRY(dalloc(c, &S.sumOrder, S.nnodes));
     TRY(dalloc(c, &S.csum, (size_t)S.nnodes * idw::CSUM));
     TRY(dalloc(c, &S.ps, S.n));
     TRY(dalloc(c, &S.normals0, 3 * (size_t)S.n));
@@ -1295,28 +1294,41 @@ int idw_initialize_warping(idw_ctx *c, const double *pts, int64_t n_pts, const d
     std::vector<int32_t> internal;
     for (int k = 0; k < S.nnodes; ++k)
         if (!H.nodes[k].leaf) internal.push_back(k);
-    // bottom-up order of the cluster data: buckets, then internal nodes by
-    // decreasing depth
-    std::vector<int32_t> sumOrder;
-    for (int k = 0; k < S.nnodes; ++k)
-        if (H.nodes[k].leaf) sumOrder.push_back(k);
-    S.nLeaves = (int)sumOrder.size();
+    // bottom-up order of the cluster data: the subtrees rooted at the cut
+    // depth (and the buckets above it), each with its deepest depth, then
+    // the internal nodes above the cut by decreasing depth
     int maxDepth = 0;
-    for (int k : internal) maxDepth = std::max(maxDepth, H.nodes[k].depth);
+    for (const auto &nd : H.nodes) maxDepth = std::max(maxDepth, nd.depth);
     if (maxDepth >= idw::MAX_LEVELS) return fail(IDW_EINVAL, "kd-tree depth %d exceeds %d", maxDepth, idw::MAX_LEVELS - 1);
-    for (int d = maxDepth; d >= 0 && !internal.empty(); --d) {
-        const int start = (int)sumOrder.size();
-        for (int k : internal)
-            if (H.nodes[k].depth == d) sumOrder.push_back(k);
-        S.lvlStart[S.nLevels] = start;
-        S.lvlCount[S.nLevels] = (int)sumOrder.size() - start;
-        ++S.nLevels;
+    const int cutDepth = std::min(maxDepth, idw::CUT_DEPTH);
+    std::vector<int32_t> cut, topOrder;
+    for (int k = 0; k < S.nnodes; ++k) {
+        const auto &nd = H.nodes[k];
+        if (nd.depth == cutDepth || (nd.leaf && nd.depth < cutDepth)) {
+            int deepest = nd.depth;
+            for (int j = k; j < nd.skip; ++j) deepest = std::max(deepest, H.nodes[j].depth);
+            cut.push_back(k);
+            cut.push_back(deepest);
+        }
     }
+    S.nCut = (int)cut.size() / 2;
+    for (int d = cutDepth - 1; d >= 0 && S.nInternal; --d) {
+        const int start = (int)topOrder.size();
+        for (int k : internal)
+            if (H.nodes[k].depth == d) topOrder.push_back(k);
+        S.top.start[S.top.n] = start;
+        S.top.count[S.top.n] = (int)topOrder.size() - start;
+        ++S.top.n;
+    }
+    TRY(dalloc(c, &S.cut, cut.size()));
+    if (!topOrder.empty()) TRY(dalloc(c, &S.topOrder, topOrder.size()));
     std::vector<double4> ps(S.n);
     for (int i = 0; i < S.n; ++i) ps[i] = make_double4(H.xu0[3 * i], H.xu0[3 * i + 1], H.xu0[3 * i + 2], 0.0);
     hipStream_t st = c->st;
     HIPCHK(hipMemcpyAsync(S.internal, internal.data(), sizeof(int32_t) * internal.size(), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(S.sumOrder, sumOrder.data(), sizeof(int32_t) * sumOrder.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(S.cut, cut.data(), sizeof(int32_t) * cut.size(), hipMemcpyHostToDevice, st));
+    if (!topOrder.empty())
+        HIPCHK(hipMemcpyAsync(S.topOrder, topOrder.data(), sizeof(int32_t) * topOrder.size(), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(S.ps, ps.data(), sizeof(double4) * S.n, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(S.xuInd, H.xuInd.data(), sizeof(int32_t) * S.n, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(S.facePtr, H.facePtr.data(), sizeof(int32_t) * (H.nface + 1), hipMemcpyHostToDevice, st));

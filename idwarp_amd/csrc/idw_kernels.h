@@ -44,6 +44,13 @@ struct WParams {
 };
 
 constexpr int MAX_LEVELS = 64;  // tree depths the cluster-data pass handles
+constexpr int CUT_DEPTH = 9;    // the cluster-data pass's split: subtrees below, one block above
+
+// levels of the internal nodes above the cut, deepest first
+struct TopLevels {
+    int n = 0;
+    int start[CUT_DEPTH] = {}, count[CUT_DEPTH] = {};
+};
 constexpr int CSUM = 16;        // per-node sums of the cluster data (15 used)
 
 struct DevSurface {
@@ -51,13 +58,14 @@ struct DevSurface {
     NodeRec *nodes = nullptr;
     WalkHdr *whdr = nullptr;  // nnodes + 1 walk headers
     int32_t *internal = nullptr;
-    // the cluster data's bottom-up order: the buckets, then the internal nodes
-    // one depth at a time from the deepest (level i: sumOrder[lvlStart[i] ..
-    // lvlStart[i] + lvlCount[i])); csum holds CSUM sums per node
-    int32_t *sumOrder = nullptr;
+    // the cluster data bottom up (csum: CSUM sums per node): below the cut
+    // depth, nCut subtrees (cut: root, deepest depth of its subtree; a bucket
+    // above the cut is a subtree of its own), then the internal nodes above
+    // the cut one depth at a time from the deepest (topOrder, levels in top)
+    int32_t *cut = nullptr, *topOrder = nullptr;
+    int nCut = 0;
+    TopLevels top;
     double *csum = nullptr;
-    int nLeaves = 0, nLevels = 0;
-    int lvlStart[MAX_LEVELS] = {}, lvlCount[MAX_LEVELS] = {};
     double4 *ps = nullptr;  // Xu0, A (tree order)
     // term records: nnodes cluster records (index = tree node), then n point
     // records (index nnodes + tree-order point); pd / pd32 point at the latter
@@ -95,7 +103,7 @@ struct DevPlan {
     unsigned int *mismatch = nullptr;
     // per group of walk waves of the mixed reverse: 1 when its seeds were
     // sparse and the group reverse took it (written by each reverse)
-    uint8_t *groupSparse = nullptr;
+    uint32_t *groupSparse = nullptr;
     // deterministic reverse: the plan's slots sorted by record (detPos) and
     // each record's first sorted position (recStart, nrec + 1)
     int32_t *detPos = nullptr;

@@ -492,13 +492,9 @@ __global__ void k_corners(int n, double threshold, const int32_t *__restrict__ n
 // record is K = mean(K), d = mean(d) + mean(K (X - p)), the form of
 // d = K X + mean(b) with b = Xu - M Xu0 (the same for point and tangent
 // records, which enter linearly).
-__global__ void __launch_bounds__(256) k_leaf_sums(int n, const int32_t *__restrict__ list,
-                                                   const NodeRec *__restrict__ nodes,
-                                                   const double4 *__restrict__ ps, const Rec64 *__restrict__ pd,
-                                                   double *__restrict__ csum) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const int node = list[t];
+__device__ __forceinline__ void leaf_sums(int node, const NodeRec *__restrict__ nodes,
+                                          const double4 *__restrict__ ps, const Rec64 *__restrict__ pd,
+                                          double *csum) {
     const NodeRec &r = nodes[node];
     double acc[15];
     for (int k = 0; k < 15; ++k) acc[k] = 0.0;
@@ -516,12 +512,8 @@ __global__ void __launch_bounds__(256) k_leaf_sums(int n, const int32_t *__restr
     for (int k = 0; k < 15; ++k) o[k] = acc[k];
 }
 
-__global__ void __launch_bounds__(256) k_node_sums(int n, const int32_t *__restrict__ list,
-                                                   const NodeRec *__restrict__ nodes, double *__restrict__ csum,
-                                                   Rec64 *__restrict__ rec, Rec32 *__restrict__ rec32) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const int node = list[t];
+__device__ __forceinline__ void node_sums(int node, const NodeRec *__restrict__ nodes, double *csum,
+                                          Rec64 *__restrict__ rec, Rec32 *__restrict__ rec32) {
     const NodeRec &r = nodes[node];
     const int ch[2] = {node + 1, nodes[node + 1].skip};
     double acc[15];
@@ -550,6 +542,39 @@ __global__ void __launch_bounds__(256) k_node_sums(int n, const int32_t *__restr
     }
     rec[node] = e;
     rec32[node] = e32;
+}
+
+// The lower tree, one block per cut root (a node at the cut depth, or a
+// bucket above it): its subtree, the node range [root, skip), one depth at a
+// time from the subtree's deepest (cut[2 b + 1]) to the root's, a barrier
+// between depths.  Every sum a node reads is its children's, written by this
+// block one depth earlier.
+__global__ void __launch_bounds__(256) k_subtree_sums(const int32_t *__restrict__ cut,
+                                                      const NodeRec *__restrict__ nodes,
+                                                      const double4 *__restrict__ ps, const Rec64 *__restrict__ pd,
+                                                      double *csum, Rec64 *__restrict__ rec,
+                                                      Rec32 *__restrict__ rec32) {
+    const int root = cut[2 * blockIdx.x], deepest = cut[2 * blockIdx.x + 1];
+    const int end = nodes[root].skip, top = nodes[root].depth;
+    for (int d = deepest; d >= top; --d) {
+        for (int k = root + (int)threadIdx.x; k < end; k += blockDim.x) {
+            if (nodes[k].depth != d) continue;
+            if (nodes[k].leaf) leaf_sums(k, nodes, ps, pd, csum);
+            else node_sums(k, nodes, csum, rec, rec32);
+        }
+        __syncthreads();
+    }
+}
+
+// The internal nodes above the cut, in one block: level i is
+// list[T.start[i] .. T.start[i] + T.count[i]), from the deepest to the root.
+__global__ void __launch_bounds__(1024) k_top_sums(TopLevels T, const int32_t *__restrict__ list,
+                                                   const NodeRec *__restrict__ nodes, double *csum,
+                                                   Rec64 *__restrict__ rec, Rec32 *__restrict__ rec32) {
+    for (int i = 0; i < T.n; ++i) {
+        for (int t = threadIdx.x; t < T.count[i]; t += blockDim.x) node_sums(list[T.start[i] + t], nodes, csum, rec, rec32);
+        __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -2114,7 +2139,7 @@ k_fast_rev_pair(int nvol, int nwave, const double4 *__restrict__ vol, const int3
                 const double *__restrict__ dxv, const double *__restrict__ den, const PlanEntry *__restrict__ plan,
                 const int64_t *__restrict__ waveOff, const int64_t *__restrict__ waveSplit, int nnodes, int nrec,
                 const Rec64 *__restrict__ rec, WParams P, Mirror M, double *__restrict__ adj,
-                const uint8_t *__restrict__ groupSparse) {
+                const uint32_t *__restrict__ groupSparse) {
     static_assert(MQ == 8 || MQ == 16 || MQ == 32, "both merge buffers are filled by one load per lane");
     __shared__ PairLds<MQ> lds[4];
     const int npair = (nwave + 1) >> 1;
@@ -2439,7 +2464,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
                                                        const int64_t *__restrict__ waveOff,
                                                        const int64_t *__restrict__ waveSplit, int nnodes, int nrec,
                                                        const Rec64 *__restrict__ rec, WParams P, Mirror M,
-                                                       double *__restrict__ adj, uint8_t *__restrict__ groupSparse) {
+                                                       double *__restrict__ adj, uint32_t *__restrict__ groupSparse) {
     static_assert(NS == 2 || NS == 4 || NS == 8, "the lanes split evenly over the walk waves");
     using Lds = GroupLds<NS>;
     constexpr int MQ = Lds::MQ;
@@ -2459,7 +2484,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
         tot += __popcll(nzm[s]);
     }
     const bool sparse = tot <= Lds::MAXS;
-    if (lane == 0) groupSparse[grp] = sparse ? 1 : 0;
+    if (lane == 0) groupSparse[grp] = sparse ? 1u : 0u;
     if (!sparse || tot == 0) return;  // the pair reverse takes a dense group; no seed: nothing to add
     // origin: the group's first node (valid: some node of the group is)
     double4 o = vol[node0];
@@ -2869,6 +2894,19 @@ __global__ void k_nodal_rev_a(int n, int fast, int useRot, const NodeRec *__rest
 
 // GETELEMENTPROPS_B per face with the seeds of all its slots summed:
 // area seed sum (sab + n_f . snb)/nPts, unit-normal seed dA_f * sum snb.
+// The fan's points are reloaded (centroid-relative) where an edge needs
+// them, and each point's seed is written once, to its face slot, then
+// shifted by the centroid's share: no per-point arrays, so nothing goes to
+// scratch memory.  The arithmetic and its order are the per-face form's
+// (point j's seed sums edge j's term and edge j-1's, from zero, in the
+// order edge 0 ... edge np-1 adds them).
+__device__ __forceinline__ void fan_point(const double *xs, const int32_t *xuInd, const int32_t *conn, int j,
+                                          const double *c, double *p) {
+#pragma clang fp contract(off)
+    load_xu(xs, xuInd, conn[j], p);
+    p[0] -= c[0]; p[1] -= c[1]; p[2] -= c[2];
+}
+
 __global__ void k_nodal_rev_face(int nface, const int32_t *__restrict__ facePtr, const int32_t *__restrict__ faceConn,
                                  const int32_t *__restrict__ xuInd, const double *__restrict__ xs,
                                  const double4 *__restrict__ fprops, const double *__restrict__ ntmp,
@@ -2877,57 +2915,83 @@ __global__ void k_nodal_rev_face(int nface, const int32_t *__restrict__ facePtr,
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= nface) return;
     const int p0 = facePtr[f], np = facePtr[f + 1] - p0;
+    const int32_t *conn = faceConn + p0;
     const double4 fp = fprops[f];
     double areab = 0.0, nrb[3] = {0.0, 0.0, 0.0};
     for (int j = 0; j < np; ++j) {
-        const double *t = ntmp + (int64_t)faceConn[p0 + j] * 8;
+        const double *t = ntmp + (int64_t)conn[j] * 8;
         areab += (t[6] + (fp.y * t[3] + fp.z * t[4] + fp.w * t[5])) / np;
         nrb[0] += t[3]; nrb[1] += t[4]; nrb[2] += t[5];
     }
     nrb[0] *= fp.x; nrb[1] *= fp.x; nrb[2] *= fp.x;
     // recompute the fan (radial vectors, cross products) of getElementProps
-    double P[20][3], c[3] = {0.0, 0.0, 0.0};
+    double c[3] = {0.0, 0.0, 0.0};
     for (int j = 0; j < np; ++j) {
-        load_xu(xs, xuInd, faceConn[p0 + j], P[j]);
-        c[0] += P[j][0]; c[1] += P[j][1]; c[2] += P[j][2];
+        double q[3];
+        load_xu(xs, xuInd, conn[j], q);
+        c[0] += q[0]; c[1] += q[1]; c[2] += q[2];
     }
     c[0] /= np; c[1] /= np; c[2] /= np;
-    for (int j = 0; j < np; ++j) { P[j][0] -= c[0]; P[j][1] -= c[1]; P[j][2] -= c[2]; }
-    double nsum[3] = {0.0, 0.0, 0.0};
+    double first[3], a[3], b[3], nsum[3] = {0.0, 0.0, 0.0};
+    fan_point(xs, xuInd, conn, 0, c, first);
+    a[0] = first[0]; a[1] = first[1]; a[2] = first[2];
     for (int j = 0; j < np; ++j) {
+        if (j + 1 < np) fan_point(xs, xuInd, conn, j + 1, c, b);
+        else { b[0] = first[0]; b[1] = first[1]; b[2] = first[2]; }
         double cr[3];
-        cross3(P[j], P[j + 1 < np ? j + 1 : 0], cr);
+        cross3(a, b, cr);
         const double cn = sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2] + 1e-15);
         nsum[0] += cr[0] / cn; nsum[1] += cr[1] / cn; nsum[2] += cr[2] / cn;
+        a[0] = b[0]; a[1] = b[1]; a[2] = b[2];
     }
     // normal = nsum / |nsum|
     const double t2 = nsum[0] * nsum[0] + nsum[1] * nsum[1] + nsum[2] * nsum[2], t = sqrt(t2);
     const double g = -((nsum[0] * nrb[0] + nsum[1] * nrb[1] + nsum[2] * nrb[2]) / (t2 * t));
     const double nsb[3] = {nrb[0] / t + nsum[0] * g, nrb[1] / t + nsum[1] * g, nrb[2] / t + nsum[2] * g};
-    double rvb[20][3];
-    for (int j = 0; j < np; ++j) rvb[j][0] = rvb[j][1] = rvb[j][2] = 0.0;
-    for (int j = 0; j < np; ++j) {
-        const int j2 = j + 1 < np ? j + 1 : 0;
+    // seed of the cross product cr = a x b of the edge (a, b)
+    auto edge_crb = [&](const double *ea, const double *eb, double *crb) {
         double cr[3];
-        cross3(P[j], P[j2], cr);
+        cross3(ea, eb, cr);
         const double cn = sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2] + 1e-15);
         // area += cn/2, nsum += cr/cn
         const double cnb = 0.5 * areab - (cr[0] * nsb[0] + cr[1] * nsb[1] + cr[2] * nsb[2]) / (cn * cn);
-        double crb[3];
         for (int k = 0; k < 3; ++k) crb[k] = nsb[k] / cn + cr[k] * (cnb / cn);
-        // cr = a x b: ab += b x crb, bb += crb x a
-        const double *a = P[j], *b = P[j2];
-        rvb[j][0] += b[1] * crb[2] - b[2] * crb[1];
-        rvb[j][1] += b[2] * crb[0] - b[0] * crb[2];
-        rvb[j][2] += b[0] * crb[1] - b[1] * crb[0];
-        rvb[j2][0] += crb[1] * a[2] - crb[2] * a[1];
-        rvb[j2][1] += crb[2] * a[0] - crb[0] * a[2];
-        rvb[j2][2] += crb[0] * a[1] - crb[1] * a[0];
-    }
+    };
+    // edge j-1 (from point j-1 to point j): its seed and its first point;
+    // before point 0 that is edge np-1
+    double prevA[3], prevCrb[3];
+    fan_point(xs, xuInd, conn, np - 1, c, prevA);
+    edge_crb(prevA, first, prevCrb);
     double cb[3] = {0.0, 0.0, 0.0};
-    for (int j = 0; j < np; ++j) { cb[0] -= rvb[j][0]; cb[1] -= rvb[j][1]; cb[2] -= rvb[j][2]; }
-    for (int j = 0; j < np; ++j)
-        for (int k = 0; k < 3; ++k) pb[3 * (int64_t)(p0 + j) + k] = rvb[j][k] + cb[k] / np;
+    a[0] = first[0]; a[1] = first[1]; a[2] = first[2];
+    for (int j = 0; j < np; ++j) {
+        if (j + 1 < np) fan_point(xs, xuInd, conn, j + 1, c, b);
+        else { b[0] = first[0]; b[1] = first[1]; b[2] = first[2]; }
+        double crb[3];
+        edge_crb(a, b, crb);
+        // cr = a x b: ab += b x crb (edge j), bb += crb x a (edge j-1's a)
+        const double xa[3] = {b[1] * crb[2] - b[2] * crb[1], b[2] * crb[0] - b[0] * crb[2],
+                              b[0] * crb[1] - b[1] * crb[0]};
+        const double xb[3] = {prevCrb[1] * prevA[2] - prevCrb[2] * prevA[1],
+                              prevCrb[2] * prevA[0] - prevCrb[0] * prevA[2],
+                              prevCrb[0] * prevA[1] - prevCrb[1] * prevA[0]};
+        double rv[3];
+        for (int k = 0; k < 3; ++k) {
+            rv[k] = 0.0;
+            if (j == 0) { rv[k] += xa[k]; rv[k] += xb[k]; }
+            else { rv[k] += xb[k]; rv[k] += xa[k]; }
+            cb[k] -= rv[k];
+        }
+        double *o = pb + 3 * (int64_t)(p0 + j);
+        o[0] = rv[0]; o[1] = rv[1]; o[2] = rv[2];
+        prevA[0] = a[0]; prevA[1] = a[1]; prevA[2] = a[2];
+        prevCrb[0] = crb[0]; prevCrb[1] = crb[1]; prevCrb[2] = crb[2];
+        a[0] = b[0]; a[1] = b[1]; a[2] = b[2];
+    }
+    for (int j = 0; j < np; ++j) {
+        double *o = pb + 3 * (int64_t)(p0 + j);
+        for (int k = 0; k < 3; ++k) o[k] = o[k] + cb[k] / np;
+    }
 }
 
 // dXs(XuInd(i)) = Bb_i + sum of the face-slot adjoints of node i
@@ -3127,10 +3191,8 @@ void launch_corners(hipStream_t st, const DevSurface &S, double threshold) {
 
 void launch_cluster_data(hipStream_t st, const DevSurface &S) {
     if (!S.nInternal) return;  // a single bucket: no cluster records
-    k_leaf_sums<<<grid_for(S.nLeaves, 256), 256, 0, st>>>(S.nLeaves, S.sumOrder, S.nodes, S.ps, S.pd, S.csum);
-    for (int i = 0; i < S.nLevels; ++i)
-        k_node_sums<<<grid_for(S.lvlCount[i], 256), 256, 0, st>>>(S.lvlCount[i], S.sumOrder + S.lvlStart[i], S.nodes,
-                                                                  S.csum, S.rec, S.rec32);
+    k_subtree_sums<<<S.nCut, 256, 0, st>>>(S.cut, S.nodes, S.ps, S.pd, S.csum, S.rec, S.rec32);
+    if (S.top.n) k_top_sums<<<1, 1024, 0, st>>>(S.top, S.topOrder, S.nodes, S.csum, S.rec, S.rec32);
 }
 
 void launch_errors(hipStream_t st, const DevSurface &S, const WParams &P, const double *sphere60) {
