@@ -1469,27 +1469,50 @@ __device__ __forceinline__ void swap_rows16(float &x, float &y) {
     y = __uint_as_float(r[1]);
 }
 
-// Add FP32 sums in the f32 16x16x4 layout (register j of lane l: row
-// 4(l/16) + j, column l%16) to FP64 sums in the f64 layout (register j:
-// row 4j + l/16): the 4 x 4 blocks over (lane row, register) are transposed
-// with two rounds of row swaps, then converted and added.
+// FP32 sums come in the f32 16x16x4 layout (register j of lane l: row
+// 4(l/16) + j, column l%16); the FP64 sums they are added to are kept in the
+// same layout while only FP32 chunks add to them, and moved to the f64
+// layout (register j: row 4j + l/16) once, by transposing the 4 x 4 blocks
+// over (lane row, register) with two rounds of row swaps.  Each sum then
+// takes the same additions in the same order as a per-chunk transpose of
+// the FP32 sums would give it.
 __device__ __forceinline__ void add_f32_sums(dvec4 &D, fvec4 F) {
-    float f0 = F[0], f1 = F[1], f2 = F[2], f3 = F[3];
-    swap_rows32(f0, f2);
-    swap_rows32(f1, f3);
-    swap_rows16(f0, f1);
-    swap_rows16(f2, f3);
-    D[0] += (double)f0;
-    D[1] += (double)f1;
-    D[2] += (double)f2;
-    D[3] += (double)f3;
+    D[0] += (double)F[0];
+    D[1] += (double)F[1];
+    D[2] += (double)F[2];
+    D[3] += (double)F[3];
+}
+__device__ __forceinline__ void swap_rows32(double &x, double &y) {
+    const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)__double2loint(x), (unsigned)__double2loint(y), false,
+                                                     false);
+    const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)__double2hiint(x), (unsigned)__double2hiint(y), false,
+                                                     false);
+    x = __hiloint2double((int)hi[0], (int)lo[0]);
+    y = __hiloint2double((int)hi[1], (int)lo[1]);
+}
+__device__ __forceinline__ void swap_rows16(double &x, double &y) {
+    const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)__double2loint(x), (unsigned)__double2loint(y), false,
+                                                     false);
+    const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)__double2hiint(x), (unsigned)__double2hiint(y), false,
+                                                     false);
+    x = __hiloint2double((int)hi[0], (int)lo[0]);
+    y = __hiloint2double((int)hi[1], (int)lo[1]);
+}
+__device__ __forceinline__ void f32_to_f64_layout(dvec4 &D) {
+    double d0 = D[0], d1 = D[1], d2 = D[2], d3 = D[3];
+    swap_rows32(d0, d2);
+    swap_rows32(d1, d3);
+    swap_rows16(d0, d1);
+    swap_rows16(d2, d3);
+    D = dvec4{d0, d1, d2, d3};
 }
 
 // Forward sums of the terms in plan entries [a, b) into D (node block t in
 // D[t]); vx/vy/vz: positions of the lane's rows (nodes 16t + lane%16).
 // The entries of the next chunk are requested while this one is summed.
 // W32 (mixed precision, FP32 terms): FP32 weights and B rows on
-// v_mfma_f32_16x16x4f32, each chunk's FP32 sums added to the FP64 sums;
+// v_mfma_f32_16x16x4f32, each chunk's FP32 sums added to the FP64 sums, which
+// W32 keeps in the f32 layout (the caller moves them to the f64 layout);
 // vx/vy/vz are then the rows' positions g relative to the wave origin and
 // g2 = |g|^2, and |r|^2 is formed in FP64 as g2 + (|y|^2 + eps) - 2 g.y
 // (y = x - o staged with the term).  Its cancellation, about
@@ -1624,6 +1647,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
             }
             fwd_terms<true, EXPM, FCH>(e0, es, plan, nnodes, nrec, rec, rec32, P, L.c32, ox, oy, oz, vx, vy, vz, g2,
                                        D);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) f32_to_f64_layout(D[t]);  // D was zero before the FP32 terms
             v = node_pos();
         }
     }
@@ -1770,8 +1795,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
             }
         }
     }
-    // D[t] register j: node 16t + 4j + R, column n.  Through LDS, 32 nodes
-    // at a time, each lane gathers its node's 13 sums (own node: row R).
+    // D[t] register j (the f32 layout: only FP32 chunks added to it): node
+    // 16t + 4R + j, column n.  Through LDS, 32 nodes at a time, each lane
+    // gathers its node's 13 sums.
     double n0 = 0.0, n1 = 0.0, n2 = 0.0, dd = 0.0;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
@@ -1779,7 +1805,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
 #pragma unroll
         for (int tt = 0; tt < 2; ++tt)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) L.acc[16 * tt + 4 * j + R][nn] = D[2 * h + tt][j];
+            for (int j = 0; j < 4; ++j) L.acc[16 * tt + 4 * R + j][nn] = D[2 * h + tt][j];
         wave_sync();
         if ((lane >> 5) == h) {
             const double *sm = L.acc[lane & 31];
